@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-1e-8 of the 823,875-DOF tet elasticity solve on one
+MI355X, FP32-inner + FP64-refinement vs plain FP64 ILU(0)-PCG.
+
+Workload: 64^3 hexes x 6 tets (274,625 nodes, 1,572,864 tets), E_e =
+exp(0.5 z_e) from the seeded portable RNG, nu = 0.3, z=0 clamped, body
+force (0,0,-1e-3), 8-colour node order.  One step = CopyValue + Decompose
++ the full solve to relative residual 1e-8 with the matrix, factors' pattern
+and right-hand side resident in HBM.  The JSON line reports the
+FP32+refinement time (value) beside the FP64 time, DOF*iterations/s, the
+HBM roofline of the dominant kernel and the CPU reference timed on a
+bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(nb: int, nnzb: int, s: int, read_y: bool) -> int:
+    """Algorithmic bytes of one 3x3 BSR MatVec: off-diagonal blocks and their
+    column indices, diagonal blocks, row starts, x read once, y written
+    (+ read once when y is scaled by beta)."""
+    ndof = 3 * nb
+    b = nnzb * (9 * s + 4) + nb * 9 * s + (nb + 1) * 4 + 2 * ndof * s
+    return b + (ndof * s if read_y else 0)
+
+
+def pcg_iter_bytes(nb: int, nnzb: int, s: int) -> int:
+    ndof = 3 * nb
+    nl = nnzb // 2
+    spmv = spmv_bytes(nb, nnzb, s, False)
+    fwd = nl * (9 * s + 4) + nb * 9 * s + nb * 8 + 2 * ndof * s
+    bwd = nl * (9 * s + 4) + nb * 8 + 2 * ndof * s
+    vec = 10 * ndof * s
+    return spmv + fwd + bwd + vec
+
+
+def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
+    """The CPU reference (oracle/_ref, the reference's own code compiled) or,
+    without it, the oracle restatement, on a bounded sample of the same
+    workload: full set-up + factorisation, then `max_cpu_iters` PCG
+    iterations; time-to-1e-8 extrapolated with the FP64 iteration count."""
+    import numpy as np
+
+    import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    t0 = time.perf_counter()
+    ind, ps = oracle.psup(prob.tet, prob.np_)
+    Sys = oracle.RefSystem(ind, ps) if kind == "reference" else oracle.System(ind, ps)
+    b = np.zeros(prob.ndof)
+    Sys.assemble(b, prob.myu, prob.lam, prob.rho, prob.g, prob.xyz, prob.tet)
+    Sys.set_fixed_bc(prob.flag)
+    oracle.set_rhs_zero(b, prob.flag)
+    t1 = time.perf_counter()
+    Sys.factor()
+    t2 = time.perf_counter()
+    _, hist = Sys.solve_pcg(b.copy(), 1e-8, max_cpu_iters)
+    t3 = time.perf_counter()
+    n_it = len(hist) - 2 if len(hist) - 1 > max_cpu_iters else len(hist) - 1
+    n_it = max(n_it, 1)
+    t_it = (t3 - t2) / n_it
+    t_solve = (t2 - t1) + t_it * iters_fp64
+    return {
+        "value": round(t_solve, 3),
+        "unit": "s (time-to-1e-8 = Decompose + FP64 PCG, iterations x measured s/iteration)",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"64^3 cube (823,875 DOF): pattern+assembly+BC {t1 - t0:.2f} s, "
+                   f"Decompose {t2 - t1:.3f} s, {n_it} PCG iterations {t3 - t2:.2f} s "
+                   f"({t_it * 1e3:.1f} ms/iter) extrapolated to {iters_fp64} iterations"),
+        "dof_iter_per_s": round(prob.ndof * n_it / (t3 - t2), 1),
+        "host": platform.processor() or platform.machine(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
+    ap.add_argument("--sigma", type=float, default=0.5)
+    ap.add_argument("--ordering", default="color8", choices=["color8", "natural"])
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--cpu-iters", type=int, default=60,
+                    help="PCG iterations of the bounded CPU sample (0 = skip)")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of "
+                                   "it (see DESIGN.md, sharding)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from delfem2_amd import solver as S
+    from delfem2_amd import workload as W
+    from delfem2_amd.problem import DeviceSystem
+
+    print(f"[bench] building {args.n}^3 problem ({args.ordering})", file=sys.stderr, flush=True)
+    prob = W.cube(args.n, sigma=args.sigma, seed=20261203, ordering=args.ordering)
+    G = DeviceSystem(prob, device=0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    assert G.assemble()
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t
+    nb, nnzb = prob.np_, G.A.ncrs
+    print(f"[bench] ndof={prob.ndof} nnzb={nnzb} setup={G.t_setup:.2f}s asm={t_asm * 1e3:.1f}ms",
+          file=sys.stderr, flush=True)
+
+    def step_fp64():
+        G.factor()
+        return G.solve(1e-8, 20000)
+
+    def step_mixed():
+        G.factor()
+        return G.solve_mixed(1e-8, inner_tol=args.inner_tol, max_outer=60, max_inner=20000)
+
+    for _ in range(args.warmup):
+        step_fp64()
+        step_mixed()
+    torch.cuda.synchronize()
+
+    # ---- timed: FP32 + FP64 refinement (the headline) ----
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        xm, hm, n_inner = step_mixed()
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+
+    # ---- timed: plain FP64 ----
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x64, h64, _ = step_fp64()
+    torch.cuda.synchronize()
+    t_fp64 = (time.perf_counter() - t0) / args.steps
+    it64 = len(h64) - 1
+    print(f"[bench] fp64 {t_fp64 * 1e3:.1f} ms ({it64} it), mixed {t_mixed * 1e3:.1f} ms "
+          f"({n_inner} inner, {len(hm) - 1} outer)", file=sys.stderr, flush=True)
+
+    err_mixed = (torch.linalg.norm(xm - x64) / torch.linalg.norm(x64)).item()
+    rr = G.b.clone()
+    G.A.MatVec(rr, -1.0, xm, 1.0)
+    relres_mixed = (torch.linalg.norm(rr) / torch.linalg.norm(G.b)).item()
+
+    # ---- dominant kernel: 3x3 BSR SpMV, timed with HIP events on its stream ----
+    xv = torch.rand(prob.ndof, dtype=torch.float64, device="cuda")
+    yv = torch.zeros_like(xv)
+    for _ in range(3):
+        G.A.MatVec(yv, 1.0, xv, 0.0)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    stream = torch.cuda.current_stream()
+    e0.record(stream)
+    for _ in range(args.spmv_reps):
+        G.A.MatVec(yv, 1.0, xv, 0.0)
+    e1.record(stream)
+    e1.synchronize()
+    spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
+    b_spmv = spmv_bytes(nb, nnzb, 8, True)
+    spmv_gbs = b_spmv / (spmv_ms * 1e-3) / 1e9
+
+    b_iter64 = pcg_iter_bytes(nb, nnzb, 8)
+    b_iter32 = pcg_iter_bytes(nb, nnzb, 4)
+    t_it64 = t_fp64 / max(it64, 1)
+
+    cpu = None
+    if args.cpu_iters > 0:
+        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(prob, it64, args.cpu_iters)
+
+    out = {
+        "metric": "time-to-1e-8 PCG at 823k DOF: FP32+refine vs FP64 (s, DOF·iter/s, %HBM)",
+        "value": round(t_mixed, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-refine",
+        "data": "synthetic (seeded voxel tet cube, lognormal E)",
+        "config": {"workload": f"tet elasticity cube {args.n}^3 hex x6 tets, {prob.ndof} DOF, "
+                               f"E~lognormal(0,{args.sigma}), nu=0.3, ILU(0)-PCG to 1e-8, "
+                               f"{args.ordering} order",
+                   "ndof": prob.ndof, "nnzb": nnzb, "ordering": args.ordering,
+                   "inner_tol": args.inner_tol},
+        "fp64": {"time_s": round(t_fp64, 5), "iterations": it64,
+                 "dof_iter_per_s": round(prob.ndof * it64 / t_fp64, 1),
+                 "hbm_frac_per_iter": round(b_iter64 / t_it64 / 1e9 / HBM_PEAK_GBS, 4)},
+        "mixed": {"time_s": round(t_mixed, 5), "inner_iterations": n_inner,
+                  "outer_iterations": len(hm) - 1,
+                  "dof_iter_per_s": round(prob.ndof * n_inner / t_mixed, 1),
+                  "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+                  "x_rel_err_vs_fp64": err_mixed, "rel_residual": relres_mixed},
+        "roofline": {"bound": "hbm", "kernel": "k_bsr_matvec<double> (3x3 BSR SpMV)",
+                     "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": b_spmv, "avg_launch_ms": round(spmv_ms, 5)},
+        "cpu_baseline": cpu,
+        "setup_s": round(G.t_setup, 3),
+        "assembly_ms": round(t_asm * 1e3, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,343 @@
+// ilu.hip -- CPreconditionerILU<double> ILU(0) on the device: symbolic set-up
+// with level scheduling (host), value copy, level-parallel numeric
+// factorisation and level-parallel triangular sweeps.
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+namespace dfm2h {
+
+// CopyValue (ls_ilu_block_sparse.cpp:1009-1047): ILU slot a2l[s] receives A
+// slot s; slots of the ILU pattern that A lacks stay 0.
+__global__ void k_copy_value(size_t ncrs_a, const int *__restrict__ a2l,
+                             const double *__restrict__ a_vcrs, double *__restrict__ l_vcrs) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncrs_a * 9) return;
+  const int d = a2l[t / 9];
+  if (d >= 0) l_vcrs[(size_t)d * 9 + t % 9] = a_vcrs[t];
+}
+
+__global__ void k_zero(double *p, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// ILU(0) numeric factorisation of one level (ls_ilu_block_sparse.cpp:286-356),
+// one thread per row, the reference's IKJ loop verbatim in arithmetic:
+//   for k in L(i) ascending, for j in U(k) ascending, (i,j) in pattern:
+//     A_ij -= A_ik * U_kj            (or D_i when j == i)
+//   D_i <- inverse(D_i) unless |det| <= 1e-30 (counted, left as is)
+//   U_ij <- D_i * U_ij
+// Row i's columns are sorted, so the slot of j in row i is found by a
+// pointer that only moves forward (j ascends for a fixed k).
+
+__device__ inline double det3(const double *a) {
+  return +a[0] * a[4] * a[8] + a[3] * a[7] * a[2] + a[6] * a[1] * a[5]
+         - a[0] * a[7] * a[5] - a[6] * a[4] * a[2] - a[3] * a[1] * a[8];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_ilu_decompose_level(const unsigned *__restrict__ rows, unsigned nrows,
+                      const unsigned *__restrict__ colind, const unsigned *__restrict__ rowptr,
+                      const unsigned *__restrict__ diaind, double *vcrs, double *vdia,
+                      int *nsing) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const unsigned i = rows[t];
+  const unsigned i0 = colind[i], i1 = colind[i + 1], idia = diaind[i];
+  for (unsigned ik = i0; ik < idia; ++ik) {
+    const unsigned k = rowptr[ik];
+    double vik[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) vik[q] = vcrs[(size_t)ik * 9 + q];
+    unsigned pos = ik + 1;
+    const unsigned k1 = colind[k + 1];
+    for (unsigned kj = diaind[k]; kj < k1; ++kj) {
+      const unsigned j = rowptr[kj];
+      double *vij;
+      if (j == i) {
+        vij = vdia + (size_t)i * 9;
+      } else {
+        while (pos < i1 && rowptr[pos] < j) ++pos;
+        if (pos >= i1 || rowptr[pos] != j) continue;
+        vij = vcrs + (size_t)pos * 9;
+      }
+      const double *vkj = vcrs + (size_t)kj * 9;
+      double u[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) u[q] = vkj[q];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        vij[a * 3 + 0] -= vik[a * 3 + 0] * u[0] + vik[a * 3 + 1] * u[3] + vik[a * 3 + 2] * u[6];
+        vij[a * 3 + 1] -= vik[a * 3 + 0] * u[1] + vik[a * 3 + 1] * u[4] + vik[a * 3 + 2] * u[7];
+        vij[a * 3 + 2] -= vik[a * 3 + 0] * u[2] + vik[a * 3 + 1] * u[5] + vik[a * 3 + 2] * u[8];
+      }
+    }
+  }
+  double d[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) d[q] = vdia[(size_t)i * 9 + q];
+  if (fabs(det3(d)) > 1.0e-30) {
+    // CalcInvMat3 (ls_ilu_block_sparse.cpp:84-103)
+    const double inv_det = 1.0 / det3(d);
+    double a[9];
+    a[0] = inv_det * (d[4] * d[8] - d[5] * d[7]);
+    a[1] = inv_det * (d[2] * d[7] - d[1] * d[8]);
+    a[2] = inv_det * (d[1] * d[5] - d[2] * d[4]);
+    a[3] = inv_det * (d[5] * d[6] - d[3] * d[8]);
+    a[4] = inv_det * (d[0] * d[8] - d[2] * d[6]);
+    a[5] = inv_det * (d[2] * d[3] - d[0] * d[5]);
+    a[6] = inv_det * (d[3] * d[7] - d[4] * d[6]);
+    a[7] = inv_det * (d[1] * d[6] - d[0] * d[7]);
+    a[8] = inv_det * (d[0] * d[4] - d[1] * d[3]);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      d[q] = a[q];
+      vdia[(size_t)i * 9 + q] = a[q];
+    }
+  } else {
+    atomicAdd(nsing, 1);
+  }
+  for (unsigned s = idia; s < i1; ++s) {
+    double *v = vcrs + (size_t)s * 9;
+    double tb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tb[q] = v[q];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      v[a * 3 + 0] = d[a * 3 + 0] * tb[0] + d[a * 3 + 1] * tb[3] + d[a * 3 + 2] * tb[6];
+      v[a * 3 + 1] = d[a * 3 + 0] * tb[1] + d[a * 3 + 1] * tb[4] + d[a * 3 + 2] * tb[7];
+      v[a * 3 + 2] = d[a * 3 + 0] * tb[2] + d[a * 3 + 1] * tb[5] + d[a * 3 + 2] * tb[8];
+    }
+  }
+}
+
+// Groups rows by dependency level.  Forward: lev(i) = 1 + max lev(j) over
+// the lower part of row i (0 when empty); backward: the same over the upper
+// part, rows visited from the last one.  Rows inside a level stay ascending.
+static void build_levels(unsigned n, const std::vector<unsigned> &colind,
+                         const std::vector<unsigned> &rowptr, const std::vector<unsigned> &diaind,
+                         bool forward, std::vector<unsigned> &levptr,
+                         std::vector<unsigned> &levrows) {
+  std::vector<unsigned> lev(n, 0);
+  unsigned nlev = 0;
+  if (forward) {
+    for (unsigned i = 0; i < n; ++i) {
+      unsigned l = 0;
+      for (unsigned s = colind[i]; s < diaind[i]; ++s) l = std::max(l, lev[rowptr[s]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  } else {
+    for (unsigned i = n; i-- > 0;) {
+      unsigned l = 0;
+      for (unsigned s = diaind[i]; s < colind[i + 1]; ++s) l = std::max(l, lev[rowptr[s]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  }
+  levptr.assign(nlev + 1, 0);
+  for (unsigned i = 0; i < n; ++i) levptr[lev[i] + 1]++;
+  for (unsigned l = 0; l < nlev; ++l) levptr[l + 1] += levptr[l];
+  levrows.resize(n);
+  std::vector<unsigned> cur(levptr.begin(), levptr.end() - 1);
+  for (unsigned i = 0; i < n; ++i) levrows[cur[lev[i]]++] = i;
+}
+
+// A slot -> ILU slot by column lookup in the ILU row (the row2crs map of
+// CopyValue), -1 where the ILU pattern lacks the column.
+static std::vector<int> build_a2l(const dfm2h_ilu_s &P, const Pattern &A) {
+  std::vector<int> a2l(A.ncrs, -1);
+  std::vector<int> row2crs(P.nblk, -1);
+  for (unsigned i = 0; i < P.nblk; ++i) {
+    for (unsigned s = P.colind[i]; s < P.colind[i + 1]; ++s) row2crs[P.rowptr[s]] = (int)s;
+    for (unsigned s = A.colind[i]; s < A.colind[i + 1]; ++s) a2l[s] = row2crs[A.rowptr[s]];
+    for (unsigned s = P.colind[i]; s < P.colind[i + 1]; ++s) row2crs[P.rowptr[s]] = -1;
+  }
+  return a2l;
+}
+
+}  // namespace dfm2h
+
+using namespace dfm2h;
+
+static void need_ilu(dfm2h_ilu P) {
+  DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null preconditioner handle");
+  DFM2H_REQUIRE(P->has_pattern, DFM2H_ERR_STATE, "SetPattern0 has not been called");
+}
+
+namespace dfm2h {
+
+void ilu_forward_levels_double(dfm2h_ilu P, const double *src, double *vec) {
+  for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l) {
+    const unsigned nrows = P->levptr_f[l + 1] - P->levptr_f[l];
+    hipLaunchKernelGGL(k_ilu_fwd_level<double>, dim3(ceil_div(nrows, kRowsPerBlock)), dim3(kBlock),
+                       0, stream(), P->levrows_f.p + P->levptr_f[l], nrows, P->colind_d.p,
+                       P->rowptr_d.p, P->diaind_d.p, P->vcrs.p, P->vdia.p, src, vec);
+  }
+  check_launch("k_ilu_fwd_level");
+}
+
+void ilu_backward_levels_double(dfm2h_ilu P, double *vec) {
+  for (size_t l = 0; l + 1 < P->levptr_b.size(); ++l) {
+    const unsigned nrows = P->levptr_b[l + 1] - P->levptr_b[l];
+    hipLaunchKernelGGL(k_ilu_bwd_level<double>, dim3(ceil_div(nrows, kRowsPerBlock)), dim3(kBlock),
+                       0, stream(), P->levrows_b.p + P->levptr_b[l], nrows, P->colind_d.p,
+                       P->rowptr_d.p, P->diaind_d.p, P->vcrs.p, vec);
+  }
+  check_launch("k_ilu_bwd_level");
+}
+
+}  // namespace dfm2h
+
+extern "C" {
+
+int dfm2h_ilu_create(dfm2h_ilu *out) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(out, DFM2H_ERR_INVALID, "null output");
+    *out = new dfm2h_ilu_s;
+  });
+}
+
+int dfm2h_ilu_destroy(dfm2h_ilu P) {
+  return guard(__func__, [&] { delete P; });
+}
+
+int dfm2h_ilu_set_pattern0(dfm2h_ilu P, dfm2h_bsr A) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P && A, DFM2H_ERR_INVALID, "null handle");
+    DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_STATE, "matrix has no pattern");
+    const Pattern &pa = A->pat;
+    P->nblk = pa.nblk;
+    P->ncrs = pa.ncrs;
+    P->colind = pa.colind;
+    P->rowptr = pa.rowptr;
+    for (unsigned i = 0; i < P->nblk; ++i)
+      std::sort(P->rowptr.begin() + P->colind[i], P->rowptr.begin() + P->colind[i + 1]);
+    P->diaind.assign(P->nblk, 0);
+    for (unsigned i = 0; i < P->nblk; ++i) {
+      P->diaind[i] = P->colind[i + 1];
+      for (unsigned s = P->colind[i]; s < P->colind[i + 1]; ++s)
+        if (P->rowptr[s] > i) {
+          P->diaind[i] = s;
+          break;
+        }
+    }
+    P->colind_d.upload(P->colind);
+    P->rowptr_d.upload(P->rowptr);
+    P->diaind_d.upload(P->diaind);
+    P->vcrs.alloc(P->ncrs * 9);
+    P->vdia.alloc((size_t)P->nblk * 9);
+    std::vector<unsigned> rows;
+    build_levels(P->nblk, P->colind, P->rowptr, P->diaind, true, P->levptr_f, rows);
+    P->levrows_f.upload(rows);
+    build_levels(P->nblk, P->colind, P->rowptr, P->diaind, false, P->levptr_b, rows);
+    P->levrows_b.upload(rows);
+    const std::vector<int> a2l = build_a2l(*P, pa);
+    P->a2l_d.upload(a2l);
+    P->mapped_for = A;
+    P->nsing_d.alloc(1);
+    P->has_pattern = true;
+    P->decomposed = false;
+    P->version++;
+    DFM2H_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int dfm2h_ilu_copy_value(dfm2h_ilu P, dfm2h_bsr A) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    DFM2H_REQUIRE(A && A->has_pattern, DFM2H_ERR_STATE, "matrix has no pattern");
+    DFM2H_REQUIRE(A->nblk == P->nblk, DFM2H_ERR_INVALID, "size mismatch");
+    if (P->mapped_for != A) {
+      const std::vector<int> a2l = build_a2l(*P, A->pat);
+      P->a2l_d.upload(a2l);
+      P->mapped_for = A;
+    }
+    if (P->ncrs) {
+      hipLaunchKernelGGL(k_zero, dim3(ceil_div(P->ncrs * 9, 256)), dim3(256), 0, stream(),
+                         P->vcrs.p, P->ncrs * 9);
+      check_launch("k_zero");
+    }
+    if (A->pat.ncrs) {
+      hipLaunchKernelGGL(k_copy_value, dim3(ceil_div(A->pat.ncrs * 9, 256)), dim3(256), 0,
+                         stream(), A->pat.ncrs, P->a2l_d.p, A->vcrs.p, P->vcrs.p);
+      check_launch("k_copy_value");
+    }
+    if (P->nblk)
+      DFM2H_HIP(hipMemcpyAsync(P->vdia.p, A->vdia.p, sizeof(double) * 9 * P->nblk,
+                               hipMemcpyDeviceToDevice, stream()));
+    P->decomposed = false;
+    P->version++;
+  });
+}
+
+int dfm2h_ilu_decompose(dfm2h_ilu P, int *ok, int *n_singular) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    P->nsing_d.zero();
+    for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l) {
+      const unsigned nrows = P->levptr_f[l + 1] - P->levptr_f[l];
+      hipLaunchKernelGGL(k_ilu_decompose_level, dim3(ceil_div(nrows, 64)), dim3(64), 0, stream(),
+                         P->levrows_f.p + P->levptr_f[l], nrows, P->colind_d.p, P->rowptr_d.p,
+                         P->diaind_d.p, P->vcrs.p, P->vdia.p, P->nsing_d.p);
+    }
+    check_launch("k_ilu_decompose_level");
+    int ns = 0;
+    DFM2H_HIP(hipMemcpyAsync(&ns, P->nsing_d.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    DFM2H_HIP(hipStreamSynchronize(stream()));
+    if (ok) *ok = ns > 10 ? 0 : 1;
+    if (n_singular) *n_singular = ns;
+    P->decomposed = true;
+    P->version++;
+  });
+}
+
+int dfm2h_ilu_forward(dfm2h_ilu P, double *vec) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
+    ilu_forward_levels_double(P, vec, vec);
+  });
+}
+
+int dfm2h_ilu_backward(dfm2h_ilu P, double *vec) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
+    ilu_backward_levels_double(P, vec);
+  });
+}
+
+int dfm2h_ilu_solve_precond(dfm2h_ilu P, double *vec) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
+    ilu_forward_levels_double(P, vec, vec);
+    ilu_backward_levels_double(P, vec);
+  });
+}
+
+int dfm2h_ilu_download(dfm2h_ilu P, unsigned *rowptr, unsigned *diaind, double *vcrs,
+                       double *vdia) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    if (rowptr) std::copy(P->rowptr.begin(), P->rowptr.end(), rowptr);
+    if (diaind) std::copy(P->diaind.begin(), P->diaind.end(), diaind);
+    if (vcrs) P->vcrs.download(vcrs, P->ncrs * 9);
+    if (vdia) P->vdia.download(vdia, (size_t)P->nblk * 9);
+    DFM2H_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int dfm2h_ilu_levels(dfm2h_ilu P, unsigned *nf, unsigned *nb) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    if (nf) *nf = (unsigned)(P->levptr_f.size() - 1);
+    if (nb) *nb = (unsigned)(P->levptr_b.size() - 1);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// runtime.hip -- device selection, stream, error strings, memory helpers,
+// host pattern build and the tet connectivity handle.
+#include "common.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace dfm2h {
+
+static thread_local std::string g_last_error;
+static hipStream_t g_stream = nullptr;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+hipStream_t stream() { return g_stream; }
+
+void check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw Error(DFM2H_ERR_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
+}
+
+}  // namespace dfm2h
+
+using namespace dfm2h;
+
+extern "C" {
+
+int dfm2h_init(int device) {
+  return guard(__func__, [&] { DFM2H_HIP(hipSetDevice(device)); });
+}
+
+int dfm2h_set_stream(void *s) {
+  g_stream = static_cast<hipStream_t>(s);
+  return DFM2H_OK;
+}
+
+void *dfm2h_get_stream(void) { return g_stream; }
+
+int dfm2h_synchronize(void) {
+  return guard(__func__, [&] { DFM2H_HIP(hipStreamSynchronize(stream())); });
+}
+
+const char *dfm2h_last_error(void) { return g_last_error.c_str(); }
+
+int dfm2h_malloc(void **p, size_t bytes) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(p, DFM2H_ERR_INVALID, "null output");
+    DFM2H_HIP(hipMalloc(p, bytes));
+  });
+}
+
+int dfm2h_free(void *p) {
+  return guard(__func__, [&] { DFM2H_HIP(hipFree(p)); });
+}
+
+int dfm2h_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  return guard(__func__, [&] {
+    DFM2H_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    DFM2H_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int dfm2h_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  return guard(__func__, [&] {
+    DFM2H_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    DFM2H_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+int dfm2h_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  return guard(__func__, [&] {
+    DFM2H_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+  });
+}
+
+// Element-around-point lists by counting sort (ascending element order per
+// point), then the distinct neighbours of each point in visiting order, then
+// an ascending sort of every row -- the set the reference builds with
+// JArray_ElSuP_MeshElem / JArrayPointSurPoint_MeshOneRingNeighborhood /
+// JArray_Sort (msh_topology_uniform.cpp:19-51, 250-316; jagarray.cpp:72-92).
+int dfm2h_psup_tet(const unsigned *tet, size_t ntet, size_t np, unsigned *psup_ind,
+                   unsigned *psup, size_t psup_cap, size_t *npsup) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(tet || ntet == 0, DFM2H_ERR_INVALID, "null tet");
+    DFM2H_REQUIRE(psup_ind, DFM2H_ERR_INVALID, "null psup_ind");
+    for (size_t i = 0; i < ntet * 4; ++i)
+      DFM2H_REQUIRE(tet[i] < np, DFM2H_ERR_INVALID, "tet node index out of range");
+    std::vector<unsigned> es_ind(np + 1, 0), es(ntet * 4);
+    for (size_t i = 0; i < ntet * 4; ++i) es_ind[tet[i] + 1]++;
+    for (size_t p = 0; p < np; ++p) es_ind[p + 1] += es_ind[p];
+    {
+      std::vector<unsigned> cur(es_ind.begin(), es_ind.end() - 1);
+      for (size_t e = 0; e < ntet; ++e)
+        for (int k = 0; k < 4; ++k) es[cur[tet[e * 4 + k]]++] = (unsigned)e;
+    }
+    std::vector<unsigned> flg(np, UINT_MAX);
+    std::vector<unsigned> out;
+    out.reserve(np * 15);
+    psup_ind[0] = 0;
+    for (size_t p = 0; p < np; ++p) {
+      flg[p] = (unsigned)p;
+      const size_t row0 = out.size();
+      for (unsigned s = es_ind[p]; s < es_ind[p + 1]; ++s) {
+        const unsigned e = es[s];
+        for (int k = 0; k < 4; ++k) {
+          const unsigned q = tet[e * 4 + k];
+          if (flg[q] != p) {
+            flg[q] = (unsigned)p;
+            out.push_back(q);
+          }
+        }
+      }
+      std::sort(out.begin() + row0, out.end());
+      psup_ind[p + 1] = (unsigned)out.size();
+    }
+    if (npsup) *npsup = out.size();
+    if (psup) {
+      DFM2H_REQUIRE(psup_cap >= out.size(), DFM2H_ERR_INVALID, "psup capacity too small");
+      std::memcpy(psup, out.data(), sizeof(unsigned) * out.size());
+    }
+  });
+}
+
+int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet, size_t np) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(out, DFM2H_ERR_INVALID, "null output");
+    DFM2H_REQUIRE(tet || ntet == 0, DFM2H_ERR_INVALID, "null tet");
+    DFM2H_REQUIRE(ntet < (1ull << 32) && np < (1ull << 32), DFM2H_ERR_INVALID,
+                  "mesh too large for 32-bit indices");
+    for (size_t i = 0; i < ntet * 4; ++i)
+      DFM2H_REQUIRE(tet[i] < np, DFM2H_ERR_INVALID, "tet node index out of range");
+    auto *M = new dfm2h_tetmesh_s;
+    try {
+      M->ntet = ntet;
+      M->np = np;
+      std::vector<unsigned> ind(np + 1, 0), es(ntet * 4);
+      for (size_t i = 0; i < ntet * 4; ++i) ind[tet[i] + 1]++;
+      for (size_t p = 0; p < np; ++p) ind[p + 1] += ind[p];
+      std::vector<unsigned> cur(ind.begin(), ind.end() - 1);
+      for (size_t e = 0; e < ntet; ++e)
+        for (int k = 0; k < 4; ++k) es[cur[tet[e * 4 + k]]++] = (unsigned)e;
+      M->tet.upload(tet, ntet * 4);
+      M->elsup_ind.upload(ind);
+      M->elsup.upload(es);
+      DFM2H_HIP(hipStreamSynchronize(stream()));
+    } catch (...) {
+      delete M;
+      throw;
+    }
+    *out = M;
+  });
+}
+
+int dfm2h_tetmesh_destroy(dfm2h_tetmesh M) {
+  return guard(__func__, [&] { delete M; });
+}
+
+}  // extern "C"

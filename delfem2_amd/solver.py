@@ -1,0 +1,302 @@
+"""Python mirror of DelFEM2's solve-path API over the HIP library.
+
+Same names and argument meaning as the reference C++ API so that callers
+(and tests) read like the reference's own code:
+
+* ``CMatrixSparse``      -- ls_block_sparse.h:24-204 (3x3 blocks, diagonal kept apart)
+* ``CPreconditionerILU`` -- ls_ilu_block_sparse.h:21-60
+* ``MergeLinSys_SolidLinear_Static_MeshTet3D`` -- fem_solidlinear.h:192-237
+* ``Solve_PCG``          -- lsitrsol.h:80-148 (fused on the device)
+* ``Dot / AddScaledVec / ScaleAndAddVec / setRHS_Zero / XPlusAY`` --
+  view_vectorx.h:61-109, vecxitrsol.cpp:226-296
+
+Vectors are torch tensors on the GPU (float64, contiguous): torch is only the
+device allocator and stream here; every computation is a HIP kernel of
+libdfm2_hip.so.  The reference's ``bool`` returns are kept (Decompose, Merge),
+other failures raise ``DFM2HError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+_initialised = False
+
+
+def init(device: int = 0) -> None:
+    """Binds the library to ``cuda:device`` and to torch's current stream."""
+    global _initialised
+    N.load()
+    torch.cuda.set_device(device)
+    N.call("dfm2h_init", device)
+    N.call("dfm2h_set_stream", C.c_void_p(torch.cuda.current_stream(device).cuda_stream))
+    _initialised = True
+
+
+def _ensure():
+    if not _initialised:
+        init(torch.cuda.current_device() if torch.cuda.is_available() else 0)
+
+
+def _dptr(t: torch.Tensor, dtype=torch.float64) -> C.c_void_p:
+    if not t.is_cuda:
+        raise ValueError("expected a GPU tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _hptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def synchronize() -> None:
+    N.call("dfm2h_synchronize")
+
+
+def JArray_PSuP_MeshElem(tet: np.ndarray, np_: int):
+    """Sorted one-ring pattern of a tet mesh (msh_topology_uniform.cpp:301-316
+    + jagarray.cpp:72-92), built on the host: (psup_ind, psup)."""
+    tet = np.ascontiguousarray(tet, np.uint32)
+    ind = np.zeros(np_ + 1, np.uint32)
+    n = C.c_size_t(0)
+    N.call("dfm2h_psup_tet", _hptr(tet), tet.size // 4, np_, _hptr(ind), None, 0, C.byref(n))
+    arr = np.zeros(max(n.value, 1), np.uint32)
+    N.call("dfm2h_psup_tet", _hptr(tet), tet.size // 4, np_, _hptr(ind), _hptr(arr), arr.size,
+           C.byref(n))
+    return ind, arr[: n.value]
+
+
+class CMatrixSparse:
+    """3x3 block-CSR matrix with separate diagonal blocks, values in HBM."""
+
+    def __init__(self):
+        _ensure()
+        self.h = None
+        self.nrowblk_ = 0
+        self.nrowdim_ = 0
+
+    def Initialize(self, nblk: int, len_: int, is_dia: bool) -> None:
+        h = C.c_void_p()
+        N.call("dfm2h_bsr_create", C.byref(h), int(nblk), int(len_), int(bool(is_dia)))
+        self._free()
+        self.h = h
+        self.nrowblk_ = int(nblk)
+        self.nrowdim_ = int(len_)
+
+    def SetPattern(self, colind: np.ndarray, rowptr: np.ndarray) -> None:
+        colind = np.ascontiguousarray(colind, np.uint32)
+        rowptr = np.ascontiguousarray(rowptr, np.uint32)
+        N.call("dfm2h_bsr_set_pattern", self.h, _hptr(colind), colind.size, _hptr(rowptr),
+               rowptr.size)
+        self.col_ind_ = colind.copy()
+        self.row_ptr_ = rowptr.copy()
+
+    def setZero(self) -> bool:
+        N.call("dfm2h_bsr_set_zero", self.h)
+        return True
+
+    @property
+    def ncrs(self) -> int:
+        n = C.c_size_t(0)
+        N.call("dfm2h_bsr_info", self.h, None, C.byref(n), None, None)
+        return n.value
+
+    def MatVec(self, y: torch.Tensor, alpha: float, x: torch.Tensor, beta: float) -> None:
+        N.call("dfm2h_bsr_matvec", self.h, _dptr(y), float(alpha), _dptr(x), float(beta))
+
+    def SetFixedBC(self, bc_flag: torch.Tensor) -> None:
+        N.call("dfm2h_bsr_set_fixed_bc", self.h, _dptr(bc_flag, torch.int32))
+
+    def upload(self, val_crs: np.ndarray | None, val_dia: np.ndarray | None) -> None:
+        vc = None if val_crs is None else np.ascontiguousarray(val_crs, np.float64)
+        vd = None if val_dia is None else np.ascontiguousarray(val_dia, np.float64)
+        N.call("dfm2h_bsr_upload", self.h, None if vc is None else _hptr(vc),
+               None if vd is None else _hptr(vd))
+
+    def download(self):
+        vc = np.zeros(self.ncrs * 9)
+        vd = np.zeros(self.nrowblk_ * 9)
+        N.call("dfm2h_bsr_download", self.h, _hptr(vc), _hptr(vd))
+        return vc, vd
+
+    def _free(self):
+        if getattr(self, "h", None):
+            N.load().dfm2h_bsr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+
+class TetMesh:
+    """Tet connectivity on the device plus the element-around-node lists that
+    the gather assembly walks (built once per mesh)."""
+
+    def __init__(self, tet: np.ndarray, np_: int):
+        _ensure()
+        tet = np.ascontiguousarray(tet, np.uint32)
+        h = C.c_void_p()
+        N.call("dfm2h_tetmesh_create", C.byref(h), _hptr(tet), tet.size // 4, int(np_))
+        self.h = h
+        self.ntet = tet.size // 4
+        self.np_ = int(np_)
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.load().dfm2h_tetmesh_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def MergeLinSys_SolidLinear_Static_MeshTet3D(mat_A: CMatrixSparse, vec_b: torch.Tensor,
+                                              myu, lam, rho: float, g, aXYZ: torch.Tensor,
+                                              mesh: TetMesh, aDisp: torch.Tensor | None = None):
+    """fem_solidlinear.h:192-237 on the device.  ``myu``/``lam`` are scalars
+    (the reference signature) or per-element float64 GPU tensors.  Returns
+    True, or False when an element edge is missing from the pattern (the
+    reference's Merge result)."""
+    g = np.ascontiguousarray(np.asarray(g, np.float64).reshape(3))
+    if isinstance(myu, torch.Tensor):
+        pm, pl, sm, sl = _dptr(myu), _dptr(lam), 0.0, 0.0
+    else:
+        pm, pl, sm, sl = None, None, float(myu), float(lam)
+    st = N.load().dfm2h_merge_linsys_solidlinear_static_tet(
+        mat_A.h, _dptr(vec_b), pm, pl, sm, sl, float(rho), _hptr(g), _dptr(aXYZ), mesh.h,
+        None if aDisp is None else _dptr(aDisp))
+    if st == N.ERR_PATTERN:
+        return False
+    N.check(st)
+    return True
+
+
+class CPreconditionerILU:
+    """Block ILU(0) with level-scheduled factorisation and sweeps."""
+
+    def __init__(self):
+        _ensure()
+        h = C.c_void_p()
+        N.call("dfm2h_ilu_create", C.byref(h))
+        self.h = h
+        self.n_singular = 0
+
+    def SetPattern0(self, m: CMatrixSparse) -> None:
+        N.call("dfm2h_ilu_set_pattern0", self.h, m.h)
+        self.nblk = m.nrowblk_
+        self.ncrs = m.ncrs
+
+    def Initialize_ILUk(self, m: CMatrixSparse, fill_level: int) -> None:
+        if fill_level != 0:
+            raise NotImplementedError("only ILU(0) (fill_level=0) is implemented")
+        self.SetPattern0(m)
+
+    def CopyValue(self, m: CMatrixSparse) -> None:
+        N.call("dfm2h_ilu_copy_value", self.h, m.h)
+
+    def Decompose(self) -> bool:
+        ok = C.c_int(0)
+        ns = C.c_int(0)
+        N.call("dfm2h_ilu_decompose", self.h, C.byref(ok), C.byref(ns))
+        self.n_singular = ns.value
+        return bool(ok.value)
+
+    def SolvePrecond(self, vec: torch.Tensor) -> None:
+        N.call("dfm2h_ilu_solve_precond", self.h, _dptr(vec))
+
+    def ForwardSubstitution(self, vec: torch.Tensor) -> None:
+        N.call("dfm2h_ilu_forward", self.h, _dptr(vec))
+
+    def BackwardSubstitution(self, vec: torch.Tensor) -> None:
+        N.call("dfm2h_ilu_backward", self.h, _dptr(vec))
+
+    def levels(self):
+        f = C.c_uint(0)
+        b = C.c_uint(0)
+        N.call("dfm2h_ilu_levels", self.h, C.byref(f), C.byref(b))
+        return f.value, b.value
+
+    def download(self):
+        rp = np.zeros(max(self.ncrs, 1), np.uint32)
+        di = np.zeros(self.nblk, np.uint32)
+        vc = np.zeros(max(self.ncrs * 9, 1))
+        vd = np.zeros(self.nblk * 9)
+        N.call("dfm2h_ilu_download", self.h, _hptr(rp), _hptr(di), _hptr(vc), _hptr(vd))
+        return rp[: self.ncrs], di, vc[: self.ncrs * 9], vd
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.load().dfm2h_ilu_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+# ---- view_vectorx.h operator set -------------------------------------------
+
+def Dot(x: torch.Tensor, y: torch.Tensor) -> float:
+    out = C.c_double(0.0)
+    N.call("dfm2h_vec_dot", _dptr(x), _dptr(y), x.numel(), C.byref(out))
+    return out.value
+
+
+def AddScaledVec(y: torch.Tensor, alpha: float, x: torch.Tensor) -> None:
+    N.call("dfm2h_vec_add_scaled", _dptr(y), float(alpha), _dptr(x), y.numel())
+
+
+def ScaleAndAddVec(y: torch.Tensor, beta: float, x: torch.Tensor) -> None:
+    N.call("dfm2h_vec_scale_and_add", _dptr(y), float(beta), _dptr(x), y.numel())
+
+
+def AddMatVec(lhs: torch.Tensor, scale_lhs: float, scale_rhs: float, mat: CMatrixSparse,
+              rhs: torch.Tensor) -> None:
+    mat.MatVec(lhs, scale_rhs, rhs, scale_lhs)
+
+
+def setRHS_Zero(vec_b: torch.Tensor, aBCFlag: torch.Tensor, iflag_nonzero: int = 0) -> None:
+    N.call("dfm2h_set_rhs_zero", _dptr(vec_b), _dptr(aBCFlag, torch.int32), vec_b.numel(),
+           int(iflag_nonzero))
+
+
+def XPlusAY(X: torch.Tensor, aBCFlag: torch.Tensor, alpha: float, Y: torch.Tensor) -> None:
+    N.call("dfm2h_xplusay", _dptr(X), X.numel(), _dptr(aBCFlag, torch.int32), float(alpha),
+           _dptr(Y))
+
+
+def Solve_PCG(r_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float, max_nitr: int,
+              mat: CMatrixSparse, ilu: CPreconditionerILU) -> list[float]:
+    """lsitrsol.h:80-148, fused on the device.  r_vec holds b and receives the
+    final residual, x_vec the solution.  Returns the residual-norm history
+    (absolute norms, entry 0 = |b|)."""
+    cap = int(max_nitr) + 2
+    hist = np.zeros(cap)
+    nh = C.c_size_t(0)
+    N.call("dfm2h_solve_pcg", mat.h, ilu.h, _dptr(r_vec), _dptr(x_vec), float(conv_ratio_tol),
+           int(max_nitr), _hptr(hist), cap, C.byref(nh))
+    return hist[: nh.value].tolist()
+
+
+def Solve_PCG_Mixed(b_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float,
+                    mat: CMatrixSparse, ilu: CPreconditionerILU, inner_tol: float = 1e-4,
+                    max_outer: int = 50, max_inner: int = 2000):
+    """FP64 iterative refinement around an FP32 ILU(0)-PCG (no reference
+    counterpart).  Returns (outer residual-norm history, total inner iterations)."""
+    cap = int(max_outer) + 2
+    hist = np.zeros(cap)
+    nh = C.c_size_t(0)
+    ni = C.c_size_t(0)
+    N.call("dfm2h_solve_pcg_mixed", mat.h, ilu.h, _dptr(b_vec), _dptr(x_vec),
+           float(conv_ratio_tol), float(inner_tol), int(max_outer), int(max_inner), _hptr(hist),
+           cap, C.byref(nh), C.byref(ni))
+    return hist[: nh.value].tolist(), ni.value

@@ -1,0 +1,204 @@
+/*
+ * dfm2_hip.h -- C ABI of the MI355X (gfx950) implementation of DelFEM2's
+ * implicit tet-FEM solve path:
+ *
+ *   tet assembly -> 3x3 block-CSR CMatrixSparse -> CPreconditionerILU ILU(0)
+ *   -> Solve_PCG
+ *
+ * Plain pointers and sizes only.  Unless stated otherwise, array arguments
+ * named *_dev are device pointers and the others are host pointers.  Every
+ * call is ordered on the library stream (dfm2h_set_stream); calls that
+ * return host values synchronise that stream.  Every function returns a
+ * status code (DFM2H_OK == 0); dfm2h_last_error() describes the last
+ * failure of the calling thread.  No C++ exception crosses this boundary.
+ *
+ * Citations name the reference function each entry point replaces, relative
+ * to the reference tree's include/delfem2/.
+ *
+ * Layout (same as CMatrixSparse<double> with 3x3 blocks):
+ *   colind[nblk+1]  row-start offsets        (reference member col_ind_)
+ *   rowptr[ncrs]    block column indices     (reference member row_ptr_)
+ *   val_crs[9*ncrs] off-diagonal blocks, row-major 3x3
+ *   val_dia[9*nblk] diagonal blocks, row-major 3x3
+ *   vectors         AoS v[3*node + d]
+ */
+#ifndef DFM2_HIP_H
+#define DFM2_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DFM2H_OK 0
+#define DFM2H_ERR_INVALID 1       /* bad handle, size or argument            */
+#define DFM2H_ERR_HIP 2           /* a HIP runtime call failed               */
+#define DFM2H_ERR_PATTERN 3       /* Merge: block missing from the pattern   */
+#define DFM2H_ERR_SINGULAR 4      /* Decompose: more than 10 singular pivots */
+#define DFM2H_ERR_NOT_CONVERGED 5 /* Krylov solve hit max_nitr               */
+#define DFM2H_ERR_NAN 6           /* residual became NaN / Inf               */
+#define DFM2H_ERR_STATE 7         /* call out of order (e.g. no pattern yet) */
+
+typedef struct dfm2h_bsr_s *dfm2h_bsr;         /* CMatrixSparse<double>, 3x3 */
+typedef struct dfm2h_ilu_s *dfm2h_ilu;         /* CPreconditionerILU<double> */
+typedef struct dfm2h_tetmesh_s *dfm2h_tetmesh; /* tet connectivity on device */
+
+/* ---- runtime ------------------------------------------------------------ */
+/* Selects the HIP device for this thread.  No reference counterpart (the
+ * reference is host-only). */
+int dfm2h_init(int device);
+/* The stream every later call is ordered on (a hipStream_t; NULL = default
+ * stream).  No reference counterpart. */
+int dfm2h_set_stream(void *hip_stream);
+void *dfm2h_get_stream(void);
+int dfm2h_synchronize(void);
+const char *dfm2h_last_error(void);
+/* Device memory helpers for C/C++ callers without their own allocator. */
+int dfm2h_malloc(void **ptr_dev, size_t bytes);
+int dfm2h_free(void *ptr_dev);
+int dfm2h_memcpy_htod(void *dst_dev, const void *src, size_t bytes);
+int dfm2h_memcpy_dtoh(void *dst, const void *src_dev, size_t bytes);
+int dfm2h_memcpy_dtod(void *dst_dev, const void *src_dev, size_t bytes);
+
+/* ---- pattern (host) ------------------------------------------------------ */
+/* msh_topology_uniform.cpp:301-316 JArray_PSuP_MeshElem followed by
+ * jagarray.cpp:72-92 JArray_Sort, for 4-node tets.  psup_ind[np+1] is
+ * filled; psup (nullable, capacity psup_cap) receives the sorted one-ring
+ * without self; *npsup receives its length. */
+int dfm2h_psup_tet(const unsigned *tet, size_t ntet, size_t np,
+                   unsigned *psup_ind, unsigned *psup, size_t psup_cap,
+                   size_t *npsup);
+
+/* ---- CMatrixSparse<double> (ls_block_sparse.h) ---------------------------- */
+/* ls_block_sparse.h:45-55 Initialize(nblk, len, is_dia); len must be 3 and
+ * is_dia 1 (the tet path). */
+int dfm2h_bsr_create(dfm2h_bsr *out, size_t nblk, unsigned len, int is_dia);
+int dfm2h_bsr_destroy(dfm2h_bsr A);
+/* ls_block_sparse.h:72-85 SetPattern(colind, ncolind, rowptr, nrowptr);
+ * host arrays, copied.  May be called once per matrix. */
+int dfm2h_bsr_set_pattern(dfm2h_bsr A, const unsigned *colind, size_t ncolind,
+                          const unsigned *rowptr, size_t nrowptr);
+/* ls_block_sparse.h:92-105 setZero. */
+int dfm2h_bsr_set_zero(dfm2h_bsr A);
+/* Host <-> device copies of val_crs_ / val_dia_ (the reference exposes them
+ * as public host vectors; here they live in HBM).  Either pointer may be NULL. */
+int dfm2h_bsr_upload(dfm2h_bsr A, const double *val_crs, const double *val_dia);
+int dfm2h_bsr_download(dfm2h_bsr A, double *val_crs, double *val_dia);
+/* Sizes and the device arrays (nullable outputs). */
+int dfm2h_bsr_info(dfm2h_bsr A, size_t *nblk, size_t *ncrs,
+                   double **val_crs_dev, double **val_dia_dev);
+/* ls_block_sparse.cpp:215-273 MatVec: y = alpha*A*x + beta*y (y is scaled
+ * by beta first, unconditionally, then off-diagonal blocks in stored order,
+ * then the diagonal block: bit-identical to the reference). */
+int dfm2h_bsr_matvec(dfm2h_bsr A, double *y_dev, double alpha,
+                     const double *x_dev, double beta);
+/* ls_block_sparse.h:147-151 SetFixedBC(bc_flag) (-> cpp:397-472):
+ * bc_flag_dev[3*nblk], nonzero = fixed DOF. */
+int dfm2h_bsr_set_fixed_bc(dfm2h_bsr A, const int *bc_flag_dev);
+/* vecxitrsol.cpp:286-296 setRHS_Zero: b[i]=0 where flag[i]!=iflag_nonzero. */
+int dfm2h_set_rhs_zero(double *b_dev, const int *flag_dev, size_t n,
+                       int iflag_nonzero);
+/* vecxitrsol.cpp:226-237 XPlusAY: X[i] += alpha*Y[i] where flag[i]==0. */
+int dfm2h_xplusay(double *X_dev, size_t n, const int *flag_dev, double alpha,
+                  const double *Y_dev);
+
+/* ---- tet assembly (fem_solidlinear.h) ------------------------------------ */
+/* Element connectivity for assembly: tet[4*ntet] (host, copied) over np
+ * nodes.  Builds the element-around-node lists in ascending element order
+ * (msh_topology_uniform.cpp:19-51) that the gather assembly walks. */
+int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet,
+                         size_t np);
+int dfm2h_tetmesh_destroy(dfm2h_tetmesh M);
+/* fem_solidlinear.h:192-237 MergeLinSys_SolidLinear_Static_MeshTet3D with
+ * EMat_SolidLinear_Static_Tet (fem_solidlinear.cpp:409-435) merged by
+ * Merge<4,4,3,3> (ls_block_sparse.h:260-310).  Additive into A and
+ * vec_b_dev, bit-identical to the reference's ascending-element loop.
+ * myu_e_dev / lambda_e_dev: per-element Lame parameters [ntet], or NULL to
+ * use the scalars myu / lambda (the reference signature).  xyz_dev [3*np],
+ * disp_dev [3*np] or NULL (zero displacement), g[3] host.
+ * Returns DFM2H_ERR_PATTERN if an element edge is missing from A's pattern
+ * (the reference's Merge returns false). */
+int dfm2h_merge_linsys_solidlinear_static_tet(
+    dfm2h_bsr A, double *vec_b_dev, const double *myu_e_dev,
+    const double *lambda_e_dev, double myu, double lambda, double rho,
+    const double *g, const double *xyz_dev, dfm2h_tetmesh M,
+    const double *disp_dev);
+
+/* ---- CPreconditionerILU<double> (ls_ilu_block_sparse.h) ------------------- */
+int dfm2h_ilu_create(dfm2h_ilu *out);
+int dfm2h_ilu_destroy(dfm2h_ilu P);
+/* ls_ilu_block_sparse.cpp:1057-1090 SetPattern0(m) (also what
+ * Initialize_ILUk(m, 0) does, :854-857): copies A's pattern, sorts each row,
+ * finds the first upper slot of each row, and schedules the rows in
+ * dependency levels for the parallel factorisation and sweeps. */
+int dfm2h_ilu_set_pattern0(dfm2h_ilu P, dfm2h_bsr A);
+/* ls_ilu_block_sparse.cpp:1009-1047 CopyValue(m). */
+int dfm2h_ilu_copy_value(dfm2h_ilu P, dfm2h_bsr A);
+/* ls_ilu_block_sparse.cpp:152-420 Decompose() (3x3 branch 286-356).  Rows
+ * are factorised level by level, each row in the reference's IKJ order, so
+ * the factors are bit-identical.  *ok = 1 (true) or 0 when more than 10
+ * pivots were singular (|det| <= 1e-30; the reference then stops, here the
+ * remaining rows are still processed and the factors are unspecified).
+ * Either output may be NULL.  Returns DFM2H_OK in both cases. */
+int dfm2h_ilu_decompose(dfm2h_ilu P, int *ok, int *n_singular);
+/* ls_ilu_block_sparse.h:38-41 SolvePrecond(vec) = ForwardSubstitution
+ * (cpp:544-573) then BackwardSubstitution (cpp:721-749), in place. */
+int dfm2h_ilu_solve_precond(dfm2h_ilu P, double *vec_dev);
+int dfm2h_ilu_forward(dfm2h_ilu P, double *vec_dev);
+int dfm2h_ilu_backward(dfm2h_ilu P, double *vec_dev);
+/* Factor values (valCrs, valDia) and the sorted pattern (rowPtr, m_diaInd);
+ * host outputs, each nullable. */
+int dfm2h_ilu_download(dfm2h_ilu P, unsigned *rowptr, unsigned *diaind,
+                       double *val_crs, double *val_dia);
+/* Number of forward / backward dependency levels of the schedule. */
+int dfm2h_ilu_levels(dfm2h_ilu P, unsigned *n_forward, unsigned *n_backward);
+
+/* ---- view_vectorx.h operator set (device vectors of n doubles) ------------ */
+/* view_vectorx.h:104-109 Dot -> host double (deterministic tree order). */
+int dfm2h_vec_dot(const double *x_dev, const double *y_dev, size_t n,
+                  double *out);
+/* view_vectorx.h:61-71 AddScaledVec: y += alpha*x. */
+int dfm2h_vec_add_scaled(double *y_dev, double alpha, const double *x_dev,
+                         size_t n);
+/* view_vectorx.h:73-83 ScaleAndAddVec: y = beta*y + x. */
+int dfm2h_vec_scale_and_add(double *y_dev, double beta, const double *x_dev,
+                            size_t n);
+/* view_vectorx.h:44-47 deep-copy operator= and :49-51 setZero. */
+int dfm2h_vec_copy(double *dst_dev, const double *src_dev, size_t n);
+int dfm2h_vec_set_zero(double *x_dev, size_t n);
+
+/* ---- fused Solve_PCG (lsitrsol.h:80-148) ---------------------------------- */
+/* Solve_PCG(r, x, Pr, p, tol, max_nitr, A, ilu) with the preconditioner P
+ * (already decomposed from A).  r_dev holds b on entry and the final
+ * residual on return; x_dev receives the solution (set to zero first).
+ * hist (host, capacity hist_cap, nullable) receives the residual-norm
+ * history, absolute norms, entry 0 = |b|; *n_hist its length (as the
+ * reference's returned vector).  The whole iteration runs on the device
+ * (captured HIP graph, scalars kept in HBM); the host only polls the
+ * convergence flag between chunks of iterations.  Returns DFM2H_OK also
+ * when max_nitr is reached (as the reference); *n_hist tells. */
+int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r_dev, double *x_dev,
+                    double tol, unsigned max_nitr, double *hist,
+                    size_t hist_cap, size_t *n_hist);
+
+/* Mixed precision (no reference counterpart; the reference has no float
+ * ILU or PCG): FP64 iterative refinement around an FP32 ILU(0)-PCG.  The
+ * outer residual r = b - A x, the update x += d and the stop test
+ * |r|/|b| < tol run in FP64; the inner PCG (FP32 BSR values, FP32 factors
+ * rounded from P's FP64 factors) solves A d = r to relative tolerance
+ * inner_tol.  b_dev is not modified; x_dev receives the solution.  hist
+ * receives the outer residual norms (entry 0 = |b|); n_inner the total
+ * number of inner PCG iterations. */
+int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
+                          double *x_dev, double tol, double inner_tol,
+                          unsigned max_outer, unsigned max_inner,
+                          double *hist, size_t hist_cap, size_t *n_hist,
+                          size_t *n_inner);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFM2_HIP_H */

@@ -1,0 +1,539 @@
+/*
+ * oracle.c -- CPU restatement of DelFEM2's implicit tet-FEM solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * Build with -ffp-contract=off: every expression below keeps the reference's
+ * association so that results are bit-identical to the reference.
+ * Citations are relative to the reference tree's include/delfem2/.
+ */
+#include "oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* gridvoxel.cpp:194-271 -- nodes x-major (:210-218), 6 Kuhn tets per voxel
+ * taken from the 8 corner indices (:228-267). */
+void orc_mesh_tet_voxel(unsigned nx, unsigned ny, unsigned nz, double *xyz,
+                        unsigned *tet) {
+  const unsigned mx = nx + 1, my = ny + 1, mz = nz + 1;
+  size_t ip = 0;
+  for (unsigned ix = 0; ix < mx; ++ix)
+    for (unsigned iy = 0; iy < my; ++iy)
+      for (unsigned iz = 0; iz < mz; ++iz) {
+        xyz[ip * 3 + 0] = (double)ix;
+        xyz[ip * 3 + 1] = (double)iy;
+        xyz[ip * 3 + 2] = (double)iz;
+        ++ip;
+      }
+  /* corner c of voxel: bit pattern (dx,dy,dz) as in the reference corner map */
+  static const unsigned cdx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+  static const unsigned cdy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+  static const unsigned cdz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  static const unsigned split[6][4] = {{0, 1, 2, 6}, {0, 1, 6, 5}, {0, 4, 5, 6},
+                                       {0, 2, 3, 6}, {0, 3, 7, 6}, {0, 4, 6, 7}};
+  size_t it = 0;
+  for (unsigned ix = 0; ix < nx; ++ix)
+    for (unsigned iy = 0; iy < ny; ++iy)
+      for (unsigned iz = 0; iz < nz; ++iz) {
+        unsigned c[8];
+        for (int k = 0; k < 8; ++k)
+          c[k] = (ix + cdx[k]) * (my * mz) + (iy + cdy[k]) * mz + (iz + cdz[k]);
+        for (int s = 0; s < 6; ++s, ++it)
+          for (int k = 0; k < 4; ++k) tet[it * 4 + k] = c[split[s][k]];
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* msh_topology_uniform.cpp:19-51 (elements around a point by counting sort,
+ * ascending element order), :250-299 (one ring without self, marked with a
+ * per-point flag), :301-316 (driver); jagarray.cpp:72-92 (ascending sort of
+ * each row -- any correct sort yields the same rows since keys are distinct). */
+size_t orc_psup(const unsigned *elem, size_t nelem, unsigned nnoel, size_t np,
+                unsigned *psup_ind, unsigned *psup) {
+  unsigned *elsup_ind = (unsigned *)calloc(np + 1, sizeof(unsigned));
+  for (size_t ie = 0; ie < nelem; ++ie)
+    for (unsigned k = 0; k < nnoel; ++k) elsup_ind[elem[ie * nnoel + k] + 1] += 1;
+  for (size_t ip = 0; ip < np; ++ip) elsup_ind[ip + 1] += elsup_ind[ip];
+  unsigned *elsup = (unsigned *)malloc(sizeof(unsigned) * (elsup_ind[np] + 1));
+  unsigned *cur = (unsigned *)malloc(sizeof(unsigned) * (np + 1));
+  memcpy(cur, elsup_ind, sizeof(unsigned) * (np + 1));
+  for (size_t ie = 0; ie < nelem; ++ie)
+    for (unsigned k = 0; k < nnoel; ++k) elsup[cur[elem[ie * nnoel + k]]++] = (unsigned)ie;
+  free(cur);
+
+  unsigned *flg = (unsigned *)malloc(sizeof(unsigned) * (np + 1));
+  for (size_t ip = 0; ip < np; ++ip) flg[ip] = UINT_MAX;
+  psup_ind[0] = 0;
+  for (size_t ip = 0; ip < np; ++ip) {
+    unsigned cnt = 0;
+    flg[ip] = (unsigned)ip;
+    for (unsigned ies = elsup_ind[ip]; ies < elsup_ind[ip + 1]; ++ies) {
+      const unsigned ie = elsup[ies];
+      for (unsigned k = 0; k < nnoel; ++k) {
+        const unsigned jp = elem[ie * nnoel + k];
+        if (flg[jp] != ip) {
+          flg[jp] = (unsigned)ip;
+          if (psup) psup[psup_ind[ip] + cnt] = jp;
+          ++cnt;
+        }
+      }
+    }
+    psup_ind[ip + 1] = psup_ind[ip] + cnt;
+    if (psup) { /* insertion sort of the row, ascending */
+      unsigned *row = psup + psup_ind[ip];
+      for (unsigned a = 1; a < cnt; ++a) {
+        const unsigned v = row[a];
+        unsigned b = a;
+        while (b > 0 && row[b - 1] > v) { row[b] = row[b - 1]; --b; }
+        row[b] = v;
+      }
+    }
+  }
+  free(flg);
+  free(elsup);
+  free(elsup_ind);
+  return psup_ind[np];
+}
+
+/* ------------------------------------------------------------------------ */
+/* femutil.cpp:202-212: signed volume, scalar triple product times 1/6. */
+double orc_tet_volume(const double *v1, const double *v2, const double *v3,
+                      const double *v4) {
+  const double one_sixth = 1.0 / 6.0;
+  return ((v2[0] - v1[0]) * ((v3[1] - v1[1]) * (v4[2] - v1[2]) - (v4[1] - v1[1]) * (v3[2] - v1[2]))
+          - (v2[1] - v1[1]) * ((v3[0] - v1[0]) * (v4[2] - v1[2]) - (v4[0] - v1[0]) * (v3[2] - v1[2]))
+          + (v2[2] - v1[2]) * ((v3[0] - v1[0]) * (v4[1] - v1[1]) - (v4[0] - v1[0]) * (v3[1] - v1[1]))) *
+         one_sixth;
+}
+
+/* femutil.cpp:215-258: gradient of the barycentric coordinate of node n is
+ * the (signed) normal of the opposite face over 6*vol.  Node n uses the other
+ * three nodes in cyclic order (n+1, n+2, n+3) and the sign alternates
+ * (-,+,-,+) with n; the component signs are (s,-s,s). */
+void orc_tet_dldx(double dldx[4][3], const double *p0, const double *p1,
+                  const double *p2, const double *p3) {
+  const double vol = orc_tet_volume(p0, p1, p2, p3);
+  const double dtmp1 = 1.0 / (vol * 6.0);
+  const double *p[4] = {p0, p1, p2, p3};
+  for (int n = 0; n < 4; ++n) {
+    const double *qa = p[(n + 1) & 3], *qb = p[(n + 2) & 3], *qc = p[(n + 3) & 3];
+    const double s = (n & 1) ? dtmp1 : -dtmp1;
+    dldx[n][0] = s * ((qb[1] - qa[1]) * (qc[2] - qa[2]) - (qc[1] - qa[1]) * (qb[2] - qa[2]));
+    dldx[n][1] = -s * ((qb[0] - qa[0]) * (qc[2] - qa[2]) - (qc[0] - qa[0]) * (qb[2] - qa[2]));
+    dldx[n][2] = s * ((qb[0] - qa[0]) * (qc[1] - qa[1]) - (qc[0] - qa[0]) * (qb[1] - qa[1]));
+  }
+}
+
+/* fem_solidlinear.cpp:409-435 with :55-79 and femutil.h:252-268.
+ * emat is zeroed here (the driver zero-clears it, fem_solidlinear.h:223) and
+ * then accumulated additively exactly as the reference does. */
+void orc_emat_tet(double emat[4][4][3][3], double eres[4][3], double myu,
+                  double lambda, const double P[4][3], const double disp[4][3]) {
+  const double vol = orc_tet_volume(P[0], P[1], P[2], P[3]);
+  double d[4][3];
+  orc_tet_dldx(d, P[0], P[1], P[2], P[3]);
+  const double w = vol;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          double v = 0.0;
+          v += w * (lambda * d[i][a] * d[j][b] + myu * d[j][a] * d[i][b]);
+          emat[i][j][a][b] = v;
+        }
+      const double dtmp1 = d[i][0] * d[j][0] + d[i][1] * d[j][1] + d[i][2] * d[j][2];
+      emat[i][j][0][0] += w * myu * dtmp1;
+      emat[i][j][1][1] += w * myu * dtmp1;
+      emat[i][j][2][2] += w * myu * dtmp1;
+    }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int a = 0; a < 3; ++a)
+        eres[i][a] += -1. * (emat[i][j][a][0] * disp[j][0] + emat[i][j][a][1] * disp[j][1] +
+                             emat[i][j][a][2] * disp[j][2]);
+}
+
+/* fem_solidlinear.h:192-237 driver, per-element Lame pair; Merge<4,4,3,3>
+ * ls_block_sparse.h:260-310 (diagonal blocks to vdia, others to the slot of
+ * column j in row i). */
+int orc_assemble_tet_static(unsigned nblk, const unsigned *colind,
+                            const unsigned *rowptr, double *vcrs, double *vdia,
+                            double *vec_b, const double *myu_e,
+                            const double *lambda_e, double rho, const double *g,
+                            const double *xyz, const unsigned *tet, size_t ntet,
+                            const double *disp) {
+  unsigned *buf = (unsigned *)malloc(sizeof(unsigned) * (nblk + 1));
+  for (unsigned i = 0; i < nblk; ++i) buf[i] = UINT_MAX;
+  int status = 0;
+  for (size_t iel = 0; iel < ntet; ++iel) {
+    const unsigned *ip = tet + iel * 4;
+    double P[4][3], U[4][3];
+    for (int k = 0; k < 4; ++k)
+      for (int a = 0; a < 3; ++a) {
+        P[k][a] = xyz[ip[k] * 3 + a];
+        U[k][a] = disp ? disp[ip[k] * 3 + a] : 0.0;
+      }
+    double eres[4][3];
+    {
+      const double vol = orc_tet_volume(P[0], P[1], P[2], P[3]);
+      for (int k = 0; k < 4; ++k) {
+        eres[k][0] = vol * rho * g[0] * 0.25;
+        eres[k][1] = vol * rho * g[1] * 0.25;
+        eres[k][2] = vol * rho * g[2] * 0.25;
+      }
+    }
+    double emat[4][4][3][3];
+    orc_emat_tet(emat, eres, myu_e[iel], lambda_e[iel], (const double(*)[3])P,
+                 (const double(*)[3])U);
+    for (int k = 0; k < 4; ++k) {
+      vec_b[ip[k] * 3 + 0] += eres[k][0];
+      vec_b[ip[k] * 3 + 1] += eres[k][1];
+      vec_b[ip[k] * 3 + 2] += eres[k][2];
+    }
+    for (int ir = 0; ir < 4; ++ir) {
+      const unsigned i = ip[ir];
+      for (unsigned s = colind[i]; s < colind[i + 1]; ++s) buf[rowptr[s]] = s;
+      for (int jc = 0; jc < 4; ++jc) {
+        const unsigned j = ip[jc];
+        double *out;
+        if (i == j) {
+          out = vdia + (size_t)i * 9;
+        } else {
+          if (buf[j] == UINT_MAX) { status = 1; continue; }
+          out = vcrs + (size_t)buf[j] * 9;
+        }
+        const double *in = &emat[ir][jc][0][0];
+        for (int k = 0; k < 9; ++k) out[k] += in[k];
+      }
+      for (unsigned s = colind[i]; s < colind[i + 1]; ++s) buf[rowptr[s]] = UINT_MAX;
+    }
+  }
+  free(buf);
+  return status;
+}
+
+/* ls_block_sparse.cpp:397-415 (diagonal), :427-444 (row), :456-472 (column). */
+void orc_set_fixed_bc(unsigned nblk, const unsigned *colind,
+                      const unsigned *rowptr, double *vcrs, double *vdia,
+                      const int *bc_flag) {
+  for (unsigned ib = 0; ib < nblk; ++ib)
+    for (unsigned il = 0; il < 3; ++il) {
+      if (bc_flag[ib * 3 + il] == 0) continue;
+      for (unsigned jl = 0; jl < 3; ++jl) {
+        vdia[ib * 9 + il * 3 + jl] = 0.0;
+        vdia[ib * 9 + jl * 3 + il] = 0.0;
+      }
+      vdia[ib * 9 + il * 3 + il] = 1.0;
+    }
+  for (unsigned ib = 0; ib < nblk; ++ib)
+    for (unsigned s = colind[ib]; s < colind[ib + 1]; ++s)
+      for (unsigned il = 0; il < 3; ++il) {
+        if (bc_flag[ib * 3 + il] == 0) continue;
+        for (unsigned jl = 0; jl < 3; ++jl) vcrs[(size_t)s * 9 + il * 3 + jl] = 0.0;
+      }
+  const unsigned ncrs = colind[nblk];
+  for (unsigned s = 0; s < ncrs; ++s) {
+    const unsigned jb = rowptr[s];
+    for (unsigned jl = 0; jl < 3; ++jl) {
+      if (bc_flag[jb * 3 + jl] == 0) continue;
+      for (unsigned il = 0; il < 3; ++il) vcrs[(size_t)s * 9 + il * 3 + jl] = 0.0;
+    }
+  }
+}
+
+/* vecxitrsol.cpp:286-296 */
+void orc_set_rhs_zero(double *b, const int *flag, size_t n, int iflag_nonzero) {
+  for (size_t i = 0; i < n; ++i) {
+    if (flag[i] == iflag_nonzero) continue;
+    b[i] = 0;
+  }
+}
+
+/* ls_block_sparse.cpp:215-222 (y *= beta first) and :103-133 (off-diagonal
+ * blocks in stored order, then the diagonal block). */
+void orc_matvec(unsigned nblk, const unsigned *colind, const unsigned *rowptr,
+                const double *vcrs, const double *vdia, double *y,
+                double alpha, const double *x, double beta) {
+  const size_t n = (size_t)nblk * 3;
+  for (size_t i = 0; i < n; ++i) y[i] *= beta;
+  for (unsigned ib = 0; ib < nblk; ++ib) {
+    const size_t i0 = (size_t)ib * 3;
+    for (unsigned s = colind[ib]; s < colind[ib + 1]; ++s) {
+      const size_t j0 = (size_t)rowptr[s] * 3;
+      const double *v = vcrs + (size_t)s * 9;
+      y[i0 + 0] += alpha * (v[0] * x[j0 + 0] + v[1] * x[j0 + 1] + v[2] * x[j0 + 2]);
+      y[i0 + 1] += alpha * (v[3] * x[j0 + 0] + v[4] * x[j0 + 1] + v[5] * x[j0 + 2]);
+      y[i0 + 2] += alpha * (v[6] * x[j0 + 0] + v[7] * x[j0 + 1] + v[8] * x[j0 + 2]);
+    }
+    const double *v = vdia + (size_t)ib * 9;
+    y[i0 + 0] += alpha * (v[0] * x[i0 + 0] + v[1] * x[i0 + 1] + v[2] * x[i0 + 2]);
+    y[i0 + 1] += alpha * (v[3] * x[i0 + 0] + v[4] * x[i0 + 1] + v[5] * x[i0 + 2]);
+    y[i0 + 2] += alpha * (v[6] * x[i0 + 0] + v[7] * x[i0 + 1] + v[8] * x[i0 + 2]);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* ls_ilu_block_sparse.cpp:1057-1090 */
+static int cmp_uint(const void *a, const void *b) {
+  const unsigned x = *(const unsigned *)a, y = *(const unsigned *)b;
+  return (x > y) - (x < y);
+}
+void orc_ilu_set_pattern0(unsigned nblk, const unsigned *colind,
+                          const unsigned *rowptr, unsigned *rowptr_out,
+                          unsigned *diaind) {
+  memcpy(rowptr_out, rowptr, sizeof(unsigned) * colind[nblk]);
+  for (unsigned ib = 0; ib < nblk; ++ib)
+    qsort(rowptr_out + colind[ib], colind[ib + 1] - colind[ib], sizeof(unsigned), cmp_uint);
+  for (unsigned ib = 0; ib < nblk; ++ib) {
+    diaind[ib] = colind[ib + 1];
+    for (unsigned s = colind[ib]; s < colind[ib + 1]; ++s)
+      if (rowptr_out[s] > ib) { diaind[ib] = s; break; }
+  }
+}
+
+/* ls_ilu_block_sparse.cpp:1009-1047 */
+void orc_ilu_copy_value(unsigned nblk, const unsigned *ilu_colind,
+                        const unsigned *ilu_rowptr, double *ilu_vcrs,
+                        double *ilu_vdia, const unsigned *a_colind,
+                        const unsigned *a_rowptr, const double *a_vcrs,
+                        const double *a_vdia) {
+  int *row2crs = (int *)malloc(sizeof(int) * (nblk + 1));
+  for (unsigned i = 0; i < nblk; ++i) row2crs[i] = -1;
+  const size_t n = (size_t)ilu_colind[nblk] * 9;
+  for (size_t i = 0; i < n; ++i) ilu_vcrs[i] = 0.0;
+  for (unsigned ib = 0; ib < nblk; ++ib) {
+    for (unsigned s = ilu_colind[ib]; s < ilu_colind[ib + 1]; ++s) row2crs[ilu_rowptr[s]] = (int)s;
+    for (unsigned s = a_colind[ib]; s < a_colind[ib + 1]; ++s) {
+      const int d = row2crs[a_rowptr[s]];
+      if (d == -1) continue;
+      for (int k = 0; k < 9; ++k) ilu_vcrs[(size_t)d * 9 + k] = a_vcrs[(size_t)s * 9 + k];
+    }
+    for (unsigned s = ilu_colind[ib]; s < ilu_colind[ib + 1]; ++s) row2crs[ilu_rowptr[s]] = -1;
+  }
+  for (size_t i = 0; i < (size_t)nblk * 9; ++i) ilu_vdia[i] = a_vdia[i];
+  free(row2crs);
+}
+
+/* ls_ilu_block_sparse.cpp:84-103 CalcInvMat3 */
+static double det3(const double *a) {
+  return +a[0] * a[4] * a[8] + a[3] * a[7] * a[2] + a[6] * a[1] * a[5]
+         - a[0] * a[7] * a[5] - a[6] * a[4] * a[2] - a[3] * a[1] * a[8];
+}
+static void inv3_inplace(double *a) {
+  const double inv_det = 1.0 / det3(a);
+  double t[9];
+  for (int i = 0; i < 9; ++i) t[i] = a[i];
+  a[0] = inv_det * (t[4] * t[8] - t[5] * t[7]);
+  a[1] = inv_det * (t[2] * t[7] - t[1] * t[8]);
+  a[2] = inv_det * (t[1] * t[5] - t[2] * t[4]);
+  a[3] = inv_det * (t[5] * t[6] - t[3] * t[8]);
+  a[4] = inv_det * (t[0] * t[8] - t[2] * t[6]);
+  a[5] = inv_det * (t[2] * t[3] - t[0] * t[5]);
+  a[6] = inv_det * (t[3] * t[7] - t[4] * t[6]);
+  a[7] = inv_det * (t[1] * t[6] - t[0] * t[7]);
+  a[8] = inv_det * (t[0] * t[4] - t[1] * t[3]);
+}
+
+/* ls_ilu_block_sparse.cpp:152-156, 286-356: row-sequential IKJ block ILU(0). */
+int orc_ilu_decompose(unsigned nblk, const unsigned *colind,
+                      const unsigned *rowptr, const unsigned *diaind,
+                      double *vcrs, double *vdia, int *n_singular) {
+  int nsing = 0;
+  int *row2crs = (int *)malloc(sizeof(int) * (nblk + 1));
+  for (unsigned i = 0; i < nblk; ++i) row2crs[i] = -1;
+  int ok = 1;
+  for (unsigned ib = 0; ib < nblk && ok; ++ib) {
+    for (unsigned s = colind[ib]; s < colind[ib + 1]; ++s) row2crs[rowptr[s]] = (int)s;
+    for (unsigned ik = colind[ib]; ik < diaind[ib]; ++ik) {
+      const unsigned kb = rowptr[ik];
+      const double *vik = vcrs + (size_t)ik * 9;
+      for (unsigned kj = diaind[kb]; kj < colind[kb + 1]; ++kj) {
+        const unsigned jb = rowptr[kj];
+        const double *vkj = vcrs + (size_t)kj * 9;
+        double *vij;
+        if (jb != ib) {
+          const int ij = row2crs[jb];
+          if (ij == -1) continue;
+          vij = vcrs + (size_t)ij * 9;
+        } else {
+          vij = vdia + (size_t)ib * 9;
+        }
+        for (int i = 0; i < 3; ++i) {
+          vij[i * 3 + 0] -= vik[i * 3 + 0] * vkj[0] + vik[i * 3 + 1] * vkj[3] + vik[i * 3 + 2] * vkj[6];
+          vij[i * 3 + 1] -= vik[i * 3 + 0] * vkj[1] + vik[i * 3 + 1] * vkj[4] + vik[i * 3 + 2] * vkj[7];
+          vij[i * 3 + 2] -= vik[i * 3 + 0] * vkj[2] + vik[i * 3 + 1] * vkj[5] + vik[i * 3 + 2] * vkj[8];
+        }
+      }
+    }
+    {
+      double *vii = vdia + (size_t)ib * 9;
+      const double det = det3(vii);
+      if (fabs(det) > 1.0e-30) {
+        inv3_inplace(vii);
+      } else {
+        ++nsing;
+        if (nsing > 10) { ok = 0; break; }
+      }
+    }
+    {
+      const double *vii = vdia + (size_t)ib * 9;
+      for (unsigned s = diaind[ib]; s < colind[ib + 1]; ++s) {
+        double *vij = vcrs + (size_t)s * 9;
+        double t[9];
+        for (int k = 0; k < 9; ++k) t[k] = vij[k];
+        for (int i = 0; i < 3; ++i) {
+          vij[i * 3 + 0] = vii[i * 3 + 0] * t[0] + vii[i * 3 + 1] * t[3] + vii[i * 3 + 2] * t[6];
+          vij[i * 3 + 1] = vii[i * 3 + 0] * t[1] + vii[i * 3 + 1] * t[4] + vii[i * 3 + 2] * t[7];
+          vij[i * 3 + 2] = vii[i * 3 + 0] * t[2] + vii[i * 3 + 1] * t[5] + vii[i * 3 + 2] * t[8];
+        }
+      }
+    }
+    for (unsigned s = colind[ib]; s < colind[ib + 1]; ++s) row2crs[rowptr[s]] = -1;
+  }
+  free(row2crs);
+  if (n_singular) *n_singular = nsing;
+  return ok;
+}
+
+/* ls_ilu_block_sparse.cpp:544-573 */
+void orc_ilu_forward(unsigned nblk, const unsigned *colind,
+                     const unsigned *rowptr, const unsigned *diaind,
+                     const double *vcrs, const double *vdia, double *vec) {
+  for (unsigned ib = 0; ib < nblk; ++ib) {
+    double t0 = vec[ib * 3 + 0], t1 = vec[ib * 3 + 1], t2 = vec[ib * 3 + 2];
+    for (unsigned s = colind[ib]; s < diaind[ib]; ++s) {
+      const size_t j0 = (size_t)rowptr[s] * 3;
+      const double *v = vcrs + (size_t)s * 9;
+      const double x0 = vec[j0 + 0], x1 = vec[j0 + 1], x2 = vec[j0 + 2];
+      t0 -= v[0] * x0 + v[1] * x1 + v[2] * x2;
+      t1 -= v[3] * x0 + v[4] * x1 + v[5] * x2;
+      t2 -= v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    const double *d = vdia + (size_t)ib * 9;
+    vec[ib * 3 + 0] = d[0] * t0 + d[1] * t1 + d[2] * t2;
+    vec[ib * 3 + 1] = d[3] * t0 + d[4] * t1 + d[5] * t2;
+    vec[ib * 3 + 2] = d[6] * t0 + d[7] * t1 + d[8] * t2;
+  }
+}
+
+/* ls_ilu_block_sparse.cpp:721-749 */
+void orc_ilu_backward(unsigned nblk, const unsigned *colind,
+                      const unsigned *rowptr, const unsigned *diaind,
+                      const double *vcrs, double *vec) {
+  for (unsigned ib = nblk - 1; ib != UINT_MAX; --ib) {
+    double t0 = vec[ib * 3 + 0], t1 = vec[ib * 3 + 1], t2 = vec[ib * 3 + 2];
+    for (unsigned s = diaind[ib]; s < colind[ib + 1]; ++s) {
+      const size_t j0 = (size_t)rowptr[s] * 3;
+      const double *v = vcrs + (size_t)s * 9;
+      const double x0 = vec[j0 + 0], x1 = vec[j0 + 1], x2 = vec[j0 + 2];
+      t0 -= v[0] * x0 + v[1] * x1 + v[2] * x2;
+      t1 -= v[3] * x0 + v[4] * x1 + v[5] * x2;
+      t2 -= v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    vec[ib * 3 + 0] = t0;
+    vec[ib * 3 + 1] = t1;
+    vec[ib * 3 + 2] = t2;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* view_vectorx.h:32-39 */
+double orc_dot(const double *x, const double *y, size_t n) {
+  double d = 0;
+  for (size_t i = 0; i < n; ++i) d += x[i] * y[i];
+  return d;
+}
+
+/* lsitrsol.h:80-148 with view_vectorx.h:61-109 (AddScaledVec y += a*x,
+ * ScaleAndAddVec y = b*y + x, AddMatVec -> MatVec(y, 1, p, 0)). */
+size_t orc_solve_pcg(double *r, double *x, double tol, unsigned max_nitr,
+                     unsigned nblk, const unsigned *a_colind,
+                     const unsigned *a_rowptr, const double *a_vcrs,
+                     const double *a_vdia, const unsigned *l_colind,
+                     const unsigned *l_rowptr, const unsigned *l_diaind,
+                     const double *l_vcrs, const double *l_vdia, double *hist,
+                     size_t hist_cap) {
+  const size_t n = (size_t)nblk * 3;
+  size_t nh = 0;
+#define PUSH(v) do { if (nh < hist_cap) hist[nh] = (v); ++nh; } while (0)
+  for (size_t i = 0; i < n; ++i) x[i] = 0;
+  const double sq0 = orc_dot(r, r, n);
+  PUSH(sqrt(sq0));
+  if (sq0 < 1.0e-30) return nh;
+  const double inv_sq0 = 1.0 / sq0;
+  double *Pr = (double *)malloc(sizeof(double) * (n + 1));
+  double *p = (double *)malloc(sizeof(double) * (n + 1));
+  memcpy(Pr, r, sizeof(double) * n);
+  orc_ilu_forward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, l_vdia, Pr);
+  orc_ilu_backward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, Pr);
+  memcpy(p, Pr, sizeof(double) * n);
+  double rPr = orc_dot(r, Pr, n);
+  int converged = 0;
+  for (unsigned it = 0; it < max_nitr; ++it) {
+    double *Ap = Pr;
+    orc_matvec(nblk, a_colind, a_rowptr, a_vcrs, a_vdia, Ap, 1.0, p, 0.0);
+    const double pAp = orc_dot(p, Ap, n);
+    const double alpha = rPr / pAp;
+    for (size_t i = 0; i < n; ++i) r[i] += -alpha * Ap[i];
+    for (size_t i = 0; i < n; ++i) x[i] += +alpha * p[i];
+    {
+      const double sq = orc_dot(r, r, n);
+      PUSH(sqrt(sq));
+      const double ratio = sqrt(sq * inv_sq0);
+      if (ratio < tol) { converged = 1; break; }
+    }
+    memcpy(Pr, r, sizeof(double) * n);
+    orc_ilu_forward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, l_vdia, Pr);
+    orc_ilu_backward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, Pr);
+    const double rPr1 = orc_dot(r, Pr, n);
+    const double beta = rPr1 / rPr;
+    rPr = rPr1;
+    for (size_t i = 0; i < n; ++i) p[i] = beta * p[i] + Pr[i];
+  }
+  if (!converged) PUSH(sqrt(orc_dot(r, r, n)));
+#undef PUSH
+  free(Pr);
+  free(p);
+  return nh;
+}
+
+/* vecxitrsol.cpp:226-237 */
+void orc_xplusay(double *X, size_t n, const int *flag, double alpha,
+                 const double *Y) {
+  for (size_t i = 0; i < n; ++i) {
+    if (flag[i] != 0) continue;
+    X[i] += alpha * Y[i];
+  }
+}
+
+/* Adjoint contraction: K_e is linear in (myu, lambda), so dK_e/dE_e is the
+ * element matrix at the E=1 Lame pair.  g = -lam_e^T K_e(E=1) u_e, summed
+ * over the 16 blocks in (i, j, a, b) order. */
+void orc_adjoint_grad_tet(double *grad, const double *mu1_e,
+                          const double *lambda1_e, const double *xyz,
+                          const unsigned *tet, size_t ntet, const double *u,
+                          const double *lam) {
+  for (size_t iel = 0; iel < ntet; ++iel) {
+    const unsigned *ip = tet + iel * 4;
+    double P[4][3], Z[4][3], emat[4][4][3][3], eres[4][3];
+    for (int k = 0; k < 4; ++k)
+      for (int a = 0; a < 3; ++a) {
+        P[k][a] = xyz[ip[k] * 3 + a];
+        Z[k][a] = 0.0;
+        eres[k][a] = 0.0;
+      }
+    orc_emat_tet(emat, eres, mu1_e[iel], lambda1_e[iel], (const double(*)[3])P,
+                 (const double(*)[3])Z);
+    double s = 0.0;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            s += lam[ip[i] * 3 + a] * emat[i][j][a][b] * u[ip[j] * 3 + b];
+    grad[iel] = -s;
+  }
+}

@@ -1,0 +1,172 @@
+"""Writes the golden fixtures of the tet-FEM solve path from the REFERENCE.
+
+Runs the reference's own code (its headers compiled by oracle/Makefile into
+oracle/_ref/libdfm2ref.so) on small seeded inputs and stores inputs and
+outputs under tests/golden/:
+
+* ``golden.npz``  -- full arrays (small cases);
+* ``golden.json`` -- SHA-256 of raw little-endian arrays of the larger cases,
+  iteration counts, residual histories and scalar summaries.
+
+Usage (needs oracle/_ref built, i.e. the reference tree at build time):
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from delfem2_amd import workload as W  # noqa: E402
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000):
+    """Full reference pipeline on one problem; returns a dict of outputs."""
+    R = oracle.ref()
+    ind = np.zeros(prob.np_ + 1, np.uint32)
+    n = R.ref_psup(prob.tet, prob.ne, prob.np_, ind, None)
+    psup = np.zeros(n, np.uint32)
+    R.ref_psup(prob.tet, prob.ne, prob.np_, ind, psup.ctypes.data)
+    sysr = oracle.RefSystem(ind, psup)
+    b = np.zeros(prob.ndof)
+    st = sysr.assemble(b, prob.myu, prob.lam, prob.rho, prob.g, prob.xyz, prob.tet, disp)
+    if prob.f_ext is not None:
+        b += prob.f_ext
+    vcrs_asm, vdia_asm = sysr.matrix()
+    b_asm = b.copy()
+    sysr.set_fixed_bc(prob.flag)
+    bb = b.copy()
+    R.ref_set_rhs_zero(bb, prob.flag, bb.size, 0)
+    vcrs, vdia = sysr.matrix()
+    # MatVec on a seeded vector
+    xv = W.uniform(777, prob.ndof) - 0.5
+    yv = W.uniform(778, prob.ndof) - 0.5
+    sysr.matvec(yv, 1.5, xv, -0.25)
+    ok = sysr.factor()
+    rp, di, lc, ld = sysr.ilu()
+    pre = bb.copy()
+    sysr.solve_precond(pre)
+    r = bb.copy()
+    x, hist = sysr.solve_pcg(r, tol, maxit)
+    return dict(psup_ind=ind, psup=psup, merge_status=st, vcrs_asm=vcrs_asm, vdia_asm=vdia_asm,
+                b_asm=b_asm, vcrs=vcrs, vdia=vdia, b=bb, matvec_x=xv, matvec_y=yv,
+                ilu_ok=ok, ilu_rowptr=rp, ilu_diaind=di, ilu_vcrs=lc, ilu_vdia=ld,
+                precond_b=pre, x=x, hist=hist, r_final=r)
+
+
+def summary(out: dict) -> dict:
+    d = {}
+    for k, v in out.items():
+        if isinstance(v, np.ndarray):
+            d[k] = {"sha256": sha(v), "size": int(v.size), "dtype": str(v.dtype)}
+        else:
+            d[k] = v if not isinstance(v, (np.bool_, bool)) else bool(v)
+    d["iterations"] = int(out["hist"].size - 1)
+    d["hist"] = [float(h) for h in out["hist"]]
+    d["x_max_abs"] = float(np.abs(out["x"]).max())
+    d["x_norm"] = float(np.linalg.norm(out["x"]))
+    return d
+
+
+def main():
+    if not oracle.ref_available():
+        raise SystemExit("oracle/_ref/libdfm2ref.so not built (needs the reference tree)")
+    R = oracle.ref()
+    arrays = {}
+    meta = {"generator": "tests/golden/make_golden.py", "source": "reference headers via "
+            "oracle/ref_shim.cpp", "cases": {}}
+
+    # --- mesh and pattern, tiny grids in full
+    for (nx, ny, nz) in [(1, 1, 1), (2, 2, 2), (2, 3, 4)]:
+        npn = (nx + 1) * (ny + 1) * (nz + 1)
+        xyz = np.zeros(npn * 3)
+        tet = np.zeros(nx * ny * nz * 24, np.uint32)
+        R.ref_mesh_tet_voxel(nx, ny, nz, xyz, tet)
+        key = f"mesh_{nx}{ny}{nz}"
+        arrays[key + "_xyz"] = xyz
+        arrays[key + "_tet"] = tet
+        ind = np.zeros(npn + 1, np.uint32)
+        n = R.ref_psup(tet, tet.size // 4, npn, ind, None)
+        ps = np.zeros(n, np.uint32)
+        R.ref_psup(tet, tet.size // 4, npn, ind, ps.ctypes.data)
+        arrays[key + "_psup_ind"] = ind
+        arrays[key + "_psup"] = ps
+
+    # --- element level: 12 seeded tets (random shape, Lame pair, displacement)
+    ne = 12
+    u = W.uniform(4242, ne * 12 * 3 + ne * 2).reshape(-1)
+    P = (u[: ne * 12] - 0.5).reshape(ne, 4, 3) + np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0],
+                                                           [0, 0, 1]])[None]
+    D = (u[ne * 12: ne * 24] - 0.5).reshape(ne, 4, 3) * 1e-2
+    ml = u[ne * 24: ne * 24 + ne * 2].reshape(ne, 2) + 0.1
+    E_out = np.zeros((ne, 4, 4, 3, 3))
+    R_out = np.zeros((ne, 4, 3))
+    DL = np.zeros((ne, 4, 3))
+    for e in range(ne):
+        em = np.zeros(144)
+        er = np.zeros(12)
+        R.ref_emat_tet(em, er, ml[e, 0], ml[e, 1], np.ascontiguousarray(P[e].reshape(-1)),
+                       np.ascontiguousarray(D[e].reshape(-1)))
+        E_out[e] = em.reshape(4, 4, 3, 3)
+        R_out[e] = er.reshape(4, 3)
+        dl = np.zeros(12)
+        p = [np.ascontiguousarray(P[e, k]) for k in range(4)]
+        R.ref_tet_dldx(dl, *p)
+        DL[e] = dl.reshape(4, 3)
+    arrays.update(elem_P=P, elem_disp=D, elem_myu_lambda=ml, elem_emat=E_out, elem_eres=R_out,
+                  elem_dldx=DL)
+
+    # --- C1: 10^3 cube, E=1, nu=0.3 (the CPU-runnable configuration)
+    p1 = W.cube(10)
+    o1 = ref_case(p1)
+    meta["cases"]["c1_natural"] = summary(o1)
+    for k in ("psup_ind", "psup", "b", "x", "hist", "precond_b", "matvec_y"):
+        arrays["c1_" + k] = o1[k]
+    # --- C1 in 8-colour node order (same physics, permuted mesh)
+    p1c = W.cube(10, ordering="color8")
+    o1c = ref_case(p1c)
+    meta["cases"]["c1_color8"] = summary(o1c)
+    arrays["c1c_x"] = o1c["x"]
+    arrays["c1c_hist"] = o1c["hist"]
+    # --- n=3, lognormal E (sigma 0.5), nonzero displacement for the -K u term
+    p3 = W.cube(3, sigma=0.5, seed=99)
+    disp3 = (W.uniform(31, p3.ndof) - 0.5) * 1e-3
+    o3 = ref_case(p3, disp=disp3)
+    arrays["n3_myu"] = p3.myu
+    arrays["n3_lam"] = p3.lam
+    arrays["n3_disp"] = disp3
+    for k in ("vcrs_asm", "vdia_asm", "b_asm", "vcrs", "vdia", "b", "ilu_vcrs", "ilu_vdia",
+              "precond_b", "x", "hist", "matvec_y"):
+        arrays["n3_" + k] = o3[k]
+    meta["cases"]["n3_sigma05"] = summary(o3)
+    # --- 16^3 sigma 0.5, natural and colour order: hashes only
+    for order in ("natural", "color8"):
+        p16 = W.cube(16, sigma=0.5, seed=20261203, ordering=order)
+        o16 = ref_case(p16)
+        meta["cases"][f"c16_sigma05_{order}"] = summary(o16)
+    # --- SIMP cantilever 8x4x4 (point-load patch, x=0 clamped)
+    pc = W.cantilever(8, 4, 4, seed=7)
+    oc = ref_case(pc)
+    meta["cases"]["cantilever844"] = summary(oc)
+    arrays["cant_x"] = oc["x"]
+
+    np.savez_compressed(os.path.join(HERE, "golden.npz"), **arrays)
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("wrote", os.path.join(HERE, "golden.npz"), os.path.join(HERE, "golden.json"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,260 @@
+"""HIP path vs the oracle on the same seeded inputs, through the C ABI.
+
+Bit-exact: assembly (A, b), fixed BCs, MatVec, ILU(0) factors, forward and
+backward sweeps (FP64, -ffp-contract=off on both sides).  Tolerance: the PCG
+solution (parallel dot products reorder sums): displacement rel-L2 <= 1e-7
+for FP64 and <= 1e-6 for FP32-inner + FP64 refinement.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from delfem2_amd import solver as S
+from delfem2_amd import workload as W
+from delfem2_amd.problem import DeviceSystem
+
+pytestmark = pytest.mark.gpu
+
+FP64_X_TOL = 1e-7
+MIXED_X_TOL = 1e-6
+
+
+def cpu_system(prob, disp=None):
+    ind, ps = oracle.psup(prob.tet, prob.np_)
+    C = oracle.System(ind, ps)
+    b = np.zeros(prob.ndof)
+    st = C.assemble(b, prob.myu, prob.lam, prob.rho, prob.g, prob.xyz, prob.tet, disp)
+    assert st == 0
+    if prob.f_ext is not None:
+        b += prob.f_ext
+    C.b_asm = b.copy()
+    C.vcrs_asm, C.vdia_asm = C.vcrs.copy(), C.vdia.copy()
+    C.set_fixed_bc(prob.flag)
+    oracle.set_rhs_zero(b, prob.flag)
+    C.b = b
+    return C
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda").to(dtype)
+
+
+CASES = {
+    "c1": lambda: (W.cube(10), None),
+    "c1_color8": lambda: (W.cube(10, ordering="color8"), None),
+    "n3_disp": lambda: (W.cube(3, sigma=0.5, seed=99),
+                        (W.uniform(31, 64 * 3) - 0.5) * 1e-3),
+    "n7_sigma1_color8": lambda: (W.cube(7, sigma=1.0, seed=5, nu_range=(0.25, 0.35),
+                                        ordering="color8"), None),
+    "cantilever844": lambda: (W.cantilever(8, 4, 4, seed=7), None),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_assembly_bc_matvec_ilu_bit_exact(gpu, case):
+    prob, disp = CASES[case]()
+    print(f"[case {case}] ndof={prob.ndof}", flush=True)
+    C = cpu_system(prob, disp)
+    G = DeviceSystem(prob)
+    # assembly (before BC)
+    G.A.setZero()
+    G.b.zero_()
+    ok = S.MergeLinSys_SolidLinear_Static_MeshTet3D(
+        G.A, G.b, G.myu, G.lam, prob.rho, prob.g, G.xyz, G.mesh,
+        None if disp is None else dev(disp))
+    assert ok
+    if G.f_ext is not None:
+        G.b.add_(G.f_ext)
+    vc, vd = G.A.download()
+    assert np.array_equal(bits(vc), bits(C.vcrs_asm))
+    assert np.array_equal(bits(vd), bits(C.vdia_asm))
+    assert np.array_equal(bits(G.b.cpu().numpy()), bits(C.b_asm))
+    # Dirichlet rows / columns and rhs
+    G.A.SetFixedBC(G.flag)
+    S.setRHS_Zero(G.b, G.flag, 0)
+    vc, vd = G.A.download()
+    assert np.array_equal(bits(vc), bits(C.vcrs))
+    assert np.array_equal(bits(vd), bits(C.vdia))
+    assert np.array_equal(bits(G.b.cpu().numpy()), bits(C.b))
+    # MatVec y = 1.5 A x - 0.25 y
+    xv = W.uniform(777, prob.ndof) - 0.5
+    yv = W.uniform(778, prob.ndof) - 0.5
+    yg = dev(yv)
+    G.A.MatVec(yg, 1.5, dev(xv), -0.25)
+    C.matvec(yv, 1.5, xv, -0.25)
+    assert np.array_equal(bits(yg.cpu().numpy()), bits(yv))
+    # ILU(0): CopyValue + Decompose
+    assert C.factor()
+    assert G.factor()
+    rp, di, lc, ld = G.ilu.download()
+    assert np.array_equal(rp, C.l_rowptr) and np.array_equal(di, C.l_diaind)
+    assert np.array_equal(bits(lc), bits(C.l_vcrs))
+    assert np.array_equal(bits(ld), bits(C.l_vdia))
+    # sweeps
+    v = W.uniform(55, prob.ndof) - 0.5
+    vg = dev(v)
+    G.ilu.ForwardSubstitution(vg)
+    vc_ = v.copy()
+    C.forward(vc_)
+    assert np.array_equal(bits(vg.cpu().numpy()), bits(vc_))
+    G.ilu.BackwardSubstitution(vg)
+    C.backward(vc_)
+    assert np.array_equal(bits(vg.cpu().numpy()), bits(vc_))
+    vg = dev(C.b)
+    G.ilu.SolvePrecond(vg)
+    pc = C.b.copy()
+    C.solve_precond(pc)
+    assert np.array_equal(bits(vg.cpu().numpy()), bits(pc))
+
+
+@pytest.mark.parametrize("case", ["c1", "c1_color8", "n7_sigma1_color8", "cantilever844"])
+def test_pcg_fp64_vs_oracle(gpu, case):
+    prob, _ = CASES[case]()
+    print(f"[case {case}] ndof={prob.ndof}", flush=True)
+    C = cpu_system(prob)
+    C.factor()
+    xc, hc = C.solve_pcg(C.b.copy(), 1e-8, 5000)
+    G = DeviceSystem(prob)
+    assert G.assemble()
+    assert G.factor()
+    x, hist, r = G.solve(1e-8, 5000)
+    xg = x.cpu().numpy()
+    assert abs(len(hist) - len(hc)) <= 1
+    assert hist[0] == pytest.approx(hc[0], rel=1e-13)
+    assert hist[-1] / hist[0] < 1e-8
+    assert np.linalg.norm(xg - xc) / np.linalg.norm(xc) <= FP64_X_TOL
+    # r on return is the final residual b - A x (to solver rounding)
+    rr = G.b.clone()
+    G.A.MatVec(rr, -1.0, x, 1.0)
+    assert torch.linalg.norm(rr - r).item() <= 1e-6 * torch.linalg.norm(G.b).item()
+
+
+def test_pcg_c1_golden(gpu):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+    G = DeviceSystem(W.cube(10))
+    G.assemble()
+    G.factor()
+    assert np.array_equal(bits(G.b.cpu().numpy()), bits(g["c1_b"]))
+    x, hist, _ = G.solve(1e-8, 1000)
+    assert len(hist) - 1 == 44
+    assert np.allclose(hist, g["c1_hist"], rtol=1e-9)
+    xr = g["c1_x"]
+    assert np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr) <= FP64_X_TOL
+
+
+@pytest.mark.parametrize("case", ["c1_color8", "n7_sigma1_color8"])
+def test_pcg_mixed_vs_oracle(gpu, case):
+    prob, _ = CASES[case]()
+    C = cpu_system(prob)
+    C.factor()
+    xc, _ = C.solve_pcg(C.b.copy(), 1e-12, 5000)
+    G = DeviceSystem(prob)
+    G.assemble()
+    G.factor()
+    x, hist, n_inner = G.solve_mixed(1e-8, inner_tol=1e-4)
+    assert hist[-1] / hist[0] < 1e-8
+    assert n_inner > 0
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xc) / np.linalg.norm(xc) <= MIXED_X_TOL
+
+
+def test_vector_ops(gpu):
+    n = 100003
+    x = W.uniform(1, n) - 0.5
+    y = W.uniform(2, n) - 0.5
+    xg, yg = dev(x), dev(y)
+    assert S.Dot(xg, yg) == pytest.approx(float(np.dot(x, y)), rel=1e-12)
+    assert S.Dot(xg, yg) == S.Dot(xg, yg)  # deterministic
+    S.AddScaledVec(yg, 0.3, xg)
+    assert np.array_equal(bits(yg.cpu().numpy()), bits(y + 0.3 * x))
+    y2 = y + 0.3 * x
+    S.ScaleAndAddVec(yg, -1.7, xg)
+    assert np.array_equal(bits(yg.cpu().numpy()), bits(-1.7 * y2 + x))
+    f = (W.uniform(3, n) < 0.3).astype(np.int32)
+    X = dev(x)
+    S.XPlusAY(X, dev(f, torch.int32), 2.0, yg)
+    ref = x.copy()
+    oracle.xplusay(ref, f, 2.0, -1.7 * y2 + x)
+    assert np.array_equal(bits(X.cpu().numpy()), bits(ref))
+
+
+def test_edge_cases(gpu):
+    prob = W.cube(3)
+    G = DeviceSystem(prob)
+    G.assemble()
+    G.factor()
+    # b = 0: early return, history [0], x = 0
+    r = torch.zeros_like(G.b)
+    x = torch.ones_like(G.b)
+    hist = S.Solve_PCG(r, x, 1e-8, 100, G.A, G.ilu)
+    assert hist == [0.0] and not x.any()
+    # max iterations: 1 + max + the reference's final push
+    x, hist, _ = G.solve(1e-30, 3)
+    assert len(hist) == 5 and hist[-1] == hist[-2]
+    # NaN in b: reported, not a hang
+    r = G.b.clone()
+    r[5] = float("nan")
+    with pytest.raises(S.N.DFM2HError) as e:
+        S.Solve_PCG(r, torch.zeros_like(r), 1e-8, 100, G.A, G.ilu)
+    assert e.value.code == S.N.ERR_NAN
+    # singular pivots: Decompose returns False past 10, like the reference
+    G.A.setZero()
+    G.ilu.CopyValue(G.A)
+    assert not G.ilu.Decompose()
+    assert G.ilu.n_singular == prob.np_
+
+
+def test_missing_pattern_entry_reported(gpu):
+    prob = W.cube(2)
+    ind, ps = oracle.psup(prob.tet, prob.np_)
+    # drop one neighbour from row 0 (and keep the pattern consistent)
+    keep = np.ones(ps.size, bool)
+    keep[ind[0]] = False
+    ps2 = ps[keep]
+    ind2 = ind.copy()
+    ind2[1:] -= 1
+    A = S.CMatrixSparse()
+    A.Initialize(prob.np_, 3, True)
+    A.SetPattern(ind2, ps2)
+    mesh = S.TetMesh(prob.tet, prob.np_)
+    b = torch.zeros(prob.ndof, dtype=torch.float64, device="cuda")
+    ok = S.MergeLinSys_SolidLinear_Static_MeshTet3D(A, b, 1.0, 1.0, 1.0, prob.g,
+                                                     dev(prob.xyz), mesh)
+    assert ok is False
+
+
+def test_assembly_deterministic_and_c2_properties(gpu):
+    # 64^3 (823,875 DOF): properties that need no CPU solve
+    prob = W.cube(64, sigma=0.5, seed=20261203, ordering="color8")
+    G = DeviceSystem(prob)
+    G.assemble()
+    vc1, vd1 = G.A.download()
+    b1 = G.b.cpu().numpy().copy()
+    G.assemble()
+    vc2, vd2 = G.A.download()
+    assert np.array_equal(bits(vc1), bits(vc2)) and np.array_equal(bits(vd1), bits(vd2))
+    assert np.array_equal(bits(b1), bits(G.b.cpu().numpy()))
+    # symmetry of the assembled operator via two random vectors: u.(A v) == v.(A u)
+    u = dev(W.uniform(1, prob.ndof) - 0.5)
+    v = dev(W.uniform(2, prob.ndof) - 0.5)
+    Au = torch.zeros_like(u)
+    Av = torch.zeros_like(u)
+    G.A.MatVec(Au, 1.0, u, 0.0)
+    G.A.MatVec(Av, 1.0, v, 0.0)
+    assert abs(S.Dot(u, Av) - S.Dot(v, Au)) <= 1e-12 * abs(S.Dot(u, Av))
+    f, b_ = G.ilu.levels()
+    assert (f, b_) == (8, 8)
+    assert G.factor()
+    x, hist, r = G.solve(1e-8, 5000)
+    rr = G.b.clone()
+    G.A.MatVec(rr, -1.0, x, 1.0)
+    assert torch.linalg.norm(rr).item() / torch.linalg.norm(G.b).item() < 2e-8
+    xm, hm, n_inner = G.solve_mixed(1e-8, inner_tol=1e-4)
+    assert hm[-1] / hm[0] < 1e-8
+    assert torch.linalg.norm(xm - x).item() / torch.linalg.norm(x).item() <= MIXED_X_TOL

@@ -85,6 +85,7 @@ SIGNATURES = {
     "dfm2h_vec_set_zero": [_vp, _sz],
     "dfm2h_solve_pcg": [_vp, _vp, _vp, _vp, _d, _u, _vp, _sz, _psz],
     "dfm2h_solve_pcg_mixed": [_vp, _vp, _vp, _vp, _d, _d, _u, _u, _vp, _sz, _psz, _psz],
+    "dfm2h_adjoint_grad_tet": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {"dfm2h_last_error": C.c_char_p, "dfm2h_get_stream": _vp}
 

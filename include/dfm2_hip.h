@@ -197,6 +197,19 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
                           double *hist, size_t hist_cap, size_t *n_hist,
                           size_t *n_inner);
 
+/* ---- adjoint (no reference counterpart) --------------------------------- */
+/* Material sensitivity of an objective J(u) with K(E) u = f and
+ * K(E) = sum_e E_e K_e^1, K_e^1 the element stiffness at the Lame pair
+ * (myu1_e, lambda1_e) of unit modulus (fem_solidlinear.cpp:409-435):
+ *   grad_dev[e] = -lam_e^T K_e^1 u_e      for every element e,
+ * where lam_dev solves K lam = dJ/du (zero on fixed DOFs; for the compliance
+ * J = f.u, lam = u).  Device arrays: xyz [3*np], myu1_e / lambda1_e / grad
+ * [ntet], u / lam [3*np]. */
+int dfm2h_adjoint_grad_tet(dfm2h_tetmesh M, const double *xyz_dev,
+                           const double *myu1_e_dev, const double *lambda1_e_dev,
+                           const double *u_dev, const double *lam_dev,
+                           double *grad_dev);
+
 #ifdef __cplusplus
 }
 #endif

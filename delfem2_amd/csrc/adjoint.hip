@@ -8,6 +8,7 @@
 // One thread per element; the 16 blocks are contracted in (i, j, a, b) order
 // as sum += (lam_ia * K_ijab) * u_jb, the order the CPU oracle uses, so the
 // result is bit-identical to it for the same u and lam.
+#include "handles.hpp"
 #include "tet.hpp"
 
 namespace dfm2h {

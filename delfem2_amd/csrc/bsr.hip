@@ -1,7 +1,7 @@
 // bsr.hip -- CMatrixSparse<double> (3x3 blocks, separate diagonal) on the
 // device: pattern, values, MatVec, Dirichlet rows/columns, and the gather
 // form of the tet assembly.
-#include "kernels.hpp"
+#include "handles.hpp"
 #include "tet.hpp"
 
 namespace dfm2h {
@@ -218,6 +218,15 @@ int dfm2h_bsr_set_pattern(dfm2h_bsr A, const unsigned *colind, size_t ncolind,
     }
     A->vcrs.alloc(nrowptr * 9);
     A->vcrs.zero();
+    {
+      std::vector<unsigned> all(A->nblk), s0(A->nblk), s1(A->nblk);
+      for (unsigned i = 0; i < A->nblk; ++i) {
+        all[i] = i;
+        s0[i] = colind[i];
+        s1[i] = colind[i + 1];
+      }
+      A->plan.build({all}, s0, s1, P.rowptr);
+    }
     A->has_pattern = true;
     A->version++;
     DFM2H_HIP(hipStreamSynchronize(stream()));
@@ -274,10 +283,11 @@ int dfm2h_bsr_matvec(dfm2h_bsr A, double *y, double alpha, const double *x, doub
     DFM2H_REQUIRE(y && x, DFM2H_ERR_INVALID, "null vector");
     DFM2H_REQUIRE(y != x, DFM2H_ERR_INVALID, "y and x must not alias");
     if (!A->nblk) return;
-    hipLaunchKernelGGL(k_bsr_matvec<double>, dim3(ceil_div(A->nblk, kRowsPerBlock)), dim3(kBlock),
-                       0, stream(), A->nblk, A->pat.colind_d.p, A->pat.rowptr_d.p, A->vcrs.p,
-                       A->vdia.p, y, alpha, x, beta);
-    check_launch("k_bsr_matvec");
+    ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, stream());
+    hipLaunchKernelGGL(k_sliced_matvec<double>, dim3(slice_grid(A->plan.nslices)), dim3(kPackBlock),
+                       0, stream(), 0u, A->plan.nslices, view(A->plan), A->pk64.val.p,
+                       A->pk64.dia.p, y, alpha, x, beta);
+    check_launch("k_sliced_matvec");
   });
 }
 

@@ -104,65 +104,7 @@ struct DBuf {
   }
 };
 
-// ---------------------------------------------------------------------------
-// handles
 
-struct Pattern {
-  unsigned nblk = 0;
-  size_t ncrs = 0;
-  std::vector<unsigned> colind, rowptr;  // host copies
-  DBuf<unsigned> colind_d, rowptr_d;
-  DBuf<unsigned> rowof_d;  // block row of each crs slot
-};
-
-}  // namespace dfm2h
-
-struct dfm2h_bsr_s {
-  unsigned nblk = 0;
-  bool has_pattern = false;
-  dfm2h::Pattern pat;
-  dfm2h::DBuf<double> vcrs, vdia;
-  // FP32 shadow of the values for mixed precision (made on demand)
-  dfm2h::DBuf<float> vcrs32, vdia32;
-  uint64_t version = 0;  // bumped whenever values change
-};
-
-struct dfm2h_tetmesh_s {
-  size_t ntet = 0, np = 0;
-  dfm2h::DBuf<unsigned> tet;        // [4*ntet]
-  dfm2h::DBuf<unsigned> elsup_ind;  // [np+1]
-  dfm2h::DBuf<unsigned> elsup;      // element ids, ascending per node
-};
-
-namespace dfm2h {
-struct PcgWork;  // pcg.hip
-void destroy_pcg_work(PcgWork *);
-}  // namespace dfm2h
-
-struct dfm2h_ilu_s {
-  unsigned nblk = 0;
-  size_t ncrs = 0;
-  bool has_pattern = false;
-  bool decomposed = false;
-  std::vector<unsigned> colind, rowptr, diaind;  // host (sorted pattern)
-  dfm2h::DBuf<unsigned> colind_d, rowptr_d, diaind_d;
-  dfm2h::DBuf<double> vcrs, vdia;
-  // A slot -> ILU slot map for CopyValue (built against `mapped_for`)
-  const dfm2h_bsr_s *mapped_for = nullptr;
-  dfm2h::DBuf<int> a2l_d;
-  // level schedules: rows grouped by dependency level, ascending inside a level
-  std::vector<unsigned> levptr_f, levptr_b;
-  dfm2h::DBuf<unsigned> levrows_f, levrows_b;
-  dfm2h::DBuf<int> nsing_d;
-  // FP32 copies of the factors (mixed precision)
-  dfm2h::DBuf<float> vcrs32, vdia32;
-  uint64_t version = 0;
-  uint64_t version32 = ~0ull;
-  dfm2h::PcgWork *work = nullptr;
-  ~dfm2h_ilu_s() { dfm2h::destroy_pcg_work(work); }
-};
-
-namespace dfm2h {
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -194,24 +136,11 @@ __device__ inline double block_reduce_sum(double v) {
   return t;
 }
 
-// Last-block-done detection for single-pass grid reductions (the
-// threadfence-reduction pattern).  Every block calls it after writing its
-// partial; exactly one block gets true, after all partials are visible.
-__device__ inline bool last_block(unsigned *ticket, unsigned nblocks) {
-  __shared__ bool am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(ticket, 1u);
-    am_last = (t == nblocks - 1);
-  }
-  __syncthreads();
-  if (am_last) __threadfence();
-  return am_last;
-}
-
 // Deterministic sum of partials[0..n) by one block of BS threads (strided
-// per-thread sums, then a fixed tree).
+// per-thread sums, then a fixed tree).  Grid reductions write one partial per
+// workgroup and a separate one-workgroup kernel sums them: the kernel
+// boundary makes the partials visible without a device-scope fence in every
+// workgroup (which on a multi-XCD part writes back L2 each time).
 template <int BS>
 __device__ inline double reduce_partials(const double *partials, unsigned n) {
   double s = 0.0;

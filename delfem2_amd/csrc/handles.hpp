@@ -1,0 +1,91 @@
+// handles.hpp -- the opaque objects behind the C ABI handles.
+//
+// Values are kept twice in HBM: in the reference's CRS layout (what
+// assembly, fixed BCs, CopyValue, Decompose and upload/download work on) and
+// packed into wavefront slices (what every iteration kernel streams).  A
+// packed copy remembers the value version it was made from and is rebuilt
+// lazily when the CRS values changed.
+#pragma once
+
+#include "packed.hpp"
+
+namespace dfm2h {
+
+struct Pattern {
+  unsigned nblk = 0;
+  size_t ncrs = 0;
+  std::vector<unsigned> colind, rowptr;  // host copies
+  DBuf<unsigned> colind_d, rowptr_d;
+  DBuf<unsigned> rowof_d;  // block row of each crs slot
+};
+
+struct PcgWork;  // pcg.hip
+void destroy_pcg_work(PcgWork *);
+
+// Packs CRS values (and optional per-row diagonal blocks) into V when the
+// source version differs from the packed one.
+template <typename T>
+void ensure_packed(const PackedPlan &P, PackedVals<T> &V, const double *crs, const double *dia,
+                   uint64_t version, hipStream_t s) {
+  if (V.version == version && (V.val.p || P.nq == 0)) return;
+  V.val.alloc(P.nq * kSliceVals);
+  if (P.nq) {
+    const size_t n = P.nq * kSliceVals;
+    hipLaunchKernelGGL(k_pack_blocks<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, P.nq, P.src.p,
+                       crs, V.val.p);
+  }
+  if (dia) {
+    V.dia.alloc((size_t)P.nslices * kSliceVals);
+    if (P.nslices) {
+      const size_t n = (size_t)P.nslices * kSliceVals;
+      hipLaunchKernelGGL(k_pack_dia<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, P.nslices,
+                         P.row.p, dia, V.dia.p);
+    }
+  }
+  check_launch("k_pack");
+  V.version = version;
+}
+
+}  // namespace dfm2h
+
+struct dfm2h_bsr_s {
+  unsigned nblk = 0;
+  bool has_pattern = false;
+  dfm2h::Pattern pat;
+  dfm2h::DBuf<double> vcrs, vdia;
+  uint64_t version = 1;  // bumped whenever values change
+  dfm2h::PackedPlan plan;  // all rows, CRS block order
+  dfm2h::PackedVals<double> pk64;
+  dfm2h::PackedVals<float> pk32;
+};
+
+struct dfm2h_tetmesh_s {
+  size_t ntet = 0, np = 0;
+  dfm2h::DBuf<unsigned> tet;        // [4*ntet]
+  dfm2h::DBuf<unsigned> elsup_ind;  // [np+1]
+  dfm2h::DBuf<unsigned> elsup;      // element ids, ascending per node
+};
+
+struct dfm2h_ilu_s {
+  unsigned nblk = 0;
+  size_t ncrs = 0;
+  bool has_pattern = false;
+  bool decomposed = false;
+  std::vector<unsigned> colind, rowptr, diaind;  // host (sorted pattern)
+  dfm2h::DBuf<unsigned> colind_d, rowptr_d, diaind_d;
+  dfm2h::DBuf<double> vcrs, vdia;
+  // A slot -> ILU slot map for CopyValue (built against `mapped_for`)
+  const dfm2h_bsr_s *mapped_for = nullptr;
+  dfm2h::DBuf<int> a2l_d;
+  // level schedules: rows grouped by dependency level, ascending inside a level
+  std::vector<unsigned> levptr_f, levptr_b;
+  dfm2h::DBuf<unsigned> levrows_f;
+  dfm2h::DBuf<int> nsing_d;
+  // sliced sweeps: forward plan (lower blocks + D^-1), backward plan (upper blocks)
+  dfm2h::PackedPlan fplan, bplan;
+  dfm2h::PackedVals<double> f64, b64;
+  dfm2h::PackedVals<float> f32, b32;
+  uint64_t version = 1;
+  dfm2h::PcgWork *work = nullptr;
+  ~dfm2h_ilu_s() { dfm2h::destroy_pcg_work(work); }
+};

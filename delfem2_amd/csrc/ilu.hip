@@ -1,7 +1,7 @@
 // ilu.hip -- CPreconditionerILU<double> ILU(0) on the device: symbolic set-up
 // with level scheduling (host), value copy, level-parallel numeric
 // factorisation and level-parallel triangular sweeps.
-#include "kernels.hpp"
+#include "handles.hpp"
 
 #include <algorithm>
 #include <numeric>
@@ -38,7 +38,7 @@ __device__ inline double det3(const double *a) {
          - a[0] * a[7] * a[5] - a[6] * a[4] * a[2] - a[3] * a[1] * a[8];
 }
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(64)
 k_ilu_decompose_level(const unsigned *__restrict__ rows, unsigned nrows,
                       const unsigned *__restrict__ colind, const unsigned *__restrict__ rowptr,
                       const unsigned *__restrict__ diaind, double *vcrs, double *vdia,
@@ -171,23 +171,24 @@ static void need_ilu(dfm2h_ilu P) {
 namespace dfm2h {
 
 void ilu_forward_levels_double(dfm2h_ilu P, const double *src, double *vec) {
-  for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l) {
-    const unsigned nrows = P->levptr_f[l + 1] - P->levptr_f[l];
-    hipLaunchKernelGGL(k_ilu_fwd_level<double>, dim3(ceil_div(nrows, kRowsPerBlock)), dim3(kBlock),
-                       0, stream(), P->levrows_f.p + P->levptr_f[l], nrows, P->colind_d.p,
-                       P->rowptr_d.p, P->diaind_d.p, P->vcrs.p, P->vdia.p, src, vec);
+  ensure_packed(P->fplan, P->f64, P->vcrs.p, P->vdia.p, P->version, stream());
+  for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
+    const unsigned nsl = P->fplan.group_slices(l);
+    hipLaunchKernelGGL(k_sliced_fwd<double>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
+                       P->fplan.group_ptr[l], nsl, view(P->fplan), P->f64.val.p, P->f64.dia.p,
+                       src, vec);
   }
-  check_launch("k_ilu_fwd_level");
+  check_launch("k_sliced_fwd");
 }
 
 void ilu_backward_levels_double(dfm2h_ilu P, double *vec) {
-  for (size_t l = 0; l + 1 < P->levptr_b.size(); ++l) {
-    const unsigned nrows = P->levptr_b[l + 1] - P->levptr_b[l];
-    hipLaunchKernelGGL(k_ilu_bwd_level<double>, dim3(ceil_div(nrows, kRowsPerBlock)), dim3(kBlock),
-                       0, stream(), P->levrows_b.p + P->levptr_b[l], nrows, P->colind_d.p,
-                       P->rowptr_d.p, P->diaind_d.p, P->vcrs.p, vec);
+  ensure_packed(P->bplan, P->b64, P->vcrs.p, nullptr, P->version, stream());
+  for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
+    const unsigned nsl = P->bplan.group_slices(l);
+    hipLaunchKernelGGL(k_sliced_bwd<double>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
+                       P->bplan.group_ptr[l], nsl, view(P->bplan), P->b64.val.p, vec);
   }
-  check_launch("k_ilu_bwd_level");
+  check_launch("k_sliced_bwd");
 }
 
 }  // namespace dfm2h
@@ -231,10 +232,28 @@ int dfm2h_ilu_set_pattern0(dfm2h_ilu P, dfm2h_bsr A) {
     P->vcrs.alloc(P->ncrs * 9);
     P->vdia.alloc((size_t)P->nblk * 9);
     std::vector<unsigned> rows;
-    build_levels(P->nblk, P->colind, P->rowptr, P->diaind, true, P->levptr_f, rows);
-    P->levrows_f.upload(rows);
-    build_levels(P->nblk, P->colind, P->rowptr, P->diaind, false, P->levptr_b, rows);
-    P->levrows_b.upload(rows);
+    {
+      std::vector<unsigned> s0(P->nblk), s1(P->nblk);
+      build_levels(P->nblk, P->colind, P->rowptr, P->diaind, true, P->levptr_f, rows);
+      P->levrows_f.upload(rows);
+      std::vector<std::vector<unsigned>> groups;
+      for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l)
+        groups.emplace_back(rows.begin() + P->levptr_f[l], rows.begin() + P->levptr_f[l + 1]);
+      for (unsigned i = 0; i < P->nblk; ++i) {
+        s0[i] = P->colind[i];
+        s1[i] = P->diaind[i];
+      }
+      P->fplan.build(groups, s0, s1, P->rowptr);
+      build_levels(P->nblk, P->colind, P->rowptr, P->diaind, false, P->levptr_b, rows);
+      groups.clear();
+      for (size_t l = 0; l + 1 < P->levptr_b.size(); ++l)
+        groups.emplace_back(rows.begin() + P->levptr_b[l], rows.begin() + P->levptr_b[l + 1]);
+      for (unsigned i = 0; i < P->nblk; ++i) {
+        s0[i] = P->diaind[i];
+        s1[i] = P->colind[i + 1];
+      }
+      P->bplan.build(groups, s0, s1, P->rowptr);
+    }
     const std::vector<int> a2l = build_a2l(*P, pa);
     P->a2l_d.upload(a2l);
     P->mapped_for = A;

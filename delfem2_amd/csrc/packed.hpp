@@ -1,0 +1,280 @@
+// packed.hpp -- sliced, wavefront-coalesced storage of a set of 3x3 block
+// rows, and the row kernels that run on it (SpMV, ILU forward / backward).
+//
+// A "plan" groups rows (all rows for the SpMV; the rows of one dependency
+// level for a sweep) into slices of 21 rows.  One wavefront owns a slice:
+// lane = c*21 + r handles component c of slice row r (lane 63 idles).  For
+// the k-th block of every row of slice s (block-slot q = off[s] + k):
+//   col [q*21 + r]                column of row r's k-th block
+//   val [q*189 + t*63 + c*21 + r] entry (c, t) of that block
+// so each of the three value loads of a block is one contiguous 504-byte
+// (FP64) run across the wavefront, and the column index is shared by the
+// three lanes of a row.  The row's own 3x3 block (diagonal of A, or D^-1 of
+// the factorisation) is held the same way at dia[s*189 + t*63 + c*21 + r].
+// Blocks of a row keep their CRS order, so every per-component sum is formed
+// in exactly the order of the reference loops (bit-identical in FP64).
+#pragma once
+
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace dfm2h {
+
+constexpr int kSliceRows = 21;
+constexpr int kSliceVals = 9 * kSliceRows;  // 189
+constexpr int kPackBlock = 256;              // 4 wavefronts = 4 slices per workgroup
+constexpr int kSlicesPerBlock = kPackBlock / kWave;
+
+// Host-built index structure of a plan; values are packed per precision.
+struct PackedPlan {
+  unsigned nslices = 0;
+  size_t nq = 0;                          // block-slots over all slices
+  std::vector<unsigned> group_ptr;        // slices of group g: [group_ptr[g], group_ptr[g+1])
+  DBuf<unsigned> off, kmax;               // per slice
+  DBuf<unsigned> row, len;                // per slice row (row = ~0u for padding)
+  DBuf<unsigned> col;                     // per block-slot x slice row
+  DBuf<int> src;                          // CRS slot feeding (q, r), -1 for padding
+  bool built = false;
+
+  // groups: list of row lists; ranges: CRS slot range [s0, s1) of each row's blocks
+  void build(const std::vector<std::vector<unsigned>> &groups, const std::vector<unsigned> &s0,
+             const std::vector<unsigned> &s1, const std::vector<unsigned> &crs_col) {
+    std::vector<unsigned> h_off, h_kmax, h_row, h_len, h_col;
+    std::vector<int> h_src;
+    group_ptr.assign(1, 0);
+    size_t q = 0;
+    for (const auto &g : groups) {
+      for (size_t b = 0; b < g.size(); b += kSliceRows) {
+        const size_t e = std::min(g.size(), b + kSliceRows);
+        unsigned km = 0;
+        for (size_t k = b; k < e; ++k) km = std::max(km, s1[g[k]] - s0[g[k]]);
+        h_off.push_back((unsigned)q);
+        h_kmax.push_back(km);
+        for (int rr = 0; rr < kSliceRows; ++rr) {
+          const size_t k = b + rr;
+          h_row.push_back(k < e ? g[k] : ~0u);
+          h_len.push_back(k < e ? s1[g[k]] - s0[g[k]] : 0u);
+        }
+        for (unsigned kk = 0; kk < km; ++kk)
+          for (int rr = 0; rr < kSliceRows; ++rr) {
+            const size_t k = b + rr;
+            if (k < e && kk < s1[g[k]] - s0[g[k]]) {
+              const unsigned s = s0[g[k]] + kk;
+              h_col.push_back(crs_col[s]);
+              h_src.push_back((int)s);
+            } else {
+              h_col.push_back(0u);
+              h_src.push_back(-1);
+            }
+          }
+        q += km;
+      }
+      group_ptr.push_back((unsigned)h_off.size());
+    }
+    nslices = (unsigned)h_off.size();
+    nq = q;
+    off.upload(h_off);
+    kmax.upload(h_kmax);
+    row.upload(h_row);
+    len.upload(h_len);
+    col.upload(h_col);
+    src.upload(h_src);
+    built = true;
+  }
+  unsigned group_slices(size_t g) const { return group_ptr[g + 1] - group_ptr[g]; }
+};
+
+// Packed values of one plan in precision T.
+template <typename T>
+struct PackedVals {
+  DBuf<T> val, dia;
+  uint64_t version = ~0ull;
+};
+
+// ---------------------------------------------------------------------------
+// packing kernels (gather from CRS storage, optional rounding to T)
+
+template <typename T>
+__global__ void k_pack_blocks(size_t nq, const int *__restrict__ src, const double *__restrict__ crs,
+                              T *__restrict__ val) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // over nq*21*9
+  if (t >= nq * kSliceVals) return;
+  const size_t q = t / kSliceVals;
+  const unsigned rem = (unsigned)(t % kSliceVals);
+  const unsigned tt = rem / 63, lane = rem % 63;
+  const unsigned c = lane / kSliceRows, r = lane % kSliceRows;
+  const int s = src[q * kSliceRows + r];
+  val[t] = s >= 0 ? (T)crs[(size_t)s * 9 + c * 3 + tt] : (T)0;
+}
+
+template <typename T>
+__global__ void k_pack_dia(unsigned nslices, const unsigned *__restrict__ row,
+                           const double *__restrict__ dia_crs, T *__restrict__ dia) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nslices * kSliceVals) return;
+  const size_t s = t / kSliceVals;
+  const unsigned rem = (unsigned)(t % kSliceVals);
+  const unsigned tt = rem / 63, lane = rem % 63;
+  const unsigned c = lane / kSliceRows, r = lane % kSliceRows;
+  const unsigned i = row[s * kSliceRows + r];
+  dia[t] = i != ~0u ? (T)dia_crs[(size_t)i * 9 + c * 3 + tt] : (T)0;
+}
+
+// ---------------------------------------------------------------------------
+// row kernels
+
+struct SliceLane {
+  unsigned s;     // slice
+  unsigned lane;  // 0..63
+  unsigned c, r;  // component, slice row (c == 3 for the idle lane)
+};
+
+__device__ inline SliceLane slice_lane(unsigned slice0) {
+  SliceLane L;
+  L.lane = threadIdx.x & (kWave - 1);
+  L.s = slice0 + blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
+  L.c = L.lane / kSliceRows;
+  L.r = L.lane - L.c * kSliceRows;
+  return L;
+}
+
+// Accumulates this lane's component over the row's blocks in stored order:
+//   SUB:  acc -= (v0*x0 + v1*x1) + v2*x2
+//   else: acc += alpha * ((v0*x0 + v1*x1) + v2*x2)
+// U blocks per round: their column indices and values are loaded first (one
+// memory round trip), then the U gathers of x (a second one), then the U
+// updates in order.
+template <typename T, bool SUB, int U>
+__device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned K, unsigned kmax,
+                                   unsigned q0, const unsigned *__restrict__ col,
+                                   const T *__restrict__ val, const T *x) {
+  for (unsigned k0 = 0; k0 < kmax; k0 += U) {
+    unsigned j[U];
+    T v[U][3], xv[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u < K) {
+        const size_t q = q0 + k0 + u;
+        j[u] = col[q * kSliceRows + L.r];
+        const T *vp = val + q * kSliceVals + L.lane;
+        v[u][0] = vp[0];
+        v[u][1] = vp[63];
+        v[u][2] = vp[126];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u < K) {
+        const T *xp = x + 3 * (size_t)j[u];
+        xv[u][0] = xp[0];
+        xv[u][1] = xp[1];
+        xv[u][2] = xp[2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u < K) {
+        const T d = v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
+        if (SUB)
+          acc -= d;
+        else
+          acc += alpha * d;
+      }
+    }
+  }
+  return acc;
+}
+
+template <typename T>
+__device__ inline T dia_row_dot(const T *__restrict__ dia, const SliceLane &L, T x0, T x1, T x2) {
+  const T *dp = dia + (size_t)L.s * kSliceVals + L.lane;
+  return dp[0] * x0 + dp[63] * x1 + dp[126] * x2;
+}
+
+struct PlanView {
+  const unsigned *off, *kmax, *row, *len, *col;
+};
+
+inline PlanView view(const PackedPlan &P) {
+  return PlanView{P.off.p, P.kmax.p, P.row.p, P.len.p, P.col.p};
+}
+
+// blocks per load round: the SpMV runs at high occupancy (fewer registers);
+// a sweep level has few wavefronts per CU, so it loads a whole row at once
+template <typename T>
+constexpr int unroll_spmv() {
+  return sizeof(T) == 8 ? 4 : 8;
+}
+template <typename T>
+constexpr int unroll_sweep() {
+  return 16;
+}
+
+// y = alpha*A*x + beta*y over the slices [slice0, slice0+nsl) (all rows)
+template <typename T>
+__global__ void __launch_bounds__(kPackBlock)
+k_sliced_matvec(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val,
+                const T *__restrict__ dia, T *__restrict__ y, T alpha, const T *__restrict__ x,
+                T beta) {
+  const SliceLane L = slice_lane(slice0);
+  if (L.s >= slice0 + nsl || L.c >= 3) return;
+  const unsigned i = P.row[L.s * kSliceRows + L.r];
+  if (i == ~0u) return;
+  T acc = y[3 * (size_t)i + L.c] * beta;
+  acc = sliced_row_sum<T, false, unroll_spmv<T>()>(acc, alpha, L, P.len[L.s * kSliceRows + L.r],
+                                                 P.kmax[L.s], P.off[L.s], P.col, val, x);
+  const T *xi = x + 3 * (size_t)i;
+  acc += alpha * dia_row_dot(dia, L, xi[0], xi[1], xi[2]);
+  y[3 * (size_t)i + L.c] = acc;
+}
+
+// forward sweep of one level: vec_i = D_i^-1 (src_i - sum L_ij vec_j)
+template <typename T>
+__device__ inline void sliced_fwd_row(const SliceLane &L, bool valid, unsigned i, const PlanView &P,
+                                      const T *__restrict__ val, const T *__restrict__ dia,
+                                      const T *src, T *vec, T &out) {
+  T t = 0;
+  if (valid) {
+    t = src[3 * (size_t)i + L.c];
+    t = sliced_row_sum<T, true, unroll_sweep<T>()>(t, (T)1, L, P.len[L.s * kSliceRows + L.r],
+                                                P.kmax[L.s], P.off[L.s], P.col, val, vec);
+  }
+  const T t0 = __shfl(t, (int)L.r, kWave);
+  const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
+  const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
+  if (valid) {
+    out = dia_row_dot(dia, L, t0, t1, t2);
+    vec[3 * (size_t)i + L.c] = out;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kPackBlock)
+k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val,
+             const T *__restrict__ dia, const T *src, T *vec) {
+  const SliceLane L = slice_lane(slice0);
+  if (L.s >= slice0 + nsl) return;  // whole wavefront
+  const unsigned i = L.c < 3 ? P.row[L.s * kSliceRows + L.r] : ~0u;
+  const bool valid = i != ~0u;
+  T out;
+  sliced_fwd_row(L, valid, i, P, val, dia, src, vec, out);
+}
+
+// backward sweep of one level: vec_i -= sum U_ij vec_j (U scaled by D^-1)
+template <typename T>
+__global__ void __launch_bounds__(kPackBlock)
+k_sliced_bwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val, T *vec) {
+  const SliceLane L = slice_lane(slice0);
+  if (L.s >= slice0 + nsl || L.c >= 3) return;
+  const unsigned i = P.row[L.s * kSliceRows + L.r];
+  if (i == ~0u) return;
+  T t = vec[3 * (size_t)i + L.c];
+  t = sliced_row_sum<T, true, unroll_sweep<T>()>(t, (T)1, L, P.len[L.s * kSliceRows + L.r],
+                                              P.kmax[L.s], P.off[L.s], P.col, val, vec);
+  vec[3 * (size_t)i + L.c] = t;
+}
+
+inline unsigned slice_grid(unsigned nsl) { return std::max(1u, ceil_div(nsl, kSlicesPerBlock)); }
+
+}  // namespace dfm2h

@@ -2,20 +2,21 @@
 // (lsitrsol.h:80-148) in FP64, plus FP32-inner / FP64-refinement.
 //
 // Fused iteration (one HIP graph node per kernel, K iterations per graph):
-//   spmv   Ap = A p, partial (p,Ap); last block: alpha = rPr/pAp
-//   update r += -alpha Ap, x += alpha p, partial (r,r); last block: history,
-//          convergence test, iteration count
+//   spmv   Ap = A p, one partial (p,Ap) per workgroup
+//   scal   alpha = rPr/pAp                          (one workgroup)
+//   update r += -alpha Ap, x += alpha p, partial (r,r)
+//   scal   history, convergence test, iteration count
 //   fwd    one kernel per dependency level: Pr = L^-1 r (reads r directly,
 //          which fuses the reference's copy Pr = r)
-//   bwd    one kernel per level: Pr = U^-1 Pr, partial (r,Pr); the last
-//          block of the last level: beta = rPr1/rPr, rPr = rPr1
+//   bwd    one kernel per level: Pr = U^-1 Pr, partial (r,Pr)
+//   scal   beta = rPr1/rPr, rPr = rPr1
 //   pupd   p = beta p + Pr
 // Every kernel returns at once when the device-side `done` flag is set, so
 // iterations launched after convergence cost only their launch.  Scalars
 // never leave HBM; the host polls `done` once per chunk, one chunk behind.
-// All grid reductions are single-pass with a fixed partition and a fixed
-// final order, hence deterministic run to run.
-#include "kernels.hpp"
+// All grid reductions use a fixed partition and a fixed final order, hence
+// are deterministic run to run.
+#include "handles.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -35,7 +36,14 @@ struct PcgScal {
   int max_nitr;
   int nan;
   int pad;
-  unsigned ticket[4];
+  // mixed precision (reliable updates)
+  double sq_b;    // |b|^2 in FP64
+  double rmax2;   // FP64 |r|^2 at the last reliable update
+  double delta2;  // update when the FP32 |r|^2 fell below delta2 * rmax2
+  int upd;        // this iteration folds x32 into x64 and recomputes r in FP64
+  int nupd;       // FP64 residuals recorded after the initial |b|
+  int max_upd;
+  int final_;     // iteration cap reached: stop after this update
 };
 
 constexpr int kVecBlock = 256;
@@ -49,21 +57,19 @@ static unsigned vec_grid(size_t n) {
 // generic deterministic dot (host-facing Dot)
 
 __global__ void __launch_bounds__(kVecBlock)
-k_dot(const double *__restrict__ x, const double *__restrict__ y, size_t n,
-      double *partials, unsigned *ticket, double *out) {
+k_dot(const double *__restrict__ x, const double *__restrict__ y, size_t n, double *partials) {
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < n;
        i += (size_t)gridDim.x * kVecBlock)
     s += x[i] * y[i];
   s = block_reduce_sum<kVecBlock>(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
-  if (last_block(ticket, gridDim.x)) {
-    const double t = reduce_partials<kVecBlock>(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      *out = t;
-      *ticket = 0;
-    }
-  }
+}
+
+__global__ void __launch_bounds__(kVecBlock)
+k_sum_partials(const double *__restrict__ partials, unsigned n, double *out) {
+  const double t = reduce_partials<kVecBlock>(partials, n);
+  if (threadIdx.x == 0) *out = t;
 }
 
 __global__ void k_add_scaled(double *y, double alpha, const double *x, size_t n) {
@@ -99,19 +105,26 @@ template <typename T>
 struct PcgArgs {
   unsigned nblk;
   size_t n;
-  // matrix
-  const unsigned *a_colind, *a_rowptr;
-  const T *a_vcrs, *a_vdia;
-  // factors
-  const unsigned *l_colind, *l_rowptr, *l_diaind;
-  const T *l_vcrs, *l_vdia;
+  // matrix (all rows, sliced) and factors (forward / backward plans, sliced)
+  PlanView a_plan;
+  unsigned a_nsl;
+  const T *a_val, *a_dia;
+  PlanView f_plan;
+  const T *f_val, *f_dia;
+  PlanView b_plan;
+  const T *b_val;
   // vectors
   T *r, *x, *Pr, *p;
   double *partials, *hist;
   PcgScal *S;
+  // mixed precision: FP64 solution, right-hand side, matrix and outer history
+  double *x64;
+  const double *b64;
+  const double *a64_val, *a64_dia;
+  double *hist64;
 };
 
-// x = 0, sq0 = (r,r), history[0]; stop at once when sq0 < 1e-30
+// x = 0, partial (r,r)
 template <typename T>
 __global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T> a) {
   double s = 0.0;
@@ -123,56 +136,131 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T> a) {
   }
   s = block_reduce_sum<kVecBlock>(s);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
-  if (last_block(&a.S->ticket[0], gridDim.x)) {
-    const double sq0 = reduce_partials<kVecBlock>(a.partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      a.S->ticket[0] = 0;
-      a.hist[0] = sqrt(sq0);
-      a.S->sq = sq0;
-      a.S->it = 0;
-      a.S->converged = 0;
-      a.S->nan = 0;
-      if (sq0 < 1.0e-30) {
-        a.S->done = 1;
-        a.S->converged = 1;
-      } else if (!(sq0 == sq0) || sq0 > 1.0e308) {
-        a.S->done = 1;
-        a.S->nan = 1;
-      } else {
-        a.S->done = 0;
-      }
-      a.S->inv_sq0 = 1.0 / sq0;
+}
+
+// Scalar steps, one workgroup each, over the partials the previous kernel
+// wrote (lsitrsol.h:95-100, 116-117, 122-125, 132-135).
+enum ScalMode {
+  kScalInit = 0,
+  kScalAlpha = 1,
+  kScalConv = 2,
+  kScalRPr0 = 3,
+  kScalBeta = 4,
+  kScalInitMixed = 5,  // |b|^2: outer history[0], rmax2
+  kScalConvMixed = 6,  // FP32 |r|^2: decide a reliable update
+  kScalReliable = 7    // FP64 |b - A x|^2: outer history, stop test
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(kVecBlock)
+k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double *hist) {
+  if (MODE != kScalInit && MODE != kScalInitMixed && S->done) return;
+  if (MODE == kScalReliable && !S->upd) return;
+  const double t = reduce_partials<kVecBlock>(partials, n);
+  if (threadIdx.x != 0) return;
+  if (MODE == kScalInit) {  // |r0|^2: history[0]; early return when < 1e-30
+    hist[0] = sqrt(t);
+    S->sq = t;
+    S->it = 0;
+    S->converged = 0;
+    S->nan = 0;
+    S->done = 0;
+    if (t < 1.0e-30) {
+      S->done = 1;
+      S->converged = 1;
+    } else if (!(t == t) || t > 1.0e308) {
+      S->done = 1;
+      S->nan = 1;
+    }
+    S->inv_sq0 = 1.0 / t;
+  } else if (MODE == kScalAlpha) {  // alpha = rPr / pAp
+    S->alpha = S->rPr / t;
+  } else if (MODE == kScalConv) {  // push |r|, stop on sqrt(|r|^2/|r0|^2) < tol
+    const int it = S->it + 1;
+    S->it = it;
+    S->sq = t;
+    hist[it] = sqrt(t);
+    if (sqrt(t * S->inv_sq0) < S->tol) {
+      S->done = 1;
+      S->converged = 1;
+    } else if (!(t == t) || t > 1.0e308) {
+      S->done = 1;
+      S->nan = 1;
+    } else if (it >= S->max_nitr) {
+      S->done = 1;
+    }
+  } else if (MODE == kScalRPr0) {
+    S->rPr = t;
+  } else if (MODE == kScalBeta) {  // beta = rPr1 / rPr; rPr = rPr1
+    S->beta = t / S->rPr;
+    S->rPr = t;
+  } else if (MODE == kScalInitMixed) {
+    hist[0] = sqrt(t);
+    S->sq_b = t;
+    S->rmax2 = t;
+    S->sq = t;
+    S->it = 0;
+    S->nupd = 0;
+    S->upd = 0;
+    S->final_ = 0;
+    S->converged = 0;
+    S->nan = 0;
+    S->done = 0;
+    if (t < 1.0e-30) {
+      S->done = 1;
+      S->converged = 1;
+    } else if (!(t == t) || t > 1.0e308) {
+      S->done = 1;
+      S->nan = 1;
+    }
+  } else if (MODE == kScalConvMixed) {
+    const int it = S->it + 1;
+    S->it = it;
+    S->sq = t;
+    if (!(t == t) || t > 1.0e308) {
+      S->done = 1;
+      S->nan = 1;
+    } else if (t < S->delta2 * S->rmax2 || sqrt(t / S->sq_b) < S->tol || it >= S->max_nitr) {
+      S->upd = 1;
+      S->final_ = it >= S->max_nitr ? 1 : 0;
+    }
+  } else {  // kScalReliable
+    const int k = S->nupd + 1;
+    S->nupd = k;
+    hist[k] = sqrt(t);
+    S->upd = 0;
+    S->rmax2 = t;
+    if (sqrt(t / S->sq_b) < S->tol) {
+      S->done = 1;
+      S->converged = 1;
+    } else if (!(t == t) || t > 1.0e308) {
+      S->done = 1;
+      S->nan = 1;
+    } else if (S->final_ || k >= S->max_upd) {
+      S->done = 1;
     }
   }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_pcg_spmv(PcgArgs<T> a) {
+__global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T> a) {
   if (a.S->done) return;
-  const RowLane L = row_lane();
-  const bool valid = L.r < kRowsPerWave && L.slot < a.nblk;
+  const SliceLane L = slice_lane(0);
+  const bool slice_ok = L.s < a.a_nsl && L.c < 3;
+  const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
   double part = 0.0;
-  if (valid) {
-    const unsigned i = L.slot, c = L.c;
+  if (i != ~0u) {
     T acc = 0;
-    const unsigned s1 = a.a_colind[i + 1];
-    for (unsigned s = a.a_colind[i]; s < s1; ++s) {
-      const size_t j = a.a_rowptr[s];
-      acc += blk_row_dot(a.a_vcrs + (size_t)s * 9 + 3 * c, a.p + 3 * j);
-    }
-    acc += blk_row_dot(a.a_vdia + (size_t)i * 9 + 3 * c, a.p + 3 * (size_t)i);
-    a.Pr[3 * (size_t)i + c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
-    part = (double)a.p[3 * (size_t)i + c] * (double)acc;
+    acc = sliced_row_sum<T, false, unroll_spmv<T>()>(acc, (T)1, L, a.a_plan.len[L.s * kSliceRows + L.r],
+                                                   a.a_plan.kmax[L.s], a.a_plan.off[L.s],
+                                                   a.a_plan.col, a.a_val, a.p);
+    const T *pi = a.p + 3 * (size_t)i;
+    acc += dia_row_dot(a.a_dia, L, pi[0], pi[1], pi[2]);
+    a.Pr[3 * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
+    part = (double)pi[L.c] * (double)acc;
   }
-  part = block_reduce_sum<kBlock>(part);
+  part = block_reduce_sum<kPackBlock>(part);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
-  if (last_block(&a.S->ticket[1], gridDim.x)) {
-    const double pAp = reduce_partials<kBlock>(a.partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      a.S->ticket[1] = 0;
-      a.S->alpha = a.S->rPr / pAp;
-    }
-  }
 }
 
 template <typename T>
@@ -189,69 +277,41 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_update(PcgArgs<T> a) {
   }
   s = block_reduce_sum<kVecBlock>(s);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
-  if (last_block(&a.S->ticket[2], gridDim.x)) {
-    const double sq = reduce_partials<kVecBlock>(a.partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      a.S->ticket[2] = 0;
-      const int it = a.S->it + 1;
-      a.S->it = it;
-      a.S->sq = sq;
-      a.hist[it] = sqrt(sq);
-      if (sqrt(sq * a.S->inv_sq0) < a.S->tol) {
-        a.S->done = 1;
-        a.S->converged = 1;
-      } else if (!(sq == sq) || sq > 1.0e308) {
-        a.S->done = 1;
-        a.S->nan = 1;
-      } else if (it >= a.S->max_nitr) {
-        a.S->done = 1;
-      }
-    }
-  }
 }
 
+// forward level: Pr = D^-1 (r - L Pr) on the level's slices (reads r
+// directly: the reference's copy Pr = r is fused away)
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_pcg_fwd_level(PcgArgs<T> a, const unsigned *__restrict__ rows, unsigned nrows) {
+__global__ void __launch_bounds__(kPackBlock)
+k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl) {
   if (a.S->done) return;
-  const RowLane L = row_lane();
-  const bool valid = L.r < kRowsPerWave && L.slot < nrows;
-  const unsigned i = valid ? rows[L.slot] : 0u;
-  fwd_row(i, L.c, valid, a.l_colind, a.l_rowptr, a.l_diaind, a.l_vcrs, a.l_vdia,
-          (const T *)a.r, a.Pr);
+  const SliceLane L = slice_lane(slice0);
+  if (L.s >= slice0 + nsl) return;  // whole wavefront
+  const unsigned i = L.c < 3 ? a.f_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  T out;
+  sliced_fwd_row(L, i != ~0u, i, a.f_plan, a.f_val, a.f_dia, (const T *)a.r, a.Pr, out);
 }
 
-// backward level + partial (r, Pr) of the rows it finalises.  The last
-// level reduces all levels' partials: prologue (mode 0) sets rPr, an
-// iteration (mode 1) sets beta = rPr1/rPr and rPr = rPr1.
+// backward level + partial (r, Pr) of the rows it finalises, written at
+// partials[part_off + workgroup]
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_pcg_bwd_level(PcgArgs<T> a, const unsigned *__restrict__ rows, unsigned nrows,
-                unsigned part_off, unsigned part_total, int last_level, int mode) {
+__global__ void __launch_bounds__(kPackBlock)
+k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, unsigned part_off) {
   if (a.S->done) return;
-  const RowLane L = row_lane();
-  const bool valid = L.r < kRowsPerWave && L.slot < nrows;
+  const SliceLane L = slice_lane(slice0);
+  const bool slice_ok = L.s < slice0 + nsl && L.c < 3;
+  const unsigned i = slice_ok ? a.b_plan.row[L.s * kSliceRows + L.r] : ~0u;
   double part = 0.0;
-  if (valid) {
-    const unsigned i = rows[L.slot];
-    const T t = bwd_row(i, L.c, a.l_colind, a.l_rowptr, a.l_diaind, a.l_vcrs, a.Pr);
+  if (i != ~0u) {
+    T t = a.Pr[3 * (size_t)i + L.c];
+    t = sliced_row_sum<T, true, unroll_sweep<T>()>(t, (T)1, L, a.b_plan.len[L.s * kSliceRows + L.r],
+                                                a.b_plan.kmax[L.s], a.b_plan.off[L.s],
+                                                a.b_plan.col, a.b_val, a.Pr);
+    a.Pr[3 * (size_t)i + L.c] = t;
     part = (double)a.r[3 * (size_t)i + L.c] * (double)t;
   }
-  part = block_reduce_sum<kBlock>(part);
+  part = block_reduce_sum<kPackBlock>(part);
   if (threadIdx.x == 0) a.partials[part_off + blockIdx.x] = part;
-  if (!last_level) return;
-  if (last_block(&a.S->ticket[3], gridDim.x)) {
-    const double rPr1 = reduce_partials<kBlock>(a.partials, part_total);
-    if (threadIdx.x == 0) {
-      a.S->ticket[3] = 0;
-      if (mode == 0) {
-        a.S->rPr = rPr1;
-      } else {
-        a.S->beta = rPr1 / a.S->rPr;
-        a.S->rPr = rPr1;
-      }
-    }
-  }
 }
 
 // p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue)
@@ -268,44 +328,52 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T> a, int mod
   }
 }
 
-// r = b - A x (FP64) written as FP32 for the inner solve, and |r|^2
-__global__ void __launch_bounds__(kBlock)
-k_residual_to_f32(unsigned nblk, const unsigned *__restrict__ colind,
-                  const unsigned *__restrict__ rowptr, const double *__restrict__ vcrs,
-                  const double *__restrict__ vdia, const double *__restrict__ b,
-                  const double *__restrict__ x, float *__restrict__ r32, double *partials,
-                  unsigned *ticket, double *sq_out) {
-  const RowLane L = row_lane();
-  const bool valid = L.r < kRowsPerWave && L.slot < nblk;
-  double part = 0.0;
-  if (valid) {
-    const unsigned i = L.slot, c = L.c;
-    double acc = 0.0;
-    const unsigned s1 = colind[i + 1];
-    for (unsigned s = colind[i]; s < s1; ++s) {
-      const size_t j = rowptr[s];
-      acc += blk_row_dot(vcrs + (size_t)s * 9 + 3 * c, x + 3 * j);
-    }
-    acc += blk_row_dot(vdia + (size_t)i * 9 + 3 * c, x + 3 * (size_t)i);
-    const double r = b[3 * (size_t)i + c] - acc;
-    r32[3 * (size_t)i + c] = (float)r;
-    part = r * r;
+// x64 = 0, x32 = 0, r32 = (float) b, partial |b|^2 (FP64)
+__global__ void __launch_bounds__(kVecBlock) k_mixed_init(PcgArgs<float> a) {
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
+       i += (size_t)gridDim.x * kVecBlock) {
+    const double v = a.b64[i];
+    s += v * v;
+    a.r[i] = (float)v;
+    a.x[i] = 0.0f;
+    a.x64[i] = 0.0;
   }
-  part = block_reduce_sum<kBlock>(part);
-  if (threadIdx.x == 0) partials[blockIdx.x] = part;
-  if (last_block(ticket, gridDim.x)) {
-    const double t = reduce_partials<kBlock>(partials, gridDim.x);
-    if (threadIdx.x == 0) {
-      *sq_out = t;
-      *ticket = 0;
-    }
+  s = block_reduce_sum<kVecBlock>(s);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+}
+
+// reliable update, part 1: x64 += x32, x32 = 0
+__global__ void __launch_bounds__(kVecBlock) k_mixed_fold(PcgArgs<float> a) {
+  if (a.S->done || !a.S->upd) return;
+  for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
+       i += (size_t)gridDim.x * kVecBlock) {
+    a.x64[i] += (double)a.x[i];
+    a.x[i] = 0.0f;
   }
 }
 
-__global__ void k_add_f32_to_f64(double *__restrict__ x, const float *__restrict__ d, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    x[i] += (double)d[i];
+// reliable update, part 2: r = b - A x64 in FP64 (sliced A, FP64 values),
+// rounded into r32, partial |r|^2
+__global__ void __launch_bounds__(kPackBlock) k_mixed_residual(PcgArgs<float> a) {
+  if (a.S->done || !a.S->upd) return;
+  const SliceLane L = slice_lane(0);
+  const bool slice_ok = L.s < a.a_nsl && L.c < 3;
+  const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  double part = 0.0;
+  if (i != ~0u) {
+    double acc = 0.0;
+    acc = sliced_row_sum<double, false, 4>(acc, 1.0, L, a.a_plan.len[L.s * kSliceRows + L.r],
+                                           a.a_plan.kmax[L.s], a.a_plan.off[L.s], a.a_plan.col,
+                                           a.a64_val, a.x64);
+    const double *xi = a.x64 + 3 * (size_t)i;
+    acc += dia_row_dot(a.a64_dia, L, xi[0], xi[1], xi[2]);
+    const double r = a.b64[3 * (size_t)i + L.c] - acc;
+    a.r[3 * (size_t)i + L.c] = (float)r;
+    part = r * r;
+  }
+  part = block_reduce_sum<kPackBlock>(part);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
 }
 
 // ---------------------------------------------------------------------------
@@ -336,8 +404,7 @@ struct PcgWork {
   PcgScal *host_status = nullptr;  // pinned, 2 entries
   PcgBufs<double> f64;
   PcgBufs<float> f32;
-  DBuf<double> b64, sq, mixed_partials;
-  DBuf<unsigned> ticket;
+  DBuf<double> b64, x64, hist64;
   PcgWork() {
     DFM2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     DFM2H_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
@@ -364,20 +431,20 @@ static PcgWork &work_of(dfm2h_ilu P) {
 }
 
 template <typename T>
-static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const T *a_vcrs,
-                            const T *a_vdia, const T *l_vcrs, const T *l_vdia) {
+static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const PackedVals<T> &av,
+                            const PackedVals<T> &fv, const PackedVals<T> &bv) {
   PcgArgs<T> a;
   a.nblk = A->nblk;
   a.n = (size_t)A->nblk * 3;
-  a.a_colind = A->pat.colind_d.p;
-  a.a_rowptr = A->pat.rowptr_d.p;
-  a.a_vcrs = a_vcrs;
-  a.a_vdia = a_vdia;
-  a.l_colind = P->colind_d.p;
-  a.l_rowptr = P->rowptr_d.p;
-  a.l_diaind = P->diaind_d.p;
-  a.l_vcrs = l_vcrs;
-  a.l_vdia = l_vdia;
+  a.a_plan = view(A->plan);
+  a.a_nsl = A->plan.nslices;
+  a.a_val = av.val.p;
+  a.a_dia = av.dia.p;
+  a.f_plan = view(P->fplan);
+  a.f_val = fv.val.p;
+  a.f_dia = fv.dia.p;
+  a.b_plan = view(P->bplan);
+  a.b_val = bv.val.p;
   a.r = B.r.p;
   a.x = B.x.p;
   a.Pr = B.Pr.p;
@@ -388,11 +455,10 @@ static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const T *a_
   return a;
 }
 
-template <typename T>
 static unsigned bwd_partial_total(dfm2h_ilu P) {
   unsigned tot = 0;
-  for (size_t l = 0; l + 1 < P->levptr_b.size(); ++l)
-    tot += std::max(1u, ceil_div(P->levptr_b[l + 1] - P->levptr_b[l], kRowsPerBlock));
+  for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l)
+    tot += slice_grid(P->bplan.group_slices(l));
   return tot;
 }
 
@@ -403,8 +469,8 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
   B.x.alloc(n);
   B.Pr.alloc(n);
   B.p.alloc(n);
-  const size_t np = std::max<size_t>({(size_t)ceil_div(A->nblk, kRowsPerBlock),
-                                      (size_t)vec_grid(n), (size_t)bwd_partial_total<T>(P), 1});
+  const size_t np = std::max<size_t>({(size_t)slice_grid(A->plan.nslices), (size_t)vec_grid(n),
+                                      (size_t)bwd_partial_total(P), 1});
   if (B.partials.n < np) B.partials.alloc(np);
   if (B.hist.n < (size_t)max_nitr + 2) B.hist.alloc((size_t)max_nitr + 2);
   if (!B.S.p) {
@@ -415,30 +481,39 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
 
 template <typename T>
 static void launch_sweeps(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s, int mode) {
-  for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l) {
-    const unsigned nrows = P->levptr_f[l + 1] - P->levptr_f[l];
-    hipLaunchKernelGGL(k_pcg_fwd_level<T>, dim3(std::max(1u, ceil_div(nrows, kRowsPerBlock))),
-                       dim3(kBlock), 0, s, a, P->levrows_f.p + P->levptr_f[l], nrows);
+  for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
+    const unsigned nsl = P->fplan.group_slices(l);
+    hipLaunchKernelGGL(k_pcg_fwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
+                       P->fplan.group_ptr[l], nsl);
   }
-  const unsigned tot = bwd_partial_total<T>(P);
   unsigned off = 0;
-  const size_t nl = P->levptr_b.size() - 1;
-  for (size_t l = 0; l < nl; ++l) {
-    const unsigned nrows = P->levptr_b[l + 1] - P->levptr_b[l];
-    const unsigned g = std::max(1u, ceil_div(nrows, kRowsPerBlock));
-    hipLaunchKernelGGL(k_pcg_bwd_level<T>, dim3(g), dim3(kBlock), 0, s, a,
-                       P->levrows_b.p + P->levptr_b[l], nrows, off, tot, (int)(l + 1 == nl), mode);
+  for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
+    const unsigned nsl = P->bplan.group_slices(l);
+    const unsigned g = slice_grid(nsl);
+    hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(g), dim3(kPackBlock), 0, s, a, P->bplan.group_ptr[l],
+                       nsl, off);
     off += g;
   }
+  if (mode == 0)
+    hipLaunchKernelGGL(k_pcg_scalar<kScalRPr0>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+                       off, a.hist);
+  else
+    hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+                       off, a.hist);
 }
 
 template <typename T>
 static void launch_iteration(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(std::max(1u, ceil_div(a.nblk, kRowsPerBlock))),
-                     dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_update<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a);
+  const unsigned gs = slice_grid(a.a_nsl);
+  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+                     gs, a.hist);
+  const unsigned gv = vec_grid(a.n);
+  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gv), dim3(kVecBlock), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalConv>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+                     gv, a.hist);
   launch_sweeps(a, P, s, 1);
-  hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
+  hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
 // Runs the fused PCG on B.r (holding b) into B.x.  Returns iterations done;
@@ -453,18 +528,21 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
   init.max_nitr = (int)max_nitr;
   DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_pcg_init<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalInit>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+                     vec_grid(a.n), a.hist);
   launch_sweeps(a, P, s, 0);
   hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
 
   // graph of `chunk` iterations, recaptured only when buffers or schedule change
   const unsigned chunk = std::max(1u, std::min(16u, max_nitr));
-  std::vector<uintptr_t> sig = {(uintptr_t)a.a_vcrs, (uintptr_t)a.a_vdia, (uintptr_t)a.l_vcrs,
-                                (uintptr_t)a.l_vdia, (uintptr_t)a.r,      (uintptr_t)a.partials,
-                                (uintptr_t)a.hist,   (uintptr_t)a.S,      (uintptr_t)P->levrows_f.p,
-                                (uintptr_t)P->levrows_b.p, (uintptr_t)chunk,
-                                (uintptr_t)P->levptr_f.size(), (uintptr_t)P->levptr_b.size(),
-                                (uintptr_t)a.nblk};
+  std::vector<uintptr_t> sig = {(uintptr_t)a.a_val, (uintptr_t)a.a_dia, (uintptr_t)a.f_val,
+                                (uintptr_t)a.f_dia, (uintptr_t)a.b_val, (uintptr_t)a.r,
+                                (uintptr_t)a.partials, (uintptr_t)a.hist, (uintptr_t)a.S,
+                                (uintptr_t)a.a_plan.col, (uintptr_t)a.f_plan.col,
+                                (uintptr_t)a.b_plan.col, (uintptr_t)chunk,
+                                (uintptr_t)P->fplan.group_ptr.size(),
+                                (uintptr_t)P->bplan.group_ptr.size(), (uintptr_t)a.nblk};
   if (!B.exec || B.sig != sig) {
     B.drop_graph();
     DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -496,34 +574,97 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
   (void)launched;
 }
 
+// One mixed iteration: FP32 PCG step; when the FP32 residual has dropped by
+// delta since the last reliable update (or looks converged), fold x32 into
+// x64, recompute r = b - A x64 in FP64, test convergence in FP64, continue
+// the same Krylov sequence from the true residual.
+static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, hipStream_t s) {
+  const unsigned gs = slice_grid(a.a_nsl);
+  const unsigned gv = vec_grid(a.n);
+  hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+                     gs, a.hist);
+  hipLaunchKernelGGL(k_pcg_update<float>, dim3(gv), dim3(kVecBlock), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalConvMixed>, dim3(1), dim3(kVecBlock), 0, s, a.S,
+                     a.partials, gv, a.hist);
+  hipLaunchKernelGGL(k_mixed_fold, dim3(gv), dim3(kVecBlock), 0, s, a);
+  hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalReliable>, dim3(1), dim3(kVecBlock), 0, s, a.S,
+                     a.partials, gs, a.hist64);
+  launch_sweeps(a, P, s, 1);
+  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
+}
+
+static void run_pcg_mixed(dfm2h_ilu P, PcgWork &W, PcgBufs<float> &B, const PcgArgs<float> &a,
+                          double tol, double delta, unsigned max_outer, unsigned max_inner,
+                          PcgScal &status) {
+  hipStream_t s = W.s;
+  PcgScal init{};
+  init.tol = tol;
+  init.max_nitr = (int)max_inner;
+  init.delta2 = delta * delta;
+  init.max_upd = (int)max_outer;
+  DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
+  const unsigned gv = vec_grid(a.n);
+  hipLaunchKernelGGL(k_mixed_init, dim3(gv), dim3(kVecBlock), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalInitMixed>, dim3(1), dim3(kVecBlock), 0, s, a.S,
+                     a.partials, gv, a.hist64);
+  launch_sweeps(a, P, s, 0);
+  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  check_launch("mixed prologue");
+  const unsigned chunk = std::max(1u, std::min(16u, max_inner));
+  std::vector<uintptr_t> sig = {(uintptr_t)a.a_val,  (uintptr_t)a.a_dia,   (uintptr_t)a.f_val,
+                                (uintptr_t)a.f_dia,  (uintptr_t)a.b_val,   (uintptr_t)a.r,
+                                (uintptr_t)a.partials, (uintptr_t)a.hist64, (uintptr_t)a.S,
+                                (uintptr_t)a.a64_val, (uintptr_t)a.a64_dia, (uintptr_t)a.x64,
+                                (uintptr_t)a.b64,    (uintptr_t)a.a_plan.col,
+                                (uintptr_t)a.f_plan.col, (uintptr_t)a.b_plan.col,
+                                (uintptr_t)chunk,    (uintptr_t)P->fplan.group_ptr.size(),
+                                (uintptr_t)P->bplan.group_ptr.size(), (uintptr_t)a.nblk};
+  if (!B.exec || B.sig != sig) {
+    B.drop_graph();
+    DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < chunk; ++k) launch_mixed_iteration(a, P, s);
+    hipError_t e = hipStreamEndCapture(s, &B.graph);
+    DFM2H_REQUIRE(e == hipSuccess, DFM2H_ERR_HIP,
+                  std::string("graph capture: ") + hipGetErrorString(e));
+    DFM2H_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    B.sig = sig;
+    B.chunk = chunk;
+  }
+  const unsigned nchunks = (max_inner + chunk - 1) / chunk + 1;
+  bool finished = false;
+  for (unsigned c = 0; c < nchunks && !finished; ++c) {
+    DFM2H_HIP(hipGraphLaunch(B.exec, s));
+    DFM2H_HIP(hipMemcpyAsync(&W.host_status[c & 1], B.S.p, sizeof(PcgScal),
+                             hipMemcpyDeviceToHost, s));
+    DFM2H_HIP(hipEventRecord(W.ev_poll[c & 1], s));
+    if (c > 0) {
+      DFM2H_HIP(hipEventSynchronize(W.ev_poll[(c - 1) & 1]));
+      finished = W.host_status[(c - 1) & 1].done != 0;
+    }
+  }
+  DFM2H_HIP(hipMemcpyAsync(&W.host_status[0], B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, s));
+  DFM2H_HIP(hipStreamSynchronize(s));
+  status = W.host_status[0];
+}
+
 static void fence_in(PcgWork &W) {
   DFM2H_HIP(hipEventRecord(W.ev_in, stream()));
   DFM2H_HIP(hipStreamWaitEvent(W.s, W.ev_in, 0));
 }
 
-// FP32 copies of A's values and P's factors, refreshed when either changed
+// Packed FP32 copies of A's values and P's factors (rounded from FP64)
 static void ensure_f32(dfm2h_bsr A, dfm2h_ilu P, hipStream_t s) {
-  const uint64_t v = (A->version << 32) ^ P->version;
-  if (P->version32 == v && A->vcrs32.p && P->vcrs32.p) return;
-  const size_t na = A->pat.ncrs * 9, nd = (size_t)A->nblk * 9, nl = P->ncrs * 9;
-  A->vcrs32.alloc(na);
-  A->vdia32.alloc(nd);
-  P->vcrs32.alloc(nl);
-  P->vdia32.alloc(nd);
-  if (na)
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(vec_grid(na)), dim3(256), 0, s,
-                       A->vcrs32.p, A->vcrs.p, na);
-  if (nd) {
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(vec_grid(nd)), dim3(256), 0, s,
-                       A->vdia32.p, A->vdia.p, nd);
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(vec_grid(nd)), dim3(256), 0, s,
-                       P->vdia32.p, P->vdia.p, nd);
-  }
-  if (nl)
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(vec_grid(nl)), dim3(256), 0, s,
-                       P->vcrs32.p, P->vcrs.p, nl);
-  check_launch("k_convert");
-  P->version32 = v;
+  ensure_packed(A->plan, A->pk32, A->vcrs.p, A->vdia.p, A->version, s);
+  ensure_packed(P->fplan, P->f32, P->vcrs.p, P->vdia.p, P->version, s);
+  ensure_packed(P->bplan, P->b32, P->vcrs.p, nullptr, P->version, s);
+}
+
+static void ensure_f64(dfm2h_bsr A, dfm2h_ilu P, hipStream_t s) {
+  ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, s);
+  ensure_packed(P->fplan, P->f64, P->vcrs.p, P->vdia.p, P->version, s);
+  ensure_packed(P->bplan, P->b64, P->vcrs.p, nullptr, P->version, s);
 }
 
 }  // namespace dfm2h
@@ -547,16 +688,12 @@ int dfm2h_vec_dot(const double *x, const double *y, size_t n, double *out) {
     }
     DFM2H_REQUIRE(x && y, DFM2H_ERR_INVALID, "null vector");
     static thread_local DBuf<double> partials, res;
-    static thread_local DBuf<unsigned> ticket;
     const unsigned g = vec_grid(n);
     if (partials.n < g) partials.alloc(kVecGridMax);
     if (!res.p) res.alloc(1);
-    if (!ticket.p) {
-      ticket.alloc(1);
-      ticket.zero();
-    }
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kVecBlock), 0, stream(), x, y, n, partials.p,
-                       ticket.p, res.p);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kVecBlock), 0, stream(), x, y, n, partials.p);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kVecBlock), 0, stream(), partials.p, g,
+                       res.p);
     check_launch("k_dot");
     DFM2H_HIP(hipMemcpyAsync(out, res.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
     DFM2H_HIP(hipStreamSynchronize(stream()));
@@ -609,7 +746,8 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
     const size_t n = (size_t)A->nblk * 3;
     fence_in(W);
     DFM2H_HIP(hipMemcpyAsync(B.r.p, r, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
-    const PcgArgs<double> a = make_args<double>(A, P, B, A->vcrs.p, A->vdia.p, P->vcrs.p, P->vdia.p);
+    ensure_f64(A, P, W.s);
+    const PcgArgs<double> a = make_args<double>(A, P, B, A->pk64, P->f64, P->b64);
     PcgScal st{};
     run_pcg(A, P, W, B, a, tol, max_nitr, st);
     DFM2H_HIP(hipMemcpyAsync(r, B.r.p, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
@@ -628,68 +766,42 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
 }
 
 int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, double tol,
-                          double inner_tol, unsigned max_outer, unsigned max_inner, double *hist,
+                          double delta, unsigned max_outer, unsigned max_inner, double *hist,
                           size_t hist_cap, size_t *n_hist, size_t *n_inner) {
   return guard(__func__, [&] {
     need_solver(A, P);
     DFM2H_REQUIRE(b && x, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_REQUIRE(delta > 0.0 && delta < 1.0, DFM2H_ERR_INVALID, "delta must be in (0, 1)");
+    DFM2H_REQUIRE(max_outer >= 1 && max_inner >= 1, DFM2H_ERR_INVALID, "caps must be >= 1");
     PcgWork &W = work_of(P);
     PcgBufs<float> &B = W.f32;
     ensure_bufs(B, A, P, max_inner);
     const size_t n = (size_t)A->nblk * 3;
-    const unsigned nbr = std::max(1u, ceil_div(A->nblk, kRowsPerBlock));
-    if (W.mixed_partials.n < nbr) W.mixed_partials.alloc(nbr);
-    if (!W.sq.p) W.sq.alloc(1);
-    if (!W.ticket.p) {
-      W.ticket.alloc(1);
-      DFM2H_HIP(hipMemsetAsync(W.ticket.p, 0, sizeof(unsigned), stream()));
-    }
+    W.b64.alloc(n);
+    W.x64.alloc(n);
+    if (W.hist64.n < (size_t)max_outer + 2) W.hist64.alloc((size_t)max_outer + 2);
     fence_in(W);
     ensure_f32(A, P, W.s);
-    hipLaunchKernelGGL(k_fill<double>, dim3(vec_grid(n)), dim3(256), 0, W.s, x, 0.0, n);
-    const PcgArgs<float> a =
-        make_args<float>(A, P, B, A->vcrs32.p, A->vdia32.p, P->vcrs32.p, P->vdia32.p);
-    std::vector<double> h;
-    size_t inner_total = 0;
-    bool nan = false;
-    double sq0 = 0.0;
-    for (unsigned k = 0; k <= max_outer; ++k) {
-      // r = b - A x (FP64), rounded to FP32 for the inner solve
-      hipLaunchKernelGGL(k_residual_to_f32, dim3(nbr), dim3(kBlock), 0, W.s, A->nblk,
-                         A->pat.colind_d.p, A->pat.rowptr_d.p, A->vcrs.p, A->vdia.p, b, x,
-                         B.r.p, W.mixed_partials.p, W.ticket.p, W.sq.p);
-      check_launch("k_residual_to_f32");
-      double sq = 0.0;
-      DFM2H_HIP(hipMemcpyAsync(&sq, W.sq.p, sizeof(double), hipMemcpyDeviceToHost, W.s));
-      DFM2H_HIP(hipStreamSynchronize(W.s));
-      h.push_back(std::sqrt(sq));
-      if (k == 0) {
-        sq0 = sq;
-        if (sq0 < 1.0e-30) break;
-      }
-      if (!(sq == sq) || sq > 1.0e308) {
-        nan = true;
-        break;
-      }
-      if (std::sqrt(sq / sq0) < tol || k == max_outer) break;
-      PcgScal st{};
-      run_pcg(A, P, W, B, a, inner_tol, max_inner, st);
-      inner_total += (size_t)st.it;
-      if (st.nan) {
-        nan = true;
-        break;
-      }
-      hipLaunchKernelGGL(k_add_f32_to_f64, dim3(vec_grid(n)), dim3(256), 0, W.s, x, B.x.p, n);
-      check_launch("k_add_f32_to_f64");
-    }
-    DFM2H_HIP(hipEventRecord(W.ev_out, W.s));
-    DFM2H_HIP(hipStreamWaitEvent(stream(), W.ev_out, 0));
+    ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, W.s);
+    DFM2H_HIP(hipMemcpyAsync(W.b64.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
+    PcgArgs<float> a = make_args<float>(A, P, B, A->pk32, P->f32, P->b32);
+    a.x64 = W.x64.p;
+    a.b64 = W.b64.p;
+    a.a64_val = A->pk64.val.p;
+    a.a64_dia = A->pk64.dia.p;
+    a.hist64 = W.hist64.p;
+    PcgScal st{};
+    run_pcg_mixed(P, W, B, a, tol, delta, max_outer, max_inner, st);
+    DFM2H_HIP(hipMemcpyAsync(x, W.x64.p, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
+    std::vector<double> h((size_t)st.nupd + 1);
+    DFM2H_HIP(hipMemcpyAsync(h.data(), W.hist64.p, h.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, W.s));
     DFM2H_HIP(hipStreamSynchronize(W.s));
     if (hist)
       for (size_t k = 0; k < h.size() && k < hist_cap; ++k) hist[k] = h[k];
     if (n_hist) *n_hist = h.size();
-    if (n_inner) *n_inner = inner_total;
-    DFM2H_REQUIRE(!nan, DFM2H_ERR_NAN, "residual became NaN or Inf");
+    if (n_inner) *n_inner = (size_t)st.it;
+    DFM2H_REQUIRE(!st.nan, DFM2H_ERR_NAN, "residual became NaN or Inf");
   });
 }
 

@@ -1,6 +1,6 @@
 // runtime.hip -- device selection, stream, error strings, memory helpers,
 // host pattern build and the tet connectivity handle.
-#include "common.hpp"
+#include "handles.hpp"
 
 #include <algorithm>
 #include <climits>

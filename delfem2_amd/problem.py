@@ -66,9 +66,9 @@ class DeviceSystem:
         hist = S.Solve_PCG(r, x, tol, max_nitr, self.A, self.ilu)
         return x, hist, r
 
-    def solve_mixed(self, tol: float = 1e-8, inner_tol: float = 1e-4, max_outer: int = 50,
-                    max_inner: int = 5000):
+    def solve_mixed(self, tol: float = 1e-8, delta: float = 1e-2, max_outer: int = 200,
+                    max_inner: int = 20000):
         x = torch.zeros_like(self.b)
-        hist, n_inner = S.Solve_PCG_Mixed(self.b, x, tol, self.A, self.ilu, inner_tol,
+        hist, n_inner = S.Solve_PCG_Mixed(self.b, x, tol, self.A, self.ilu, delta,
                                           max_outer, max_inner)
         return x, hist, n_inner

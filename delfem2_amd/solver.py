@@ -288,15 +288,16 @@ def Solve_PCG(r_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float, m
 
 
 def Solve_PCG_Mixed(b_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float,
-                    mat: CMatrixSparse, ilu: CPreconditionerILU, inner_tol: float = 1e-4,
-                    max_outer: int = 50, max_inner: int = 2000):
-    """FP64 iterative refinement around an FP32 ILU(0)-PCG (no reference
-    counterpart).  Returns (outer residual-norm history, total inner iterations)."""
+                    mat: CMatrixSparse, ilu: CPreconditionerILU, delta: float = 1e-2,
+                    max_outer: int = 200, max_inner: int = 20000):
+    """FP32 ILU(0)-PCG inside FP64 refinement by reliable updates (no
+    reference counterpart): refine whenever the FP32 residual fell by `delta`.
+    Returns (FP64 residual-norm history, FP32 iterations)."""
     cap = int(max_outer) + 2
     hist = np.zeros(cap)
     nh = C.c_size_t(0)
     ni = C.c_size_t(0)
     N.call("dfm2h_solve_pcg_mixed", mat.h, ilu.h, _dptr(b_vec), _dptr(x_vec),
-           float(conv_ratio_tol), float(inner_tol), int(max_outer), int(max_inner), _hptr(hist),
+           float(conv_ratio_tol), float(delta), int(max_outer), int(max_inner), _hptr(hist),
            cap, C.byref(nh), C.byref(ni))
     return hist[: nh.value].tolist(), ni.value

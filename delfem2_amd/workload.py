@@ -43,6 +43,16 @@ def color8_permutation(nx: int, ny: int, nz: int):
     return np.argsort(color, kind="stable").astype(np.int64)
 
 
+def color4_permutation(nx: int, ny: int, nz: int):
+    """new->old node map grouping nodes by (ix+iy+iz) mod 4, stable inside a
+    colour.  Every Kuhn-split edge offset has a coordinate sum of 1, 2 or 3
+    (or its negative), so this is a valid 4-colouring: 4 dependency levels
+    per triangular sweep instead of 8."""
+    g = np.indices((nx + 1, ny + 1, nz + 1)).reshape(3, -1)
+    color = (g[0] + g[1] + g[2]) % 4
+    return np.argsort(color, kind="stable").astype(np.int64)
+
+
 def permute_mesh(xyz: np.ndarray, tet: np.ndarray, new2old: np.ndarray):
     """Relabel nodes: node ``k`` of the result is node ``new2old[k]`` of the input."""
     old2new = np.empty_like(new2old)
@@ -129,8 +139,9 @@ class Problem:
 def _order(xyz, tet, nx, ny, nz, ordering):
     if ordering == "natural":
         return xyz, tet, None
-    if ordering == "color8":
-        n2o = color8_permutation(nx, ny, nz)
+    if ordering in ("color8", "color4"):
+        perm = color8_permutation if ordering == "color8" else color4_permutation
+        n2o = perm(nx, ny, nz)
         xyz2, tet2, _ = permute_mesh(xyz, tet, n2o)
         return xyz2, tet2, n2o
     raise ValueError(ordering)

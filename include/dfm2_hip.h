@@ -184,15 +184,20 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r_dev, double *x_dev,
                     size_t hist_cap, size_t *n_hist);
 
 /* Mixed precision (no reference counterpart; the reference has no float
- * ILU or PCG): FP64 iterative refinement around an FP32 ILU(0)-PCG.  The
- * outer residual r = b - A x, the update x += d and the stop test
- * |r|/|b| < tol run in FP64; the inner PCG (FP32 BSR values, FP32 factors
- * rounded from P's FP64 factors) solves A d = r to relative tolerance
- * inner_tol.  b_dev is not modified; x_dev receives the solution.  hist
- * receives the outer residual norms (entry 0 = |b|); n_inner the total
- * number of inner PCG iterations. */
+ * ILU or PCG): an FP32 ILU(0)-PCG (FP32 BSR values, FP32 factors rounded
+ * from P's FP64 factors) inside FP64 iterative refinement by reliable
+ * updates.  Whenever the FP32 residual has fallen by the factor `delta`
+ * (0 < delta < 1) since the last refinement step, or looks converged, the
+ * FP32 correction is folded into the FP64 solution x, the residual
+ * r = b - A x is recomputed in FP64 (FP64 values of A), the stop test
+ * |r|/|b| < tol is made in FP64, and the FP32 iteration continues from the
+ * true residual with its search direction kept (no Krylov restart).
+ * b_dev is not modified; x_dev receives the solution.  hist receives the
+ * FP64 residual norms (entry 0 = |b|, then one per refinement step); n_inner
+ * the number of FP32 iterations.  max_outer caps the refinement steps,
+ * max_inner the FP32 iterations. */
 int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
-                          double *x_dev, double tol, double inner_tol,
+                          double *x_dev, double tol, double delta,
                           unsigned max_outer, unsigned max_inner,
                           double *hist, size_t hist_cap, size_t *n_hist,
                           size_t *n_inner);

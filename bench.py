@@ -35,14 +35,29 @@ def spmv_bytes(nb: int, nnzb: int, s: int, read_y: bool) -> int:
     return b + (ndof * s if read_y else 0)
 
 
-def pcg_iter_bytes(nb: int, nnzb: int, s: int) -> int:
+def kernel_bytes(nb: int, nnzb: int, s: int) -> dict:
+    """Algorithmic bytes of one launch class of the fused PCG iteration (s =
+    bytes per scalar; column indices 4 B; nL = nnzb/2 lower = upper blocks):
+      spmv   off-diagonal blocks + indices, diagonal blocks, row starts,
+             p read, Ap written
+      fwd    lower blocks + indices, D^-1 blocks, 8 B per row of schedule,
+             r read, Pr written          (whole sweep, all levels)
+      bwd    upper blocks + indices, 8 B per row, Pr read and written
+      update r, Ap, x, p read; r, x written
+      pupd   p, Pr read; p written"""
     ndof = 3 * nb
     nl = nnzb // 2
-    spmv = spmv_bytes(nb, nnzb, s, False)
-    fwd = nl * (9 * s + 4) + nb * 9 * s + nb * 8 + 2 * ndof * s
-    bwd = nl * (9 * s + 4) + nb * 8 + 2 * ndof * s
-    vec = 10 * ndof * s
-    return spmv + fwd + bwd + vec
+    return {
+        "spmv": spmv_bytes(nb, nnzb, s, False),
+        "fwd_sweep": nl * (9 * s + 4) + nb * 9 * s + nb * 8 + 2 * ndof * s,
+        "bwd_sweep": nl * (9 * s + 4) + nb * 8 + 2 * ndof * s,
+        "update": 6 * ndof * s,
+        "pupdate": 3 * ndof * s,
+    }
+
+
+def pcg_iter_bytes(nb: int, nnzb: int, s: int) -> int:
+    return sum(kernel_bytes(nb, nnzb, s).values())
 
 
 def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
@@ -93,10 +108,24 @@ def main():
     ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
     ap.add_argument("--delta", type=float, default=1e-2,
                     help="mixed: refine in FP64 whenever the FP32 residual fell by this factor")
-    ap.add_argument("--cpu-iters", type=int, default=60,
+    ap.add_argument("--cpu-iters", type=int, default=200,
                     help="PCG iterations of the bounded CPU sample (0 = skip)")
-    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--spmv-reps", type=int, default=50,
+                    help="launches per kernel class when timing the fused kernels")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MB (the known-size "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+
+    if args.copy_mb:
+        import torch
+        src = torch.ones(args.copy_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 8, "launches": 5}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of "
@@ -158,25 +187,44 @@ def main():
     G.A.MatVec(rr, -1.0, xm, 1.0)
     relres_mixed = (torch.linalg.norm(rr) / torch.linalg.norm(G.b)).item()
 
-    # ---- dominant kernel: 3x3 BSR SpMV, timed with HIP events on its stream ----
-    xv = torch.rand(prob.ndof, dtype=torch.float64, device="cuda")
-    yv = torch.zeros_like(xv)
-    for _ in range(3):
-        G.A.MatVec(yv, 1.0, xv, 0.0)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    stream = torch.cuda.current_stream()
-    e0.record(stream)
-    for _ in range(args.spmv_reps):
-        G.A.MatVec(yv, 1.0, xv, 0.0)
-    e1.record(stream)
-    e1.synchronize()
-    spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
-    b_spmv = spmv_bytes(nb, nnzb, 8, True)
-    spmv_gbs = b_spmv / (spmv_ms * 1e-3) / 1e9
+    # ---- per-kernel times of the fused iterations: HIP events on the solver
+    # stream, right after the timed solves (same buffers, same kernels) ----
+    nlev_f, nlev_b = G.ilu.levels()
+    nlev = {"fwd_sweep": nlev_f, "bwd_sweep": nlev_b}
+    kern = {}
+    for prec, fp32, s in (("fp32", True, 4), ("fp64", False, 8)):
+        if fp32:
+            step_mixed()
+        else:
+            step_fp64()
+        ms = S.pcg_kernel_times(G.A, G.ilu, fp32, args.spmv_reps)
+        kb = kernel_bytes(nb, nnzb, s)
+        kern[prec] = {}
+        for k, b in kb.items():
+            launches = nlev.get(k, 1)
+            kern[prec][k] = {"ms_per_call": round(ms[k], 5), "launches_per_call": launches,
+                             "bytes_per_call": b,
+                             "gbs": round(b / (ms[k] * 1e-3) / 1e9, 1)}
+        for k in ("alpha_step", "conv_step") + (("refine_step",) if fp32 else ()):
+            kern[prec][k + "_ms"] = round(ms[k], 5)
+    # dominant kernel of the headline (FP32 + refinement) solve: largest time per iteration
+    dom = max(("spmv", "fwd_sweep", "bwd_sweep", "update", "pupdate"),
+              key=lambda k: kern["fp32"][k]["ms_per_call"])
+    dk = kern["fp32"][dom]
+    launch_ms = dk["ms_per_call"] / dk["launches_per_call"]
+    launch_bytes = dk["bytes_per_call"] / dk["launches_per_call"]
+    dom_gbs = launch_bytes / (launch_ms * 1e-3) / 1e9
+    kname = {"spmv": "k_pcg_spmv<float>", "fwd_sweep": "k_pcg_fwd<float>",
+             "bwd_sweep": "k_pcg_bwd<float>", "update": "k_pcg_update<float>",
+             "pupdate": "k_pcg_pupdate<float>"}[dom]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if kname in t:
+            traffic = t[kname]
 
     b_iter64 = pcg_iter_bytes(nb, nnzb, 8)
-    b_iter32 = pcg_iter_bytes(nb, nnzb, 4)
     t_it64 = t_fp64 / max(it64, 1)
 
     cpu = None
@@ -210,10 +258,11 @@ def main():
                   "dof_iter_per_s": round(prob.ndof * n_inner / t_mixed, 1),
                   "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
                   "x_rel_err_vs_fp64": err_mixed, "rel_residual": relres_mixed},
-        "roofline": {"bound": "hbm", "kernel": "k_bsr_matvec<double> (3x3 BSR SpMV)",
-                     "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": b_spmv, "avg_launch_ms": round(spmv_ms, 5)},
+        "roofline": {"bound": "hbm", "kernel": kname,
+                     "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_launch": int(launch_bytes), "avg_launch_ms": round(launch_ms, 5)},
+        "kernels": kern,
         "cpu_baseline": cpu,
         "setup_s": round(G.t_setup, 3),
         "assembly_ms": round(t_asm * 1e3, 3),

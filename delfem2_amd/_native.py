@@ -10,7 +10,7 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdfm2_hip.so")
+LIB_PATH = os.environ.get("DFM2H_LIBRARY", os.path.join(HERE, "libdfm2_hip.so"))
 HEADER = os.path.join(os.path.dirname(HERE), "include", "dfm2_hip.h")
 
 OK = 0
@@ -86,6 +86,7 @@ SIGNATURES = {
     "dfm2h_solve_pcg": [_vp, _vp, _vp, _vp, _d, _u, _vp, _sz, _psz],
     "dfm2h_solve_pcg_mixed": [_vp, _vp, _vp, _vp, _d, _d, _u, _u, _vp, _sz, _psz, _psz],
     "dfm2h_adjoint_grad_tet": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "dfm2h_pcg_kernel_times": [_vp, _vp, _i, _u, _vp],
 }
 _RESTYPE = {"dfm2h_last_error": C.c_char_p, "dfm2h_get_stream": _vp}
 

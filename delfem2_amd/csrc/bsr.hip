@@ -284,7 +284,7 @@ int dfm2h_bsr_matvec(dfm2h_bsr A, double *y, double alpha, const double *x, doub
     DFM2H_REQUIRE(y != x, DFM2H_ERR_INVALID, "y and x must not alias");
     if (!A->nblk) return;
     ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, stream());
-    hipLaunchKernelGGL(k_sliced_matvec<double>, dim3(slice_grid(A->plan.nslices)), dim3(kPackBlock),
+    hipLaunchKernelGGL((k_sliced_matvec<double, 3>), dim3(slice_grid(A->plan.nslices)), dim3(kPackBlock),
                        0, stream(), 0u, A->plan.nslices, view(A->plan), A->pk64.val.p,
                        A->pk64.dia.p, y, alpha, x, beta);
     check_launch("k_sliced_matvec");

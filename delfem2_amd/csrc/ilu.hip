@@ -174,7 +174,7 @@ void ilu_forward_levels_double(dfm2h_ilu P, const double *src, double *vec) {
   ensure_packed(P->fplan, P->f64, P->vcrs.p, P->vdia.p, P->version, stream());
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
-    hipLaunchKernelGGL(k_sliced_fwd<double>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
+    hipLaunchKernelGGL((k_sliced_fwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
                        P->fplan.group_ptr[l], nsl, view(P->fplan), P->f64.val.p, P->f64.dia.p,
                        src, vec);
   }
@@ -185,7 +185,7 @@ void ilu_backward_levels_double(dfm2h_ilu P, double *vec) {
   ensure_packed(P->bplan, P->b64, P->vcrs.p, nullptr, P->version, stream());
   for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
-    hipLaunchKernelGGL(k_sliced_bwd<double>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
+    hipLaunchKernelGGL((k_sliced_bwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
                        P->bplan.group_ptr[l], nsl, view(P->bplan), P->b64.val.p, vec);
   }
   check_launch("k_sliced_bwd");

@@ -5,12 +5,16 @@
 // level for a sweep) into slices of 21 rows.  One wavefront owns a slice:
 // lane = c*21 + r handles component c of slice row r (lane 63 idles).  For
 // the k-th block of every row of slice s (block-slot q = off[s] + k):
-//   col [q*21 + r]                column of row r's k-th block
-//   val [q*189 + t*63 + c*21 + r] entry (c, t) of that block
-// so each of the three value loads of a block is one contiguous 504-byte
-// (FP64) run across the wavefront, and the column index is shared by the
-// three lanes of a row.  The row's own 3x3 block (diagonal of A, or D^-1 of
-// the factorisation) is held the same way at dia[s*189 + t*63 + c*21 + r].
+//   col [q*21 + r]                      column of row r's k-th block
+//   val                                 entry (c, t) of that block, for
+//                                       lane = c*21 + r at
+//     FP64: [(q*63 + lane)*3 + t]       (a lane's block row is contiguous)
+//     FP32: [q*189 + t*63 + lane]       (planar)
+// so every value load of a wavefront covers one contiguous run, and the
+// column index is shared by the three lanes of a row.  The row's own 3x3
+// block (diagonal of A, or D^-1 of the factorisation) is held the same way
+// with s in place of q.  Vectors are read with a node stride VS: 3 (the
+// reference's AoS layout) or 4 (padded).
 // Blocks of a row keep their CRS order, so every per-component sum is formed
 // in exactly the order of the reference loops (bit-identical in FP64).
 #pragma once
@@ -20,6 +24,15 @@
 #include "common.hpp"
 
 namespace dfm2h {
+
+// Value layout inside a block-slot, chosen by measurement on MI355X:
+// FP32 planar, val[q*189 + t*63 + lane] (three dword loads, each one
+// contiguous 252-byte run across the wavefront); FP64 per-lane triples,
+// val[(q*63 + lane)*3 + t] (one dwordx4 + dwordx2 per lane).
+template <typename T>
+constexpr bool planar_vals() {
+  return sizeof(T) == 4;
+}
 
 constexpr int kSliceRows = 21;
 constexpr int kSliceVals = 9 * kSliceRows;  // 189
@@ -102,7 +115,8 @@ __global__ void k_pack_blocks(size_t nq, const int *__restrict__ src, const doub
   if (t >= nq * kSliceVals) return;
   const size_t q = t / kSliceVals;
   const unsigned rem = (unsigned)(t % kSliceVals);
-  const unsigned tt = rem / 63, lane = rem % 63;
+  const unsigned lane = planar_vals<T>() ? rem % 63 : rem / 3;
+  const unsigned tt = planar_vals<T>() ? rem / 63 : rem % 3;
   const unsigned c = lane / kSliceRows, r = lane % kSliceRows;
   const int s = src[q * kSliceRows + r];
   val[t] = s >= 0 ? (T)crs[(size_t)s * 9 + c * 3 + tt] : (T)0;
@@ -115,7 +129,8 @@ __global__ void k_pack_dia(unsigned nslices, const unsigned *__restrict__ row,
   if (t >= (size_t)nslices * kSliceVals) return;
   const size_t s = t / kSliceVals;
   const unsigned rem = (unsigned)(t % kSliceVals);
-  const unsigned tt = rem / 63, lane = rem % 63;
+  const unsigned lane = planar_vals<T>() ? rem % 63 : rem / 3;
+  const unsigned tt = planar_vals<T>() ? rem / 63 : rem % 3;
   const unsigned c = lane / kSliceRows, r = lane % kSliceRows;
   const unsigned i = row[s * kSliceRows + r];
   dia[t] = i != ~0u ? (T)dia_crs[(size_t)i * 9 + c * 3 + tt] : (T)0;
@@ -139,43 +154,63 @@ __device__ inline SliceLane slice_lane(unsigned slice0) {
   return L;
 }
 
+template <typename T>
+struct Trip {
+  T a, b, c;
+};
+template <typename T>
+struct alignas(4 * sizeof(T)) Quad {
+  T a, b, c, d;
+};
+
+// the three values of lane `lane`'s block row in block-slot q
+template <typename T>
+__device__ inline Trip<T> load_trip(const T *__restrict__ base, size_t q, unsigned lane) {
+  if (planar_vals<T>()) {
+    const T *p = base + q * kSliceVals + lane;
+    return Trip<T>{p[0], p[63], p[126]};
+  }
+  return *reinterpret_cast<const Trip<T> *>(base + (q * 63 + lane) * 3);
+}
+
+// node j of a vector with node stride VS
+template <typename T, int VS>
+__device__ inline Trip<T> load_node(const T *x, size_t j) {
+  if (VS == 4) {
+    const Quad<T> v = *reinterpret_cast<const Quad<T> *>(x + 4 * j);
+    return Trip<T>{v.a, v.b, v.c};
+  }
+  return Trip<T>{x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+}
+
 // Accumulates this lane's component over the row's blocks in stored order:
 //   SUB:  acc -= (v0*x0 + v1*x1) + v2*x2
 //   else: acc += alpha * ((v0*x0 + v1*x1) + v2*x2)
 // U blocks per round: their column indices and values are loaded first (one
 // memory round trip), then the U gathers of x (a second one), then the U
 // updates in order.
-template <typename T, bool SUB, int U>
+template <typename T, bool SUB, int U, int VS>
 __device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned K, unsigned kmax,
                                    unsigned q0, const unsigned *__restrict__ col,
                                    const T *__restrict__ val, const T *x) {
   for (unsigned k0 = 0; k0 < kmax; k0 += U) {
     unsigned j[U];
-    T v[U][3], xv[U][3];
+    Trip<T> v[U], xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (k0 + u < K) {
         const size_t q = q0 + k0 + u;
         j[u] = col[q * kSliceRows + L.r];
-        const T *vp = val + q * kSliceVals + L.lane;
-        v[u][0] = vp[0];
-        v[u][1] = vp[63];
-        v[u][2] = vp[126];
+        v[u] = load_trip(val, q, L.lane);
       }
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (k0 + u < K) {
-        const T *xp = x + 3 * (size_t)j[u];
-        xv[u][0] = xp[0];
-        xv[u][1] = xp[1];
-        xv[u][2] = xp[2];
-      }
-    }
+    for (int u = 0; u < U; ++u)
+      if (k0 + u < K) xv[u] = load_node<T, VS>(x, j[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (k0 + u < K) {
-        const T d = v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
+        const T d = v[u].a * xv[u].a + v[u].b * xv[u].b + v[u].c * xv[u].c;
         if (SUB)
           acc -= d;
         else
@@ -188,8 +223,8 @@ __device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned 
 
 template <typename T>
 __device__ inline T dia_row_dot(const T *__restrict__ dia, const SliceLane &L, T x0, T x1, T x2) {
-  const T *dp = dia + (size_t)L.s * kSliceVals + L.lane;
-  return dp[0] * x0 + dp[63] * x1 + dp[126] * x2;
+  const Trip<T> d = load_trip(dia, L.s, L.lane);
+  return d.a * x0 + d.b * x1 + d.c * x2;
 }
 
 struct PlanView {
@@ -212,7 +247,7 @@ constexpr int unroll_sweep() {
 }
 
 // y = alpha*A*x + beta*y over the slices [slice0, slice0+nsl) (all rows)
-template <typename T>
+template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
 k_sliced_matvec(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val,
                 const T *__restrict__ dia, T *__restrict__ y, T alpha, const T *__restrict__ x,
@@ -221,35 +256,35 @@ k_sliced_matvec(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__
   if (L.s >= slice0 + nsl || L.c >= 3) return;
   const unsigned i = P.row[L.s * kSliceRows + L.r];
   if (i == ~0u) return;
-  T acc = y[3 * (size_t)i + L.c] * beta;
-  acc = sliced_row_sum<T, false, unroll_spmv<T>()>(acc, alpha, L, P.len[L.s * kSliceRows + L.r],
-                                                 P.kmax[L.s], P.off[L.s], P.col, val, x);
-  const T *xi = x + 3 * (size_t)i;
-  acc += alpha * dia_row_dot(dia, L, xi[0], xi[1], xi[2]);
-  y[3 * (size_t)i + L.c] = acc;
+  T acc = y[VS * (size_t)i + L.c] * beta;
+  acc = sliced_row_sum<T, false, unroll_spmv<T>(), VS>(
+      acc, alpha, L, P.len[L.s * kSliceRows + L.r], P.kmax[L.s], P.off[L.s], P.col, val, x);
+  const Trip<T> xi = load_node<T, VS>(x, i);
+  acc += alpha * dia_row_dot(dia, L, xi.a, xi.b, xi.c);
+  y[VS * (size_t)i + L.c] = acc;
 }
 
 // forward sweep of one level: vec_i = D_i^-1 (src_i - sum L_ij vec_j)
-template <typename T>
+template <typename T, int VS>
 __device__ inline void sliced_fwd_row(const SliceLane &L, bool valid, unsigned i, const PlanView &P,
                                       const T *__restrict__ val, const T *__restrict__ dia,
                                       const T *src, T *vec, T &out) {
   T t = 0;
   if (valid) {
-    t = src[3 * (size_t)i + L.c];
-    t = sliced_row_sum<T, true, unroll_sweep<T>()>(t, (T)1, L, P.len[L.s * kSliceRows + L.r],
-                                                P.kmax[L.s], P.off[L.s], P.col, val, vec);
+    t = src[VS * (size_t)i + L.c];
+    t = sliced_row_sum<T, true, unroll_sweep<T>(), VS>(
+        t, (T)1, L, P.len[L.s * kSliceRows + L.r], P.kmax[L.s], P.off[L.s], P.col, val, vec);
   }
   const T t0 = __shfl(t, (int)L.r, kWave);
   const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
   const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
   if (valid) {
     out = dia_row_dot(dia, L, t0, t1, t2);
-    vec[3 * (size_t)i + L.c] = out;
+    vec[VS * (size_t)i + L.c] = out;
   }
 }
 
-template <typename T>
+template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
 k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val,
              const T *__restrict__ dia, const T *src, T *vec) {
@@ -258,21 +293,21 @@ k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ va
   const unsigned i = L.c < 3 ? P.row[L.s * kSliceRows + L.r] : ~0u;
   const bool valid = i != ~0u;
   T out;
-  sliced_fwd_row(L, valid, i, P, val, dia, src, vec, out);
+  sliced_fwd_row<T, VS>(L, valid, i, P, val, dia, src, vec, out);
 }
 
 // backward sweep of one level: vec_i -= sum U_ij vec_j (U scaled by D^-1)
-template <typename T>
+template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
 k_sliced_bwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val, T *vec) {
   const SliceLane L = slice_lane(slice0);
   if (L.s >= slice0 + nsl || L.c >= 3) return;
   const unsigned i = P.row[L.s * kSliceRows + L.r];
   if (i == ~0u) return;
-  T t = vec[3 * (size_t)i + L.c];
-  t = sliced_row_sum<T, true, unroll_sweep<T>()>(t, (T)1, L, P.len[L.s * kSliceRows + L.r],
-                                              P.kmax[L.s], P.off[L.s], P.col, val, vec);
-  vec[3 * (size_t)i + L.c] = t;
+  T t = vec[VS * (size_t)i + L.c];
+  t = sliced_row_sum<T, true, unroll_sweep<T>(), VS>(
+      t, (T)1, L, P.len[L.s * kSliceRows + L.r], P.kmax[L.s], P.off[L.s], P.col, val, vec);
+  vec[VS * (size_t)i + L.c] = t;
 }
 
 inline unsigned slice_grid(unsigned nsl) { return std::max(1u, ceil_div(nsl, kSlicesPerBlock)); }

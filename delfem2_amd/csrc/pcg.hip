@@ -24,7 +24,7 @@
 namespace dfm2h {
 
 struct PcgScal {
-  double rPr;
+  double rPr;  // (r, M^-1 r)
   double alpha;
   double beta;
   double inv_sq0;
@@ -48,10 +48,15 @@ struct PcgScal {
 
 constexpr int kVecBlock = 256;
 constexpr unsigned kVecGridMax = 2048;
+constexpr int kScalBlock = 1024;  // the one-workgroup scalar steps
+// node stride of the fused solvers' internal vectors: 3, the caller's
+// layout (measured faster on MI355X than a padded stride of 4)
+constexpr int kVS = 3;
 
 static unsigned vec_grid(size_t n) {
   return std::max(1u, std::min(kVecGridMax, ceil_div(n, kVecBlock)));
 }
+
 
 // ---------------------------------------------------------------------------
 // generic deterministic dot (host-facing Dot)
@@ -98,6 +103,24 @@ __global__ void k_convert(TO *__restrict__ y, const TI *__restrict__ x, size_t n
     y[i] = (TO)x[i];
 }
 
+// node stride 3 (caller's vectors) <-> kVS (the fused solvers' vectors)
+template <typename TO, typename TI>
+__global__ void k_pad4(TO *__restrict__ y4, const TI *__restrict__ x3, size_t nnode) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnode * kVS;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t node = i / kVS;
+    const unsigned c = (unsigned)(i % kVS);
+    y4[i] = c < 3 ? (TO)x3[node * 3 + c] : (TO)0;
+  }
+}
+
+template <typename TO, typename TI>
+__global__ void k_unpad4(TO *__restrict__ y3, const TI *__restrict__ x4, size_t nnode) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnode * 3;
+       i += (size_t)gridDim.x * blockDim.x)
+    y3[i] = (TO)x4[(i / 3) * kVS + i % 3];
+}
+
 // ---------------------------------------------------------------------------
 // fused PCG kernels
 
@@ -116,6 +139,7 @@ struct PcgArgs {
   // vectors
   T *r, *x, *Pr, *p;
   double *partials, *hist;
+  double *partials2;  // (r,r) partials of the update kernel
   PcgScal *S;
   // mixed precision: FP64 solution, right-hand side, matrix and outer history
   double *x64;
@@ -152,11 +176,11 @@ enum ScalMode {
 };
 
 template <int MODE>
-__global__ void __launch_bounds__(kVecBlock)
+__global__ void __launch_bounds__(kScalBlock)
 k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double *hist) {
   if (MODE != kScalInit && MODE != kScalInitMixed && S->done) return;
   if (MODE == kScalReliable && !S->upd) return;
-  const double t = reduce_partials<kVecBlock>(partials, n);
+  const double t = reduce_partials<kScalBlock>(partials, n);
   if (threadIdx.x != 0) return;
   if (MODE == kScalInit) {  // |r0|^2: history[0]; early return when < 1e-30
     hist[0] = sqrt(t);
@@ -175,6 +199,11 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double
     S->inv_sq0 = 1.0 / t;
   } else if (MODE == kScalAlpha) {  // alpha = rPr / pAp
     S->alpha = S->rPr / t;
+  } else if (MODE == kScalRPr0) {
+    S->rPr = t;
+  } else if (MODE == kScalBeta) {  // beta = rPr1 / rPr; rPr = rPr1
+    S->beta = t / S->rPr;
+    S->rPr = t;
   } else if (MODE == kScalConv) {  // push |r|, stop on sqrt(|r|^2/|r0|^2) < tol
     const int it = S->it + 1;
     S->it = it;
@@ -189,11 +218,6 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double
     } else if (it >= S->max_nitr) {
       S->done = 1;
     }
-  } else if (MODE == kScalRPr0) {
-    S->rPr = t;
-  } else if (MODE == kScalBeta) {  // beta = rPr1 / rPr; rPr = rPr1
-    S->beta = t / S->rPr;
-    S->rPr = t;
   } else if (MODE == kScalInitMixed) {
     hist[0] = sqrt(t);
     S->sq_b = t;
@@ -251,13 +275,13 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T> a) {
   double part = 0.0;
   if (i != ~0u) {
     T acc = 0;
-    acc = sliced_row_sum<T, false, unroll_spmv<T>()>(acc, (T)1, L, a.a_plan.len[L.s * kSliceRows + L.r],
-                                                   a.a_plan.kmax[L.s], a.a_plan.off[L.s],
-                                                   a.a_plan.col, a.a_val, a.p);
-    const T *pi = a.p + 3 * (size_t)i;
-    acc += dia_row_dot(a.a_dia, L, pi[0], pi[1], pi[2]);
-    a.Pr[3 * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
-    part = (double)pi[L.c] * (double)acc;
+    acc = sliced_row_sum<T, false, unroll_spmv<T>(), kVS>(
+        acc, (T)1, L, a.a_plan.len[L.s * kSliceRows + L.r], a.a_plan.kmax[L.s],
+        a.a_plan.off[L.s], a.a_plan.col, a.a_val, a.p);
+    const Trip<T> pi = load_node<T, kVS>(a.p, i);
+    acc += dia_row_dot(a.a_dia, L, pi.a, pi.b, pi.c);
+    a.Pr[kVS * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
+    part = (double)a.p[kVS * (size_t)i + L.c] * (double)acc;
   }
   part = block_reduce_sum<kPackBlock>(part);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
@@ -276,7 +300,7 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_update(PcgArgs<T> a) {
     s += (double)rn * (double)rn;
   }
   s = block_reduce_sum<kVecBlock>(s);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+  if (threadIdx.x == 0) a.partials2[blockIdx.x] = s;
 }
 
 // forward level: Pr = D^-1 (r - L Pr) on the level's slices (reads r
@@ -289,7 +313,7 @@ k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl) {
   if (L.s >= slice0 + nsl) return;  // whole wavefront
   const unsigned i = L.c < 3 ? a.f_plan.row[L.s * kSliceRows + L.r] : ~0u;
   T out;
-  sliced_fwd_row(L, i != ~0u, i, a.f_plan, a.f_val, a.f_dia, (const T *)a.r, a.Pr, out);
+  sliced_fwd_row<T, kVS>(L, i != ~0u, i, a.f_plan, a.f_val, a.f_dia, (const T *)a.r, a.Pr, out);
 }
 
 // backward level + partial (r, Pr) of the rows it finalises, written at
@@ -303,12 +327,12 @@ k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, unsigned part_off) {
   const unsigned i = slice_ok ? a.b_plan.row[L.s * kSliceRows + L.r] : ~0u;
   double part = 0.0;
   if (i != ~0u) {
-    T t = a.Pr[3 * (size_t)i + L.c];
-    t = sliced_row_sum<T, true, unroll_sweep<T>()>(t, (T)1, L, a.b_plan.len[L.s * kSliceRows + L.r],
-                                                a.b_plan.kmax[L.s], a.b_plan.off[L.s],
-                                                a.b_plan.col, a.b_val, a.Pr);
-    a.Pr[3 * (size_t)i + L.c] = t;
-    part = (double)a.r[3 * (size_t)i + L.c] * (double)t;
+    T t = a.Pr[kVS * (size_t)i + L.c];
+    t = sliced_row_sum<T, true, unroll_sweep<T>(), kVS>(
+        t, (T)1, L, a.b_plan.len[L.s * kSliceRows + L.r], a.b_plan.kmax[L.s],
+        a.b_plan.off[L.s], a.b_plan.col, a.b_val, a.Pr);
+    a.Pr[kVS * (size_t)i + L.c] = t;
+    part = (double)a.r[kVS * (size_t)i + L.c] * (double)t;
   }
   part = block_reduce_sum<kPackBlock>(part);
   if (threadIdx.x == 0) a.partials[part_off + blockIdx.x] = part;
@@ -363,13 +387,13 @@ __global__ void __launch_bounds__(kPackBlock) k_mixed_residual(PcgArgs<float> a)
   double part = 0.0;
   if (i != ~0u) {
     double acc = 0.0;
-    acc = sliced_row_sum<double, false, 4>(acc, 1.0, L, a.a_plan.len[L.s * kSliceRows + L.r],
-                                           a.a_plan.kmax[L.s], a.a_plan.off[L.s], a.a_plan.col,
-                                           a.a64_val, a.x64);
-    const double *xi = a.x64 + 3 * (size_t)i;
-    acc += dia_row_dot(a.a64_dia, L, xi[0], xi[1], xi[2]);
-    const double r = a.b64[3 * (size_t)i + L.c] - acc;
-    a.r[3 * (size_t)i + L.c] = (float)r;
+    acc = sliced_row_sum<double, false, 4, kVS>(acc, 1.0, L, a.a_plan.len[L.s * kSliceRows + L.r],
+                                              a.a_plan.kmax[L.s], a.a_plan.off[L.s],
+                                              a.a_plan.col, a.a64_val, a.x64);
+    const Trip<double> xi = load_node<double, kVS>(a.x64, i);
+    acc += dia_row_dot(a.a64_dia, L, xi.a, xi.b, xi.c);
+    const double r = a.b64[kVS * (size_t)i + L.c] - acc;
+    a.r[kVS * (size_t)i + L.c] = (float)r;
     part = r * r;
   }
   part = block_reduce_sum<kPackBlock>(part);
@@ -382,7 +406,7 @@ __global__ void __launch_bounds__(kPackBlock) k_mixed_residual(PcgArgs<float> a)
 template <typename T>
 struct PcgBufs {
   DBuf<T> r, x, Pr, p;
-  DBuf<double> partials, hist;
+  DBuf<double> partials, partials2, hist;
   DBuf<PcgScal> S;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -435,7 +459,7 @@ static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Packe
                             const PackedVals<T> &fv, const PackedVals<T> &bv) {
   PcgArgs<T> a;
   a.nblk = A->nblk;
-  a.n = (size_t)A->nblk * 3;
+  a.n = (size_t)A->nblk * kVS;
   a.a_plan = view(A->plan);
   a.a_nsl = A->plan.nslices;
   a.a_val = av.val.p;
@@ -450,6 +474,7 @@ static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Packe
   a.Pr = B.Pr.p;
   a.p = B.p.p;
   a.partials = B.partials.p;
+  a.partials2 = B.partials2.p;
   a.hist = B.hist.p;
   a.S = B.S.p;
   return a;
@@ -462,16 +487,25 @@ static unsigned bwd_partial_total(dfm2h_ilu P) {
   return tot;
 }
 
+// Vectors are padded to 4 components per node; the pad entries are zeroed
+// here and never written by the solver kernels.
 template <typename T>
 static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_nitr) {
-  const size_t n = (size_t)A->nblk * 3;
-  B.r.alloc(n);
-  B.x.alloc(n);
-  B.Pr.alloc(n);
-  B.p.alloc(n);
+  const size_t n = (size_t)A->nblk * kVS;
+  if (B.r.n != n) {
+    B.r.alloc(n);
+    B.x.alloc(n);
+    B.Pr.alloc(n);
+    B.p.alloc(n);
+    B.r.zero();
+    B.x.zero();
+    B.Pr.zero();
+    B.p.zero();
+  }
   const size_t np = std::max<size_t>({(size_t)slice_grid(A->plan.nslices), (size_t)vec_grid(n),
                                       (size_t)bwd_partial_total(P), 1});
   if (B.partials.n < np) B.partials.alloc(np);
+  if (B.partials2.n < kVecGridMax) B.partials2.alloc(kVecGridMax);
   if (B.hist.n < (size_t)max_nitr + 2) B.hist.alloc((size_t)max_nitr + 2);
   if (!B.S.p) {
     B.S.alloc(1);
@@ -479,8 +513,9 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
   }
 }
 
+// forward then backward levels; returns the number of (r,Pr) partials
 template <typename T>
-static void launch_sweeps(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s, int mode) {
+static unsigned launch_sweeps(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
     hipLaunchKernelGGL(k_pcg_fwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
@@ -494,25 +529,22 @@ static void launch_sweeps(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s, int m
                        nsl, off);
     off += g;
   }
-  if (mode == 0)
-    hipLaunchKernelGGL(k_pcg_scalar<kScalRPr0>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
-                       off, a.hist);
-  else
-    hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
-                       off, a.hist);
+  return off;
 }
 
 template <typename T>
 static void launch_iteration(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
   const unsigned gs = slice_grid(a.a_nsl);
   hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
                      gs, a.hist);
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(gv), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalConv>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+  hipLaunchKernelGGL(k_pcg_scalar<kScalConv>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials2,
                      gv, a.hist);
-  launch_sweeps(a, P, s, 1);
+  const unsigned nb = launch_sweeps(a, P, s);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
+                     nb, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
@@ -528,9 +560,11 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
   init.max_nitr = (int)max_nitr;
   DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_pcg_init<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalInit>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+  hipLaunchKernelGGL(k_pcg_scalar<kScalInit>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
                      vec_grid(a.n), a.hist);
-  launch_sweeps(a, P, s, 0);
+  const unsigned nb0 = launch_sweeps(a, P, s);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalRPr0>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
+                     nb0, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
 
@@ -538,7 +572,7 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
   const unsigned chunk = std::max(1u, std::min(16u, max_nitr));
   std::vector<uintptr_t> sig = {(uintptr_t)a.a_val, (uintptr_t)a.a_dia, (uintptr_t)a.f_val,
                                 (uintptr_t)a.f_dia, (uintptr_t)a.b_val, (uintptr_t)a.r,
-                                (uintptr_t)a.partials, (uintptr_t)a.hist, (uintptr_t)a.S,
+                                (uintptr_t)a.partials, (uintptr_t)a.partials2, (uintptr_t)a.hist, (uintptr_t)a.S,
                                 (uintptr_t)a.a_plan.col, (uintptr_t)a.f_plan.col,
                                 (uintptr_t)a.b_plan.col, (uintptr_t)chunk,
                                 (uintptr_t)P->fplan.group_ptr.size(),
@@ -582,16 +616,18 @@ static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, hipStre
   const unsigned gs = slice_grid(a.a_nsl);
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kVecBlock), 0, s, a.S, a.partials,
+  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
                      gs, a.hist);
   hipLaunchKernelGGL(k_pcg_update<float>, dim3(gv), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalConvMixed>, dim3(1), dim3(kVecBlock), 0, s, a.S,
-                     a.partials, gv, a.hist);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalConvMixed>, dim3(1), dim3(kScalBlock), 0, s, a.S,
+                     a.partials2, gv, a.hist);
   hipLaunchKernelGGL(k_mixed_fold, dim3(gv), dim3(kVecBlock), 0, s, a);
   hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalReliable>, dim3(1), dim3(kVecBlock), 0, s, a.S,
+  hipLaunchKernelGGL(k_pcg_scalar<kScalReliable>, dim3(1), dim3(kScalBlock), 0, s, a.S,
                      a.partials, gs, a.hist64);
-  launch_sweeps(a, P, s, 1);
+  const unsigned nb = launch_sweeps(a, P, s);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
+                     nb, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
@@ -607,15 +643,17 @@ static void run_pcg_mixed(dfm2h_ilu P, PcgWork &W, PcgBufs<float> &B, const PcgA
   DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL(k_mixed_init, dim3(gv), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalInitMixed>, dim3(1), dim3(kVecBlock), 0, s, a.S,
+  hipLaunchKernelGGL(k_pcg_scalar<kScalInitMixed>, dim3(1), dim3(kScalBlock), 0, s, a.S,
                      a.partials, gv, a.hist64);
-  launch_sweeps(a, P, s, 0);
-  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  const unsigned nb0 = launch_sweeps(a, P, s);
+  hipLaunchKernelGGL(k_pcg_scalar<kScalRPr0>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
+                     nb0, a.hist);
+  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("mixed prologue");
   const unsigned chunk = std::max(1u, std::min(16u, max_inner));
   std::vector<uintptr_t> sig = {(uintptr_t)a.a_val,  (uintptr_t)a.a_dia,   (uintptr_t)a.f_val,
                                 (uintptr_t)a.f_dia,  (uintptr_t)a.b_val,   (uintptr_t)a.r,
-                                (uintptr_t)a.partials, (uintptr_t)a.hist64, (uintptr_t)a.S,
+                                (uintptr_t)a.partials, (uintptr_t)a.partials2, (uintptr_t)a.hist64, (uintptr_t)a.S,
                                 (uintptr_t)a.a64_val, (uintptr_t)a.a64_dia, (uintptr_t)a.x64,
                                 (uintptr_t)a.b64,    (uintptr_t)a.a_plan.col,
                                 (uintptr_t)a.f_plan.col, (uintptr_t)a.b_plan.col,
@@ -743,15 +781,19 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
     PcgWork &W = work_of(P);
     PcgBufs<double> &B = W.f64;
     ensure_bufs(B, A, P, max_nitr);
-    const size_t n = (size_t)A->nblk * 3;
+    const size_t nn = A->nblk;
     fence_in(W);
-    DFM2H_HIP(hipMemcpyAsync(B.r.p, r, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
+    hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s, B.r.p,
+                       r, nn);
     ensure_f64(A, P, W.s);
     const PcgArgs<double> a = make_args<double>(A, P, B, A->pk64, P->f64, P->b64);
     PcgScal st{};
     run_pcg(A, P, W, B, a, tol, max_nitr, st);
-    DFM2H_HIP(hipMemcpyAsync(r, B.r.p, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
-    DFM2H_HIP(hipMemcpyAsync(x, B.x.p, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
+    hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, r,
+                       B.r.p, nn);
+    hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x,
+                       B.x.p, nn);
+    check_launch("k_unpad4");
     // history: it+1 norms, plus the reference's final push when not converged
     std::vector<double> h((size_t)st.it + 1);
     DFM2H_HIP(hipMemcpyAsync(h.data(), B.hist.p, h.size() * sizeof(double),
@@ -776,14 +818,19 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
     PcgWork &W = work_of(P);
     PcgBufs<float> &B = W.f32;
     ensure_bufs(B, A, P, max_inner);
-    const size_t n = (size_t)A->nblk * 3;
-    W.b64.alloc(n);
-    W.x64.alloc(n);
+    const size_t nn = A->nblk;
+    if (W.b64.n != kVS * nn) {
+      W.b64.alloc(kVS * nn);
+      W.x64.alloc(kVS * nn);
+      W.b64.zero();
+      W.x64.zero();
+    }
     if (W.hist64.n < (size_t)max_outer + 2) W.hist64.alloc((size_t)max_outer + 2);
     fence_in(W);
     ensure_f32(A, P, W.s);
     ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, W.s);
-    DFM2H_HIP(hipMemcpyAsync(W.b64.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
+    hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s,
+                       W.b64.p, b, nn);
     PcgArgs<float> a = make_args<float>(A, P, B, A->pk32, P->f32, P->b32);
     a.x64 = W.x64.p;
     a.b64 = W.b64.p;
@@ -792,7 +839,9 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
     a.hist64 = W.hist64.p;
     PcgScal st{};
     run_pcg_mixed(P, W, B, a, tol, delta, max_outer, max_inner, st);
-    DFM2H_HIP(hipMemcpyAsync(x, W.x64.p, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
+    hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x,
+                       W.x64.p, nn);
+    check_launch("k_unpad4");
     std::vector<double> h((size_t)st.nupd + 1);
     DFM2H_HIP(hipMemcpyAsync(h.data(), W.hist64.p, h.size() * sizeof(double),
                              hipMemcpyDeviceToHost, W.s));
@@ -802,6 +851,117 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
     if (n_hist) *n_hist = h.size();
     if (n_inner) *n_inner = (size_t)st.it;
     DFM2H_REQUIRE(!st.nan, DFM2H_ERR_NAN, "residual became NaN or Inf");
+  });
+}
+
+int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, double *ms) {
+  return guard(__func__, [&] {
+    need_solver(A, P);
+    DFM2H_REQUIRE(ms && nrep > 0, DFM2H_ERR_INVALID, "bad argument");
+    PcgWork &W = work_of(P);
+    // Runs nrep iterations with direct launches and an event between
+    // consecutive kernel classes, so every class is timed inside the real
+    // dependency chain (as in the captured graph).  Convergence and
+    // refinement are disabled for the duration.
+    auto run = [&](auto tag, auto &B, const auto &a) {
+      using T = decltype(tag);
+      PcgScal st{};
+      DFM2H_HIP(hipMemcpyAsync(&st, B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, W.s));
+      DFM2H_HIP(hipStreamSynchronize(W.s));
+      PcgScal tst = st;
+      tst.done = 0;
+      tst.upd = 0;
+      tst.tol = 0.0;
+      tst.delta2 = 0.0;
+      tst.max_nitr = 1 << 30;
+      DFM2H_HIP(hipMemcpyAsync(B.S.p, &tst, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
+      const unsigned gs = slice_grid(a.a_nsl), gv = vec_grid(a.n);
+      constexpr int kSeg = 8;  // spmv, alpha, update, conv, refine, fwd, bwd, beta+pupdate
+      hipEvent_t ev[kSeg + 1];
+      for (auto &e : ev) DFM2H_HIP(hipEventCreate(&e));
+      double acc[kSeg] = {0};
+      for (unsigned rep = 0; rep <= nrep; ++rep) {
+        DFM2H_HIP(hipEventRecord(ev[0], W.s));
+        hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, W.s, a);
+        DFM2H_HIP(hipEventRecord(ev[1], W.s));
+        hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kScalBlock), 0, W.s, a.S,
+                           a.partials, gs, a.hist);
+        DFM2H_HIP(hipEventRecord(ev[2], W.s));
+        hipLaunchKernelGGL(k_pcg_update<T>, dim3(gv), dim3(kVecBlock), 0, W.s, a);
+        DFM2H_HIP(hipEventRecord(ev[3], W.s));
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL(k_pcg_scalar<kScalConvMixed>, dim3(1), dim3(kScalBlock), 0, W.s,
+                             a.S, a.partials2, gv, a.hist);
+        else
+          hipLaunchKernelGGL(k_pcg_scalar<kScalConv>, dim3(1), dim3(kScalBlock), 0, W.s, a.S,
+                             a.partials2, gv, a.hist);
+        DFM2H_HIP(hipEventRecord(ev[4], W.s));
+        if constexpr (sizeof(T) == 4) {  // the refinement kernels, no-ops unless a refinement is due
+          const PcgArgs<float> &af = a;
+          hipLaunchKernelGGL(k_mixed_fold, dim3(gv), dim3(kVecBlock), 0, W.s, af);
+          hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, W.s, af);
+          hipLaunchKernelGGL(k_pcg_scalar<kScalReliable>, dim3(1), dim3(kScalBlock), 0, W.s,
+                             a.S, a.partials, gs, a.hist64);
+        }
+        DFM2H_HIP(hipEventRecord(ev[5], W.s));
+        for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
+          const unsigned nsl = P->fplan.group_slices(l);
+          hipLaunchKernelGGL(k_pcg_fwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, W.s, a,
+                             P->fplan.group_ptr[l], nsl);
+        }
+        DFM2H_HIP(hipEventRecord(ev[6], W.s));
+        unsigned off = 0;
+        for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
+          const unsigned nsl = P->bplan.group_slices(l);
+          const unsigned g = slice_grid(nsl);
+          hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(g), dim3(kPackBlock), 0, W.s, a,
+                             P->bplan.group_ptr[l], nsl, off);
+          off += g;
+        }
+        DFM2H_HIP(hipEventRecord(ev[7], W.s));
+        hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kScalBlock), 0, W.s, a.S,
+                           a.partials, off, a.hist);
+        hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, W.s, a, 1);
+        DFM2H_HIP(hipEventRecord(ev[8], W.s));
+        DFM2H_HIP(hipEventSynchronize(ev[8]));
+        if (rep == 0) continue;  // warm-up iteration
+        for (int k = 0; k < kSeg; ++k) {
+          float t = 0.f;
+          DFM2H_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+          acc[k] += t;
+        }
+      }
+      check_launch("kernel timing");
+      for (auto &e : ev) (void)hipEventDestroy(e);
+      // ms[] = spmv, update, fwd sweep, bwd sweep, pupdate (incl. its beta
+      // step), scalar step (alpha), refinement kernels (FP32 only)
+      ms[0] = acc[0] / nrep;
+      ms[1] = acc[2] / nrep;
+      ms[2] = acc[5] / nrep;
+      ms[3] = acc[6] / nrep;
+      ms[4] = acc[7] / nrep;
+      ms[5] = acc[1] / nrep;
+      ms[6] = acc[4] / nrep;
+      ms[7] = acc[3] / nrep;
+      st.done = 1;
+      DFM2H_HIP(hipMemcpyAsync(B.S.p, &st, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
+      DFM2H_HIP(hipStreamSynchronize(W.s));
+    };
+    if (fp32) {
+      DFM2H_REQUIRE(W.f32.S.p && A->pk32.val.p && W.x64.p, DFM2H_ERR_STATE,
+                    "run a mixed solve first");
+      PcgArgs<float> a = make_args<float>(A, P, W.f32, A->pk32, P->f32, P->b32);
+      a.x64 = W.x64.p;
+      a.b64 = W.b64.p;
+      a.a64_val = A->pk64.val.p;
+      a.a64_dia = A->pk64.dia.p;
+      a.hist64 = W.hist64.p;
+      run(float{}, W.f32, a);
+    } else {
+      DFM2H_REQUIRE(W.f64.S.p && A->pk64.val.p, DFM2H_ERR_STATE, "run an FP64 solve first");
+      PcgArgs<double> a = make_args<double>(A, P, W.f64, A->pk64, P->f64, P->b64);
+      run(double{}, W.f64, a);
+    }
   });
 }
 

@@ -287,6 +287,16 @@ def Solve_PCG(r_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float, m
     return hist[: nh.value].tolist()
 
 
+def pcg_kernel_times(mat: CMatrixSparse, ilu: CPreconditionerILU, fp32: bool,
+                     nrep: int = 20) -> dict:
+    """Average ms per launch of each fused-iteration kernel class (HIP events
+    on the solver stream), after a solve of that precision."""
+    ms = np.zeros(8)
+    N.call("dfm2h_pcg_kernel_times", mat.h, ilu.h, int(bool(fp32)), int(nrep), _hptr(ms))
+    return dict(zip(["spmv", "update", "fwd_sweep", "bwd_sweep", "pupdate", "alpha_step",
+                     "refine_step", "conv_step"], ms.tolist()))
+
+
 def Solve_PCG_Mixed(b_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float,
                     mat: CMatrixSparse, ilu: CPreconditionerILU, delta: float = 1e-2,
                     max_outer: int = 200, max_inner: int = 20000):

@@ -202,6 +202,18 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
                           double *hist, size_t hist_cap, size_t *n_hist,
                           size_t *n_inner);
 
+/* Measurement (no reference counterpart): average time in ms of each kernel
+ * class inside the fused iteration, from HIP events recorded between
+ * consecutive kernels on the solver's own stream over nrep iterations run
+ * right after a solve of the same precision (fp32 = 1: the mixed solver's
+ * FP32 iteration); convergence and refinement are disabled meanwhile.
+ * ms[8] = { SpMV, r/x update, forward sweep (all levels), backward sweep
+ * (all levels), beta step + p update, alpha step, refinement kernels (FP32;
+ * no-ops when no refinement is due), convergence step }.  The kernels run on
+ * the solver's scratch vectors; nothing the caller owns is touched. */
+int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep,
+                           double *ms);
+
 /* ---- adjoint (no reference counterpart) --------------------------------- */
 /* Material sensitivity of an objective J(u) with K(E) u = f and
  * K(E) = sum_e E_e K_e^1, K_e^1 the element stiffness at the Lame pair

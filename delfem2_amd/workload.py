@@ -197,7 +197,7 @@ def cantilever(nx: int = 96, ny: int = 48, nz: int = 48, seed: int = 20261204,
         f = np.ascontiguousarray(f.reshape(-1, 3)[n2o].reshape(-1))
     p = Problem(f"cantilever{nx}x{ny}x{nz}", xyz, tet, myu, lam, flag,
                 g=np.zeros(3), f_ext=f, grid=(nx, ny, nz), ordering=ordering)
-    p.extra.update(density=dens, E=E, new2old=n2o, seed=seed)
+    p.extra.update(density=dens, E=E, nu=np.full(ne, 0.3), new2old=n2o, seed=seed)
     return p
 
 

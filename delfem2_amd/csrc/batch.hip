@@ -1,0 +1,1018 @@
+// batch.hip -- many independent tet systems solved together.
+//
+// The meshes of a batch are the diagonal blocks of one union system
+// (A = diag(A_0 .. A_{n-1}), ILU(0) factors likewise), assembled and
+// factorised with the ordinary entry points.  The solver runs one
+// preconditioned CG per mesh, all in the same kernels:
+//   * the packed plans are rebuilt with every slice inside one mesh: the
+//     matrix plan groups rows by mesh, the sweep plans by (level, mesh), so
+//     one sweep launch per dependency level serves every mesh;
+//   * every reduction writes one partial per slice; a scalar kernel runs one
+//     wavefront per mesh over that mesh's slices (a segmented, fixed-order
+//     and therefore deterministic sum) and keeps alpha, beta, (r, Pr), the
+//     iteration count and the convergence state per mesh;
+//   * a wavefront whose mesh has converged returns at once, so a mesh stops
+//     costing bandwidth the iteration after it converges;
+//   * iterations are captured in a HIP graph (16 per graph); the host polls
+//     the number of finished meshes one graph behind.
+// FP64 and FP32-inner / FP64-reliable-update forms are both provided.  The
+// launch shape (wavefronts per workgroup, slices per wavefront, blocks per
+// load round) is chosen by a built-in autotuner per (DOF bucket,
+// precision), cached in-process and optionally in a text file.
+#include "handles.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <mutex>
+
+namespace dfm2h {
+namespace {
+
+constexpr int kVSB = 3;            // node stride of the batch vectors (caller layout)
+constexpr int kMeshPerBlock = 4;   // scalar kernels: one wavefront per mesh
+constexpr unsigned kChunk = 16;    // iterations per captured graph
+
+struct MeshScal {
+  double rPr, alpha, beta, inv_sq0, sq_b, rmax2, res;
+  int it, done, converged, nan, upd, nupd, final_, pad;
+};
+
+struct BatchGlobal {
+  double tol, delta2;
+  int max_nitr, max_upd;
+  unsigned ndone, pad;
+};
+
+struct BPlan {
+  const unsigned *off, *kmax, *row, *len, *col, *mesh;
+};
+
+template <typename T>
+struct BArgs {
+  unsigned nmesh, nlev_b;
+  BPlan A, F, B;
+  const T *a_val, *a_dia, *f_val, *f_dia, *b_val;
+  const double *a64_val, *a64_dia;
+  T *r, *x, *Pr, *p;
+  double *x64;
+  const double *b64;
+  double *partA, *partA2, *partB;
+  MeshScal *M;
+  BatchGlobal *G;
+  const unsigned *a_mptr, *b_gptr;
+};
+
+// Launch-shape variants the autotuner chooses from:
+//   WPB  wavefronts (slices in flight) per workgroup
+//   SPW  slices handled by one wavefront, one after the other
+//   US   blocks per load round in the SpMV / residual (FP32, FP64)
+//   UW   blocks per load round in the triangular sweeps
+template <int WPB_, int SPW_, int US32_, int US64_, int UW_>
+struct Var {
+  static constexpr int WPB = WPB_, SPW = SPW_, UW = UW_;
+  template <typename T>
+  static constexpr int US() {
+    return sizeof(T) == 8 ? US64_ : US32_;
+  }
+};
+using V0 = Var<4, 1, 8, 4, 16>;
+using V1 = Var<4, 1, 4, 2, 8>;
+using V2 = Var<2, 1, 8, 4, 16>;
+using V3 = Var<8, 1, 8, 4, 16>;
+using V4 = Var<4, 2, 8, 4, 16>;
+using V5 = Var<1, 1, 4, 4, 8>;
+constexpr int kNumVariants = 6;
+const char *const kVariantNames[kNumVariants] = {
+    "wpb4_spw1_u8/4_w16", "wpb4_spw1_u4/2_w8", "wpb2_spw1_u8/4_w16",
+    "wpb8_spw1_u8/4_w16", "wpb4_spw2_u8/4_w16", "wpb1_spw1_u4/4_w8"};
+
+template <class V>
+__device__ inline SliceLane lane_of(unsigned s) {
+  SliceLane L;
+  L.lane = threadIdx.x & (kWave - 1);
+  L.s = s;
+  L.c = L.lane / kSliceRows;
+  L.r = L.lane - L.c * kSliceRows;
+  return L;
+}
+
+// slices [slice0, slice0 + nsl): wavefront w of the grid takes SPW
+// consecutive ones
+template <class V>
+__device__ inline unsigned wave_first(unsigned slice0) {
+  return slice0 + (blockIdx.x * V::WPB + threadIdx.x / kWave) * V::SPW;
+}
+
+template <class V>
+inline unsigned grid_of(unsigned nsl) {
+  return std::max(1u, ceil_div(nsl, (size_t)V::WPB * V::SPW));
+}
+
+// ---------------------------------------------------------------------------
+// slice kernels (every one walks the slices of one plan; a wavefront is
+// uniform in slice, hence in mesh)
+
+// r = b, x = 0 (and x64 = 0), partial |b|^2
+template <typename T, class V>
+__global__ void __launch_bounds__(V::WPB * kWave) kb_init(BArgs<T> a, unsigned nsl) {
+  const unsigned s0 = wave_first<V>(0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= nsl) return;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    double part = 0.0;
+    if (i != ~0u) {
+      const size_t e = kVSB * (size_t)i + L.c;
+      const double v = a.b64[e];
+      part = v * v;
+      a.r[e] = (T)v;
+      a.x[e] = (T)0;
+      if (sizeof(T) == 4) a.x64[e] = 0.0;
+    }
+    part = wave_reduce_sum(part);
+    if (L.lane == 0) a.partA[s] = part;
+  }
+}
+
+// Ap -> Pr, partial (p, Ap)
+template <typename T, class V>
+__global__ void __launch_bounds__(V::WPB * kWave) kb_spmv(BArgs<T> a, unsigned nsl) {
+  const unsigned s0 = wave_first<V>(0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= nsl) return;
+    if (a.M[a.A.mesh[s]].done) continue;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    double part = 0.0;
+    if (i != ~0u) {
+      T acc = sliced_row_sum<T, false, V::template US<T>(), kVSB>(
+          (T)0, (T)1, L, a.A.len[s * kSliceRows + L.r], a.A.kmax[s], a.A.off[s], a.A.col,
+          a.a_val, a.p);
+      const Trip<T> pi = load_node<T, kVSB>(a.p, i);
+      acc += dia_row_dot(a.a_dia, L, pi.a, pi.b, pi.c);
+      const size_t e = kVSB * (size_t)i + L.c;
+      a.Pr[e] = acc;
+      part = (double)a.p[e] * (double)acc;
+    }
+    part = wave_reduce_sum(part);
+    if (L.lane == 0) a.partA[s] = part;
+  }
+}
+
+// r -= alpha Ap, x += alpha p, partial |r|^2
+template <typename T, class V>
+__global__ void __launch_bounds__(V::WPB * kWave) kb_update(BArgs<T> a, unsigned nsl) {
+  const unsigned s0 = wave_first<V>(0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= nsl) return;
+    const MeshScal *S = a.M + a.A.mesh[s];
+    if (S->done) continue;
+    const double alpha = S->alpha;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    double part = 0.0;
+    if (i != ~0u) {
+      const size_t e = kVSB * (size_t)i + L.c;
+      const T rn = (T)((double)a.r[e] + (-alpha) * (double)a.Pr[e]);
+      a.r[e] = rn;
+      a.x[e] = (T)((double)a.x[e] + alpha * (double)a.p[e]);
+      part = (double)rn * (double)rn;
+    }
+    part = wave_reduce_sum(part);
+    if (L.lane == 0) a.partA2[s] = part;
+  }
+}
+
+// forward level: Pr = D^-1 (r - L Pr)
+template <typename T, class V>
+__global__ void __launch_bounds__(V::WPB * kWave)
+kb_fwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
+  const unsigned s0 = wave_first<V>(slice0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= slice0 + nsl) return;
+    if (a.M[a.F.mesh[s]].done) continue;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.F.row[s * kSliceRows + L.r] : ~0u;
+    const bool valid = i != ~0u;
+    T t = 0;
+    if (valid) {
+      t = a.r[kVSB * (size_t)i + L.c];
+      t = sliced_row_sum<T, true, V::UW, kVSB>(t, (T)1, L, a.F.len[s * kSliceRows + L.r],
+                                               a.F.kmax[s], a.F.off[s], a.F.col, a.f_val, a.Pr);
+    }
+    const T t0 = __shfl(t, (int)L.r, kWave);
+    const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
+    const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
+    if (valid) a.Pr[kVSB * (size_t)i + L.c] = dia_row_dot(a.f_dia, L, t0, t1, t2);
+  }
+}
+
+// backward level: Pr -= U Pr, partial (r, Pr) of the rows it finalises
+template <typename T, class V>
+__global__ void __launch_bounds__(V::WPB * kWave)
+kb_bwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
+  const unsigned s0 = wave_first<V>(slice0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= slice0 + nsl) return;
+    if (a.M[a.B.mesh[s]].done) continue;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.B.row[s * kSliceRows + L.r] : ~0u;
+    double part = 0.0;
+    if (i != ~0u) {
+      const size_t e = kVSB * (size_t)i + L.c;
+      T t = a.Pr[e];
+      t = sliced_row_sum<T, true, V::UW, kVSB>(t, (T)1, L, a.B.len[s * kSliceRows + L.r],
+                                               a.B.kmax[s], a.B.off[s], a.B.col, a.b_val, a.Pr);
+      a.Pr[e] = t;
+      part = (double)a.r[e] * (double)t;
+    }
+    part = wave_reduce_sum(part);
+    if (L.lane == 0) a.partB[s] = part;
+  }
+}
+
+// p = beta p + Pr (mode 1) or p = Pr (mode 0)
+template <typename T, class V>
+__global__ void __launch_bounds__(V::WPB * kWave)
+kb_pupdate(BArgs<T> a, unsigned nsl, int mode) {
+  const unsigned s0 = wave_first<V>(0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= nsl) return;
+    const MeshScal *S = a.M + a.A.mesh[s];
+    if (S->done) continue;
+    const double beta = S->beta;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    if (i == ~0u) continue;
+    const size_t e = kVSB * (size_t)i + L.c;
+    a.p[e] = mode == 0 ? a.Pr[e] : (T)(beta * (double)a.p[e] + (double)a.Pr[e]);
+  }
+}
+
+// reliable update, part 1 (meshes due one): x64 += x32, x32 = 0
+template <class V>
+__global__ void __launch_bounds__(V::WPB * kWave) kb_fold(BArgs<float> a, unsigned nsl) {
+  const unsigned s0 = wave_first<V>(0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= nsl) return;
+    const MeshScal *S = a.M + a.A.mesh[s];
+    if (S->done || !S->upd) continue;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    if (i == ~0u) continue;
+    const size_t e = kVSB * (size_t)i + L.c;
+    a.x64[e] += (double)a.x[e];
+    a.x[e] = 0.0f;
+  }
+}
+
+// reliable update, part 2: r = b - A x64 in FP64, rounded into r32, partial |r|^2
+template <class V>
+__global__ void __launch_bounds__(V::WPB * kWave) kb_residual(BArgs<float> a, unsigned nsl) {
+  const unsigned s0 = wave_first<V>(0);
+  for (int k = 0; k < V::SPW; ++k) {
+    const unsigned s = s0 + k;
+    if (s >= nsl) return;
+    const MeshScal *S = a.M + a.A.mesh[s];
+    if (S->done || !S->upd) continue;
+    const SliceLane L = lane_of<V>(s);
+    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    double part = 0.0;
+    if (i != ~0u) {
+      double acc = sliced_row_sum<double, false, V::template US<double>(), kVSB>(
+          0.0, 1.0, L, a.A.len[s * kSliceRows + L.r], a.A.kmax[s], a.A.off[s], a.A.col,
+          a.a64_val, a.x64);
+      const Trip<double> xi = load_node<double, kVSB>(a.x64, i);
+      acc += dia_row_dot(a.a64_dia, L, xi.a, xi.b, xi.c);
+      const size_t e = kVSB * (size_t)i + L.c;
+      const double r = a.b64[e] - acc;
+      a.r[e] = (float)r;
+      part = r * r;
+    }
+    part = wave_reduce_sum(part);
+    if (L.lane == 0) a.partA[s] = part;
+  }
+}
+
+__global__ void kb_fill(double *y, double v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// per-mesh scalar steps: one wavefront per mesh sums that mesh's slice
+// partials (lane-strided, then a fixed butterfly) and updates its state
+
+enum BMode {
+  kBInit = 0,         // |r0|^2 (FP64 form)
+  kBAlpha = 1,        // alpha = rPr / pAp
+  kBConv = 2,         // FP64 |r|^2: stop test
+  kBRPr0 = 3,         // first (r, Pr)
+  kBBeta = 4,         // beta = rPr1 / rPr
+  kBInitMixed = 5,    // |b|^2
+  kBConvMixed = 6,    // FP32 |r|^2: decide a reliable update
+  kBReliable = 7      // FP64 |b - A x|^2: stop test
+};
+
+__device__ inline void mark_done(MeshScal *S, BatchGlobal *G) {
+  S->done = 1;
+  atomicAdd(&G->ndone, 1u);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> a) {
+  // (the argument block is precision-independent here; the caller passes a
+  // float-typed view of the same control pointers)
+  const unsigned m = blockIdx.x * kMeshPerBlock + threadIdx.x / kWave;
+  if (m >= a.nmesh) return;
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  MeshScal *S = a.M + m;
+  constexpr bool kInit = MODE == kBInit || MODE == kBInitMixed;
+  if (!kInit && S->done) return;
+  if (MODE == kBReliable && !S->upd) return;
+  double t = 0.0;
+  if (MODE == kBRPr0 || MODE == kBBeta) {
+    for (unsigned l = 0; l < a.nlev_b; ++l) {
+      const unsigned g = l * a.nmesh + m;
+      for (unsigned q = a.b_gptr[g] + lane; q < a.b_gptr[g + 1]; q += kWave) t += a.partB[q];
+    }
+  } else {
+    const double *src = (MODE == kBConv || MODE == kBConvMixed) ? a.partA2 : a.partA;
+    for (unsigned q = a.a_mptr[m] + lane; q < a.a_mptr[m + 1]; q += kWave) t += src[q];
+  }
+  t = wave_reduce_sum(t);
+  if (lane != 0) return;
+  BatchGlobal *G = a.G;
+  const bool bad = !(t == t) || t > 1.0e308;
+  if (MODE == kBInit || MODE == kBInitMixed) {
+    S->it = 0;
+    S->nupd = 0;
+    S->upd = 0;
+    S->final_ = 0;
+    S->converged = 0;
+    S->nan = 0;
+    S->done = 0;
+    S->sq_b = t;
+    S->rmax2 = t;
+    S->inv_sq0 = 1.0 / t;
+    S->res = 1.0;
+    if (t < 1.0e-30) {
+      S->converged = 1;
+      S->res = 0.0;
+      mark_done(S, G);
+    } else if (bad) {
+      S->nan = 1;
+      mark_done(S, G);
+    }
+  } else if (MODE == kBAlpha) {
+    S->alpha = S->rPr / t;
+  } else if (MODE == kBRPr0) {
+    S->rPr = t;
+  } else if (MODE == kBBeta) {
+    S->beta = t / S->rPr;
+    S->rPr = t;
+  } else if (MODE == kBConv) {
+    const int it = S->it + 1;
+    S->it = it;
+    S->res = sqrt(t * S->inv_sq0);
+    if (S->res < G->tol) {
+      S->converged = 1;
+      mark_done(S, G);
+    } else if (bad) {
+      S->nan = 1;
+      mark_done(S, G);
+    } else if (it >= G->max_nitr) {
+      mark_done(S, G);
+    }
+  } else if (MODE == kBConvMixed) {
+    const int it = S->it + 1;
+    S->it = it;
+    if (bad) {
+      S->nan = 1;
+      mark_done(S, G);
+    } else if (t < G->delta2 * S->rmax2 || sqrt(t / S->sq_b) < G->tol || it >= G->max_nitr) {
+      S->upd = 1;
+      S->final_ = it >= G->max_nitr ? 1 : 0;
+    }
+  } else {  // kBReliable
+    const int k = S->nupd + 1;
+    S->nupd = k;
+    S->upd = 0;
+    S->rmax2 = t;
+    S->res = sqrt(t / S->sq_b);
+    if (S->res < G->tol) {
+      S->converged = 1;
+      mark_done(S, G);
+    } else if (bad) {
+      S->nan = 1;
+      mark_done(S, G);
+    } else if (S->final_ || k >= G->max_upd) {
+      mark_done(S, G);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dfm2h
+
+using namespace dfm2h;
+
+// ---------------------------------------------------------------------------
+// host side
+
+template <typename T>
+struct BatchBufs {
+  DBuf<T> r, x, Pr, p;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> sig;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    sig.clear();
+  }
+  ~BatchBufs() { drop_graph(); }
+};
+
+struct dfm2h_batch_s {
+  unsigned nmesh = 0, nblk = 0;
+  dfm2h_bsr A = nullptr;
+  dfm2h_ilu P = nullptr;
+  uint64_t a_serial = 0, p_serial = 0;  // pattern serials the plans were built from
+  bool has_b = false;
+  std::vector<unsigned> node_offset;
+  PackedPlan aplan, fplan, bplan;
+  DBuf<unsigned> a_mesh, f_mesh, b_mesh, a_mptr, b_gptr;
+  unsigned nlev_f = 0, nlev_b = 0;
+  PackedVals<double> a64, f64, b64v;
+  PackedVals<float> a32, f32, b32;
+  BatchBufs<double> s64;
+  BatchBufs<float> s32;
+  DBuf<double> x64, b64, partA, partA2, partB;
+  DBuf<MeshScal> M;
+  DBuf<BatchGlobal> G;
+  int variant[2] = {-1, -1};  // [fp32] chosen launch shape, -1 = autotune
+  hipStream_t s = nullptr;
+  hipEvent_t ev_in = nullptr, ev_poll[2] = {nullptr, nullptr}, ev_t0 = nullptr, ev_t1 = nullptr;
+  BatchGlobal *host_g = nullptr;  // pinned, 2 entries
+  ~dfm2h_batch_s() {
+    s64.drop_graph();
+    s32.drop_graph();
+    if (host_g) (void)hipHostFree(host_g);
+    for (hipEvent_t e : {ev_in, ev_poll[0], ev_poll[1], ev_t0, ev_t1})
+      if (e) (void)hipEventDestroy(e);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace dfm2h {
+namespace {
+
+// groups of rows -> per-slice mesh ids
+std::vector<unsigned> slice_mesh(const PackedPlan &P, unsigned nmesh, bool level_major) {
+  std::vector<unsigned> out(P.nslices);
+  const size_t ng = P.group_ptr.size() - 1;
+  for (size_t g = 0; g < ng; ++g)
+    for (unsigned s = P.group_ptr[g]; s < P.group_ptr[g + 1]; ++s)
+      out[s] = level_major ? (unsigned)(g % nmesh) : (unsigned)g;
+  return out;
+}
+
+// rows of each level split by mesh: groups[l * nmesh + m]
+std::vector<std::vector<unsigned>> level_mesh_groups(const std::vector<unsigned> &levptr,
+                                                     const std::vector<unsigned> &rows,
+                                                     const std::vector<unsigned> &node_offset) {
+  const size_t nm = node_offset.size() - 1;
+  const size_t nl = levptr.size() - 1;
+  std::vector<std::vector<unsigned>> groups(nl * nm);
+  for (size_t l = 0; l < nl; ++l) {
+    size_t m = 0;
+    for (unsigned k = levptr[l]; k < levptr[l + 1]; ++k) {
+      const unsigned i = rows[k];  // ascending inside a level
+      while (i >= node_offset[m + 1]) ++m;
+      groups[l * nm + m].push_back(i);
+    }
+  }
+  return groups;
+}
+
+template <typename T>
+BArgs<T> make_bargs(dfm2h_batch B, BatchBufs<T> &S, const PackedVals<T> &av,
+                    const PackedVals<T> &fv, const PackedVals<T> &bv) {
+  BArgs<T> a{};
+  a.nmesh = B->nmesh;
+  a.nlev_b = B->nlev_b;
+  a.A = BPlan{B->aplan.off.p, B->aplan.kmax.p, B->aplan.row.p, B->aplan.len.p, B->aplan.col.p,
+              B->a_mesh.p};
+  a.F = BPlan{B->fplan.off.p, B->fplan.kmax.p, B->fplan.row.p, B->fplan.len.p, B->fplan.col.p,
+              B->f_mesh.p};
+  a.B = BPlan{B->bplan.off.p, B->bplan.kmax.p, B->bplan.row.p, B->bplan.len.p, B->bplan.col.p,
+              B->b_mesh.p};
+  a.a_val = av.val.p;
+  a.a_dia = av.dia.p;
+  a.f_val = fv.val.p;
+  a.f_dia = fv.dia.p;
+  a.b_val = bv.val.p;
+  a.a64_val = B->a64.val.p;
+  a.a64_dia = B->a64.dia.p;
+  a.r = S.r.p;
+  a.x = S.x.p;
+  a.Pr = S.Pr.p;
+  a.p = S.p.p;
+  a.x64 = B->x64.p;
+  a.b64 = B->b64.p;
+  a.partA = B->partA.p;
+  a.partA2 = B->partA2.p;
+  a.partB = B->partB.p;
+  a.M = B->M.p;
+  a.G = B->G.p;
+  a.a_mptr = B->a_mptr.p;
+  a.b_gptr = B->b_gptr.p;
+  return a;
+}
+
+// the scalar kernels only touch control pointers: view any BArgs as BArgs<float>
+template <typename T>
+BArgs<float> ctl_view(const BArgs<T> &a) {
+  BArgs<float> c{};
+  c.nmesh = a.nmesh;
+  c.nlev_b = a.nlev_b;
+  c.partA = a.partA;
+  c.partA2 = a.partA2;
+  c.partB = a.partB;
+  c.M = a.M;
+  c.G = a.G;
+  c.a_mptr = a.a_mptr;
+  c.b_gptr = a.b_gptr;
+  return c;
+}
+
+template <int MODE, typename T>
+void launch_scalar(const BArgs<T> &a, hipStream_t s) {
+  hipLaunchKernelGGL(kb_scalar<MODE>, dim3(ceil_div(a.nmesh, kMeshPerBlock)),
+                     dim3(kMeshPerBlock * kWave), 0, s, ctl_view(a));
+}
+
+template <typename T, class V>
+void launch_sweeps(const BArgs<T> &a, dfm2h_batch B, hipStream_t s) {
+  const unsigned nm = B->nmesh;
+  for (unsigned l = 0; l < B->nlev_f; ++l) {
+    const unsigned s0 = B->fplan.group_ptr[(size_t)l * nm];
+    const unsigned nsl = B->fplan.group_ptr[(size_t)(l + 1) * nm] - s0;
+    if (!nsl) continue;
+    hipLaunchKernelGGL((kb_fwd<T, V>), dim3(grid_of<V>(nsl)), dim3(V::WPB * kWave), 0, s, a, s0,
+                       nsl);
+  }
+  for (unsigned l = 0; l < B->nlev_b; ++l) {
+    const unsigned s0 = B->bplan.group_ptr[(size_t)l * nm];
+    const unsigned nsl = B->bplan.group_ptr[(size_t)(l + 1) * nm] - s0;
+    if (!nsl) continue;
+    hipLaunchKernelGGL((kb_bwd<T, V>), dim3(grid_of<V>(nsl)), dim3(V::WPB * kWave), 0, s, a, s0,
+                       nsl);
+  }
+}
+
+template <typename T, class V>
+void launch_prologue(const BArgs<T> &a, dfm2h_batch B, hipStream_t s) {
+  const unsigned na = B->aplan.nslices;
+  const dim3 ga(grid_of<V>(na)), bs(V::WPB * kWave);
+  hipLaunchKernelGGL((kb_init<T, V>), ga, bs, 0, s, a, na);
+  if (sizeof(T) == 4)
+    launch_scalar<kBInitMixed>(a, s);
+  else
+    launch_scalar<kBInit>(a, s);
+  launch_sweeps<T, V>(a, B, s);
+  launch_scalar<kBRPr0>(a, s);
+  hipLaunchKernelGGL((kb_pupdate<T, V>), ga, bs, 0, s, a, na, 0);
+}
+
+template <typename T, class V>
+void launch_iteration(const BArgs<T> &a, dfm2h_batch B, hipStream_t s) {
+  const unsigned na = B->aplan.nslices;
+  const dim3 ga(grid_of<V>(na)), bs(V::WPB * kWave);
+  hipLaunchKernelGGL((kb_spmv<T, V>), ga, bs, 0, s, a, na);
+  launch_scalar<kBAlpha>(a, s);
+  hipLaunchKernelGGL((kb_update<T, V>), ga, bs, 0, s, a, na);
+  if constexpr (sizeof(T) == 4) {
+    launch_scalar<kBConvMixed>(a, s);
+    hipLaunchKernelGGL((kb_fold<V>), ga, bs, 0, s, a, na);
+    hipLaunchKernelGGL((kb_residual<V>), ga, bs, 0, s, a, na);
+    launch_scalar<kBReliable>(a, s);
+  } else {
+    launch_scalar<kBConv>(a, s);
+  }
+  launch_sweeps<T, V>(a, B, s);
+  launch_scalar<kBBeta>(a, s);
+  hipLaunchKernelGGL((kb_pupdate<T, V>), ga, bs, 0, s, a, na, 1);
+}
+
+template <typename T>
+using LaunchFn = void (*)(const BArgs<T> &, dfm2h_batch, hipStream_t);
+
+template <typename T>
+LaunchFn<T> prologue_fn(int v) {
+  switch (v) {
+    case 0: return launch_prologue<T, V0>;
+    case 1: return launch_prologue<T, V1>;
+    case 2: return launch_prologue<T, V2>;
+    case 3: return launch_prologue<T, V3>;
+    case 4: return launch_prologue<T, V4>;
+    default: return launch_prologue<T, V5>;
+  }
+}
+
+template <typename T>
+LaunchFn<T> iteration_fn(int v) {
+  switch (v) {
+    case 0: return launch_iteration<T, V0>;
+    case 1: return launch_iteration<T, V1>;
+    case 2: return launch_iteration<T, V2>;
+    case 3: return launch_iteration<T, V3>;
+    case 4: return launch_iteration<T, V4>;
+    default: return launch_iteration<T, V5>;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// autotuner cache: (DOF bucket = floor(log2 ndof), fp32) -> variant.
+// In-process map; when DFM2H_TUNE_CACHE names a file, entries are read from
+// it once and every new entry is appended ("bucket fp32 variant ms").
+
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, int> g_tune;
+bool g_tune_loaded = false;
+
+const char *tune_file() { return std::getenv("DFM2H_TUNE_CACHE"); }
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char *f = tune_file();
+  if (!f) return;
+  FILE *fp = std::fopen(f, "r");
+  if (!fp) return;
+  int b, p, v;
+  double ms;
+  while (std::fscanf(fp, "%d %d %d %lf", &b, &p, &v, &ms) == 4)
+    if (v >= 0 && v < kNumVariants) g_tune[{b, p}] = v;
+  std::fclose(fp);
+}
+
+int tune_lookup(int bucket, int fp32) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  auto it = g_tune.find({bucket, fp32});
+  return it == g_tune.end() ? -1 : it->second;
+}
+
+void tune_store(int bucket, int fp32, int v, double ms) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  g_tune[{bucket, fp32}] = v;
+  if (const char *f = tune_file()) {
+    if (FILE *fp = std::fopen(f, "a")) {
+      std::fprintf(fp, "%d %d %d %.6f\n", bucket, fp32, v, ms);
+      std::fclose(fp);
+    }
+  }
+}
+
+int dof_bucket(dfm2h_batch B) {
+  const double n = 3.0 * (double)B->nblk;
+  return n < 1.0 ? 0 : (int)std::floor(std::log2(n));
+}
+
+// values packed for the batch plans, from the current A and P
+template <typename T>
+void ensure_batch_vals(dfm2h_batch B, PackedVals<T> &av, PackedVals<T> &fv, PackedVals<T> &bv) {
+  dfm2h_bsr A = B->A;
+  dfm2h_ilu P = B->P;
+  ensure_packed(B->aplan, av, A->vcrs.p, A->vdia.p, A->version, B->s);
+  ensure_packed(B->fplan, fv, P->vcrs.p, P->vdia.p, P->version, B->s);
+  ensure_packed(B->bplan, bv, P->vcrs.p, nullptr, P->version, B->s);
+}
+
+template <typename T>
+void ensure_batch_bufs(dfm2h_batch B, BatchBufs<T> &S) {
+  const size_t n = (size_t)kVSB * B->nblk;
+  if (S.r.n != n) {
+    S.r.alloc(n);
+    S.x.alloc(n);
+    S.Pr.alloc(n);
+    S.p.alloc(n);
+  }
+}
+
+// Runs one whole solve of every mesh with variant v on the batch stream;
+// b is already in B->b64.  Returns when every mesh is done.
+template <typename T>
+void run_batch(dfm2h_batch B, BatchBufs<T> &S, const BArgs<T> &a, int v, double tol,
+               double delta, unsigned max_nitr) {
+  hipStream_t s = B->s;
+  BatchGlobal g{};
+  g.tol = tol;
+  g.delta2 = delta * delta;
+  g.max_nitr = (int)max_nitr;
+  g.max_upd = (int)max_nitr;
+  g.ndone = 0;
+  DFM2H_HIP(hipMemcpyAsync(B->G.p, &g, sizeof(BatchGlobal), hipMemcpyHostToDevice, s));
+  prologue_fn<T>(v)(a, B, s);
+  check_launch("batch prologue");
+  const unsigned chunk = std::max(1u, std::min(kChunk, max_nitr));
+  std::vector<uintptr_t> sig = {(uintptr_t)v,        (uintptr_t)chunk,    (uintptr_t)a.a_val,
+                                (uintptr_t)a.a_dia,  (uintptr_t)a.f_val,  (uintptr_t)a.f_dia,
+                                (uintptr_t)a.b_val,  (uintptr_t)a.a64_val, (uintptr_t)a.a64_dia,
+                                (uintptr_t)a.r,      (uintptr_t)a.x64,    (uintptr_t)a.b64,
+                                (uintptr_t)a.partA,  (uintptr_t)a.M};
+  if (!S.exec || S.sig != sig) {
+    S.drop_graph();
+    DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < chunk; ++k) iteration_fn<T>(v)(a, B, s);
+    hipError_t e = hipStreamEndCapture(s, &S.graph);
+    DFM2H_REQUIRE(e == hipSuccess, DFM2H_ERR_HIP,
+                  std::string("graph capture: ") + hipGetErrorString(e));
+    DFM2H_HIP(hipGraphInstantiate(&S.exec, S.graph, nullptr, nullptr, 0));
+    S.sig = sig;
+  }
+  // the FP32 form may take one extra chunk for the final reliable update
+  const unsigned nchunks = (max_nitr + chunk - 1) / chunk + (sizeof(T) == 4 ? 1 : 0);
+  bool finished = false;
+  for (unsigned c = 0; c < nchunks && !finished; ++c) {
+    DFM2H_HIP(hipGraphLaunch(S.exec, s));
+    DFM2H_HIP(hipMemcpyAsync(&B->host_g[c & 1], B->G.p, sizeof(BatchGlobal),
+                             hipMemcpyDeviceToHost, s));
+    DFM2H_HIP(hipEventRecord(B->ev_poll[c & 1], s));
+    if (c > 0) {
+      DFM2H_HIP(hipEventSynchronize(B->ev_poll[(c - 1) & 1]));
+      finished = B->host_g[(c - 1) & 1].ndone >= B->nmesh;
+    }
+  }
+  DFM2H_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+BArgs<T> prepare(dfm2h_batch B, BatchBufs<T> &S);
+
+template <>
+BArgs<double> prepare<double>(dfm2h_batch B, BatchBufs<double> &S) {
+  ensure_batch_bufs(B, S);
+  ensure_batch_vals(B, B->a64, B->f64, B->b64v);
+  return make_bargs<double>(B, S, B->a64, B->f64, B->b64v);
+}
+
+template <>
+BArgs<float> prepare<float>(dfm2h_batch B, BatchBufs<float> &S) {
+  ensure_batch_bufs(B, S);
+  ensure_batch_vals(B, B->a32, B->f32, B->b32);
+  ensure_packed(B->aplan, B->a64, B->A->vcrs.p, B->A->vdia.p, B->A->version, B->s);
+  return make_bargs<float>(B, S, B->a32, B->f32, B->b32);
+}
+
+// Times a short fixed-length solve (tolerance 0, `nit` iterations per
+// mesh) for every variant and returns the fastest; ms[v] = time per
+// iteration of variant v.
+template <typename T>
+int autotune(dfm2h_batch B, BatchBufs<T> &S, double *ms, size_t ms_cap) {
+  const unsigned nit = 2 * kChunk;
+  int best = 0;
+  double best_ms = 1e300;
+  for (int v = 0; v < kNumVariants; ++v) {
+    const BArgs<T> a = prepare<T>(B, S);
+    double t_ms = 0.0;
+    for (int rep = 0; rep < 2; ++rep) {  // first run captures and warms up
+      DFM2H_HIP(hipEventRecord(B->ev_t0, B->s));
+      run_batch<T>(B, S, a, v, 0.0, 1e-2, nit);
+      DFM2H_HIP(hipEventRecord(B->ev_t1, B->s));
+      DFM2H_HIP(hipEventSynchronize(B->ev_t1));
+      float t = 0.f;
+      DFM2H_HIP(hipEventElapsedTime(&t, B->ev_t0, B->ev_t1));
+      t_ms = (double)t / nit;
+    }
+    if (ms && (size_t)v < ms_cap) ms[v] = t_ms;
+    if (t_ms < best_ms) {
+      best_ms = t_ms;
+      best = v;
+    }
+  }
+  S.drop_graph();
+  tune_store(dof_bucket(B), sizeof(T) == 4 ? 1 : 0, best, best_ms);
+  return best;
+}
+
+void need_batch(dfm2h_batch B) {
+  DFM2H_REQUIRE(B, DFM2H_ERR_INVALID, "null handle");
+  DFM2H_REQUIRE(B->A->pattern_serial == B->a_serial && B->P->pattern_serial == B->p_serial,
+                DFM2H_ERR_STATE, "matrix / preconditioner pattern changed since batch_create");
+  DFM2H_REQUIRE(B->P->decomposed, DFM2H_ERR_STATE, "preconditioner not decomposed");
+}
+
+}  // namespace
+}  // namespace dfm2h
+
+extern "C" {
+
+int dfm2h_batch_create(dfm2h_batch *out, dfm2h_bsr A, dfm2h_ilu P, const unsigned *node_offset,
+                       size_t nmesh) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(out && A && P && node_offset, DFM2H_ERR_INVALID, "null argument");
+    *out = nullptr;
+    DFM2H_REQUIRE(A->has_pattern && P->has_pattern, DFM2H_ERR_STATE, "pattern not set");
+    DFM2H_REQUIRE(A->nblk == P->nblk, DFM2H_ERR_INVALID, "matrix / preconditioner size mismatch");
+    DFM2H_REQUIRE(nmesh >= 1 && nmesh < (1u << 24), DFM2H_ERR_INVALID, "bad mesh count");
+    const unsigned nblk = A->nblk;
+    std::vector<unsigned> off(node_offset, node_offset + nmesh + 1);
+    DFM2H_REQUIRE(off[0] == 0 && off[nmesh] == nblk, DFM2H_ERR_INVALID,
+                  "node_offset must start at 0 and end at the node count");
+    for (size_t m = 0; m < nmesh; ++m)
+      DFM2H_REQUIRE(off[m] <= off[m + 1], DFM2H_ERR_INVALID, "node_offset must be non-decreasing");
+    // every block of row i must stay inside i's mesh (the union is block diagonal)
+    {
+      size_t m = 0;
+      for (unsigned i = 0; i < nblk; ++i) {
+        while (i >= off[m + 1]) ++m;
+        for (unsigned q = A->pat.colind[i]; q < A->pat.colind[i + 1]; ++q) {
+          const unsigned j = A->pat.rowptr[q];
+          DFM2H_REQUIRE(j >= off[m] && j < off[m + 1], DFM2H_ERR_PATTERN,
+                        "matrix couples two meshes (row " + std::to_string(i) + ", column " +
+                            std::to_string(j) + ")");
+        }
+      }
+    }
+    auto B = std::make_unique<dfm2h_batch_s>();
+    B->nmesh = (unsigned)nmesh;
+    B->nblk = nblk;
+    B->A = A;
+    B->P = P;
+    B->a_serial = A->pattern_serial;
+    B->p_serial = P->pattern_serial;
+    B->node_offset = off;
+    // matrix plan: one group per mesh
+    {
+      std::vector<std::vector<unsigned>> groups(nmesh);
+      for (size_t m = 0; m < nmesh; ++m)
+        for (unsigned i = off[m]; i < off[m + 1]; ++i) groups[m].push_back(i);
+      std::vector<unsigned> s0(nblk), s1(nblk);
+      for (unsigned i = 0; i < nblk; ++i) {
+        s0[i] = A->pat.colind[i];
+        s1[i] = A->pat.colind[i + 1];
+      }
+      B->aplan.build(groups, s0, s1, A->pat.rowptr);
+      B->a_mesh.upload(slice_mesh(B->aplan, B->nmesh, false));
+      B->a_mptr.upload(B->aplan.group_ptr);
+    }
+    // sweep plans: one group per (level, mesh), level-major
+    {
+      std::vector<unsigned> levptr, rows, s0(nblk), s1(nblk);
+      build_levels(nblk, P->colind, P->rowptr, P->diaind, true, levptr, rows);
+      B->nlev_f = (unsigned)levptr.size() - 1;
+      for (unsigned i = 0; i < nblk; ++i) {
+        s0[i] = P->colind[i];
+        s1[i] = P->diaind[i];
+      }
+      B->fplan.build(level_mesh_groups(levptr, rows, off), s0, s1, P->rowptr);
+      B->f_mesh.upload(slice_mesh(B->fplan, B->nmesh, true));
+      build_levels(nblk, P->colind, P->rowptr, P->diaind, false, levptr, rows);
+      B->nlev_b = (unsigned)levptr.size() - 1;
+      for (unsigned i = 0; i < nblk; ++i) {
+        s0[i] = P->diaind[i];
+        s1[i] = P->colind[i + 1];
+      }
+      B->bplan.build(level_mesh_groups(levptr, rows, off), s0, s1, P->rowptr);
+      B->b_mesh.upload(slice_mesh(B->bplan, B->nmesh, true));
+      B->b_gptr.upload(B->bplan.group_ptr);
+    }
+    const size_t n = (size_t)kVSB * nblk;
+    B->x64.alloc(std::max<size_t>(n, 1));
+    B->b64.alloc(std::max<size_t>(n, 1));
+    B->partA.alloc(std::max<size_t>(B->aplan.nslices, 1));
+    B->partA2.alloc(std::max<size_t>(B->aplan.nslices, 1));
+    B->partB.alloc(std::max<size_t>(B->bplan.nslices, 1));
+    B->M.alloc(nmesh);
+    B->G.alloc(1);
+    DFM2H_HIP(hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking));
+    DFM2H_HIP(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
+    DFM2H_HIP(hipEventCreateWithFlags(&B->ev_poll[0], hipEventDisableTiming));
+    DFM2H_HIP(hipEventCreateWithFlags(&B->ev_poll[1], hipEventDisableTiming));
+    DFM2H_HIP(hipEventCreate(&B->ev_t0));
+    DFM2H_HIP(hipEventCreate(&B->ev_t1));
+    DFM2H_HIP(hipHostMalloc((void **)&B->host_g, 2 * sizeof(BatchGlobal), hipHostMallocDefault));
+    DFM2H_HIP(hipStreamSynchronize(stream()));
+    *out = B.release();
+  });
+}
+
+int dfm2h_batch_destroy(dfm2h_batch B) {
+  return guard(__func__, [&] { delete B; });
+}
+
+int dfm2h_batch_info(dfm2h_batch B, size_t *nmesh, size_t *nslices, unsigned *n_forward,
+                     unsigned *n_backward) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(B, DFM2H_ERR_INVALID, "null handle");
+    if (nmesh) *nmesh = B->nmesh;
+    if (nslices) *nslices = B->aplan.nslices;
+    if (n_forward) *n_forward = B->nlev_f;
+    if (n_backward) *n_backward = B->nlev_b;
+  });
+}
+
+int dfm2h_batch_set_variant(dfm2h_batch B, int fp32, int variant) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(B, DFM2H_ERR_INVALID, "null handle");
+    DFM2H_REQUIRE(variant >= -1 && variant < kNumVariants, DFM2H_ERR_INVALID, "bad variant");
+    B->variant[fp32 ? 1 : 0] = variant;
+  });
+}
+
+int dfm2h_batch_get_variant(dfm2h_batch B, int fp32, int *variant, const char **name) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(B && variant, DFM2H_ERR_INVALID, "null argument");
+    const int v = B->variant[fp32 ? 1 : 0];
+    *variant = v;
+    if (name) *name = v >= 0 ? kVariantNames[v] : "autotune";
+  });
+}
+
+int dfm2h_batch_tune(dfm2h_batch B, int fp32, int *best, double *ms, size_t ms_cap,
+                     size_t *n_variants) {
+  return guard(__func__, [&] {
+    need_batch(B);
+    DFM2H_HIP(hipEventRecord(B->ev_in, stream()));
+    DFM2H_HIP(hipStreamWaitEvent(B->s, B->ev_in, 0));
+    const size_t n = (size_t)kVSB * B->nblk;
+    if (!B->has_b && n) {  // no right-hand side seen yet: time on b = 1
+      hipLaunchKernelGGL(kb_fill, dim3(std::min<size_t>(2048, ceil_div(n, 256))), dim3(256), 0,
+                         B->s, B->b64.p, 1.0, n);
+      check_launch("kb_fill");
+    }
+    const int v = fp32 ? autotune<float>(B, B->s32, ms, ms_cap)
+                       : autotune<double>(B, B->s64, ms, ms_cap);
+    B->variant[fp32 ? 1 : 0] = v;
+    if (best) *best = v;
+    if (n_variants) *n_variants = kNumVariants;
+  });
+}
+
+int dfm2h_batch_solve(dfm2h_batch B, const double *b, double *x, double tol, int fp32,
+                      double delta, unsigned max_nitr, unsigned *iters, double *relres,
+                      int *state) {
+  return guard(__func__, [&] {
+    need_batch(B);
+    DFM2H_REQUIRE(b && x, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_REQUIRE(max_nitr >= 1, DFM2H_ERR_INVALID, "max_nitr must be >= 1");
+    DFM2H_REQUIRE(!fp32 || (delta > 0.0 && delta < 1.0), DFM2H_ERR_INVALID,
+                  "delta must be in (0, 1)");
+    const size_t n = (size_t)kVSB * B->nblk;
+    DFM2H_HIP(hipEventRecord(B->ev_in, stream()));
+    DFM2H_HIP(hipStreamWaitEvent(B->s, B->ev_in, 0));
+    if (n)
+      DFM2H_HIP(hipMemcpyAsync(B->b64.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, B->s));
+    B->has_b = true;
+    const int p = fp32 ? 1 : 0;
+    int v = B->variant[p];
+    if (v < 0) v = tune_lookup(dof_bucket(B), p);
+    if (fp32) {
+      if (v < 0) v = autotune<float>(B, B->s32, nullptr, 0);
+      const BArgs<float> a = prepare<float>(B, B->s32);
+      run_batch<float>(B, B->s32, a, v, tol, delta, max_nitr);
+      if (n)
+        DFM2H_HIP(hipMemcpyAsync(x, B->x64.p, n * sizeof(double), hipMemcpyDeviceToDevice, B->s));
+    } else {
+      if (v < 0) v = autotune<double>(B, B->s64, nullptr, 0);
+      const BArgs<double> a = prepare<double>(B, B->s64);
+      run_batch<double>(B, B->s64, a, v, tol, 0.5, max_nitr);
+      if (n)
+        DFM2H_HIP(hipMemcpyAsync(x, B->s64.x.p, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 B->s));
+    }
+    B->variant[p] = v;
+    std::vector<MeshScal> h(B->nmesh);
+    DFM2H_HIP(hipMemcpyAsync(h.data(), B->M.p, h.size() * sizeof(MeshScal),
+                             hipMemcpyDeviceToHost, B->s));
+    DFM2H_HIP(hipStreamSynchronize(B->s));
+    bool any_nan = false;
+    for (unsigned m = 0; m < B->nmesh; ++m) {
+      if (iters) iters[m] = (unsigned)h[m].it;
+      if (relres) relres[m] = h[m].res;
+      if (state) state[m] = h[m].nan ? -1 : h[m].converged ? 1 : 0;
+      any_nan |= h[m].nan != 0;
+    }
+    DFM2H_REQUIRE(!any_nan, DFM2H_ERR_NAN, "a mesh's residual became NaN or Inf");
+  });
+}
+
+}  // extern "C"

@@ -228,6 +228,7 @@ int dfm2h_bsr_set_pattern(dfm2h_bsr A, const unsigned *colind, size_t ncolind,
       A->plan.build({all}, s0, s1, P.rowptr);
     }
     A->has_pattern = true;
+    A->pattern_serial++;
     A->version++;
     DFM2H_HIP(hipStreamSynchronize(stream()));
   });

@@ -46,6 +46,38 @@ void ensure_packed(const PackedPlan &P, PackedVals<T> &V, const double *crs, con
   V.version = version;
 }
 
+// Groups rows by dependency level.  Forward: lev(i) = 1 + max lev(j) over
+// the lower part of row i (0 when empty); backward: the same over the upper
+// part, rows visited from the last one.  Rows inside a level stay ascending.
+inline void build_levels(unsigned n, const std::vector<unsigned> &colind,
+                         const std::vector<unsigned> &rowptr, const std::vector<unsigned> &diaind,
+                         bool forward, std::vector<unsigned> &levptr,
+                         std::vector<unsigned> &levrows) {
+  std::vector<unsigned> lev(n, 0);
+  unsigned nlev = 0;
+  if (forward) {
+    for (unsigned i = 0; i < n; ++i) {
+      unsigned l = 0;
+      for (unsigned s = colind[i]; s < diaind[i]; ++s) l = std::max(l, lev[rowptr[s]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  } else {
+    for (unsigned i = n; i-- > 0;) {
+      unsigned l = 0;
+      for (unsigned s = diaind[i]; s < colind[i + 1]; ++s) l = std::max(l, lev[rowptr[s]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  }
+  levptr.assign(nlev + 1, 0);
+  for (unsigned i = 0; i < n; ++i) levptr[lev[i] + 1]++;
+  for (unsigned l = 0; l < nlev; ++l) levptr[l + 1] += levptr[l];
+  levrows.resize(n);
+  std::vector<unsigned> cur(levptr.begin(), levptr.end() - 1);
+  for (unsigned i = 0; i < n; ++i) levrows[cur[lev[i]]++] = i;
+}
+
 }  // namespace dfm2h
 
 struct dfm2h_bsr_s {
@@ -54,6 +86,7 @@ struct dfm2h_bsr_s {
   dfm2h::Pattern pat;
   dfm2h::DBuf<double> vcrs, vdia;
   uint64_t version = 1;  // bumped whenever values change
+  uint64_t pattern_serial = 0;  // bumped whenever the pattern is set
   dfm2h::PackedPlan plan;  // all rows, CRS block order
   dfm2h::PackedVals<double> pk64;
   dfm2h::PackedVals<float> pk32;
@@ -86,6 +119,7 @@ struct dfm2h_ilu_s {
   dfm2h::PackedVals<double> f64, b64;
   dfm2h::PackedVals<float> f32, b32;
   uint64_t version = 1;
+  uint64_t pattern_serial = 0;  // bumped whenever the pattern is set
   dfm2h::PcgWork *work = nullptr;
   ~dfm2h_ilu_s() { dfm2h::destroy_pcg_work(work); }
 };

@@ -114,38 +114,6 @@ k_ilu_decompose_level(const unsigned *__restrict__ rows, unsigned nrows,
   }
 }
 
-// Groups rows by dependency level.  Forward: lev(i) = 1 + max lev(j) over
-// the lower part of row i (0 when empty); backward: the same over the upper
-// part, rows visited from the last one.  Rows inside a level stay ascending.
-static void build_levels(unsigned n, const std::vector<unsigned> &colind,
-                         const std::vector<unsigned> &rowptr, const std::vector<unsigned> &diaind,
-                         bool forward, std::vector<unsigned> &levptr,
-                         std::vector<unsigned> &levrows) {
-  std::vector<unsigned> lev(n, 0);
-  unsigned nlev = 0;
-  if (forward) {
-    for (unsigned i = 0; i < n; ++i) {
-      unsigned l = 0;
-      for (unsigned s = colind[i]; s < diaind[i]; ++s) l = std::max(l, lev[rowptr[s]] + 1);
-      lev[i] = l;
-      nlev = std::max(nlev, l + 1);
-    }
-  } else {
-    for (unsigned i = n; i-- > 0;) {
-      unsigned l = 0;
-      for (unsigned s = diaind[i]; s < colind[i + 1]; ++s) l = std::max(l, lev[rowptr[s]] + 1);
-      lev[i] = l;
-      nlev = std::max(nlev, l + 1);
-    }
-  }
-  levptr.assign(nlev + 1, 0);
-  for (unsigned i = 0; i < n; ++i) levptr[lev[i] + 1]++;
-  for (unsigned l = 0; l < nlev; ++l) levptr[l + 1] += levptr[l];
-  levrows.resize(n);
-  std::vector<unsigned> cur(levptr.begin(), levptr.end() - 1);
-  for (unsigned i = 0; i < n; ++i) levrows[cur[lev[i]]++] = i;
-}
-
 // A slot -> ILU slot by column lookup in the ILU row (the row2crs map of
 // CopyValue), -1 where the ILU pattern lacks the column.
 static std::vector<int> build_a2l(const dfm2h_ilu_s &P, const Pattern &A) {
@@ -259,6 +227,7 @@ int dfm2h_ilu_set_pattern0(dfm2h_ilu P, dfm2h_bsr A) {
     P->mapped_for = A;
     P->nsing_d.alloc(1);
     P->has_pattern = true;
+    P->pattern_serial++;
     P->decomposed = false;
     P->version++;
     DFM2H_HIP(hipStreamSynchronize(stream()));

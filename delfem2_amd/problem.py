@@ -72,3 +72,24 @@ class DeviceSystem:
         hist, n_inner = S.Solve_PCG_Mixed(self.b, x, tol, self.A, self.ilu, delta,
                                           max_outer, max_inner)
         return x, hist, n_inner
+
+
+class BatchSystem(DeviceSystem):
+    """The union system of a ``workload.Batch`` plus its batched solver."""
+
+    def __init__(self, bat, device: int = 0):
+        super().__init__(bat.union, device)
+        self.batch = bat
+        self.solver = None
+
+    def factor(self) -> bool:
+        ok = super().factor()
+        if self.solver is None:
+            self.solver = S.BatchSolver(self.A, self.ilu, self.batch.node_offset)
+        return ok
+
+    def solve_batch(self, tol: float = 1e-8, fp32: bool = False, delta: float = 1e-2,
+                    max_nitr: int = 20000):
+        x = torch.zeros_like(self.b)
+        it, rr, st = self.solver.solve(self.b, x, tol, fp32, delta, max_nitr)
+        return x, it, rr, st

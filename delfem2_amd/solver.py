@@ -311,3 +311,66 @@ def Solve_PCG_Mixed(b_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: fl
            float(conv_ratio_tol), float(delta), int(max_outer), int(max_inner), _hptr(hist),
            cap, C.byref(nh), C.byref(ni))
     return hist[: nh.value].tolist(), ni.value
+
+
+class BatchSolver:
+    """Many independent meshes solved together (no reference counterpart).
+
+    ``mat`` / ``ilu`` hold the union system (assembled, BCs set, decomposed
+    with the ordinary calls); mesh m owns nodes ``node_offset[m] ..
+    node_offset[m+1]-1``.  Each mesh runs its own ILU(0)-PCG (lsitrsol.h:80-148)
+    inside shared kernels; converged meshes drop out.
+    """
+
+    def __init__(self, mat: CMatrixSparse, ilu: CPreconditionerILU, node_offset):
+        _ensure()
+        off = np.ascontiguousarray(node_offset, np.uint32)
+        h = C.c_void_p()
+        N.call("dfm2h_batch_create", C.byref(h), mat.h, ilu.h, _hptr(off), off.size - 1)
+        self.h = h
+        self.nmesh = off.size - 1
+        self._keep = (mat, ilu)  # the batch refers to both
+
+    def info(self) -> dict:
+        nm, ns = C.c_size_t(0), C.c_size_t(0)
+        f, b = C.c_uint(0), C.c_uint(0)
+        N.call("dfm2h_batch_info", self.h, C.byref(nm), C.byref(ns), C.byref(f), C.byref(b))
+        return {"nmesh": nm.value, "nslices": ns.value, "levels": (f.value, b.value)}
+
+    def solve(self, b: torch.Tensor, x: torch.Tensor, tol: float, fp32: bool = False,
+              delta: float = 1e-2, max_nitr: int = 20000):
+        """Returns (iterations, relative residuals, state) per mesh; state is 1
+        converged, 0 iteration cap, -1 NaN."""
+        it = np.zeros(self.nmesh, np.uint32)
+        rr = np.zeros(self.nmesh)
+        st = np.zeros(self.nmesh, np.int32)
+        N.call("dfm2h_batch_solve", self.h, _dptr(b), _dptr(x), float(tol), int(bool(fp32)),
+               float(delta), int(max_nitr), _hptr(it), _hptr(rr), _hptr(st))
+        return it, rr, st
+
+    def tune(self, fp32: bool):
+        """Runs the launch-shape autotuner; returns (best variant, ms per
+        iteration of every variant)."""
+        best = C.c_int(-1)
+        ms = np.zeros(16)
+        nv = C.c_size_t(0)
+        N.call("dfm2h_batch_tune", self.h, int(bool(fp32)), C.byref(best), _hptr(ms), ms.size,
+               C.byref(nv))
+        return best.value, ms[: nv.value]
+
+    def set_variant(self, fp32: bool, variant: int) -> None:
+        N.call("dfm2h_batch_set_variant", self.h, int(bool(fp32)), int(variant))
+
+    def variant(self, fp32: bool):
+        v = C.c_int(-1)
+        name = C.c_char_p()
+        N.call("dfm2h_batch_get_variant", self.h, int(bool(fp32)), C.byref(v), C.byref(name))
+        return v.value, name.value.decode()
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.load().dfm2h_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

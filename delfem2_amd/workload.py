@@ -205,3 +205,108 @@ def batch_grids(n_mesh: int = 512, lo: int = 4, hi: int = 24, seed: int = 202612
     """Per-mesh (nx, ny, nz) drawn uniformly from {lo..hi}^3."""
     u = uniform(seed, 3 * n_mesh).reshape(n_mesh, 3)
     return (lo + np.floor(u * (hi - lo + 1))).astype(np.int64)
+
+
+def tet_volumes(xyz: np.ndarray, tet: np.ndarray) -> np.ndarray:
+    """Signed tet volumes, operand for operand as femutil.cpp:202-212
+    (TetVolume3D): ((a - b) + c) * (1/6) with the cofactor products in that
+    order."""
+    P = xyz.reshape(-1, 3)[tet.reshape(-1, 4).astype(np.int64)]
+    d1, d2, d3 = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]
+    a = d1[:, 0] * (d2[:, 1] * d3[:, 2] - d3[:, 1] * d2[:, 2])
+    b = d1[:, 1] * (d2[:, 0] * d3[:, 2] - d3[:, 0] * d2[:, 2])
+    c = d1[:, 2] * (d2[:, 0] * d3[:, 1] - d3[:, 0] * d2[:, 1])
+    return ((a - b) + c) * (1.0 / 6.0)
+
+
+def body_force_load(xyz: np.ndarray, tet: np.ndarray, rho: float, g) -> np.ndarray:
+    """Nodal load of a uniform body force rho*g: every element adds
+    vol*rho*g*0.25 to each of its four nodes, elements in ascending order
+    (the right-hand side fem_solidlinear.h:192-237 builds with zero
+    displacement, summed in the same order)."""
+    vol = tet_volumes(xyz, tet)
+    b = np.zeros(xyz.size)
+    nodes = tet.reshape(-1).astype(np.int64)
+    for d in range(3):
+        e = np.repeat(vol * rho * float(g[d]) * 0.25, 4)
+        np.add.at(b, nodes * 3 + d, e)
+    return b
+
+
+@dataclass
+class Batch:
+    """Independent meshes stacked into one block-diagonal union problem.
+
+    ``union.f_ext`` carries every mesh's body-force load (the union itself
+    has g = 0); mesh m owns nodes ``node_offset[m]:node_offset[m+1]`` and
+    elements ``elem_offset[m]:elem_offset[m+1]``."""
+    union: Problem
+    node_offset: np.ndarray   # uint32 [n+1]
+    elem_offset: np.ndarray   # int64 [n+1]
+    grids: np.ndarray         # int64 [n, 3]
+    g: np.ndarray             # float64 [n, 3]
+    index: np.ndarray         # which meshes of the full set these are
+
+    @property
+    def nmesh(self) -> int:
+        return self.grids.shape[0]
+
+    def mesh(self, m: int) -> Problem:
+        """Mesh m as a standalone problem (body force rho*g_m, no f_ext)."""
+        U = self.union
+        n0, n1 = int(self.node_offset[m]), int(self.node_offset[m + 1])
+        e0, e1 = int(self.elem_offset[m]), int(self.elem_offset[m + 1])
+        tet = np.ascontiguousarray((U.tet[4 * e0:4 * e1].astype(np.int64) - n0).astype(np.uint32))
+        return Problem(f"mesh{int(self.index[m])}", np.ascontiguousarray(U.xyz[3 * n0:3 * n1]), tet,
+                       np.ascontiguousarray(U.myu[e0:e1]), np.ascontiguousarray(U.lam[e0:e1]),
+                       np.ascontiguousarray(U.flag[3 * n0:3 * n1]), rho=U.rho,
+                       g=self.g[m].copy(), grid=tuple(int(v) for v in self.grids[m]),
+                       ordering=U.ordering)
+
+    def mesh_slice(self, m: int) -> slice:
+        return slice(3 * int(self.node_offset[m]), 3 * int(self.node_offset[m + 1]))
+
+
+def batch(n_mesh: int = 512, lo: int = 4, hi: int = 24, seed: int = 20261205,
+          sigma: float = 0.5, nu: float = 0.3, ordering: str = "color4",
+          select=None) -> Batch:
+    """``n_mesh`` unit-cube meshes with (nx, ny, nz) hexes drawn uniformly from
+    {lo..hi}^3 (6 tets per hex), z=0 face clamped, E_e = exp(sigma*z_e)
+    (lognormal), nu fixed, rho = 1 and a per-mesh body force g_m uniform in
+    [-1e-3, 1e-3]^3.  Each mesh is ordered on its own (``ordering``).
+    ``select`` keeps a subset of the meshes (same definitions)."""
+    grids_all = batch_grids(n_mesh, lo, hi, seed)
+    g_all = (2.0 * uniform(seed + 1, 3 * n_mesh) - 1.0).reshape(n_mesh, 3) * 1e-3
+    index = np.arange(n_mesh) if select is None else np.asarray(select, np.int64)
+    xyzs, tets, myus, lams, flags, fs = [], [], [], [], [], []
+    noff, eoff = [0], [0]
+    for m in index:
+        nx, ny, nz = (int(v) for v in grids_all[m])
+        xyz_i, tet = mesh_tet_voxel(nx, ny, nz)
+        iz = xyz_i[2::3].copy()
+        xyz = xyz_i.reshape(-1, 3) / np.array([nx, ny, nz], np.float64)
+        xyz = np.ascontiguousarray(xyz.reshape(-1))
+        ne = tet.size // 4
+        E = np.exp(sigma * normal(seed + 1000003 * (int(m) + 1), ne))
+        myu, lam = lame(E, np.full(ne, float(nu)))
+        flag = np.zeros(xyz.size, np.int32)
+        flag.reshape(-1, 3)[iz == 0] = 1
+        xyz, tet, n2o = _order(xyz, tet, nx, ny, nz, ordering)
+        if n2o is not None:
+            flag = np.ascontiguousarray(flag.reshape(-1, 3)[n2o].reshape(-1))
+        f = body_force_load(xyz, tet, 1.0, g_all[m])
+        xyzs.append(xyz)
+        tets.append(tet.astype(np.int64) + noff[-1])
+        myus.append(myu)
+        lams.append(lam)
+        flags.append(flag)
+        fs.append(f)
+        noff.append(noff[-1] + xyz.size // 3)
+        eoff.append(eoff[-1] + ne)
+    union = Problem(f"batch{len(index)}", np.concatenate(xyzs),
+                    np.ascontiguousarray(np.concatenate(tets).astype(np.uint32)),
+                    np.concatenate(myus), np.concatenate(lams), np.concatenate(flags),
+                    rho=1.0, g=np.zeros(3), f_ext=np.concatenate(fs), ordering=ordering)
+    union.extra.update(seed=seed, sigma=sigma)
+    return Batch(union, np.array(noff, np.uint32), np.array(eoff, np.int64),
+                 grids_all[index].copy(), g_all[index].copy(), index)

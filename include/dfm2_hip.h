@@ -44,6 +44,7 @@ extern "C" {
 typedef struct dfm2h_bsr_s *dfm2h_bsr;         /* CMatrixSparse<double>, 3x3 */
 typedef struct dfm2h_ilu_s *dfm2h_ilu;         /* CPreconditionerILU<double> */
 typedef struct dfm2h_tetmesh_s *dfm2h_tetmesh; /* tet connectivity on device */
+typedef struct dfm2h_batch_s *dfm2h_batch;     /* many meshes, one solver     */
 
 /* ---- runtime ------------------------------------------------------------ */
 /* Selects the HIP device for this thread.  No reference counterpart (the
@@ -226,6 +227,51 @@ int dfm2h_adjoint_grad_tet(dfm2h_tetmesh M, const double *xyz_dev,
                            const double *myu1_e_dev, const double *lambda1_e_dev,
                            const double *u_dev, const double *lam_dev,
                            double *grad_dev);
+
+/* ---- batched solve (no reference counterpart) --------------------------- */
+/* Many independent meshes solved together.  A and P hold the union system:
+ * mesh m owns nodes [node_offset[m], node_offset[m+1]) and no block of A
+ * couples two meshes (checked: DFM2H_ERR_PATTERN).  A and P are assembled,
+ * given fixed BCs, copied and decomposed with the ordinary calls; the batch
+ * keeps its own mesh-aligned packed plans, built here from the patterns
+ * (rebuild the batch after setting a new pattern: DFM2H_ERR_STATE).  Values
+ * are re-packed whenever A or P changed.  node_offset: host, nmesh+1
+ * entries. */
+int dfm2h_batch_create(dfm2h_batch *out, dfm2h_bsr A, dfm2h_ilu P,
+                       const unsigned *node_offset, size_t nmesh);
+int dfm2h_batch_destroy(dfm2h_batch B);
+/* nslices: wavefront slices of the matrix plan; n_forward / n_backward:
+ * dependency levels of the union factors (= sweep launches per iteration). */
+int dfm2h_batch_info(dfm2h_batch B, size_t *nmesh, size_t *nslices,
+                     unsigned *n_forward, unsigned *n_backward);
+/* One ILU(0)-PCG per mesh, all meshes in the same kernels (Solve_PCG of
+ * lsitrsol.h:80-148 per mesh).  b_dev, x_dev: union vectors [3*np], device.
+ * fp32 = 0: FP64 throughout, stop when |r_k| / |r_0| < tol per mesh (the
+ * reference's test); fp32 = 1: FP32 SpMV / sweeps / vectors inside FP64
+ * reliable updates (x64 += x32, r = b - A x64 in FP64 whenever the FP32
+ * residual fell by delta), stop when the FP64 |b - A x| / |b| < tol.
+ * max_nitr caps the (inner) iterations of each mesh.  Optional host outputs
+ * per mesh: iters (inner iterations), relres (last relative residual tested),
+ * state (1 converged, 0 iteration cap, -1 NaN).  Returns DFM2H_ERR_NAN if any
+ * mesh produced NaN / Inf; meshes that hit the cap are reported in state.
+ * The first solve of a (DOF bucket, precision) with no variant set runs the
+ * autotuner (dfm2h_batch_tune). */
+int dfm2h_batch_solve(dfm2h_batch B, const double *b_dev, double *x_dev,
+                      double tol, int fp32, double delta, unsigned max_nitr,
+                      unsigned *iters, double *relres, int *state);
+/* Autotuner: times a 32-iteration solve of every launch-shape variant
+ * (wavefronts per workgroup, slices per wavefront, blocks per load round of
+ * the SpMV and of the sweeps) on this batch, keeps the fastest for this
+ * batch and for its (floor(log2 ndof), precision) key; ms[v] = ms per
+ * iteration of variant v (ms_cap entries at most).  The key -> variant map
+ * lives for the process; when the environment variable DFM2H_TUNE_CACHE
+ * names a file it is read from and appended to that file as well. */
+int dfm2h_batch_tune(dfm2h_batch B, int fp32, int *best, double *ms,
+                     size_t ms_cap, size_t *n_variants);
+/* variant -1 = autotune on the next solve; 0 .. n_variants-1 = fixed. */
+int dfm2h_batch_set_variant(dfm2h_batch B, int fp32, int variant);
+int dfm2h_batch_get_variant(dfm2h_batch B, int fp32, int *variant,
+                            const char **name);
 
 #ifdef __cplusplus
 }

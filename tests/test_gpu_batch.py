@@ -1,0 +1,171 @@
+"""Batched solve: many independent meshes in one solver, through the C ABI.
+
+Per mesh, against the oracle run on that mesh alone:
+* union assembly, BCs and ILU(0) factors restricted to the mesh: bit-exact
+  (the union is block diagonal, so its rows are the mesh's rows);
+* FP64 batch solution vs the oracle's Solve_PCG at the same tolerance
+  (1e-8): displacement rel-L2 <= 1e-7, iteration counts within 1;
+* FP32-inner / FP64-refined batch solution vs the oracle solved to 1e-12:
+  rel-L2 <= 1e-6.
+At the full 512-mesh size (5.1M DOF): every mesh converges, its true FP64
+residual |b - A x| / |b| is below the tolerance (x 1.5 for the recursive
+residual), and three sampled meshes (the smallest, the largest, a random
+one) match the oracle as above.  Every launch-shape variant of the
+autotuner gives bit-identical results (the summation order does not depend
+on the launch shape).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from delfem2_amd import solver as S
+from delfem2_amd import workload as W
+from delfem2_amd.problem import BatchSystem
+
+pytestmark = pytest.mark.gpu
+
+FP64_X_TOL = 1e-7
+MIXED_X_TOL = 1e-6
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def oracle_mesh(prob):
+    ind, ps = oracle.psup(prob.tet, prob.np_)
+    C = oracle.System(ind, ps)
+    b = np.zeros(prob.ndof)
+    assert C.assemble(b, prob.myu, prob.lam, prob.rho, prob.g, prob.xyz, prob.tet) == 0
+    C.set_fixed_bc(prob.flag)
+    oracle.set_rhs_zero(b, prob.flag)
+    C.b = b
+    C.factor()
+    return C
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def small():
+    bat = W.batch(12, lo=2, hi=7, seed=77)
+    print(f"[small batch] meshes={bat.nmesh} ndof={bat.union.ndof}", flush=True)
+    G = BatchSystem(bat)
+    assert G.assemble()
+    assert G.factor()
+    return bat, G
+
+
+def test_union_bit_exact_per_mesh(gpu, small):
+    bat, G = small
+    b = G.b.cpu().numpy()
+    rp, di, vc, vd = G.ilu.download()
+    colind = G.psup_ind
+    for m in range(bat.nmesh):
+        p = bat.mesh(m)
+        C = oracle_mesh(p)
+        assert np.array_equal(bits(b[bat.mesh_slice(m)]), bits(C.b))
+        n0, n1 = int(bat.node_offset[m]), int(bat.node_offset[m + 1])
+        assert np.array_equal(bits(vd[9 * n0:9 * n1]), bits(C.l_vdia))
+        q0, q1 = int(colind[n0]), int(colind[n1])
+        assert np.array_equal(rp[q0:q1].astype(np.int64) - n0, C.l_rowptr.astype(np.int64))
+        assert np.array_equal(bits(vc[9 * q0:9 * q1]), bits(C.l_vcrs))
+
+
+def test_fp64_and_mixed_vs_oracle(gpu, small):
+    bat, G = small
+    x, it, rr, st = G.solve_batch(1e-8, fp32=False)
+    xm, itm, rrm, stm = G.solve_batch(1e-8, fp32=True)
+    x, xm = x.cpu().numpy(), xm.cpu().numpy()
+    assert np.all(st == 1) and np.all(stm == 1)
+    assert np.all(rr < 1e-8) and np.all(rrm < 1e-8)
+    info = G.solver.info()
+    assert info["nmesh"] == bat.nmesh and info["levels"] == (4, 4)
+    for m in range(bat.nmesh):
+        C = oracle_mesh(bat.mesh(m))
+        xc, hc = C.solve_pcg(C.b.copy(), 1e-8, 5000)
+        assert abs(int(it[m]) - (len(hc) - 1)) <= 1, (m, it[m], len(hc) - 1)
+        assert rel(x[bat.mesh_slice(m)], xc) <= FP64_X_TOL, m
+        xe, _ = C.solve_pcg(C.b.copy(), 1e-12, 5000)
+        assert rel(xm[bat.mesh_slice(m)], xe) <= MIXED_X_TOL, m
+
+
+def test_variants_bit_identical_and_tuner(gpu, small):
+    bat, G = small
+    ref = {}
+    for fp32 in (False, True):
+        best, ms = G.solver.tune(fp32)
+        assert 0 <= best < ms.size and np.all(ms > 0)
+        assert G.solver.variant(fp32)[0] == best
+        for v in range(ms.size):
+            G.solver.set_variant(fp32, v)
+            x, it, rr, st = G.solve_batch(1e-8, fp32=fp32)
+            xh = x.cpu().numpy()
+            if fp32 not in ref:
+                ref[fp32] = (xh, it)
+            assert np.array_equal(bits(xh), bits(ref[fp32][0])), (fp32, v)
+            assert np.array_equal(it, ref[fp32][1])
+        G.solver.set_variant(fp32, -1)
+    # deterministic run to run (autotuned variant, looked up from the cache)
+    x1 = G.solve_batch(1e-8, fp32=True)[0]
+    x2 = G.solve_batch(1e-8, fp32=True)[0]
+    assert torch.equal(x1, x2)
+
+
+def test_edge_cases(gpu):
+    # one mesh with a zero load converges at once with x = 0; a batch of one
+    bat = W.batch(3, lo=2, hi=3, seed=5)
+    bat.union.f_ext[bat.mesh_slice(1)] = 0.0
+    G = BatchSystem(bat)
+    G.assemble()
+    G.factor()
+    for fp32 in (False, True):
+        x, it, rr, st = G.solve_batch(1e-8, fp32=fp32)
+        assert np.all(st == 1)
+        assert it[1] == 0 and torch.count_nonzero(x[bat.mesh_slice(1)]).item() == 0
+        assert it[0] > 0 and it[2] > 0
+    # iteration cap: state 0, iterations == cap
+    x, it, rr, st = G.solve_batch(1e-14, fp32=False, max_nitr=3)
+    assert np.all(it[[0, 2]] == 3) and np.all(st[[0, 2]] == 0)
+    one = S.BatchSolver(G.A, G.ilu, [0, G.prob.np_])
+    with pytest.raises(S.N.DFM2HError):  # meshes 0..2 are coupled inside "mesh 0"
+        S.BatchSolver(G.A, G.ilu, [0, 5, G.prob.np_])
+    assert one.info()["nmesh"] == 1
+    ito, _, sto = one.solve(G.b, torch.zeros_like(G.b), 1e-8)
+    assert sto[0] == 1 and ito[0] > 0
+
+
+def test_c5_512_meshes(gpu):
+    """The full batching workload: 512 meshes, 4..24 hexes per axis."""
+    bat = W.batch()
+    print(f"[c5] meshes={bat.nmesh} ndof={bat.union.ndof}", flush=True)
+    G = BatchSystem(bat)
+    assert G.assemble()
+    assert G.factor()
+    bn = G.b.cpu().numpy()
+    off = bat.node_offset.astype(np.int64) * 3
+    sample = [int(np.argmin(np.diff(off))), int(np.argmax(np.diff(off))), 137]
+    for fp32 in (False, True):
+        x, it, rr, st = G.solve_batch(1e-8, fp32=fp32)
+        assert np.all(st == 1), np.flatnonzero(st != 1)
+        r = G.b.clone()
+        G.A.MatVec(r, -1.0, x, 1.0)
+        rh = r.cpu().numpy()
+        num = np.sqrt(np.add.reduceat(rh * rh, off[:-1]))
+        den = np.sqrt(np.add.reduceat(bn * bn, off[:-1]))
+        print(f"fp32={fp32} iters min/max {it.min()}/{it.max()} true relres max "
+              f"{(num / den).max():.2e}", flush=True)
+        assert np.all(num / den < 1.5e-8)
+        xh = x.cpu().numpy()
+        for m in sample:
+            C = oracle_mesh(bat.mesh(m))
+            if fp32:
+                xe, _ = C.solve_pcg(C.b.copy(), 1e-12, 5000)
+                assert rel(xh[bat.mesh_slice(m)], xe) <= MIXED_X_TOL, m
+            else:
+                xc, hc = C.solve_pcg(C.b.copy(), 1e-8, 5000)
+                assert abs(int(it[m]) - (len(hc) - 1)) <= 1
+                assert rel(xh[bat.mesh_slice(m)], xc) <= FP64_X_TOL, m

@@ -54,3 +54,38 @@ def test_cube_config_sizes():
     assert np.isclose(c.f_ext.sum(), -1.0) and (c.f_ext != 0).sum() == 9
     g = W.batch_grids(512)
     assert g.min() >= 4 and g.max() <= 24 and g.shape == (512, 3)
+
+
+def test_batch_layout_and_rhs_bit_exact():
+    """Union layout of a batch, and its per-mesh body-force load equal, bit
+    for bit, to the right-hand side the oracle's Merge builds for that mesh
+    alone (zero displacement, g = g_m)."""
+    import oracle
+    bat = W.batch(10, lo=2, hi=6, seed=3)
+    U = bat.union
+    off = bat.node_offset.astype(np.int64)
+    assert off[0] == 0 and off[-1] == U.np_ and np.all(np.diff(off) > 0)
+    assert np.all(np.abs(bat.g) <= 1e-3)
+    for m in range(bat.nmesh):
+        e0, e1 = bat.elem_offset[m], bat.elem_offset[m + 1]
+        t = U.tet[4 * e0:4 * e1]
+        assert t.min() >= off[m] and t.max() < off[m + 1]
+        p = bat.mesh(m)
+        nx, ny, nz = bat.grids[m]
+        assert p.np_ == (nx + 1) * (ny + 1) * (nz + 1) and p.ne == 6 * nx * ny * nz
+        ind, ps = oracle.psup(p.tet, p.np_)
+        C = oracle.System(ind, ps)
+        b = np.zeros(p.ndof)
+        assert C.assemble(b, p.myu, p.lam, p.rho, p.g, p.xyz, p.tet) == 0
+        assert np.array_equal(b.view(np.uint64), U.f_ext[bat.mesh_slice(m)].view(np.uint64))
+    # a subset keeps the full set's per-mesh definitions
+    sub = W.batch(10, lo=2, hi=6, seed=3, select=[4, 7])
+    assert np.array_equal(sub.grids, bat.grids[[4, 7]])
+    assert np.array_equal(sub.union.f_ext[sub.mesh_slice(1)], U.f_ext[bat.mesh_slice(7)])
+
+
+def test_batch_c5_sizes():
+    g = W.batch_grids()
+    assert g.shape == (512, 3) and g.min() == 4 and g.max() == 24
+    ndof = 3 * np.prod(g + 1, axis=1)
+    assert 4.5e6 < ndof.sum() < 6e6

@@ -98,11 +98,116 @@ def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
     }
 
 
+def cpu_baseline_batch(bat, it64, budget_s: float = 15.0):
+    """The CPU reference (or the oracle) solving meshes of the batch one after
+    another (pattern + assembly + Decompose + FP64 PCG to 1e-8, single
+    thread) until `budget_s` has passed; the whole batch extrapolated by the
+    per-mesh work weight ndof_m x iterations_m (FP64 GPU counts)."""
+    import numpy as np
+
+    import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    ndof = np.diff(bat.node_offset.astype(np.int64)) * 3
+    w = ndof * np.maximum(it64.astype(np.int64), 1)
+    t0 = time.perf_counter()
+    done = 0
+    for m in range(bat.nmesh):
+        p = bat.mesh(m)
+        ind, ps = oracle.psup(p.tet, p.np_)
+        Sys = oracle.RefSystem(ind, ps) if kind == "reference" else oracle.System(ind, ps)
+        b = np.zeros(p.ndof)
+        Sys.assemble(b, p.myu, p.lam, p.rho, p.g, p.xyz, p.tet)
+        Sys.set_fixed_bc(p.flag)
+        oracle.set_rhs_zero(b, p.flag)
+        Sys.factor()
+        Sys.solve_pcg(b, 1e-8, 20000)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    t = time.perf_counter() - t0
+    est = t * w.sum() / w[:done].sum()
+    return {"value": round(est, 3),
+            "unit": "s (all meshes: set-up + Decompose + FP64 PCG to 1e-8, extrapolated)",
+            "cores": 1, "kind": kind,
+            "sample": f"first {done} of {bat.nmesh} meshes solved in {t:.2f} s; scaled by "
+                      f"sum(ndof x iterations) of all meshes / of the sample",
+            "host": platform.processor() or platform.machine()}
+
+
+def bench_batch(args):
+    """512 independent cube meshes (4..24 hexes per axis, lognormal E, random
+    body forces), one batched solve to 1e-8 per step: CopyValue + Decompose of
+    the union, then the batch solver (FP32 + FP64 reliable updates; FP64
+    alongside)."""
+    import numpy as np
+    import torch
+
+    from delfem2_amd import workload as W
+    from delfem2_amd.problem import BatchSystem
+
+    print("[bench] building the 512-mesh batch", file=sys.stderr, flush=True)
+    bat = W.batch()
+    G = BatchSystem(bat, device=0)
+    assert G.assemble()
+    assert G.factor()
+    ndof = np.diff(bat.node_offset.astype(np.int64)) * 3
+    tune = {}
+    for fp32 in (True, False):
+        best, ms = G.solver.tune(fp32)
+        tune["fp32" if fp32 else "fp64"] = {"best": G.solver.variant(fp32)[1],
+                                            "ms_per_iter": [round(v, 4) for v in ms]}
+
+    def step(fp32):
+        G.factor()
+        return G.solve_batch(1e-8, fp32=fp32, delta=args.delta)
+
+    for _ in range(args.warmup):
+        step(True)
+        step(False)
+    res = {}
+    for name, fp32 in (("mixed", True), ("fp64", False)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            x, it, rr, st = step(fp32)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        assert np.all(st == 1)
+        res[name] = {"time_s": round(t, 5), "iters_min": int(it.min()),
+                     "iters_max": int(it.max()), "iters_mean": round(float(it.mean()), 2),
+                     "dof_iter_per_s": round(float((ndof * it).sum()) / t, 1),
+                     "_it": it}
+        print(f"[bench] batch {name} {t * 1e3:.1f} ms, iterations {it.min()}..{it.max()}",
+              file=sys.stderr, flush=True)
+    it64 = res["fp64"].pop("_it")
+    res["mixed"].pop("_it")
+    res["mixed"]["speedup_vs_fp64"] = round(res["fp64"]["time_s"] / res["mixed"]["time_s"], 3)
+    cpu = cpu_baseline_batch(bat, it64) if args.cpu_iters > 0 else None
+    out = {
+        "metric": "time-to-1e-8 of 512 independent tet meshes, batched FP32+refine (s)",
+        "value": res["mixed"]["time_s"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(res["mixed"]["time_s"] * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-inner/f64-refine", "data": "synthetic (seeded voxel tet cubes)",
+        "config": {"workload": f"512 cube meshes, 4..24 hexes per axis x6 tets, "
+                               f"{int(ndof.sum())} DOF total, E~lognormal(0,0.5), nu=0.3, "
+                               f"g~U[-1e-3,1e-3]^3, color4 order, ILU(0)-PCG to 1e-8 per mesh",
+                   "ndof": int(ndof.sum()), "nmesh": bat.nmesh,
+                   "ndof_min": int(ndof.min()), "ndof_max": int(ndof.max())},
+        "mixed": res["mixed"], "fp64": res["fp64"], "autotune": tune,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cube", choices=["cube", "batch512"],
+                    help="cube: the 823,875-DOF single solve (headline); batch512: 512 "
+                         "independent meshes in one batched solve")
     ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
     ap.add_argument("--sigma", type=float, default=0.5)
     ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
@@ -131,6 +236,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of "
                                    "it (see DESIGN.md, sharding)", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload == "batch512":
+        bench_batch(args)
+        return
 
     import torch
 

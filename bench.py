@@ -35,29 +35,49 @@ def spmv_bytes(nb: int, nnzb: int, s: int, read_y: bool) -> int:
     return b + (ndof * s if read_y else 0)
 
 
-def kernel_bytes(nb: int, nnzb: int, s: int) -> dict:
+def kernel_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0) -> dict:
     """Algorithmic bytes of one launch class of the fused PCG iteration (s =
-    bytes per scalar; column indices 4 B; nL = nnzb/2 lower = upper blocks):
+    bytes per scalar; column indices 4 B; nnzb off-diagonal blocks, nL =
+    nnzb/2 lower = upper blocks; n0 rows without lower blocks = forward level 0, computed by the
+    update; nu0 rows without upper blocks = backward level 0, no work):
       spmv   off-diagonal blocks + indices, diagonal blocks, row starts,
              p read, Ap written
-      fwd    lower blocks + indices, D^-1 blocks, 8 B per row of schedule,
-             r read, Pr written          (whole sweep, all levels)
-      bwd    upper blocks + indices, 8 B per row, Pr read and written
-      update r, Ap, x, p read; r, x written
+      update r, Ap, x, p read; r, x written; D^-1 of the n0 level-0 rows
+             read and their Pr written; two 4 B masks per 21-row slice
+      fwd    (levels >= 1) lower blocks + indices, D^-1 blocks, 8 B per row
+             of schedule, r read, Pr written
+      bwd    (levels >= 1) upper blocks + indices, 8 B per row, Pr read and
+             written, r read for (r, Pr)
       pupd   p, Pr read; p written"""
     ndof = 3 * nb
     nl = nnzb // 2
+    nf, nbk = nb - n0, nb - nu0
     return {
         "spmv": spmv_bytes(nb, nnzb, s, False),
-        "fwd_sweep": nl * (9 * s + 4) + nb * 9 * s + nb * 8 + 2 * ndof * s,
-        "bwd_sweep": nl * (9 * s + 4) + nb * 8 + 2 * ndof * s,
-        "update": 6 * ndof * s,
+        "update": 6 * ndof * s + n0 * (9 * s + 3 * s) + 8 * ((nb + 20) // 21),
+        "fwd_sweep": nl * (9 * s + 4) + nf * 9 * s + nf * 8 + 2 * 3 * nf * s,
+        "bwd_sweep": nl * (9 * s + 4) + nbk * 8 + 3 * 3 * nbk * s,
         "pupdate": 3 * ndof * s,
     }
 
 
-def pcg_iter_bytes(nb: int, nnzb: int, s: int) -> int:
-    return sum(kernel_bytes(nb, nnzb, s).values())
+def level0_rows(psup_ind, psup, nb: int):
+    """(rows with no lower neighbour, rows with no upper neighbour) of the
+    one-ring pattern: the sizes of forward / backward level 0."""
+    import numpy as np
+    ind = np.asarray(psup_ind, np.int64)
+    ps = np.asarray(psup, np.int64)
+    cnt = np.diff(ind)
+    rows = np.repeat(np.arange(nb), cnt)
+    has_lo = np.zeros(nb, bool)
+    has_up = np.zeros(nb, bool)
+    has_lo[rows[ps < rows]] = True
+    has_up[rows[ps > rows]] = True
+    return int((~has_lo).sum()), int((~has_up).sum())
+
+
+def pcg_iter_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0) -> int:
+    return sum(kernel_bytes(nb, nnzb, s, n0, nu0).values())
 
 
 def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
@@ -299,7 +319,8 @@ def main():
     # ---- per-kernel times of the fused iterations: HIP events on the solver
     # stream, right after the timed solves (same buffers, same kernels) ----
     nlev_f, nlev_b = G.ilu.levels()
-    nlev = {"fwd_sweep": nlev_f, "bwd_sweep": nlev_b}
+    nlev = {"fwd_sweep": max(nlev_f - 1, 1), "bwd_sweep": max(nlev_b - 1, 1)}
+    n0, nu0 = level0_rows(G.psup_ind, G.psup, nb)
     kern = {}
     for prec, fp32, s in (("fp32", True, 4), ("fp64", False, 8)):
         if fp32:
@@ -307,14 +328,14 @@ def main():
         else:
             step_fp64()
         ms = S.pcg_kernel_times(G.A, G.ilu, fp32, args.spmv_reps)
-        kb = kernel_bytes(nb, nnzb, s)
+        kb = kernel_bytes(nb, nnzb, s, n0, nu0)
         kern[prec] = {}
         for k, b in kb.items():
             launches = nlev.get(k, 1)
             kern[prec][k] = {"ms_per_call": round(ms[k], 5), "launches_per_call": launches,
                              "bytes_per_call": b,
                              "gbs": round(b / (ms[k] * 1e-3) / 1e9, 1)}
-        for k in ("alpha_step", "conv_step") + (("refine_step",) if fp32 else ()):
+        for k in ("alpha_step",) + (("refine_step", "conv_step") if fp32 else ()):
             kern[prec][k + "_ms"] = round(ms[k], 5)
     # dominant kernel of the headline (FP32 + refinement) solve: largest time per iteration
     dom = max(("spmv", "fwd_sweep", "bwd_sweep", "update", "pupdate"),
@@ -333,7 +354,7 @@ def main():
         if kname in t:
             traffic = t[kname]
 
-    b_iter64 = pcg_iter_bytes(nb, nnzb, 8)
+    b_iter64 = pcg_iter_bytes(nb, nnzb, 8, n0, nu0)
     t_it64 = t_fp64 / max(it64, 1)
 
     cpu = None

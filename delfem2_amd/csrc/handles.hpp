@@ -116,6 +116,10 @@ struct dfm2h_ilu_s {
   dfm2h::DBuf<int> nsing_d;
   // sliced sweeps: forward plan (lower blocks + D^-1), backward plan (upper blocks)
   dfm2h::PackedPlan fplan, bplan;
+  // per forward slice: bit r set when slice row r has no upper blocks (its
+  // forward result is final); per forward level: whether it has such rows
+  dfm2h::DBuf<unsigned> fumask;
+  std::vector<char> flev_u0;
   dfm2h::PackedVals<double> f64, b64;
   dfm2h::PackedVals<float> f32, b32;
   uint64_t version = 1;

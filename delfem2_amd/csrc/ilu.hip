@@ -212,6 +212,21 @@ int dfm2h_ilu_set_pattern0(dfm2h_ilu P, dfm2h_bsr A) {
         s1[i] = P->diaind[i];
       }
       P->fplan.build(groups, s0, s1, P->rowptr);
+      {
+        std::vector<unsigned> um;
+        P->flev_u0.assign(groups.size(), 0);
+        for (size_t l = 0; l < groups.size(); ++l)
+          for (size_t b = 0; b < groups[l].size(); b += kSliceRows) {
+            unsigned m = 0;
+            for (size_t k = b; k < std::min(groups[l].size(), b + kSliceRows); ++k) {
+              const unsigned i = groups[l][k];
+              if (P->diaind[i] == P->colind[i + 1]) m |= 1u << (k - b);
+            }
+            um.push_back(m);
+            if (m) P->flev_u0[l] = 1;
+          }
+        P->fumask.upload(um);
+      }
       build_levels(P->nblk, P->colind, P->rowptr, P->diaind, false, P->levptr_b, rows);
       groups.clear();
       for (size_t l = 0; l + 1 < P->levptr_b.size(); ++l)

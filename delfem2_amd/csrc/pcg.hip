@@ -44,6 +44,8 @@ struct PcgScal {
   int nupd;       // FP64 residuals recorded after the initial |b|
   int max_upd;
   int final_;     // iteration cap reached: stop after this update
+  int fold;       // the p update folds x32 into x64 (set by a reliable update)
+  int pad2;
 };
 
 constexpr int kVecBlock = 256;
@@ -123,6 +125,18 @@ __global__ void k_unpad4(TO *__restrict__ y3, const TI *__restrict__ x4, size_t 
 
 // ---------------------------------------------------------------------------
 // fused PCG kernels
+//
+// Two launches of the reference's sweep structure are folded into
+// neighbours, without changing any arithmetic:
+//  * forward level 0 (rows with no lower blocks: Pr_i = D_i^-1 r_i) is
+//    computed by the kernel that produces r (the r/x update, or the FP64
+//    residual of a reliable update), from the row's three r components
+//    exchanged inside the wavefront;
+//  * backward level 0 (rows with no upper blocks) changes nothing; its share
+//    of (r, Pr) is summed where those rows' Pr is final (the forward level
+//    that computes them).
+// Every partial of (r, Pr) of an iteration goes to `partials3`:
+//   [backward levels >= 1 | forward levels with final rows | update].
 
 template <typename T>
 struct PcgArgs {
@@ -136,10 +150,18 @@ struct PcgArgs {
   const T *f_val, *f_dia;
   PlanView b_plan;
   const T *b_val;
+  // forward level 0 inside the update: per matrix slice, the rows that are
+  // level 0 (l0mask) and those that are also backward level 0 (isomask),
+  // and their D^-1 blocks packed in the matrix-slice layout
+  const unsigned *l0mask, *isomask;
+  const T *dinv0;
+  const unsigned *fumask;  // per forward slice: rows with no upper blocks
   // vectors
   T *r, *x, *Pr, *p;
   double *partials, *hist;
   double *partials2;  // (r,r) partials of the update kernel
+  double *partials3;  // (r,Pr) partials of the fused iteration
+  unsigned upd_part_off;
   PcgScal *S;
   // mixed precision: FP64 solution, right-hand side, matrix and outer history
   double *x64;
@@ -162,25 +184,30 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T> a) {
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }
 
-// Scalar steps, one workgroup each, over the partials the previous kernel
+// Scalar steps, one workgroup each, over the partials the previous kernels
 // wrote (lsitrsol.h:95-100, 116-117, 122-125, 132-135).
 enum ScalMode {
   kScalInit = 0,
   kScalAlpha = 1,
-  kScalConv = 2,
+  kScalConvBeta = 2,   // FP64 iteration: |r| history / stop test, then beta
   kScalRPr0 = 3,
-  kScalBeta = 4,
   kScalInitMixed = 5,  // |b|^2: outer history[0], rmax2
   kScalConvMixed = 6,  // FP32 |r|^2: decide a reliable update
-  kScalReliable = 7    // FP64 |b - A x|^2: outer history, stop test
+  kScalBetaMixed = 7   // FP64 |b - A x|^2 when an update ran: outer history,
+                       // stop test; then beta
 };
 
+// pa/na: first partial set, pb/nb: second (ConvBeta, BetaMixed)
 template <int MODE>
 __global__ void __launch_bounds__(kScalBlock)
-k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double *hist) {
+k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
+             const double *__restrict__ pb, unsigned nb, double *hist) {
   if (MODE != kScalInit && MODE != kScalInitMixed && S->done) return;
-  if (MODE == kScalReliable && !S->upd) return;
-  const double t = reduce_partials<kScalBlock>(partials, n);
+  const bool upd = MODE == kScalBetaMixed ? S->upd != 0 : true;  // block-uniform
+  const double t = upd ? reduce_partials<kScalBlock>(pa, na) : 0.0;
+  const double t2 = (MODE == kScalConvBeta || MODE == kScalBetaMixed)
+                        ? reduce_partials<kScalBlock>(pb, nb)
+                        : 0.0;
   if (threadIdx.x != 0) return;
   if (MODE == kScalInit) {  // |r0|^2: history[0]; early return when < 1e-30
     hist[0] = sqrt(t);
@@ -201,10 +228,8 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double
     S->alpha = S->rPr / t;
   } else if (MODE == kScalRPr0) {
     S->rPr = t;
-  } else if (MODE == kScalBeta) {  // beta = rPr1 / rPr; rPr = rPr1
-    S->beta = t / S->rPr;
-    S->rPr = t;
-  } else if (MODE == kScalConv) {  // push |r|, stop on sqrt(|r|^2/|r0|^2) < tol
+  } else if (MODE == kScalConvBeta) {
+    // push |r|, stop on sqrt(|r|^2/|r0|^2) < tol; else beta = rPr1/rPr
     const int it = S->it + 1;
     S->it = it;
     S->sq = t;
@@ -217,6 +242,9 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double
       S->nan = 1;
     } else if (it >= S->max_nitr) {
       S->done = 1;
+    } else {
+      S->beta = t2 / S->rPr;
+      S->rPr = t2;
     }
   } else if (MODE == kScalInitMixed) {
     hist[0] = sqrt(t);
@@ -226,6 +254,7 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double
     S->it = 0;
     S->nupd = 0;
     S->upd = 0;
+    S->fold = 0;
     S->final_ = 0;
     S->converged = 0;
     S->nan = 0;
@@ -241,6 +270,7 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double
     const int it = S->it + 1;
     S->it = it;
     S->sq = t;
+    S->fold = 0;
     if (!(t == t) || t > 1.0e308) {
       S->done = 1;
       S->nan = 1;
@@ -248,20 +278,27 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ partials, unsigned n, double
       S->upd = 1;
       S->final_ = it >= S->max_nitr ? 1 : 0;
     }
-  } else {  // kScalReliable
-    const int k = S->nupd + 1;
-    S->nupd = k;
-    hist[k] = sqrt(t);
-    S->upd = 0;
-    S->rmax2 = t;
-    if (sqrt(t / S->sq_b) < S->tol) {
-      S->done = 1;
-      S->converged = 1;
-    } else if (!(t == t) || t > 1.0e308) {
-      S->done = 1;
-      S->nan = 1;
-    } else if (S->final_ || k >= S->max_upd) {
-      S->done = 1;
+  } else {  // kScalBetaMixed
+    if (upd) {
+      const int k = S->nupd + 1;
+      S->nupd = k;
+      hist[k] = sqrt(t);
+      S->upd = 0;
+      S->fold = 1;
+      S->rmax2 = t;
+      if (sqrt(t / S->sq_b) < S->tol) {
+        S->done = 1;
+        S->converged = 1;
+      } else if (!(t == t) || t > 1.0e308) {
+        S->done = 1;
+        S->nan = 1;
+      } else if (S->final_ || k >= S->max_upd) {
+        S->done = 1;
+      }
+    }
+    if (!S->done) {
+      S->beta = t2 / S->rPr;
+      S->rPr = t2;
     }
   }
 }
@@ -287,40 +324,81 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T> a) {
   if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
 }
 
+// Forward level 0 for the level-0 rows of matrix slice L.s, given every
+// lane's new r component rv (wavefront-uniform call): Pr_i = D_i^-1 r_i, the
+// same products in the same order as the sweep.  Rows that are also
+// backward level 0 add r_i . Pr_i to srp.
 template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_pcg_update(PcgArgs<T> a) {
-  if (a.S->done) return;
-  const double alpha = a.S->alpha;
-  double s = 0.0;
-  for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
-       i += (size_t)gridDim.x * kVecBlock) {
-    const T rn = (T)((double)a.r[i] + (-alpha) * (double)a.Pr[i]);
-    a.r[i] = rn;
-    a.x[i] = (T)((double)a.x[i] + alpha * (double)a.p[i]);
-    s += (double)rn * (double)rn;
+__device__ inline void fwd_level0(const PcgArgs<T> &a, const SliceLane &L, unsigned i, T rv,
+                                  double &srp) {
+  const unsigned m = a.l0mask[L.s];
+  if (!m) return;
+  const T t0 = __shfl(rv, (int)L.r, kWave);
+  const T t1 = __shfl(rv, (int)L.r + kSliceRows, kWave);
+  const T t2 = __shfl(rv, (int)L.r + 2 * kSliceRows, kWave);
+  if (i != ~0u && ((m >> L.r) & 1u)) {
+    const T z = dia_row_dot(a.dinv0, L, t0, t1, t2);
+    a.Pr[kVS * (size_t)i + L.c] = z;
+    if ((a.isomask[L.s] >> L.r) & 1u) srp += (double)rv * (double)z;
   }
-  s = block_reduce_sum<kVecBlock>(s);
-  if (threadIdx.x == 0) a.partials2[blockIdx.x] = s;
 }
 
-// forward level: Pr = D^-1 (r - L Pr) on the level's slices (reads r
-// directly: the reference's copy Pr = r is fused away)
+// r += -alpha Ap, x += alpha p, partial (r,r); then forward level 0.
+// One wavefront per matrix slice (rows in order, components in lanes).
 template <typename T>
+__global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T> a) {
+  if (a.S->done) return;
+  const double alpha = a.S->alpha;
+  const SliceLane L = slice_lane(0);
+  const bool sl = L.s < a.a_nsl;
+  const unsigned i = sl && L.c < 3 ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  double s2 = 0.0, srp = 0.0;
+  T rn = 0;
+  if (i != ~0u) {
+    const size_t e = kVS * (size_t)i + L.c;
+    rn = (T)((double)a.r[e] + (-alpha) * (double)a.Pr[e]);
+    a.r[e] = rn;
+    a.x[e] = (T)((double)a.x[e] + alpha * (double)a.p[e]);
+    s2 = (double)rn * (double)rn;
+  }
+  if (sl) fwd_level0(a, L, i, rn, srp);
+  s2 = block_reduce_sum<kPackBlock>(s2);
+  srp = block_reduce_sum<kPackBlock>(srp);
+  if (threadIdx.x == 0) {
+    a.partials2[blockIdx.x] = s2;
+    a.partials3[a.upd_part_off + blockIdx.x] = srp;
+  }
+}
+
+// forward level >= 1: Pr = D^-1 (r - L Pr) on the level's slices (reads r
+// directly: the reference's copy Pr = r is fused away).  PART: the level
+// holds rows with no upper blocks; their (r, Pr) is summed here.
+template <typename T, bool PART>
 __global__ void __launch_bounds__(kPackBlock)
-k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl) {
+k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
   const SliceLane L = slice_lane(slice0);
-  if (L.s >= slice0 + nsl) return;  // whole wavefront
-  const unsigned i = L.c < 3 ? a.f_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  T out;
-  sliced_fwd_row<T, kVS>(L, i != ~0u, i, a.f_plan, a.f_val, a.f_dia, (const T *)a.r, a.Pr, out);
+  const bool sl = L.s < slice0 + nsl;
+  if (!PART && !sl) return;  // whole wavefront
+  const unsigned i = sl && L.c < 3 ? a.f_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  T out = 0;
+  if (sl)
+    sliced_fwd_row<T, kVS>(L, i != ~0u, i, a.f_plan, a.f_val, a.f_dia, (const T *)a.r, a.Pr,
+                           out);
+  if (PART) {
+    double part = 0.0;
+    if (i != ~0u && ((a.fumask[L.s] >> L.r) & 1u))
+      part = (double)a.r[kVS * (size_t)i + L.c] * (double)out;
+    part = block_reduce_sum<kPackBlock>(part);
+    if (threadIdx.x == 0) dst[blockIdx.x] = part;
+  }
 }
 
 // backward level + partial (r, Pr) of the rows it finalises, written at
-// partials[part_off + workgroup]
+// dst[workgroup]
 template <typename T>
 __global__ void __launch_bounds__(kPackBlock)
-k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, unsigned part_off) {
+k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
   const SliceLane L = slice_lane(slice0);
   const bool slice_ok = L.s < slice0 + nsl && L.c < 3;
@@ -335,20 +413,26 @@ k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, unsigned part_off) {
     part = (double)a.r[kVS * (size_t)i + L.c] * (double)t;
   }
   part = block_reduce_sum<kPackBlock>(part);
-  if (threadIdx.x == 0) a.partials[part_off + blockIdx.x] = part;
+  if (threadIdx.x == 0) dst[blockIdx.x] = part;
 }
 
-// p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue)
+// p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue).  FP32 form: after
+// a reliable update also folds x64 += x32, x32 = 0.
 template <typename T>
 __global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T> a, int mode) {
   if (a.S->done) return;
   const double beta = a.S->beta;
+  const bool fold = sizeof(T) == 4 && a.S->fold;
   for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
        i += (size_t)gridDim.x * kVecBlock) {
     if (mode == 0)
       a.p[i] = a.Pr[i];
     else
       a.p[i] = (T)(beta * (double)a.p[i] + (double)a.Pr[i]);
+    if (fold) {
+      a.x64[i] += (double)a.x[i];
+      a.x[i] = (T)0;
+    }
   }
 }
 
@@ -367,37 +451,53 @@ __global__ void __launch_bounds__(kVecBlock) k_mixed_init(PcgArgs<float> a) {
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }
 
-// reliable update, part 1: x64 += x32, x32 = 0
-__global__ void __launch_bounds__(kVecBlock) k_mixed_fold(PcgArgs<float> a) {
-  if (a.S->done || !a.S->upd) return;
-  for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
-       i += (size_t)gridDim.x * kVecBlock) {
-    a.x64[i] += (double)a.x[i];
-    a.x[i] = 0.0f;
-  }
-}
-
-// reliable update, part 2: r = b - A x64 in FP64 (sliced A, FP64 values),
-// rounded into r32, partial |r|^2
+// Reliable update (when due): r = b - A (x64 + x32) in FP64 (sliced A, FP64
+// values), rounded into r32, partial |r|^2; then forward level 0 from the
+// new r (overwriting the update kernel's).  x32 is folded into x64 later,
+// by the p update.
 __global__ void __launch_bounds__(kPackBlock) k_mixed_residual(PcgArgs<float> a) {
   if (a.S->done || !a.S->upd) return;
   const SliceLane L = slice_lane(0);
-  const bool slice_ok = L.s < a.a_nsl && L.c < 3;
-  const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  double part = 0.0;
+  const bool sl = L.s < a.a_nsl;
+  const unsigned i = sl && L.c < 3 ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  double part = 0.0, srp = 0.0;
+  float rf = 0.0f;
   if (i != ~0u) {
+    const unsigned K = a.a_plan.len[L.s * kSliceRows + L.r];
+    const unsigned q0 = a.a_plan.off[L.s];
     double acc = 0.0;
-    acc = sliced_row_sum<double, false, 4, kVS>(acc, 1.0, L, a.a_plan.len[L.s * kSliceRows + L.r],
-                                              a.a_plan.kmax[L.s], a.a_plan.off[L.s],
-                                              a.a_plan.col, a.a64_val, a.x64);
-    const Trip<double> xi = load_node<double, kVS>(a.x64, i);
-    acc += dia_row_dot(a.a64_dia, L, xi.a, xi.b, xi.c);
-    const double r = a.b64[kVS * (size_t)i + L.c] - acc;
-    a.r[kVS * (size_t)i + L.c] = (float)r;
+    for (unsigned k = 0; k < K; ++k) {
+      const size_t q = q0 + k;
+      const unsigned j = a.a_plan.col[q * kSliceRows + L.r];
+      const Trip<double> v = load_trip(a.a64_val, q, L.lane);
+      const double x0 = a.x64[kVS * (size_t)j] + (double)a.x[kVS * (size_t)j];
+      const double x1 = a.x64[kVS * (size_t)j + 1] + (double)a.x[kVS * (size_t)j + 1];
+      const double x2 = a.x64[kVS * (size_t)j + 2] + (double)a.x[kVS * (size_t)j + 2];
+      acc += v.a * x0 + v.b * x1 + v.c * x2;
+    }
+    const size_t ii = kVS * (size_t)i;
+    acc += dia_row_dot(a.a64_dia, L, a.x64[ii] + (double)a.x[ii],
+                       a.x64[ii + 1] + (double)a.x[ii + 1], a.x64[ii + 2] + (double)a.x[ii + 2]);
+    const double r = a.b64[ii + L.c] - acc;
+    rf = (float)r;
+    a.r[ii + L.c] = rf;
     part = r * r;
   }
+  if (sl) fwd_level0(a, L, i, rf, srp);
   part = block_reduce_sum<kPackBlock>(part);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
+  srp = block_reduce_sum<kPackBlock>(srp);
+  if (threadIdx.x == 0) {
+    a.partials[blockIdx.x] = part;
+    a.partials3[a.upd_part_off + blockIdx.x] = srp;
+  }
+}
+
+// x = x64 + x32 (the last fold may not have run)
+__global__ void k_mixed_out(double *__restrict__ x, const double *__restrict__ x64,
+                            const float *__restrict__ x32, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = x64[i] + (double)x32[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -406,7 +506,7 @@ __global__ void __launch_bounds__(kPackBlock) k_mixed_residual(PcgArgs<float> a)
 template <typename T>
 struct PcgBufs {
   DBuf<T> r, x, Pr, p;
-  DBuf<double> partials, partials2, hist;
+  DBuf<double> partials, partials2, partials3, hist;
   DBuf<PcgScal> S;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -422,6 +522,18 @@ struct PcgBufs {
   ~PcgBufs() { drop_graph(); }
 };
 
+// Per (matrix, factors) pair: the level-0 masks and D^-1 blocks the update
+// kernel needs, and where every (r, Pr) partial of an iteration goes.
+struct Fusion {
+  const void *A = nullptr;
+  uint64_t a_serial = ~0ull, p_serial = ~0ull, serial = 0;
+  DBuf<unsigned> l0mask, isomask, row_l0;
+  PackedVals<double> d64;
+  PackedVals<float> d32;
+  std::vector<unsigned> fwd_off, bwd_off;  // per level (>= 1): offset in partials3
+  unsigned upd_off = 0, n3 = 0;
+};
+
 struct PcgWork {
   hipStream_t s = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_poll[2] = {nullptr, nullptr};
@@ -429,6 +541,7 @@ struct PcgWork {
   PcgBufs<double> f64;
   PcgBufs<float> f32;
   DBuf<double> b64, x64, hist64;
+  Fusion fu;
   PcgWork() {
     DFM2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     DFM2H_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
@@ -454,10 +567,71 @@ static PcgWork &work_of(dfm2h_ilu P) {
   return *P->work;
 }
 
+// Builds the fused-iteration data for (A, P) when either pattern changed.
+// Matrix slice s holds rows 21 s .. 21 s + 20 (the matrix plan is one group
+// of all rows in order).  Row i is forward level 0 iff it has no lower
+// blocks, backward level 0 iff it has no upper blocks.
+static void ensure_fusion(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W) {
+  Fusion &F = W.fu;
+  if (F.A == A && F.a_serial == A->pattern_serial && F.p_serial == P->pattern_serial) return;
+  const unsigned nsl = A->plan.nslices, n = A->nblk;
+  std::vector<unsigned> l0(nsl, 0), iso(nsl, 0), row(std::max<size_t>((size_t)nsl * kSliceRows, 1), ~0u);
+  for (unsigned s = 0; s < nsl; ++s)
+    for (unsigned r = 0; r < (unsigned)kSliceRows; ++r) {
+      const unsigned i = s * kSliceRows + r;
+      if (i >= n || P->diaind[i] != P->colind[i]) continue;
+      l0[s] |= 1u << r;
+      row[i] = i;
+      if (P->diaind[i] == P->colind[i + 1]) iso[s] |= 1u << r;
+    }
+  F.l0mask.upload(l0);
+  F.isomask.upload(iso);
+  F.row_l0.upload(row);
+  F.d64.version = ~0ull;
+  F.d32.version = ~0ull;
+  unsigned off = 0;
+  const size_t nlb = P->bplan.group_ptr.size() - 1, nlf = P->fplan.group_ptr.size() - 1;
+  F.bwd_off.assign(nlb, ~0u);
+  for (size_t l = 1; l < nlb; ++l) {
+    F.bwd_off[l] = off;
+    off += slice_grid(P->bplan.group_slices(l));
+  }
+  F.fwd_off.assign(nlf, ~0u);
+  for (size_t l = 1; l < nlf; ++l)
+    if (P->flev_u0[l]) {
+      F.fwd_off[l] = off;
+      off += slice_grid(P->fplan.group_slices(l));
+    }
+  F.upd_off = off;
+  off += slice_grid(nsl);
+  F.n3 = off;
+  F.A = A;
+  F.a_serial = A->pattern_serial;
+  F.p_serial = P->pattern_serial;
+  F.serial++;
+}
+
+// D^-1 of the level-0 rows in matrix-slice layout, precision T
+template <typename T>
+static void ensure_dinv(dfm2h_bsr A, dfm2h_ilu P, PackedVals<T> &V, hipStream_t s) {
+  if (V.version == P->version && (V.dia.p || A->plan.nslices == 0)) return;
+  const unsigned nsl = A->plan.nslices;
+  V.dia.alloc((size_t)nsl * kSliceVals);
+  if (nsl) {
+    const size_t n = (size_t)nsl * kSliceVals;
+    hipLaunchKernelGGL(k_pack_dia<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, nsl,
+                       P->work->fu.row_l0.p, P->vdia.p, V.dia.p);
+    check_launch("k_pack_dia");
+  }
+  V.version = P->version;
+}
+
 template <typename T>
 static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const PackedVals<T> &av,
-                            const PackedVals<T> &fv, const PackedVals<T> &bv) {
-  PcgArgs<T> a;
+                            const PackedVals<T> &fv, const PackedVals<T> &bv,
+                            const PackedVals<T> &dv) {
+  const Fusion &F = P->work->fu;
+  PcgArgs<T> a{};
   a.nblk = A->nblk;
   a.n = (size_t)A->nblk * kVS;
   a.a_plan = view(A->plan);
@@ -469,12 +643,18 @@ static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Packe
   a.f_dia = fv.dia.p;
   a.b_plan = view(P->bplan);
   a.b_val = bv.val.p;
+  a.l0mask = F.l0mask.p;
+  a.isomask = F.isomask.p;
+  a.dinv0 = dv.dia.p;
+  a.fumask = P->fumask.p;
   a.r = B.r.p;
   a.x = B.x.p;
   a.Pr = B.Pr.p;
   a.p = B.p.p;
   a.partials = B.partials.p;
   a.partials2 = B.partials2.p;
+  a.partials3 = B.partials3.p;
+  a.upd_part_off = F.upd_off;
   a.hist = B.hist.p;
   a.S = B.S.p;
   return a;
@@ -487,8 +667,7 @@ static unsigned bwd_partial_total(dfm2h_ilu P) {
   return tot;
 }
 
-// Vectors are padded to 4 components per node; the pad entries are zeroed
-// here and never written by the solver kernels.
+// Scratch vectors and partial buffers; the vectors are zeroed once here.
 template <typename T>
 static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_nitr) {
   const size_t n = (size_t)A->nblk * kVS;
@@ -502,10 +681,12 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
     B.Pr.zero();
     B.p.zero();
   }
-  const size_t np = std::max<size_t>({(size_t)slice_grid(A->plan.nslices), (size_t)vec_grid(n),
-                                      (size_t)bwd_partial_total(P), 1});
+  const size_t gs = slice_grid(A->plan.nslices);
+  const size_t np = std::max<size_t>({gs, (size_t)vec_grid(n), (size_t)bwd_partial_total(P), 1});
   if (B.partials.n < np) B.partials.alloc(np);
-  if (B.partials2.n < kVecGridMax) B.partials2.alloc(kVecGridMax);
+  if (B.partials2.n < gs) B.partials2.alloc(gs);
+  const size_t n3 = std::max<size_t>(P->work->fu.n3, 1);
+  if (B.partials3.n < n3) B.partials3.alloc(n3);
   if (B.hist.n < (size_t)max_nitr + 2) B.hist.alloc((size_t)max_nitr + 2);
   if (!B.S.p) {
     B.S.alloc(1);
@@ -513,164 +694,90 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
   }
 }
 
-// forward then backward levels; returns the number of (r,Pr) partials
+// Prologue form: every forward and backward level, (r,Pr) partials into
+// `partials` (one per backward workgroup); returns their count.
 template <typename T>
-static unsigned launch_sweeps(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
+static unsigned launch_sweeps_full(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
-    hipLaunchKernelGGL(k_pcg_fwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
-                       P->fplan.group_ptr[l], nsl);
+    hipLaunchKernelGGL((k_pcg_fwd<T, false>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
+                       P->fplan.group_ptr[l], nsl, (double *)nullptr);
   }
   unsigned off = 0;
   for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
     const unsigned g = slice_grid(nsl);
     hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(g), dim3(kPackBlock), 0, s, a, P->bplan.group_ptr[l],
-                       nsl, off);
+                       nsl, a.partials + off);
     off += g;
   }
   return off;
 }
 
+// Iteration form: forward levels >= 1 then backward levels >= 1 (level 0 of
+// each is fused away), (r,Pr) partials into partials3.
+template <typename T>
+static void launch_sweeps_fused(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
+  const Fusion &F = P->work->fu;
+  for (size_t l = 1; l + 1 < P->fplan.group_ptr.size(); ++l) {
+    const unsigned nsl = P->fplan.group_slices(l);
+    if (F.fwd_off[l] != ~0u)
+      hipLaunchKernelGGL((k_pcg_fwd<T, true>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
+                         P->fplan.group_ptr[l], nsl, a.partials3 + F.fwd_off[l]);
+    else
+      hipLaunchKernelGGL((k_pcg_fwd<T, false>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
+                         a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
+  }
+  for (size_t l = 1; l + 1 < P->bplan.group_ptr.size(); ++l) {
+    const unsigned nsl = P->bplan.group_slices(l);
+    hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
+                       P->bplan.group_ptr[l], nsl, a.partials3 + F.bwd_off[l]);
+  }
+}
+
+template <int MODE>
+static void launch_scalar(hipStream_t s, PcgScal *S, const double *pa, unsigned na,
+                          const double *pb, unsigned nb, double *hist) {
+  hipLaunchKernelGGL(k_pcg_scalar<MODE>, dim3(1), dim3(kScalBlock), 0, s, S, pa, na, pb, nb,
+                     hist);
+}
+
+// One FP64 iteration (lsitrsol.h:107-146): spmv, alpha, update (+ forward
+// level 0), forward levels >= 1, backward levels >= 1, stop test + beta,
+// p update.
 template <typename T>
 static void launch_iteration(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
   const unsigned gs = slice_grid(a.a_nsl);
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
-                     gs, a.hist);
   const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gv), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalConv>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials2,
-                     gv, a.hist);
-  const unsigned nb = launch_sweeps(a, P, s);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
-                     nb, a.hist);
+  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
+  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  launch_sweeps_fused(a, P, s);
+  launch_scalar<kScalConvBeta>(s, a.S, a.partials2, gs, a.partials3, P->work->fu.n3, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
-// Runs the fused PCG on B.r (holding b) into B.x.  Returns iterations done;
-// fills status.
+// Captures `chunk` iterations into B's graph unless its signature matches.
+template <typename T, class F>
+static void ensure_graph(PcgBufs<T> &B, hipStream_t s, const std::vector<uintptr_t> &sig,
+                         unsigned chunk, F &&one_iteration) {
+  if (B.exec && B.sig == sig) return;
+  B.drop_graph();
+  DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (unsigned k = 0; k < chunk; ++k) one_iteration();
+  hipError_t e = hipStreamEndCapture(s, &B.graph);
+  DFM2H_REQUIRE(e == hipSuccess, DFM2H_ERR_HIP,
+                std::string("graph capture: ") + hipGetErrorString(e));
+  DFM2H_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+  B.sig = sig;
+  B.chunk = chunk;
+}
+
+// Launches the captured chunks until the device says done (polled one chunk
+// behind, so the device never idles) or nchunks ran; returns the status.
 template <typename T>
-static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const PcgArgs<T> &a,
-                    double tol, unsigned max_nitr, PcgScal &status) {
+static void run_chunks(PcgWork &W, PcgBufs<T> &B, unsigned nchunks, PcgScal &status) {
   hipStream_t s = W.s;
-  // scalars: tol and cap (a plain copy into the scalar block, not a memset)
-  PcgScal init{};
-  init.tol = tol;
-  init.max_nitr = (int)max_nitr;
-  DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_pcg_init<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalInit>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
-                     vec_grid(a.n), a.hist);
-  const unsigned nb0 = launch_sweeps(a, P, s);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalRPr0>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
-                     nb0, a.hist);
-  hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
-  check_launch("pcg prologue");
-
-  // graph of `chunk` iterations, recaptured only when buffers or schedule change
-  const unsigned chunk = std::max(1u, std::min(16u, max_nitr));
-  std::vector<uintptr_t> sig = {(uintptr_t)a.a_val, (uintptr_t)a.a_dia, (uintptr_t)a.f_val,
-                                (uintptr_t)a.f_dia, (uintptr_t)a.b_val, (uintptr_t)a.r,
-                                (uintptr_t)a.partials, (uintptr_t)a.partials2, (uintptr_t)a.hist, (uintptr_t)a.S,
-                                (uintptr_t)a.a_plan.col, (uintptr_t)a.f_plan.col,
-                                (uintptr_t)a.b_plan.col, (uintptr_t)chunk,
-                                (uintptr_t)P->fplan.group_ptr.size(),
-                                (uintptr_t)P->bplan.group_ptr.size(), (uintptr_t)a.nblk};
-  if (!B.exec || B.sig != sig) {
-    B.drop_graph();
-    DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (unsigned k = 0; k < chunk; ++k) launch_iteration(a, P, s);
-    hipError_t e = hipStreamEndCapture(s, &B.graph);
-    DFM2H_REQUIRE(e == hipSuccess, DFM2H_ERR_HIP,
-                  std::string("graph capture: ") + hipGetErrorString(e));
-    DFM2H_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
-    B.sig = sig;
-    B.chunk = chunk;
-  }
-  const unsigned nchunks = (max_nitr + chunk - 1) / chunk;
-  unsigned launched = 0;
-  bool finished = false;
-  for (unsigned c = 0; c < nchunks && !finished; ++c) {
-    DFM2H_HIP(hipGraphLaunch(B.exec, s));
-    DFM2H_HIP(hipMemcpyAsync(&W.host_status[c & 1], B.S.p, sizeof(PcgScal),
-                             hipMemcpyDeviceToHost, s));
-    DFM2H_HIP(hipEventRecord(W.ev_poll[c & 1], s));
-    ++launched;
-    if (c > 0) {  // poll one chunk behind so the device never idles
-      DFM2H_HIP(hipEventSynchronize(W.ev_poll[(c - 1) & 1]));
-      finished = W.host_status[(c - 1) & 1].done != 0;
-    }
-  }
-  DFM2H_HIP(hipMemcpyAsync(&W.host_status[0], B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, s));
-  DFM2H_HIP(hipStreamSynchronize(s));
-  status = W.host_status[0];
-  (void)launched;
-}
-
-// One mixed iteration: FP32 PCG step; when the FP32 residual has dropped by
-// delta since the last reliable update (or looks converged), fold x32 into
-// x64, recompute r = b - A x64 in FP64, test convergence in FP64, continue
-// the same Krylov sequence from the true residual.
-static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, hipStream_t s) {
-  const unsigned gs = slice_grid(a.a_nsl);
-  const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
-                     gs, a.hist);
-  hipLaunchKernelGGL(k_pcg_update<float>, dim3(gv), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalConvMixed>, dim3(1), dim3(kScalBlock), 0, s, a.S,
-                     a.partials2, gv, a.hist);
-  hipLaunchKernelGGL(k_mixed_fold, dim3(gv), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalReliable>, dim3(1), dim3(kScalBlock), 0, s, a.S,
-                     a.partials, gs, a.hist64);
-  const unsigned nb = launch_sweeps(a, P, s);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
-                     nb, a.hist);
-  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
-}
-
-static void run_pcg_mixed(dfm2h_ilu P, PcgWork &W, PcgBufs<float> &B, const PcgArgs<float> &a,
-                          double tol, double delta, unsigned max_outer, unsigned max_inner,
-                          PcgScal &status) {
-  hipStream_t s = W.s;
-  PcgScal init{};
-  init.tol = tol;
-  init.max_nitr = (int)max_inner;
-  init.delta2 = delta * delta;
-  init.max_upd = (int)max_outer;
-  DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
-  const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL(k_mixed_init, dim3(gv), dim3(kVecBlock), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalInitMixed>, dim3(1), dim3(kScalBlock), 0, s, a.S,
-                     a.partials, gv, a.hist64);
-  const unsigned nb0 = launch_sweeps(a, P, s);
-  hipLaunchKernelGGL(k_pcg_scalar<kScalRPr0>, dim3(1), dim3(kScalBlock), 0, s, a.S, a.partials,
-                     nb0, a.hist);
-  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(vec_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
-  check_launch("mixed prologue");
-  const unsigned chunk = std::max(1u, std::min(16u, max_inner));
-  std::vector<uintptr_t> sig = {(uintptr_t)a.a_val,  (uintptr_t)a.a_dia,   (uintptr_t)a.f_val,
-                                (uintptr_t)a.f_dia,  (uintptr_t)a.b_val,   (uintptr_t)a.r,
-                                (uintptr_t)a.partials, (uintptr_t)a.partials2, (uintptr_t)a.hist64, (uintptr_t)a.S,
-                                (uintptr_t)a.a64_val, (uintptr_t)a.a64_dia, (uintptr_t)a.x64,
-                                (uintptr_t)a.b64,    (uintptr_t)a.a_plan.col,
-                                (uintptr_t)a.f_plan.col, (uintptr_t)a.b_plan.col,
-                                (uintptr_t)chunk,    (uintptr_t)P->fplan.group_ptr.size(),
-                                (uintptr_t)P->bplan.group_ptr.size(), (uintptr_t)a.nblk};
-  if (!B.exec || B.sig != sig) {
-    B.drop_graph();
-    DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (unsigned k = 0; k < chunk; ++k) launch_mixed_iteration(a, P, s);
-    hipError_t e = hipStreamEndCapture(s, &B.graph);
-    DFM2H_REQUIRE(e == hipSuccess, DFM2H_ERR_HIP,
-                  std::string("graph capture: ") + hipGetErrorString(e));
-    DFM2H_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
-    B.sig = sig;
-    B.chunk = chunk;
-  }
-  const unsigned nchunks = (max_inner + chunk - 1) / chunk + 1;
   bool finished = false;
   for (unsigned c = 0; c < nchunks && !finished; ++c) {
     DFM2H_HIP(hipGraphLaunch(B.exec, s));
@@ -687,6 +794,81 @@ static void run_pcg_mixed(dfm2h_ilu P, PcgWork &W, PcgBufs<float> &B, const PcgA
   status = W.host_status[0];
 }
 
+template <typename T>
+static std::vector<uintptr_t> graph_sig(const PcgArgs<T> &a, dfm2h_bsr A, dfm2h_ilu P,
+                                        unsigned chunk) {
+  return {(uintptr_t)a.a_val,    (uintptr_t)a.a_dia,     (uintptr_t)a.f_val,
+          (uintptr_t)a.f_dia,    (uintptr_t)a.b_val,     (uintptr_t)a.dinv0,
+          (uintptr_t)a.r,        (uintptr_t)a.partials,  (uintptr_t)a.partials2,
+          (uintptr_t)a.partials3, (uintptr_t)a.hist,     (uintptr_t)a.S,
+          (uintptr_t)a.a64_val,  (uintptr_t)a.a64_dia,   (uintptr_t)a.x64,
+          (uintptr_t)a.b64,      (uintptr_t)a.hist64,    (uintptr_t)chunk,
+          (uintptr_t)A,          (uintptr_t)A->pattern_serial, (uintptr_t)P->pattern_serial,
+          (uintptr_t)P->work->fu.serial};
+}
+
+// Runs the fused PCG on B.r (holding b) into B.x; fills status.
+template <typename T>
+static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const PcgArgs<T> &a,
+                    double tol, unsigned max_nitr, PcgScal &status) {
+  hipStream_t s = W.s;
+  // scalars: tol and cap (a plain copy into the scalar block, not a memset)
+  PcgScal init{};
+  init.tol = tol;
+  init.max_nitr = (int)max_nitr;
+  DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
+  const unsigned gv = vec_grid(a.n);
+  hipLaunchKernelGGL(k_pcg_init<T>, dim3(gv), dim3(kVecBlock), 0, s, a);
+  launch_scalar<kScalInit>(s, a.S, a.partials, gv, nullptr, 0, a.hist);
+  const unsigned nb0 = launch_sweeps_full(a, P, s);
+  launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
+  hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  check_launch("pcg prologue");
+  const unsigned chunk = std::max(1u, std::min(16u, max_nitr));
+  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk, [&] { launch_iteration(a, P, s); });
+  run_chunks(W, B, (max_nitr + chunk - 1) / chunk, status);
+}
+
+// One mixed iteration: FP32 PCG step; when the FP32 residual has dropped by
+// delta since the last reliable update (or looks converged), recompute
+// r = b - A (x64 + x32) in FP64, test convergence in FP64 (in the beta
+// step), fold x32 into x64 (in the p update) and continue the same Krylov
+// sequence from the true residual.
+static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, hipStream_t s) {
+  const unsigned gs = slice_grid(a.a_nsl);
+  const unsigned gv = vec_grid(a.n);
+  hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
+  hipLaunchKernelGGL(k_pcg_update<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  launch_scalar<kScalConvMixed>(s, a.S, a.partials2, gs, nullptr, 0, a.hist);
+  hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, s, a);
+  launch_sweeps_fused(a, P, s);
+  launch_scalar<kScalBetaMixed>(s, a.S, a.partials, gs, a.partials3, P->work->fu.n3, a.hist64);
+  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
+}
+
+static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<float> &B,
+                          const PcgArgs<float> &a, double tol, double delta, unsigned max_outer,
+                          unsigned max_inner, PcgScal &status) {
+  hipStream_t s = W.s;
+  PcgScal init{};
+  init.tol = tol;
+  init.max_nitr = (int)max_inner;
+  init.delta2 = delta * delta;
+  init.max_upd = (int)max_outer;
+  DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
+  const unsigned gv = vec_grid(a.n);
+  hipLaunchKernelGGL(k_mixed_init, dim3(gv), dim3(kVecBlock), 0, s, a);
+  launch_scalar<kScalInitMixed>(s, a.S, a.partials, gv, nullptr, 0, a.hist64);
+  const unsigned nb0 = launch_sweeps_full(a, P, s);
+  launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
+  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  check_launch("mixed prologue");
+  const unsigned chunk = std::max(1u, std::min(16u, max_inner));
+  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk, [&] { launch_mixed_iteration(a, P, s); });
+  run_chunks(W, B, (max_inner + chunk - 1) / chunk + 1, status);
+}
+
 static void fence_in(PcgWork &W) {
   DFM2H_HIP(hipEventRecord(W.ev_in, stream()));
   DFM2H_HIP(hipStreamWaitEvent(W.s, W.ev_in, 0));
@@ -697,12 +879,14 @@ static void ensure_f32(dfm2h_bsr A, dfm2h_ilu P, hipStream_t s) {
   ensure_packed(A->plan, A->pk32, A->vcrs.p, A->vdia.p, A->version, s);
   ensure_packed(P->fplan, P->f32, P->vcrs.p, P->vdia.p, P->version, s);
   ensure_packed(P->bplan, P->b32, P->vcrs.p, nullptr, P->version, s);
+  ensure_dinv(A, P, P->work->fu.d32, s);
 }
 
 static void ensure_f64(dfm2h_bsr A, dfm2h_ilu P, hipStream_t s) {
   ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, s);
   ensure_packed(P->fplan, P->f64, P->vcrs.p, P->vdia.p, P->version, s);
   ensure_packed(P->bplan, P->b64, P->vcrs.p, nullptr, P->version, s);
+  ensure_dinv(A, P, P->work->fu.d64, s);
 }
 
 }  // namespace dfm2h
@@ -780,13 +964,14 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
     DFM2H_REQUIRE(r && x, DFM2H_ERR_INVALID, "null vector");
     PcgWork &W = work_of(P);
     PcgBufs<double> &B = W.f64;
+    ensure_fusion(A, P, W);
     ensure_bufs(B, A, P, max_nitr);
     const size_t nn = A->nblk;
     fence_in(W);
     hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s, B.r.p,
                        r, nn);
     ensure_f64(A, P, W.s);
-    const PcgArgs<double> a = make_args<double>(A, P, B, A->pk64, P->f64, P->b64);
+    const PcgArgs<double> a = make_args<double>(A, P, B, A->pk64, P->f64, P->b64, W.fu.d64);
     PcgScal st{};
     run_pcg(A, P, W, B, a, tol, max_nitr, st);
     hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, r,
@@ -817,6 +1002,7 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
     DFM2H_REQUIRE(max_outer >= 1 && max_inner >= 1, DFM2H_ERR_INVALID, "caps must be >= 1");
     PcgWork &W = work_of(P);
     PcgBufs<float> &B = W.f32;
+    ensure_fusion(A, P, W);
     ensure_bufs(B, A, P, max_inner);
     const size_t nn = A->nblk;
     if (W.b64.n != kVS * nn) {
@@ -831,17 +1017,17 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
     ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, W.s);
     hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s,
                        W.b64.p, b, nn);
-    PcgArgs<float> a = make_args<float>(A, P, B, A->pk32, P->f32, P->b32);
+    PcgArgs<float> a = make_args<float>(A, P, B, A->pk32, P->f32, P->b32, W.fu.d32);
     a.x64 = W.x64.p;
     a.b64 = W.b64.p;
     a.a64_val = A->pk64.val.p;
     a.a64_dia = A->pk64.dia.p;
     a.hist64 = W.hist64.p;
     PcgScal st{};
-    run_pcg_mixed(P, W, B, a, tol, delta, max_outer, max_inner, st);
-    hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x,
-                       W.x64.p, nn);
-    check_launch("k_unpad4");
+    run_pcg_mixed(A, P, W, B, a, tol, delta, max_outer, max_inner, st);
+    hipLaunchKernelGGL(k_mixed_out, dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x, W.x64.p, B.x.p,
+                       3 * nn);
+    check_launch("k_mixed_out");
     std::vector<double> h((size_t)st.nupd + 1);
     DFM2H_HIP(hipMemcpyAsync(h.data(), W.hist64.p, h.size() * sizeof(double),
                              hipMemcpyDeviceToHost, W.s));
@@ -871,12 +1057,15 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       PcgScal tst = st;
       tst.done = 0;
       tst.upd = 0;
+      tst.fold = 0;
       tst.tol = 0.0;
       tst.delta2 = 0.0;
       tst.max_nitr = 1 << 30;
       DFM2H_HIP(hipMemcpyAsync(B.S.p, &tst, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
       const unsigned gs = slice_grid(a.a_nsl), gv = vec_grid(a.n);
-      constexpr int kSeg = 8;  // spmv, alpha, update, conv, refine, fwd, bwd, beta+pupdate
+      const unsigned n3 = W.fu.n3;
+      // segments: spmv | alpha | update | conv (FP32) | refine (FP32) | fwd | bwd | beta+pupdate
+      constexpr int kSeg = 8;
       hipEvent_t ev[kSeg + 1];
       for (auto &e : ev) DFM2H_HIP(hipEventCreate(&e));
       double acc[kSeg] = {0};
@@ -884,43 +1073,36 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         DFM2H_HIP(hipEventRecord(ev[0], W.s));
         hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[1], W.s));
-        hipLaunchKernelGGL(k_pcg_scalar<kScalAlpha>, dim3(1), dim3(kScalBlock), 0, W.s, a.S,
-                           a.partials, gs, a.hist);
+        launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[2], W.s));
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(gv), dim3(kVecBlock), 0, W.s, a);
+        hipLaunchKernelGGL(k_pcg_update<T>, dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[3], W.s));
         if constexpr (sizeof(T) == 4)
-          hipLaunchKernelGGL(k_pcg_scalar<kScalConvMixed>, dim3(1), dim3(kScalBlock), 0, W.s,
-                             a.S, a.partials2, gv, a.hist);
-        else
-          hipLaunchKernelGGL(k_pcg_scalar<kScalConv>, dim3(1), dim3(kScalBlock), 0, W.s, a.S,
-                             a.partials2, gv, a.hist);
+          launch_scalar<kScalConvMixed>(W.s, a.S, a.partials2, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[4], W.s));
-        if constexpr (sizeof(T) == 4) {  // the refinement kernels, no-ops unless a refinement is due
-          const PcgArgs<float> &af = a;
-          hipLaunchKernelGGL(k_mixed_fold, dim3(gv), dim3(kVecBlock), 0, W.s, af);
-          hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, W.s, af);
-          hipLaunchKernelGGL(k_pcg_scalar<kScalReliable>, dim3(1), dim3(kScalBlock), 0, W.s,
-                             a.S, a.partials, gs, a.hist64);
-        }
+        if constexpr (sizeof(T) == 4)  // a no-op unless a refinement is due
+          hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[5], W.s));
-        for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
+        for (size_t l = 1; l + 1 < P->fplan.group_ptr.size(); ++l) {
           const unsigned nsl = P->fplan.group_slices(l);
-          hipLaunchKernelGGL(k_pcg_fwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, W.s, a,
-                             P->fplan.group_ptr[l], nsl);
+          if (W.fu.fwd_off[l] != ~0u)
+            hipLaunchKernelGGL((k_pcg_fwd<T, true>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0,
+                               W.s, a, P->fplan.group_ptr[l], nsl, a.partials3 + W.fu.fwd_off[l]);
+          else
+            hipLaunchKernelGGL((k_pcg_fwd<T, false>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0,
+                               W.s, a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
         }
         DFM2H_HIP(hipEventRecord(ev[6], W.s));
-        unsigned off = 0;
-        for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
+        for (size_t l = 1; l + 1 < P->bplan.group_ptr.size(); ++l) {
           const unsigned nsl = P->bplan.group_slices(l);
-          const unsigned g = slice_grid(nsl);
-          hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(g), dim3(kPackBlock), 0, W.s, a,
-                             P->bplan.group_ptr[l], nsl, off);
-          off += g;
+          hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, W.s, a,
+                             P->bplan.group_ptr[l], nsl, a.partials3 + W.fu.bwd_off[l]);
         }
         DFM2H_HIP(hipEventRecord(ev[7], W.s));
-        hipLaunchKernelGGL(k_pcg_scalar<kScalBeta>, dim3(1), dim3(kScalBlock), 0, W.s, a.S,
-                           a.partials, off, a.hist);
+        if constexpr (sizeof(T) == 4)
+          launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials, gs, a.partials3, n3, a.hist64);
+        else
+          launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
         hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, W.s, a, 1);
         DFM2H_HIP(hipEventRecord(ev[8], W.s));
         DFM2H_HIP(hipEventSynchronize(ev[8]));
@@ -933,24 +1115,22 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       }
       check_launch("kernel timing");
       for (auto &e : ev) (void)hipEventDestroy(e);
-      // ms[] = spmv, update, fwd sweep, bwd sweep, pupdate (incl. its beta
-      // step), scalar step (alpha), refinement kernels (FP32 only)
-      ms[0] = acc[0] / nrep;
-      ms[1] = acc[2] / nrep;
-      ms[2] = acc[5] / nrep;
-      ms[3] = acc[6] / nrep;
-      ms[4] = acc[7] / nrep;
-      ms[5] = acc[1] / nrep;
-      ms[6] = acc[4] / nrep;
-      ms[7] = acc[3] / nrep;
+      ms[0] = acc[0] / nrep;  // SpMV
+      ms[1] = acc[2] / nrep;  // update + forward level 0
+      ms[2] = acc[5] / nrep;  // forward levels >= 1
+      ms[3] = acc[6] / nrep;  // backward levels >= 1
+      ms[4] = acc[7] / nrep;  // (stop test +) beta step + p update
+      ms[5] = acc[1] / nrep;  // alpha step
+      ms[6] = acc[4] / nrep;  // reliable-update residual (FP32)
+      ms[7] = acc[3] / nrep;  // update decision (FP32)
       st.done = 1;
       DFM2H_HIP(hipMemcpyAsync(B.S.p, &st, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
       DFM2H_HIP(hipStreamSynchronize(W.s));
     };
     if (fp32) {
-      DFM2H_REQUIRE(W.f32.S.p && A->pk32.val.p && W.x64.p, DFM2H_ERR_STATE,
+      DFM2H_REQUIRE(W.f32.S.p && A->pk32.val.p && W.x64.p && W.fu.A == A, DFM2H_ERR_STATE,
                     "run a mixed solve first");
-      PcgArgs<float> a = make_args<float>(A, P, W.f32, A->pk32, P->f32, P->b32);
+      PcgArgs<float> a = make_args<float>(A, P, W.f32, A->pk32, P->f32, P->b32, W.fu.d32);
       a.x64 = W.x64.p;
       a.b64 = W.b64.p;
       a.a64_val = A->pk64.val.p;
@@ -958,8 +1138,9 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       a.hist64 = W.hist64.p;
       run(float{}, W.f32, a);
     } else {
-      DFM2H_REQUIRE(W.f64.S.p && A->pk64.val.p, DFM2H_ERR_STATE, "run an FP64 solve first");
-      PcgArgs<double> a = make_args<double>(A, P, W.f64, A->pk64, P->f64, P->b64);
+      DFM2H_REQUIRE(W.f64.S.p && A->pk64.val.p && W.fu.A == A, DFM2H_ERR_STATE,
+                    "run an FP64 solve first");
+      PcgArgs<double> a = make_args<double>(A, P, W.f64, A->pk64, P->f64, P->b64, W.fu.d64);
       run(double{}, W.f64, a);
     }
   });

@@ -335,7 +335,7 @@ def main():
             kern[prec][k] = {"ms_per_call": round(ms[k], 5), "launches_per_call": launches,
                              "bytes_per_call": b,
                              "gbs": round(b / (ms[k] * 1e-3) / 1e9, 1)}
-        for k in ("alpha_step",) + (("refine_step", "conv_step") if fp32 else ()):
+        for k in ("alpha_step",):
             kern[prec][k + "_ms"] = round(ms[k], 5)
     # dominant kernel of the headline (FP32 + refinement) solve: largest time per iteration
     dom = max(("spmv", "fwd_sweep", "bwd_sweep", "update", "pupdate"),

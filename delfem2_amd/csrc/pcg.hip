@@ -40,12 +40,10 @@ struct PcgScal {
   double sq_b;    // |b|^2 in FP64
   double rmax2;   // FP64 |r|^2 at the last reliable update
   double delta2;  // update when the FP32 |r|^2 fell below delta2 * rmax2
-  int upd;        // this iteration folds x32 into x64 and recomputes r in FP64
+  int upd;        // the next SpMV recomputes r in FP64, the next update folds x32
   int nupd;       // FP64 residuals recorded after the initial |b|
   int max_upd;
   int final_;     // iteration cap reached: stop after this update
-  int fold;       // the p update folds x32 into x64 (set by a reliable update)
-  int pad2;
 };
 
 constexpr int kVecBlock = 256;
@@ -191,21 +189,23 @@ enum ScalMode {
   kScalAlpha = 1,
   kScalConvBeta = 2,   // FP64 iteration: |r| history / stop test, then beta
   kScalRPr0 = 3,
-  kScalInitMixed = 5,  // |b|^2: outer history[0], rmax2
-  kScalConvMixed = 6,  // FP32 |r|^2: decide a reliable update
-  kScalBetaMixed = 7   // FP64 |b - A x|^2 when an update ran: outer history,
-                       // stop test; then beta
+  kScalInitMixed = 5,   // |b|^2: outer history[0], rmax2
+  kScalAlphaMixed = 6,  // FP64 |b - A x|^2 when a reliable update ran: outer
+                        // history, stop test; then alpha
+  kScalBetaMixed = 7    // FP32 |r|^2: decide a reliable update for the next
+                        // iteration; then beta
 };
 
-// pa/na: first partial set, pb/nb: second (ConvBeta, BetaMixed)
+// pa/na: first partial set, pb/nb: second (ConvBeta, AlphaMixed, BetaMixed)
 template <int MODE>
 __global__ void __launch_bounds__(kScalBlock)
 k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
              const double *__restrict__ pb, unsigned nb, double *hist) {
   if (MODE != kScalInit && MODE != kScalInitMixed && S->done) return;
-  const bool upd = MODE == kScalBetaMixed ? S->upd != 0 : true;  // block-uniform
-  const double t = upd ? reduce_partials<kScalBlock>(pa, na) : 0.0;
-  const double t2 = (MODE == kScalConvBeta || MODE == kScalBetaMixed)
+  const bool upd = S->upd != 0;  // block-uniform
+  const double t = reduce_partials<kScalBlock>(pa, na);
+  const double t2 = (MODE == kScalConvBeta || MODE == kScalBetaMixed ||
+                     (MODE == kScalAlphaMixed && upd))
                         ? reduce_partials<kScalBlock>(pb, nb)
                         : 0.0;
   if (threadIdx.x != 0) return;
@@ -254,7 +254,6 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
     S->it = 0;
     S->nupd = 0;
     S->upd = 0;
-    S->fold = 0;
     S->final_ = 0;
     S->converged = 0;
     S->nan = 0;
@@ -266,50 +265,77 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
       S->done = 1;
       S->nan = 1;
     }
-  } else if (MODE == kScalConvMixed) {
-    const int it = S->it + 1;
-    S->it = it;
-    S->sq = t;
-    S->fold = 0;
-    if (!(t == t) || t > 1.0e308) {
-      S->done = 1;
-      S->nan = 1;
-    } else if (t < S->delta2 * S->rmax2 || sqrt(t / S->sq_b) < S->tol || it >= S->max_nitr) {
-      S->upd = 1;
-      S->final_ = it >= S->max_nitr ? 1 : 0;
-    }
-  } else {  // kScalBetaMixed
+  } else if (MODE == kScalAlphaMixed) {
+    // t: (p, Ap); t2: FP64 |b - A x|^2 of the reliable update, if one ran
     if (upd) {
       const int k = S->nupd + 1;
       S->nupd = k;
-      hist[k] = sqrt(t);
-      S->upd = 0;
-      S->fold = 1;
-      S->rmax2 = t;
-      if (sqrt(t / S->sq_b) < S->tol) {
+      hist[k] = sqrt(t2);
+      S->rmax2 = t2;
+      if (sqrt(t2 / S->sq_b) < S->tol) {
         S->done = 1;
         S->converged = 1;
-      } else if (!(t == t) || t > 1.0e308) {
+      } else if (!(t2 == t2) || t2 > 1.0e308) {
         S->done = 1;
         S->nan = 1;
       } else if (S->final_ || k >= S->max_upd) {
         S->done = 1;
       }
     }
-    if (!S->done) {
-      S->beta = t2 / S->rPr;
-      S->rPr = t2;
+    S->alpha = S->rPr / t;
+  } else {  // kScalBetaMixed: t: FP32 |r|^2 after the update, t2: (r, Pr)
+    const int it = S->it + 1;
+    S->it = it;
+    S->sq = t;
+    S->upd = 0;
+    if (!(t == t) || t > 1.0e308) {
+      S->done = 1;
+      S->nan = 1;
+      return;
     }
+    if (t < S->delta2 * S->rmax2 || sqrt(t / S->sq_b) < S->tol || it >= S->max_nitr) {
+      S->upd = 1;
+      S->final_ = it >= S->max_nitr ? 1 : 0;
+    }
+    S->beta = t2 / S->rPr;
+    S->rPr = t2;
   }
 }
 
+// Reliable update, row part: r_i = b_i - (A (x64 + x32))_i in FP64 (FP64
+// matrix values), rounded into r32; returns r_i^2.
+__device__ inline double residual_row(const PcgArgs<float> &a, const SliceLane &L, unsigned i) {
+  const unsigned K = a.a_plan.len[L.s * kSliceRows + L.r];
+  const unsigned q0 = a.a_plan.off[L.s];
+  double acc = 0.0;
+  for (unsigned k = 0; k < K; ++k) {
+    const size_t q = q0 + k;
+    const size_t j = kVS * (size_t)a.a_plan.col[q * kSliceRows + L.r];
+    const Trip<double> v = load_trip(a.a64_val, q, L.lane);
+    const double x0 = a.x64[j] + (double)a.x[j];
+    const double x1 = a.x64[j + 1] + (double)a.x[j + 1];
+    const double x2 = a.x64[j + 2] + (double)a.x[j + 2];
+    acc += v.a * x0 + v.b * x1 + v.c * x2;
+  }
+  const size_t ii = kVS * (size_t)i;
+  acc += dia_row_dot(a.a64_dia, L, a.x64[ii] + (double)a.x[ii],
+                     a.x64[ii + 1] + (double)a.x[ii + 1], a.x64[ii + 2] + (double)a.x[ii + 2]);
+  const double r = a.b64[ii + L.c] - acc;
+  a.r[ii + L.c] = (float)r;
+  return r * r;
+}
+
+// Ap -> Pr, partial (p, Ap).  FP32 form with a reliable update due: also
+// r = b - A x in FP64 (partials2: |r|^2), for the update kernel that
+// follows.
 template <typename T>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T> a) {
   if (a.S->done) return;
+  const bool refine = sizeof(T) == 4 && a.S->upd;  // grid-uniform
   const SliceLane L = slice_lane(0);
   const bool slice_ok = L.s < a.a_nsl && L.c < 3;
   const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  double part = 0.0;
+  double part = 0.0, part2 = 0.0;
   if (i != ~0u) {
     T acc = 0;
     acc = sliced_row_sum<T, false, unroll_spmv<T>(), kVS>(
@@ -319,9 +345,15 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T> a) {
     acc += dia_row_dot(a.a_dia, L, pi.a, pi.b, pi.c);
     a.Pr[kVS * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
     part = (double)a.p[kVS * (size_t)i + L.c] * (double)acc;
+    if constexpr (sizeof(T) == 4)
+      if (refine) part2 = residual_row(a, L, i);
   }
   part = block_reduce_sum<kPackBlock>(part);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = part;
+  if (refine) part2 = block_reduce_sum<kPackBlock>(part2);
+  if (threadIdx.x == 0) {
+    a.partials[blockIdx.x] = part;
+    if (refine) a.partials2[blockIdx.x] = part2;
+  }
 }
 
 // Forward level 0 for the level-0 rows of matrix slice L.s, given every
@@ -345,6 +377,7 @@ __device__ inline void fwd_level0(const PcgArgs<T> &a, const SliceLane &L, unsig
 
 // r += -alpha Ap, x += alpha p, partial (r,r); then forward level 0.
 // One wavefront per matrix slice (rows in order, components in lanes).
+// FP32 form right after a reliable update: x32 is first folded into x64.
 template <typename T>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T> a) {
   if (a.S->done) return;
@@ -352,13 +385,19 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T> a) {
   const SliceLane L = slice_lane(0);
   const bool sl = L.s < a.a_nsl;
   const unsigned i = sl && L.c < 3 ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  const bool fold = sizeof(T) == 4 && a.S->upd;  // a reliable update just ran
   double s2 = 0.0, srp = 0.0;
   T rn = 0;
   if (i != ~0u) {
     const size_t e = kVS * (size_t)i + L.c;
     rn = (T)((double)a.r[e] + (-alpha) * (double)a.Pr[e]);
     a.r[e] = rn;
-    a.x[e] = (T)((double)a.x[e] + alpha * (double)a.p[e]);
+    if (fold) {  // x64 += x32, x32 = alpha p
+      a.x64[e] += (double)a.x[e];
+      a.x[e] = (T)(alpha * (double)a.p[e]);
+    } else {
+      a.x[e] = (T)((double)a.x[e] + alpha * (double)a.p[e]);
+    }
     s2 = (double)rn * (double)rn;
   }
   if (sl) fwd_level0(a, L, i, rn, srp);
@@ -416,23 +455,17 @@ k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
   if (threadIdx.x == 0) dst[blockIdx.x] = part;
 }
 
-// p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue).  FP32 form: after
-// a reliable update also folds x64 += x32, x32 = 0.
+// p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue)
 template <typename T>
 __global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T> a, int mode) {
   if (a.S->done) return;
   const double beta = a.S->beta;
-  const bool fold = sizeof(T) == 4 && a.S->fold;
   for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
        i += (size_t)gridDim.x * kVecBlock) {
     if (mode == 0)
       a.p[i] = a.Pr[i];
     else
       a.p[i] = (T)(beta * (double)a.p[i] + (double)a.Pr[i]);
-    if (fold) {
-      a.x64[i] += (double)a.x[i];
-      a.x[i] = (T)0;
-    }
   }
 }
 
@@ -451,48 +484,7 @@ __global__ void __launch_bounds__(kVecBlock) k_mixed_init(PcgArgs<float> a) {
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }
 
-// Reliable update (when due): r = b - A (x64 + x32) in FP64 (sliced A, FP64
-// values), rounded into r32, partial |r|^2; then forward level 0 from the
-// new r (overwriting the update kernel's).  x32 is folded into x64 later,
-// by the p update.
-__global__ void __launch_bounds__(kPackBlock) k_mixed_residual(PcgArgs<float> a) {
-  if (a.S->done || !a.S->upd) return;
-  const SliceLane L = slice_lane(0);
-  const bool sl = L.s < a.a_nsl;
-  const unsigned i = sl && L.c < 3 ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  double part = 0.0, srp = 0.0;
-  float rf = 0.0f;
-  if (i != ~0u) {
-    const unsigned K = a.a_plan.len[L.s * kSliceRows + L.r];
-    const unsigned q0 = a.a_plan.off[L.s];
-    double acc = 0.0;
-    for (unsigned k = 0; k < K; ++k) {
-      const size_t q = q0 + k;
-      const unsigned j = a.a_plan.col[q * kSliceRows + L.r];
-      const Trip<double> v = load_trip(a.a64_val, q, L.lane);
-      const double x0 = a.x64[kVS * (size_t)j] + (double)a.x[kVS * (size_t)j];
-      const double x1 = a.x64[kVS * (size_t)j + 1] + (double)a.x[kVS * (size_t)j + 1];
-      const double x2 = a.x64[kVS * (size_t)j + 2] + (double)a.x[kVS * (size_t)j + 2];
-      acc += v.a * x0 + v.b * x1 + v.c * x2;
-    }
-    const size_t ii = kVS * (size_t)i;
-    acc += dia_row_dot(a.a64_dia, L, a.x64[ii] + (double)a.x[ii],
-                       a.x64[ii + 1] + (double)a.x[ii + 1], a.x64[ii + 2] + (double)a.x[ii + 2]);
-    const double r = a.b64[ii + L.c] - acc;
-    rf = (float)r;
-    a.r[ii + L.c] = rf;
-    part = r * r;
-  }
-  if (sl) fwd_level0(a, L, i, rf, srp);
-  part = block_reduce_sum<kPackBlock>(part);
-  srp = block_reduce_sum<kPackBlock>(srp);
-  if (threadIdx.x == 0) {
-    a.partials[blockIdx.x] = part;
-    a.partials3[a.upd_part_off + blockIdx.x] = srp;
-  }
-}
-
-// x = x64 + x32 (the last fold may not have run)
+// x = x64 + x32 (the solution at the last reliable update, or the cap)
 __global__ void k_mixed_out(double *__restrict__ x, const double *__restrict__ x64,
                             const float *__restrict__ x32, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -829,21 +821,21 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
   run_chunks(W, B, (max_nitr + chunk - 1) / chunk, status);
 }
 
-// One mixed iteration: FP32 PCG step; when the FP32 residual has dropped by
-// delta since the last reliable update (or looks converged), recompute
-// r = b - A (x64 + x32) in FP64, test convergence in FP64 (in the beta
-// step), fold x32 into x64 (in the p update) and continue the same Krylov
-// sequence from the true residual.
+// One mixed iteration: an FP32 PCG step whose beta step also decides
+// whether the next iteration starts with a reliable update (the FP32
+// residual fell by delta since the last one, looks converged, or the cap is
+// reached).  That update runs inside the next SpMV: r = b - A (x64 + x32) in
+// FP64, tested in FP64 by the alpha step (stop when |r| / |b| < tol); the
+// r/x update then folds x32 into x64 and the same Krylov sequence goes on
+// from the true residual.
 static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, hipStream_t s) {
   const unsigned gs = slice_grid(a.a_nsl);
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
+  launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64);
   hipLaunchKernelGGL(k_pcg_update<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  launch_scalar<kScalConvMixed>(s, a.S, a.partials2, gs, nullptr, 0, a.hist);
-  hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, s);
-  launch_scalar<kScalBetaMixed>(s, a.S, a.partials, gs, a.partials3, P->work->fu.n3, a.hist64);
+  launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, gs, a.partials3, P->work->fu.n3, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
@@ -1057,14 +1049,13 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       PcgScal tst = st;
       tst.done = 0;
       tst.upd = 0;
-      tst.fold = 0;
       tst.tol = 0.0;
       tst.delta2 = 0.0;
       tst.max_nitr = 1 << 30;
       DFM2H_HIP(hipMemcpyAsync(B.S.p, &tst, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
       const unsigned gs = slice_grid(a.a_nsl), gv = vec_grid(a.n);
       const unsigned n3 = W.fu.n3;
-      // segments: spmv | alpha | update | conv (FP32) | refine (FP32) | fwd | bwd | beta+pupdate
+      // segments: spmv | alpha | update | (two empty) | fwd | bwd | beta+pupdate
       constexpr int kSeg = 8;
       hipEvent_t ev[kSeg + 1];
       for (auto &e : ev) DFM2H_HIP(hipEventCreate(&e));
@@ -1073,15 +1064,14 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         DFM2H_HIP(hipEventRecord(ev[0], W.s));
         hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[1], W.s));
-        launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
+        if constexpr (sizeof(T) == 4)
+          launch_scalar<kScalAlphaMixed>(W.s, a.S, a.partials, gs, a.partials2, gs, a.hist64);
+        else
+          launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[2], W.s));
         hipLaunchKernelGGL(k_pcg_update<T>, dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[3], W.s));
-        if constexpr (sizeof(T) == 4)
-          launch_scalar<kScalConvMixed>(W.s, a.S, a.partials2, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[4], W.s));
-        if constexpr (sizeof(T) == 4)  // a no-op unless a refinement is due
-          hipLaunchKernelGGL(k_mixed_residual, dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[5], W.s));
         for (size_t l = 1; l + 1 < P->fplan.group_ptr.size(); ++l) {
           const unsigned nsl = P->fplan.group_slices(l);
@@ -1100,7 +1090,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         }
         DFM2H_HIP(hipEventRecord(ev[7], W.s));
         if constexpr (sizeof(T) == 4)
-          launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials, gs, a.partials3, n3, a.hist64);
+          launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
         else
           launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
         hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, W.s, a, 1);
@@ -1121,8 +1111,8 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       ms[3] = acc[6] / nrep;  // backward levels >= 1
       ms[4] = acc[7] / nrep;  // (stop test +) beta step + p update
       ms[5] = acc[1] / nrep;  // alpha step
-      ms[6] = acc[4] / nrep;  // reliable-update residual (FP32)
-      ms[7] = acc[3] / nrep;  // update decision (FP32)
+      ms[6] = acc[4] / nrep;  // (no separate refinement kernel: 0)
+      ms[7] = acc[3] / nrep;  // (no separate decision step: 0)
       st.done = 1;
       DFM2H_HIP(hipMemcpyAsync(B.S.p, &st, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
       DFM2H_HIP(hipStreamSynchronize(W.s));

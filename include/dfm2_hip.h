@@ -210,10 +210,10 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
  * FP32 iteration); convergence and refinement are disabled meanwhile.
  * ms[8] = { SpMV, r/x update with forward level 0, forward levels >= 1,
  * backward levels >= 1 (backward level 0 has no work: its (r, Pr) share is
- * summed in the forward sweep), stop test (FP64) + beta step + p update,
- * alpha step, reliable-update residual (FP32; a no-op unless due), update
- * decision (FP32) }.  The kernels run on the solver's scratch vectors;
- * nothing the caller owns is touched. */
+ * summed in the forward sweep), stop test + beta step + p update, alpha
+ * step, 0, 0 } (the FP32 reliable-update residual runs inside the SpMV when
+ * due, its decision inside the beta step).  The kernels run on the solver's
+ * scratch vectors; nothing the caller owns is touched. */
 int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep,
                            double *ms);
 

@@ -354,6 +354,11 @@ def main():
         if kname in t:
             traffic = t[kname]
 
+    shapes = {}
+    for prec, fp32 in (("fp32", True), ("fp64", False)):
+        v, name, tms = G.ilu.sweep_shape(fp32)
+        shapes[prec] = {"shape": name, "tune_ms_per_sweep_pair": [round(t, 5) for t in tms]}
+
     b_iter64 = pcg_iter_bytes(nb, nnzb, 8, n0, nu0)
     t_it64 = t_fp64 / max(it64, 1)
 
@@ -393,6 +398,7 @@ def main():
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": int(launch_bytes), "avg_launch_ms": round(launch_ms, 5)},
         "kernels": kern,
+        "sweep_shape": shapes,
         "cpu_baseline": cpu,
         "setup_s": round(G.t_setup, 3),
         "assembly_ms": round(t_asm * 1e3, 3),

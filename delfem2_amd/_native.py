@@ -87,6 +87,8 @@ SIGNATURES = {
     "dfm2h_solve_pcg_mixed": [_vp, _vp, _vp, _vp, _d, _d, _u, _u, _vp, _sz, _psz, _psz],
     "dfm2h_adjoint_grad_tet": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "dfm2h_pcg_kernel_times": [_vp, _vp, _i, _u, _vp],
+    "dfm2h_pcg_set_shape": [_vp, _i, _i],
+    "dfm2h_pcg_get_shape": [_vp, _i, _pi, C.POINTER(C.c_char_p), _vp, _sz, _psz],
     "dfm2h_batch_create": [_pvp, _vp, _vp, _vp, _sz],
     "dfm2h_batch_destroy": [_vp],
     "dfm2h_batch_info": [_vp, _psz, _psz, _pu, _pu],

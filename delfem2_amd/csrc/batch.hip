@@ -25,9 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
-#include <map>
 #include <memory>
-#include <mutex>
 
 namespace dfm2h {
 namespace {
@@ -647,54 +645,7 @@ LaunchFn<T> iteration_fn(int v) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// autotuner cache: (DOF bucket = floor(log2 ndof), fp32) -> variant.
-// In-process map; when DFM2H_TUNE_CACHE names a file, entries are read from
-// it once and every new entry is appended ("bucket fp32 variant ms").
-
-std::mutex g_tune_mu;
-std::map<std::pair<int, int>, int> g_tune;
-bool g_tune_loaded = false;
-
-const char *tune_file() { return std::getenv("DFM2H_TUNE_CACHE"); }
-
-void tune_load_locked() {
-  if (g_tune_loaded) return;
-  g_tune_loaded = true;
-  const char *f = tune_file();
-  if (!f) return;
-  FILE *fp = std::fopen(f, "r");
-  if (!fp) return;
-  int b, p, v;
-  double ms;
-  while (std::fscanf(fp, "%d %d %d %lf", &b, &p, &v, &ms) == 4)
-    if (v >= 0 && v < kNumVariants) g_tune[{b, p}] = v;
-  std::fclose(fp);
-}
-
-int tune_lookup(int bucket, int fp32) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  tune_load_locked();
-  auto it = g_tune.find({bucket, fp32});
-  return it == g_tune.end() ? -1 : it->second;
-}
-
-void tune_store(int bucket, int fp32, int v, double ms) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  tune_load_locked();
-  g_tune[{bucket, fp32}] = v;
-  if (const char *f = tune_file()) {
-    if (FILE *fp = std::fopen(f, "a")) {
-      std::fprintf(fp, "%d %d %d %.6f\n", bucket, fp32, v, ms);
-      std::fclose(fp);
-    }
-  }
-}
-
-int dof_bucket(dfm2h_batch B) {
-  const double n = 3.0 * (double)B->nblk;
-  return n < 1.0 ? 0 : (int)std::floor(std::log2(n));
-}
+int batch_bucket(dfm2h_batch B) { return dof_bucket(3 * (size_t)B->nblk); }
 
 // values packed for the batch plans, from the current A and P
 template <typename T>
@@ -809,7 +760,7 @@ int autotune(dfm2h_batch B, BatchBufs<T> &S, double *ms, size_t ms_cap) {
     }
   }
   S.drop_graph();
-  tune_store(dof_bucket(B), sizeof(T) == 4 ? 1 : 0, best, best_ms);
+  tune_store("batch", batch_bucket(B), sizeof(T) == 4 ? 1 : 0, best, best_ms);
   return best;
 }
 
@@ -984,7 +935,7 @@ int dfm2h_batch_solve(dfm2h_batch B, const double *b, double *x, double tol, int
     B->has_b = true;
     const int p = fp32 ? 1 : 0;
     int v = B->variant[p];
-    if (v < 0) v = tune_lookup(dof_bucket(B), p);
+    if (v < 0) v = tune_lookup("batch", batch_bucket(B), p, kNumVariants);
     if (fp32) {
       if (v < 0) v = autotune<float>(B, B->s32, nullptr, 0);
       const BArgs<float> a = prepare<float>(B, B->s32);

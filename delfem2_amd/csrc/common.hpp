@@ -68,6 +68,15 @@ hipStream_t stream();
 // checks the last kernel launch
 void check_launch(const char *what);
 
+// Autotuner cache: (solver kind, floor(log2 ndof), fp32) -> launch variant.
+// Kept for the process; when the environment variable DFM2H_TUNE_CACHE
+// names a file, entries are read from it once and new ones appended
+// ("kind bucket fp32 variant ms" per line).  Lookup returns -1 when absent
+// or when the stored variant is not below n_variants.
+int tune_lookup(const char *kind, int bucket, int fp32, int n_variants);
+void tune_store(const char *kind, int bucket, int fp32, int variant, double ms);
+int dof_bucket(size_t ndof);
+
 // ---------------------------------------------------------------------------
 // device buffer (RAII over hipMalloc)
 

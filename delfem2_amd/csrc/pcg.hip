@@ -409,25 +409,102 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T> a) {
   }
 }
 
+// Sweep launch shapes: SPLIT wavefronts per slice (SPLIT = 2: the two
+// wavefronts take the even / odd blocks of every row and add their sums
+// through LDS, halving each wavefront's chain of dependent loads), U blocks
+// per load round.  With SPLIT = 1 the per-component sums run in stored
+// block order, exactly as the reference's loops.
+template <int SPLIT>
+__device__ inline SliceLane sweep_lane(unsigned slice0, unsigned &half, unsigned &slot) {
+  constexpr unsigned SPB = kPackBlock / kWave / SPLIT;  // slices per workgroup
+  const unsigned w = threadIdx.x / kWave;
+  half = SPLIT == 1 ? 0u : w / SPB;
+  slot = w % SPB;
+  SliceLane L;
+  L.lane = threadIdx.x & (kWave - 1);
+  L.s = slice0 + blockIdx.x * SPB + slot;
+  L.c = L.lane / kSliceRows;
+  L.r = L.lane - L.c * kSliceRows;
+  return L;
+}
+
+// acc (-)= sum over the row's blocks k = h, h + H, h + 2H, ... (H halves)
+template <typename T, int U, int H>
+__device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, unsigned K,
+                                        unsigned kmax, unsigned q0,
+                                        const unsigned *__restrict__ col,
+                                        const T *__restrict__ val, const T *x) {
+  for (unsigned k0 = h; k0 < kmax; k0 += H * U) {
+    unsigned j[U];
+    Trip<T> v[U], xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned k = k0 + H * u;
+      if (k < K) {
+        const size_t q = q0 + k;
+        j[u] = col[q * kSliceRows + L.r];
+        v[u] = load_trip(val, q, L.lane);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + H * u < K) xv[u] = load_node<T, kVS>(x, j[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + H * u < K) acc -= v[u].a * xv[u].a + v[u].b * xv[u].b + v[u].c * xv[u].c;
+  }
+  return acc;
+}
+
+// the halves' sums meet in half 0
+template <typename T, int SPLIT>
+__device__ inline T join_halves(T t, unsigned half, unsigned slot, unsigned lane) {
+  if (SPLIT == 1) return t;
+  constexpr unsigned SPB = kPackBlock / kWave / SPLIT;
+  __shared__ T xs[SPB][kWave];
+  if (half == 1) xs[slot][lane] = t;
+  __syncthreads();
+  return half == 0 ? t + xs[slot][lane] : t;
+}
+
+template <int SPLIT>
+inline unsigned sweep_grid(unsigned nsl) {
+  return std::max(1u, ceil_div(nsl, kPackBlock / kWave / SPLIT));
+}
+
 // forward level >= 1: Pr = D^-1 (r - L Pr) on the level's slices (reads r
 // directly: the reference's copy Pr = r is fused away).  PART: the level
 // holds rows with no upper blocks; their (r, Pr) is summed here.
-template <typename T, bool PART>
+template <typename T, bool PART, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
 k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
-  const SliceLane L = slice_lane(slice0);
+  unsigned half, slot;
+  const SliceLane L = sweep_lane<SPLIT>(slice0, half, slot);
   const bool sl = L.s < slice0 + nsl;
-  if (!PART && !sl) return;  // whole wavefront
+  if (SPLIT == 1 && !PART && !sl) return;  // whole wavefront
   const unsigned i = sl && L.c < 3 ? a.f_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  T out = 0;
-  if (sl)
-    sliced_fwd_row<T, kVS>(L, i != ~0u, i, a.f_plan, a.f_val, a.f_dia, (const T *)a.r, a.Pr,
-                           out);
+  T t = 0;
+  if (i != ~0u) {
+    if (half == 0) t = a.r[kVS * (size_t)i + L.c];
+    t = sliced_row_sub_part<T, U, SPLIT>(t, L, half, a.f_plan.len[L.s * kSliceRows + L.r],
+                                         a.f_plan.kmax[L.s], a.f_plan.off[L.s], a.f_plan.col,
+                                         a.f_val, a.Pr);
+  }
+  t = join_halves<T, SPLIT>(t, half, slot, L.lane);
+  double part = 0.0;
+  if (half == 0 && sl) {
+    const T t0 = __shfl(t, (int)L.r, kWave);
+    const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
+    const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
+    if (i != ~0u) {
+      const T out = dia_row_dot(a.f_dia, L, t0, t1, t2);
+      a.Pr[kVS * (size_t)i + L.c] = out;
+      if (PART && ((a.fumask[L.s] >> L.r) & 1u))
+        part = (double)a.r[kVS * (size_t)i + L.c] * (double)out;
+    }
+  }
   if (PART) {
-    double part = 0.0;
-    if (i != ~0u && ((a.fumask[L.s] >> L.r) & 1u))
-      part = (double)a.r[kVS * (size_t)i + L.c] * (double)out;
     part = block_reduce_sum<kPackBlock>(part);
     if (threadIdx.x == 0) dst[blockIdx.x] = part;
   }
@@ -435,19 +512,24 @@ k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
 
 // backward level + partial (r, Pr) of the rows it finalises, written at
 // dst[workgroup]
-template <typename T>
+template <typename T, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
 k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
-  const SliceLane L = slice_lane(slice0);
-  const bool slice_ok = L.s < slice0 + nsl && L.c < 3;
-  const unsigned i = slice_ok ? a.b_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  double part = 0.0;
+  unsigned half, slot;
+  const SliceLane L = sweep_lane<SPLIT>(slice0, half, slot);
+  const bool sl = L.s < slice0 + nsl;
+  const unsigned i = sl && L.c < 3 ? a.b_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  T t = 0;
   if (i != ~0u) {
-    T t = a.Pr[kVS * (size_t)i + L.c];
-    t = sliced_row_sum<T, true, unroll_sweep<T>(), kVS>(
-        t, (T)1, L, a.b_plan.len[L.s * kSliceRows + L.r], a.b_plan.kmax[L.s],
-        a.b_plan.off[L.s], a.b_plan.col, a.b_val, a.Pr);
+    if (half == 0) t = a.Pr[kVS * (size_t)i + L.c];
+    t = sliced_row_sub_part<T, U, SPLIT>(t, L, half, a.b_plan.len[L.s * kSliceRows + L.r],
+                                         a.b_plan.kmax[L.s], a.b_plan.off[L.s], a.b_plan.col,
+                                         a.b_val, a.Pr);
+  }
+  t = join_halves<T, SPLIT>(t, half, slot, L.lane);
+  double part = 0.0;
+  if (half == 0 && i != ~0u) {
     a.Pr[kVS * (size_t)i + L.c] = t;
     part = (double)a.r[kVS * (size_t)i + L.c] * (double)t;
   }
@@ -522,9 +604,47 @@ struct Fusion {
   DBuf<unsigned> l0mask, isomask, row_l0;
   PackedVals<double> d64;
   PackedVals<float> d32;
-  std::vector<unsigned> fwd_off, bwd_off;  // per level (>= 1): offset in partials3
+};
+
+// Where every (r, Pr) partial of a fused iteration goes (partials3), for a
+// sweep shape with SPLIT wavefronts per slice.
+struct SweepLayout {
+  std::vector<unsigned> fwd_off, bwd_off;  // per level (>= 1), ~0u: none
   unsigned upd_off = 0, n3 = 0;
 };
+
+static unsigned split_grid(unsigned nsl, int split) {
+  return std::max(1u, ceil_div(nsl, kPackBlock / kWave / split));
+}
+
+static SweepLayout sweep_layout(dfm2h_bsr A, dfm2h_ilu P, int split) {
+  SweepLayout Lo;
+  unsigned off = 0;
+  const size_t nlb = P->bplan.group_ptr.size() - 1, nlf = P->fplan.group_ptr.size() - 1;
+  Lo.bwd_off.assign(nlb, ~0u);
+  for (size_t l = 1; l < nlb; ++l) {
+    Lo.bwd_off[l] = off;
+    off += split_grid(P->bplan.group_slices(l), split);
+  }
+  Lo.fwd_off.assign(nlf, ~0u);
+  for (size_t l = 1; l < nlf; ++l)
+    if (P->flev_u0[l]) {
+      Lo.fwd_off[l] = off;
+      off += split_grid(P->fplan.group_slices(l), split);
+    }
+  Lo.upd_off = off;
+  off += slice_grid(A->plan.nslices);
+  Lo.n3 = off;
+  return Lo;
+}
+
+// Sweep shapes of the fused iteration: {wavefronts per slice, blocks per
+// load round}.  Shape 0 keeps the reference's summation order.
+constexpr int kNumSweepShapes = 6;
+constexpr int kSweepSplit[kNumSweepShapes] = {1, 1, 2, 2, 1, 2};
+constexpr int kSweepU[kNumSweepShapes] = {8, 4, 8, 4, 16, 16};
+const char *const kSweepNames[kNumSweepShapes] = {"split1_u8", "split1_u4", "split2_u8",
+                                                  "split2_u4", "split1_u16", "split2_u16"};
 
 struct PcgWork {
   hipStream_t s = nullptr;
@@ -534,6 +654,8 @@ struct PcgWork {
   PcgBufs<float> f32;
   DBuf<double> b64, x64, hist64;
   Fusion fu;
+  int shape[2] = {-1, -1};          // sweep shape per precision [fp32]; -1: autotune
+  std::vector<double> tune_ms[2];   // last autotuner timings (ms per sweep pair)
   PcgWork() {
     DFM2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     DFM2H_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
@@ -581,22 +703,6 @@ static void ensure_fusion(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W) {
   F.row_l0.upload(row);
   F.d64.version = ~0ull;
   F.d32.version = ~0ull;
-  unsigned off = 0;
-  const size_t nlb = P->bplan.group_ptr.size() - 1, nlf = P->fplan.group_ptr.size() - 1;
-  F.bwd_off.assign(nlb, ~0u);
-  for (size_t l = 1; l < nlb; ++l) {
-    F.bwd_off[l] = off;
-    off += slice_grid(P->bplan.group_slices(l));
-  }
-  F.fwd_off.assign(nlf, ~0u);
-  for (size_t l = 1; l < nlf; ++l)
-    if (P->flev_u0[l]) {
-      F.fwd_off[l] = off;
-      off += slice_grid(P->fplan.group_slices(l));
-    }
-  F.upd_off = off;
-  off += slice_grid(nsl);
-  F.n3 = off;
   F.A = A;
   F.a_serial = A->pattern_serial;
   F.p_serial = P->pattern_serial;
@@ -646,7 +752,7 @@ static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Packe
   a.partials = B.partials.p;
   a.partials2 = B.partials2.p;
   a.partials3 = B.partials3.p;
-  a.upd_part_off = F.upd_off;
+  a.upd_part_off = 0;  // set per sweep shape
   a.hist = B.hist.p;
   a.S = B.S.p;
   return a;
@@ -677,7 +783,7 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
   const size_t np = std::max<size_t>({gs, (size_t)vec_grid(n), (size_t)bwd_partial_total(P), 1});
   if (B.partials.n < np) B.partials.alloc(np);
   if (B.partials2.n < gs) B.partials2.alloc(gs);
-  const size_t n3 = std::max<size_t>(P->work->fu.n3, 1);
+  const size_t n3 = std::max<size_t>(sweep_layout(A, P, 2).n3, 1);  // the larger layout
   if (B.partials3.n < n3) B.partials3.alloc(n3);
   if (B.hist.n < (size_t)max_nitr + 2) B.hist.alloc((size_t)max_nitr + 2);
   if (!B.S.p) {
@@ -692,38 +798,53 @@ template <typename T>
 static unsigned launch_sweeps_full(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
-    hipLaunchKernelGGL((k_pcg_fwd<T, false>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
-                       P->fplan.group_ptr[l], nsl, (double *)nullptr);
+    hipLaunchKernelGGL((k_pcg_fwd<T, false, 1, 8>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
+                       a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
   }
   unsigned off = 0;
   for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
     const unsigned g = slice_grid(nsl);
-    hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(g), dim3(kPackBlock), 0, s, a, P->bplan.group_ptr[l],
-                       nsl, a.partials + off);
+    hipLaunchKernelGGL((k_pcg_bwd<T, 1, 8>), dim3(g), dim3(kPackBlock), 0, s, a,
+                       P->bplan.group_ptr[l], nsl, a.partials + off);
     off += g;
   }
   return off;
 }
 
 // Iteration form: forward levels >= 1 then backward levels >= 1 (level 0 of
-// each is fused away), (r,Pr) partials into partials3.
-template <typename T>
-static void launch_sweeps_fused(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
-  const Fusion &F = P->work->fu;
-  for (size_t l = 1; l + 1 < P->fplan.group_ptr.size(); ++l) {
+// each is fused away), (r,Pr) partials into partials3 at layout Lo.
+template <typename T, int SPLIT, int U>
+static void launch_sweeps_shape(const PcgArgs<T> &a, dfm2h_ilu P, const SweepLayout &Lo,
+                                hipStream_t s, int which) {
+  for (size_t l = 1; (which & 1) && l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
-    if (F.fwd_off[l] != ~0u)
-      hipLaunchKernelGGL((k_pcg_fwd<T, true>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
-                         P->fplan.group_ptr[l], nsl, a.partials3 + F.fwd_off[l]);
+    const dim3 g(split_grid(nsl, SPLIT));
+    if (Lo.fwd_off[l] != ~0u)
+      hipLaunchKernelGGL((k_pcg_fwd<T, true, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
+                         P->fplan.group_ptr[l], nsl, a.partials3 + Lo.fwd_off[l]);
     else
-      hipLaunchKernelGGL((k_pcg_fwd<T, false>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
-                         a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
+      hipLaunchKernelGGL((k_pcg_fwd<T, false, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
+                         P->fplan.group_ptr[l], nsl, (double *)nullptr);
   }
-  for (size_t l = 1; l + 1 < P->bplan.group_ptr.size(); ++l) {
+  for (size_t l = 1; (which & 2) && l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
-    hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s, a,
-                       P->bplan.group_ptr[l], nsl, a.partials3 + F.bwd_off[l]);
+    hipLaunchKernelGGL((k_pcg_bwd<T, SPLIT, U>), dim3(split_grid(nsl, SPLIT)), dim3(kPackBlock),
+                       0, s, a, P->bplan.group_ptr[l], nsl, a.partials3 + Lo.bwd_off[l]);
+  }
+}
+
+// which: 1 forward, 2 backward, 3 both
+template <typename T>
+static void launch_sweeps_fused(const PcgArgs<T> &a, dfm2h_ilu P, int shape,
+                                const SweepLayout &Lo, hipStream_t s, int which = 3) {
+  switch (shape) {
+    case 0: launch_sweeps_shape<T, 1, 8>(a, P, Lo, s, which); break;
+    case 1: launch_sweeps_shape<T, 1, 4>(a, P, Lo, s, which); break;
+    case 2: launch_sweeps_shape<T, 2, 8>(a, P, Lo, s, which); break;
+    case 3: launch_sweeps_shape<T, 2, 4>(a, P, Lo, s, which); break;
+    case 4: launch_sweeps_shape<T, 1, 16>(a, P, Lo, s, which); break;
+    default: launch_sweeps_shape<T, 2, 16>(a, P, Lo, s, which); break;
   }
 }
 
@@ -738,14 +859,15 @@ static void launch_scalar(hipStream_t s, PcgScal *S, const double *pa, unsigned 
 // level 0), forward levels >= 1, backward levels >= 1, stop test + beta,
 // p update.
 template <typename T>
-static void launch_iteration(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
+static void launch_iteration(const PcgArgs<T> &a, dfm2h_ilu P, int shape, const SweepLayout &Lo,
+                             hipStream_t s) {
   const unsigned gs = slice_grid(a.a_nsl);
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  launch_sweeps_fused(a, P, s);
-  launch_scalar<kScalConvBeta>(s, a.S, a.partials2, gs, a.partials3, P->work->fu.n3, a.hist);
+  launch_sweeps_fused(a, P, shape, Lo, s);
+  launch_scalar<kScalConvBeta>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
@@ -796,12 +918,70 @@ static std::vector<uintptr_t> graph_sig(const PcgArgs<T> &a, dfm2h_bsr A, dfm2h_
           (uintptr_t)a.a64_val,  (uintptr_t)a.a64_dia,   (uintptr_t)a.x64,
           (uintptr_t)a.b64,      (uintptr_t)a.hist64,    (uintptr_t)chunk,
           (uintptr_t)A,          (uintptr_t)A->pattern_serial, (uintptr_t)P->pattern_serial,
-          (uintptr_t)P->work->fu.serial};
+          (uintptr_t)P->work->fu.serial, (uintptr_t)P->work->shape[sizeof(T) == 4 ? 1 : 0]};
+}
+
+// Autotuner of the sweep shape: times kRep forward + backward sweep pairs
+// of every shape on the current vectors and returns the fastest (FP64: of
+// the one-wavefront-per-slice shapes, whose sums run in the reference's
+// order).  The sweeps only rewrite Pr, which the prologue recomputes
+// afterwards.
+template <typename T>
+static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T> &a0,
+                       std::vector<double> &ms) {
+  constexpr int kRep = 20;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DFM2H_HIP(hipEventCreate(&e0));
+  DFM2H_HIP(hipEventCreate(&e1));
+  ms.assign(kNumSweepShapes, 0.0);
+  int best = 0;
+  for (int v = 0; v < kNumSweepShapes; ++v) {
+    // FP64 keeps the reference's summation order (one wavefront per slice)
+    if (sizeof(T) == 8 && kSweepSplit[v] != 1) continue;
+    const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[v]);
+    PcgArgs<T> a = a0;
+    a.upd_part_off = Lo.upd_off;
+    launch_sweeps_fused(a, P, v, Lo, W.s);  // warm-up
+    DFM2H_HIP(hipEventRecord(e0, W.s));
+    for (int k = 0; k < kRep; ++k) launch_sweeps_fused(a, P, v, Lo, W.s);
+    DFM2H_HIP(hipEventRecord(e1, W.s));
+    DFM2H_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    DFM2H_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[v] = (double)t / kRep;
+    if (ms[v] < ms[best]) best = v;
+  }
+  check_launch("sweep autotuner");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+// The sweep shape of this solve: the handle's, else the cached one for
+// (DOF bucket, precision), else measured now.  Called after the
+// prologue's first scalar step; a solve that is already done skips tuning.
+template <typename T>
+static int resolve_shape(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
+                         const PcgArgs<T> &a) {
+  const int p = sizeof(T) == 4 ? 1 : 0;
+  if (W.shape[p] >= 0) return W.shape[p];
+  const int bucket = dof_bucket(3 * (size_t)A->nblk);
+  int v = tune_lookup("pcg", bucket, p, kNumSweepShapes);
+  if (v < 0) {
+    PcgScal st{};
+    DFM2H_HIP(hipMemcpyAsync(&st, B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, W.s));
+    DFM2H_HIP(hipStreamSynchronize(W.s));
+    if (st.done) return 0;
+    v = tune_sweeps<T>(A, P, W, a, W.tune_ms[p]);
+    tune_store("pcg", bucket, p, v, W.tune_ms[p][v]);
+  }
+  W.shape[p] = v;
+  return v;
 }
 
 // Runs the fused PCG on B.r (holding b) into B.x; fills status.
 template <typename T>
-static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const PcgArgs<T> &a,
+static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const PcgArgs<T> &a0,
                     double tol, unsigned max_nitr, PcgScal &status) {
   hipStream_t s = W.s;
   // scalars: tol and cap (a plain copy into the scalar block, not a memset)
@@ -809,15 +989,20 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
   init.tol = tol;
   init.max_nitr = (int)max_nitr;
   DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
-  const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL(k_pcg_init<T>, dim3(gv), dim3(kVecBlock), 0, s, a);
-  launch_scalar<kScalInit>(s, a.S, a.partials, gv, nullptr, 0, a.hist);
+  const unsigned gv = vec_grid(a0.n);
+  hipLaunchKernelGGL(k_pcg_init<T>, dim3(gv), dim3(kVecBlock), 0, s, a0);
+  launch_scalar<kScalInit>(s, a0.S, a0.partials, gv, nullptr, 0, a0.hist);
+  const int shape = resolve_shape(A, P, W, B, a0);
+  const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
+  PcgArgs<T> a = a0;
+  a.upd_part_off = Lo.upd_off;
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
   const unsigned chunk = std::max(1u, std::min(16u, max_nitr));
-  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk, [&] { launch_iteration(a, P, s); });
+  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk,
+               [&] { launch_iteration(a, P, shape, Lo, s); });
   run_chunks(W, B, (max_nitr + chunk - 1) / chunk, status);
 }
 
@@ -828,19 +1013,20 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
 // FP64, tested in FP64 by the alpha step (stop when |r| / |b| < tol); the
 // r/x update then folds x32 into x64 and the same Krylov sequence goes on
 // from the true residual.
-static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, hipStream_t s) {
+static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, int shape,
+                                   const SweepLayout &Lo, hipStream_t s) {
   const unsigned gs = slice_grid(a.a_nsl);
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64);
   hipLaunchKernelGGL(k_pcg_update<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
-  launch_sweeps_fused(a, P, s);
-  launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, gs, a.partials3, P->work->fu.n3, a.hist);
+  launch_sweeps_fused(a, P, shape, Lo, s);
+  launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
 static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<float> &B,
-                          const PcgArgs<float> &a, double tol, double delta, unsigned max_outer,
+                          const PcgArgs<float> &a0, double tol, double delta, unsigned max_outer,
                           unsigned max_inner, PcgScal &status) {
   hipStream_t s = W.s;
   PcgScal init{};
@@ -849,15 +1035,20 @@ static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<float> &
   init.delta2 = delta * delta;
   init.max_upd = (int)max_outer;
   DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
-  const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL(k_mixed_init, dim3(gv), dim3(kVecBlock), 0, s, a);
-  launch_scalar<kScalInitMixed>(s, a.S, a.partials, gv, nullptr, 0, a.hist64);
+  const unsigned gv = vec_grid(a0.n);
+  hipLaunchKernelGGL(k_mixed_init, dim3(gv), dim3(kVecBlock), 0, s, a0);
+  launch_scalar<kScalInitMixed>(s, a0.S, a0.partials, gv, nullptr, 0, a0.hist64);
+  const int shape = resolve_shape(A, P, W, B, a0);
+  const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
+  PcgArgs<float> a = a0;
+  a.upd_part_off = Lo.upd_off;
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
   check_launch("mixed prologue");
   const unsigned chunk = std::max(1u, std::min(16u, max_inner));
-  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk, [&] { launch_mixed_iteration(a, P, s); });
+  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk,
+               [&] { launch_mixed_iteration(a, P, shape, Lo, s); });
   run_chunks(W, B, (max_inner + chunk - 1) / chunk + 1, status);
 }
 
@@ -1032,6 +1223,30 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
   });
 }
 
+int dfm2h_pcg_set_shape(dfm2h_ilu P, int fp32, int shape) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null handle");
+    DFM2H_REQUIRE(shape >= -1 && shape < kNumSweepShapes, DFM2H_ERR_INVALID, "bad shape");
+    PcgWork &W = work_of(P);
+    W.shape[fp32 ? 1 : 0] = shape;
+    W.f64.drop_graph();
+    W.f32.drop_graph();
+  });
+}
+
+int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name, double *tune_ms,
+                        size_t cap, size_t *n_shapes) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P && shape, DFM2H_ERR_INVALID, "null argument");
+    PcgWork &W = work_of(P);
+    const int p = fp32 ? 1 : 0;
+    *shape = W.shape[p];
+    if (name) *name = W.shape[p] >= 0 ? kSweepNames[W.shape[p]] : "autotune";
+    for (size_t k = 0; tune_ms && k < cap && k < W.tune_ms[p].size(); ++k) tune_ms[k] = W.tune_ms[p][k];
+    if (n_shapes) *n_shapes = kNumSweepShapes;
+  });
+}
+
 int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, double *ms) {
   return guard(__func__, [&] {
     need_solver(A, P);
@@ -1041,8 +1256,12 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
     // consecutive kernel classes, so every class is timed inside the real
     // dependency chain (as in the captured graph).  Convergence and
     // refinement are disabled for the duration.
-    auto run = [&](auto tag, auto &B, const auto &a) {
+    auto run = [&](auto tag, auto &B, const auto &a0) {
       using T = decltype(tag);
+      const int shape = std::max(0, W.shape[sizeof(T) == 4 ? 1 : 0]);
+      const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
+      auto a = a0;
+      a.upd_part_off = Lo.upd_off;
       PcgScal st{};
       DFM2H_HIP(hipMemcpyAsync(&st, B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, W.s));
       DFM2H_HIP(hipStreamSynchronize(W.s));
@@ -1054,7 +1273,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       tst.max_nitr = 1 << 30;
       DFM2H_HIP(hipMemcpyAsync(B.S.p, &tst, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
       const unsigned gs = slice_grid(a.a_nsl), gv = vec_grid(a.n);
-      const unsigned n3 = W.fu.n3;
+      const unsigned n3 = Lo.n3;
       // segments: spmv | alpha | update | (two empty) | fwd | bwd | beta+pupdate
       constexpr int kSeg = 8;
       hipEvent_t ev[kSeg + 1];
@@ -1073,21 +1292,9 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         DFM2H_HIP(hipEventRecord(ev[3], W.s));
         DFM2H_HIP(hipEventRecord(ev[4], W.s));
         DFM2H_HIP(hipEventRecord(ev[5], W.s));
-        for (size_t l = 1; l + 1 < P->fplan.group_ptr.size(); ++l) {
-          const unsigned nsl = P->fplan.group_slices(l);
-          if (W.fu.fwd_off[l] != ~0u)
-            hipLaunchKernelGGL((k_pcg_fwd<T, true>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0,
-                               W.s, a, P->fplan.group_ptr[l], nsl, a.partials3 + W.fu.fwd_off[l]);
-          else
-            hipLaunchKernelGGL((k_pcg_fwd<T, false>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0,
-                               W.s, a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
-        }
+        launch_sweeps_fused(a, P, shape, Lo, W.s, 1);
         DFM2H_HIP(hipEventRecord(ev[6], W.s));
-        for (size_t l = 1; l + 1 < P->bplan.group_ptr.size(); ++l) {
-          const unsigned nsl = P->bplan.group_slices(l);
-          hipLaunchKernelGGL(k_pcg_bwd<T>, dim3(slice_grid(nsl)), dim3(kPackBlock), 0, W.s, a,
-                             P->bplan.group_ptr[l], nsl, a.partials3 + W.fu.bwd_off[l]);
-        }
+        launch_sweeps_fused(a, P, shape, Lo, W.s, 2);
         DFM2H_HIP(hipEventRecord(ev[7], W.s));
         if constexpr (sizeof(T) == 4)
           launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);

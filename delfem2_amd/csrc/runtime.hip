@@ -4,6 +4,12 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace dfm2h {
 
@@ -18,6 +24,49 @@ void check_launch(const char *what) {
   if (e != hipSuccess)
     throw Error(DFM2H_ERR_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
 }
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<std::string, int, int>, int> g_tune;
+bool g_tune_loaded = false;
+
+void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char *f = std::getenv("DFM2H_TUNE_CACHE");
+  if (!f) return;
+  FILE *fp = std::fopen(f, "r");
+  if (!fp) return;
+  char kind[32];
+  int b, p, v;
+  double ms;
+  while (std::fscanf(fp, "%31s %d %d %d %lf", kind, &b, &p, &v, &ms) == 5)
+    g_tune[std::make_tuple(std::string(kind), b, p)] = v;
+  std::fclose(fp);
+}
+}  // namespace
+
+int tune_lookup(const char *kind, int bucket, int fp32, int n_variants) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  auto it = g_tune.find(std::make_tuple(std::string(kind), bucket, fp32));
+  if (it == g_tune.end() || it->second < 0 || it->second >= n_variants) return -1;
+  return it->second;
+}
+
+void tune_store(const char *kind, int bucket, int fp32, int variant, double ms) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  g_tune[std::make_tuple(std::string(kind), bucket, fp32)] = variant;
+  if (const char *f = std::getenv("DFM2H_TUNE_CACHE")) {
+    if (FILE *fp = std::fopen(f, "a")) {
+      std::fprintf(fp, "%s %d %d %d %.6f\n", kind, bucket, fp32, variant, ms);
+      std::fclose(fp);
+    }
+  }
+}
+
+int dof_bucket(size_t ndof) { return ndof < 1 ? 0 : (int)std::floor(std::log2((double)ndof)); }
 
 }  // namespace dfm2h
 

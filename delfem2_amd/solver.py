@@ -226,6 +226,20 @@ class CPreconditionerILU:
         N.call("dfm2h_ilu_levels", self.h, C.byref(f), C.byref(b))
         return f.value, b.value
 
+    def set_sweep_shape(self, fp32: bool, shape: int) -> None:
+        """Sweep launch shape of the fused solvers (-1 = autotune)."""
+        N.call("dfm2h_pcg_set_shape", self.h, int(bool(fp32)), int(shape))
+
+    def sweep_shape(self, fp32: bool):
+        """(shape, name, autotuner ms per sweep pair of every shape)."""
+        v = C.c_int(-1)
+        name = C.c_char_p()
+        ms = np.zeros(16)
+        n = C.c_size_t(0)
+        N.call("dfm2h_pcg_get_shape", self.h, int(bool(fp32)), C.byref(v), C.byref(name),
+               _hptr(ms), ms.size, C.byref(n))
+        return v.value, name.value.decode(), ms[: n.value]
+
     def download(self):
         rp = np.zeros(max(self.ncrs, 1), np.uint32)
         di = np.zeros(self.nblk, np.uint32)

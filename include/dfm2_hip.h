@@ -203,6 +203,21 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
                           double *hist, size_t hist_cap, size_t *n_hist,
                           size_t *n_inner);
 
+/* Launch shape of the fused solvers' triangular sweeps (no reference
+ * counterpart): shape s = {wavefronts per 21-row slice, blocks per load
+ * round} from { {1,8}, {1,4}, {2,8}, {2,4}, {1,16}, {2,16} }; with two
+ * wavefronts per slice each takes every other block of a row and the two
+ * sums are added (a different rounding order from the reference's loops;
+ * shapes 0, 1, 4 keep it).  -1 (the default): the first solve of each
+ * precision times every shape (FP64: shapes 0, 1, 4 only) on its own
+ * sweeps and keeps the fastest, per
+ * (floor(log2 ndof), precision) for the process (and in the file named by
+ * DFM2H_TUNE_CACHE, if set).  get_shape: current shape, its name, and the
+ * autotuner's ms per forward+backward sweep pair of every shape (0 where
+ * not measured); n_shapes = 6. */
+int dfm2h_pcg_set_shape(dfm2h_ilu P, int fp32, int shape);
+int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name,
+                        double *tune_ms, size_t cap, size_t *n_shapes);
 /* Measurement (no reference counterpart): average time in ms of each kernel
  * class inside the fused iteration, from HIP events recorded between
  * consecutive kernels on the solver's own stream over nrep iterations run

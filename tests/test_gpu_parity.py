@@ -260,3 +260,43 @@ def test_assembly_deterministic_and_c2_properties(gpu):
     xm, hm, n_inner = G.solve_mixed(1e-8, delta=1e-2)
     assert hm[-1] / hm[0] < 1e-8
     assert torch.linalg.norm(xm - x).item() / torch.linalg.norm(x).item() <= MIXED_X_TOL
+
+
+def test_sweep_shapes_and_autotuner(gpu):
+    """Every sweep shape of the fused solvers gives the oracle's solution
+    (FP64 shapes 0/1/4 keep the summation order: same iterations as the
+    default); the autotuner fills its timings and picks an allowed shape."""
+    prob = W.cube(12, sigma=0.5, seed=11, ordering="color4")
+    C = cpu_system(prob)
+    C.factor()
+    xc, hc = C.solve_pcg(C.b.copy(), 1e-8, 5000)
+    xe, _ = C.solve_pcg(C.b.copy(), 1e-12, 5000)
+    G = DeviceSystem(prob)
+    G.assemble()
+    G.factor()
+    for fp32 in (False, True):
+        G.ilu.set_sweep_shape(fp32, -1)
+    x0, h0, _ = G.solve(1e-8, 5000)
+    xm0, _, _ = G.solve_mixed(1e-8)
+    for fp32 in (False, True):
+        v, name, ms = G.ilu.sweep_shape(fp32)
+        assert ms.size == 6 and 0 <= v < 6, (v, name)
+        if not fp32:
+            assert v in (0, 1, 4)
+    ref64 = None
+    for shape in range(6):
+        for fp32 in (False, True):
+            G.ilu.set_sweep_shape(fp32, shape)
+        x, hist, _ = G.solve(1e-8, 5000)
+        assert abs(len(hist) - len(hc)) <= 1
+        assert np.linalg.norm(x.cpu().numpy() - xc) / np.linalg.norm(xc) <= FP64_X_TOL
+        if shape in (0, 1, 4):  # same summation order: same bits
+            if ref64 is None:
+                ref64 = x.clone()
+            assert torch.equal(x, ref64)
+        xm, hm, _ = G.solve_mixed(1e-8)
+        assert hm[-1] / hm[0] < 1e-8
+        assert np.linalg.norm(xm.cpu().numpy() - xe) / np.linalg.norm(xe) <= MIXED_X_TOL
+    assert torch.equal(x0, ref64)
+    for fp32 in (False, True):
+        G.ilu.set_sweep_shape(fp32, -1)

@@ -110,7 +110,7 @@ def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
         "unit": "s (time-to-1e-8 = Decompose + FP64 PCG, iterations x measured s/iteration)",
         "cores": 1,
         "kind": kind,
-        "sample": (f"64^3 cube (823,875 DOF): pattern+assembly+BC {t1 - t0:.2f} s, "
+        "sample": (f"{prob.name} ({prob.ndof} DOF): pattern+assembly+BC {t1 - t0:.2f} s, "
                    f"Decompose {t2 - t1:.3f} s, {n_it} PCG iterations {t3 - t2:.2f} s "
                    f"({t_it * 1e3:.1f} ms/iter) extrapolated to {iters_fp64} iterations"),
         "dof_iter_per_s": round(prob.ndof * n_it / (t3 - t2), 1),
@@ -220,13 +220,88 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_cube128(args):
+    """128^3 hexes x 6 tets (6,440,067 DOF), E lognormal(0, 1.0), nu ~ U[0.25,
+    0.35], 4-colour order: FP32 inner + FP64 reliable updates to relative
+    residual 1e-10 (value) beside plain FP64 ILU(0)-PCG to 1e-10; one step =
+    CopyValue + Decompose + solve.  Error of the mixed solution against an
+    FP64 solve to 1e-12 is reported."""
+    import torch
+
+    from delfem2_amd import workload as W
+    from delfem2_amd.problem import DeviceSystem
+
+    print("[bench] building the 128^3 problem", file=sys.stderr, flush=True)
+    prob = W.cube(128, sigma=1.0, nu_range=(0.25, 0.35), ordering="color4")
+    G = DeviceSystem(prob, device=0)
+    assert G.assemble()
+    nb, nnzb = prob.np_, G.A.ncrs
+
+    def step(fp32):
+        G.factor()
+        if fp32:
+            return G.solve_mixed(1e-10, delta=args.delta, max_outer=400, max_inner=40000)
+        return G.solve(1e-10, 40000)
+
+    for _ in range(args.warmup):
+        step(True)
+        step(False)
+    res = {}
+    for name, fp32 in (("mixed", True), ("fp64", False)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = step(fp32)
+        torch.cuda.synchronize()
+        res[name] = ((time.perf_counter() - t0) / args.steps, out)
+    t_m, (xm, hm, n_inner) = res["mixed"]
+    t_64, (x64, h64, _) = res["fp64"]
+    it64 = len(h64) - 1
+    xr, _, _ = G.solve(1e-12, 40000)
+    err = (torch.linalg.norm(xm - xr) / torch.linalg.norm(xr)).item()
+    err64 = (torch.linalg.norm(x64 - xr) / torch.linalg.norm(xr)).item()
+    rr = G.b.clone()
+    G.A.MatVec(rr, -1.0, xm, 1.0)
+    relres = (torch.linalg.norm(rr) / torch.linalg.norm(G.b)).item()
+    print(f"[bench] 128^3 mixed {t_m * 1e3:.1f} ms ({n_inner} inner), fp64 {t_64 * 1e3:.1f} ms "
+          f"({it64} it), err {err:.2e}", file=sys.stderr, flush=True)
+    cpu = None
+    if args.cpu_iters > 0:
+        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(prob, it64, min(args.cpu_iters, 20))
+        cpu["unit"] = "s (time-to-1e-10 = Decompose + FP64 PCG, iterations x measured s/iteration)"
+    n0, nu0 = level0_rows(G.psup_ind, G.psup, nb)
+    b32 = pcg_iter_bytes(nb, nnzb, 4, n0, nu0)
+    out = {
+        "metric": "time-to-1e-10 PCG at 6.44M DOF: FP32+refine vs FP64 (s)",
+        "value": round(t_m, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_m * 1e3, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32-inner/f64-refine",
+        "data": "synthetic (seeded voxel tet cube, lognormal E, uniform nu)",
+        "config": {"workload": f"tet elasticity cube 128^3 hex x6 tets, {prob.ndof} DOF, "
+                               f"E~lognormal(0,1.0), nu~U[0.25,0.35], ILU(0)-PCG to 1e-10, "
+                               f"color4 order", "ndof": prob.ndof, "nnzb": nnzb},
+        "mixed": {"time_s": round(t_m, 5), "inner_iterations": n_inner,
+                  "outer_iterations": len(hm) - 1, "true_rel_residual": relres,
+                  "x_rel_err_vs_fp64_1e-12": err,
+                  "dof_iter_per_s": round(prob.ndof * n_inner / t_m, 1),
+                  "hbm_frac_per_iter": round(b32 * n_inner / t_m / 1e9 / HBM_PEAK_GBS, 4),
+                  "speedup_vs_fp64": round(t_64 / t_m, 3)},
+        "fp64": {"time_s": round(t_64, 5), "iterations": it64, "x_rel_err_vs_fp64_1e-12": err64,
+                 "dof_iter_per_s": round(prob.ndof * it64 / t_64, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cube", choices=["cube", "batch512"],
-                    help="cube: the 823,875-DOF single solve (headline); batch512: 512 "
+    ap.add_argument("--workload", default="cube", choices=["cube", "cube128", "batch512"],
+                    help="cube: the 823,875-DOF single solve to 1e-8 (headline); cube128: "
+                         "the 6,440,067-DOF mixed-precision solve to 1e-10; batch512: 512 "
                          "independent meshes in one batched solve")
     ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
     ap.add_argument("--sigma", type=float, default=0.5)
@@ -259,6 +334,9 @@ def main():
 
     if args.workload == "batch512":
         bench_batch(args)
+        return
+    if args.workload == "cube128":
+        bench_cube128(args)
         return
 
     import torch

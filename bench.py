@@ -234,6 +234,7 @@ def bench_cube128(args):
     print("[bench] building the 128^3 problem", file=sys.stderr, flush=True)
     prob = W.cube(128, sigma=1.0, nu_range=(0.25, 0.35), ordering="color4")
     G = DeviceSystem(prob, device=0)
+    G.ilu.set_mixed_options(args.mixed_vectors == 64, args.update_period)
     assert G.assemble()
     nb, nnzb = prob.np_, G.A.ncrs
 
@@ -276,7 +277,9 @@ def bench_cube128(args):
         "metric": "time-to-1e-10 PCG at 6.44M DOF: FP32+refine vs FP64 (s)",
         "value": round(t_m, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t_m * 1e3, 3), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32-inner/f64-refine",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-inner/f64-refine" if args.mixed_vectors == 32 else
+                 "f32-matrix+factors/f64-vectors+refine",
         "data": "synthetic (seeded voxel tet cube, lognormal E, uniform nu)",
         "config": {"workload": f"tet elasticity cube 128^3 hex x6 tets, {prob.ndof} DOF, "
                                f"E~lognormal(0,1.0), nu~U[0.25,0.35], ILU(0)-PCG to 1e-10, "
@@ -306,8 +309,14 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
     ap.add_argument("--sigma", type=float, default=0.5)
     ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
-    ap.add_argument("--delta", type=float, default=1e-2,
+    ap.add_argument("--delta", type=float, default=1e-1,
                     help="mixed: refine in FP64 whenever the FP32 residual fell by this factor")
+    ap.add_argument("--update-period", type=int, default=50,
+                    help="mixed: force a reliable update after this many inner iterations "
+                         "(0: only the delta rule)")
+    ap.add_argument("--mixed-vectors", type=int, default=32, choices=[32, 64],
+                    help="Krylov vector precision of the mixed solver (matrix and factors "
+                         "are FP32 either way)")
     ap.add_argument("--cpu-iters", type=int, default=200,
                     help="PCG iterations of the bounded CPU sample (0 = skip)")
     ap.add_argument("--spmv-reps", type=int, default=50,
@@ -348,6 +357,7 @@ def main():
     print(f"[bench] building {args.n}^3 problem ({args.ordering})", file=sys.stderr, flush=True)
     prob = W.cube(args.n, sigma=args.sigma, seed=20261203, ordering=args.ordering)
     G = DeviceSystem(prob, device=0)
+    G.ilu.set_mixed_options(args.mixed_vectors == 64, args.update_period)
     torch.cuda.synchronize()
     t = time.perf_counter()
     assert G.assemble()
@@ -456,7 +466,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-inner/f64-refine",
+        "dtype": "f32-inner/f64-refine" if args.mixed_vectors == 32 else
+                 "f32-matrix+factors/f64-vectors+refine",
         "data": "synthetic (seeded voxel tet cube, lognormal E)",
         "config": {"workload": f"tet elasticity cube {args.n}^3 hex x6 tets, {prob.ndof} DOF, "
                                f"E~lognormal(0,{args.sigma}), nu=0.3, ILU(0)-PCG to 1e-8, "

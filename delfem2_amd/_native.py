@@ -88,6 +88,7 @@ SIGNATURES = {
     "dfm2h_adjoint_grad_tet": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "dfm2h_pcg_kernel_times": [_vp, _vp, _i, _u, _vp],
     "dfm2h_pcg_set_shape": [_vp, _i, _i],
+    "dfm2h_pcg_set_mixed_options": [_vp, _i, _u],
     "dfm2h_pcg_get_shape": [_vp, _i, _pi, C.POINTER(C.c_char_p), _vp, _sz, _psz],
     "dfm2h_batch_create": [_pvp, _vp, _vp, _vp, _sz],
     "dfm2h_batch_destroy": [_vp],
@@ -95,6 +96,7 @@ SIGNATURES = {
     "dfm2h_batch_solve": [_vp, _vp, _vp, _d, _i, _d, _u, _vp, _vp, _vp],
     "dfm2h_batch_tune": [_vp, _i, _pi, _vp, _sz, _psz],
     "dfm2h_batch_set_variant": [_vp, _i, _i],
+    "dfm2h_batch_set_update_period": [_vp, _u],
     "dfm2h_batch_get_variant": [_vp, _i, _pi, C.POINTER(C.c_char_p)],
 }
 _RESTYPE = {"dfm2h_last_error": C.c_char_p, "dfm2h_get_stream": _vp}

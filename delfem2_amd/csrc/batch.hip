@@ -36,13 +36,14 @@ constexpr unsigned kChunk = 16;    // iterations per captured graph
 
 struct MeshScal {
   double rPr, alpha, beta, inv_sq0, sq_b, rmax2, res;
-  int it, done, converged, nan, upd, nupd, final_, pad;
+  int it, done, converged, nan, upd, nupd, final_, it_upd;
 };
 
 struct BatchGlobal {
   double tol, delta2;
   int max_nitr, max_upd;
-  unsigned ndone, pad;
+  unsigned ndone;
+  int period;  // FP32 form: force a reliable update after this many iterations
 };
 
 struct BPlan {
@@ -359,6 +360,7 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
     S->nupd = 0;
     S->upd = 0;
     S->final_ = 0;
+    S->it_upd = 0;
     S->converged = 0;
     S->nan = 0;
     S->done = 0;
@@ -400,9 +402,11 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
     if (bad) {
       S->nan = 1;
       mark_done(S, G);
-    } else if (t < G->delta2 * S->rmax2 || sqrt(t / S->sq_b) < G->tol || it >= G->max_nitr) {
+    } else if (t < G->delta2 * S->rmax2 || sqrt(t / S->sq_b) < G->tol || it >= G->max_nitr ||
+               (G->period > 0 && it - S->it_upd >= G->period)) {
       S->upd = 1;
       S->final_ = it >= G->max_nitr ? 1 : 0;
+      S->it_upd = it;
     }
   } else {  // kBReliable
     const int k = S->nupd + 1;
@@ -464,6 +468,7 @@ struct dfm2h_batch_s {
   DBuf<MeshScal> M;
   DBuf<BatchGlobal> G;
   int variant[2] = {-1, -1};  // [fp32] chosen launch shape, -1 = autotune
+  unsigned period = 50;       // FP32 form: forced reliable-update period
   hipStream_t s = nullptr;
   hipEvent_t ev_in = nullptr, ev_poll[2] = {nullptr, nullptr}, ev_t0 = nullptr, ev_t1 = nullptr;
   BatchGlobal *host_g = nullptr;  // pinned, 2 entries
@@ -680,6 +685,7 @@ void run_batch(dfm2h_batch B, BatchBufs<T> &S, const BArgs<T> &a, int v, double 
   g.max_nitr = (int)max_nitr;
   g.max_upd = (int)max_nitr;
   g.ndone = 0;
+  g.period = (int)B->period;
   DFM2H_HIP(hipMemcpyAsync(B->G.p, &g, sizeof(BatchGlobal), hipMemcpyHostToDevice, s));
   prologue_fn<T>(v)(a, B, s);
   check_launch("batch prologue");
@@ -878,6 +884,13 @@ int dfm2h_batch_info(dfm2h_batch B, size_t *nmesh, size_t *nslices, unsigned *n_
     if (nslices) *nslices = B->aplan.nslices;
     if (n_forward) *n_forward = B->nlev_f;
     if (n_backward) *n_backward = B->nlev_b;
+  });
+}
+
+int dfm2h_batch_set_update_period(dfm2h_batch B, unsigned update_period) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(B, DFM2H_ERR_INVALID, "null handle");
+    B->period = update_period;
   });
 }
 

@@ -188,14 +188,16 @@ __device__ inline Trip<T> load_node(const T *x, size_t j) {
 //   else: acc += alpha * ((v0*x0 + v1*x1) + v2*x2)
 // U blocks per round: their column indices and values are loaded first (one
 // memory round trip), then the U gathers of x (a second one), then the U
-// updates in order.
-template <typename T, bool SUB, int U, int VS>
-__device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned K, unsigned kmax,
-                                   unsigned q0, const unsigned *__restrict__ col,
-                                   const T *__restrict__ val, const T *x) {
+// updates in order.  Values are stored in precision M and widened to the
+// vector precision T before the products (M = T: no conversion).
+template <typename T, typename M, bool SUB, int U, int VS>
+__device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigned K,
+                                     unsigned kmax, unsigned q0, const unsigned *__restrict__ col,
+                                     const M *__restrict__ val, const T *x) {
   for (unsigned k0 = 0; k0 < kmax; k0 += U) {
     unsigned j[U];
-    Trip<T> v[U], xv[U];
+    Trip<M> v[U];
+    Trip<T> xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (k0 + u < K) {
@@ -210,7 +212,7 @@ __device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned 
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (k0 + u < K) {
-        const T d = v[u].a * xv[u].a + v[u].b * xv[u].b + v[u].c * xv[u].c;
+        const T d = (T)v[u].a * xv[u].a + (T)v[u].b * xv[u].b + (T)v[u].c * xv[u].c;
         if (SUB)
           acc -= d;
         else
@@ -221,10 +223,24 @@ __device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned 
   return acc;
 }
 
+template <typename T, bool SUB, int U, int VS>
+__device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned K, unsigned kmax,
+                                   unsigned q0, const unsigned *__restrict__ col,
+                                   const T *__restrict__ val, const T *x) {
+  return sliced_row_sum_m<T, T, SUB, U, VS>(acc, alpha, L, K, kmax, q0, col, val, x);
+}
+
+// this lane's row of a stored 3x3 block (precision M) times (x0, x1, x2)
+template <typename T, typename M>
+__device__ inline T dia_row_dot_m(const M *__restrict__ dia, const SliceLane &L, T x0, T x1,
+                                  T x2) {
+  const Trip<M> d = load_trip(dia, L.s, L.lane);
+  return (T)d.a * x0 + (T)d.b * x1 + (T)d.c * x2;
+}
+
 template <typename T>
 __device__ inline T dia_row_dot(const T *__restrict__ dia, const SliceLane &L, T x0, T x1, T x2) {
-  const Trip<T> d = load_trip(dia, L.s, L.lane);
-  return d.a * x0 + d.b * x1 + d.c * x2;
+  return dia_row_dot_m<T, T>(dia, L, x0, x1, x2);
 }
 
 struct PlanView {

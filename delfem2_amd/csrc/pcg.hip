@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace dfm2h {
 
@@ -44,6 +45,8 @@ struct PcgScal {
   int nupd;       // FP64 residuals recorded after the initial |b|
   int max_upd;
   int final_;     // iteration cap reached: stop after this update
+  int it_upd;     // inner iteration of the last reliable update
+  int period;     // force an update after this many inner iterations (0: off)
 };
 
 constexpr int kVecBlock = 256;
@@ -136,23 +139,32 @@ __global__ void k_unpad4(TO *__restrict__ y3, const TI *__restrict__ x4, size_t 
 // Every partial of (r, Pr) of an iteration goes to `partials3`:
 //   [backward levels >= 1 | forward levels with final rows | update].
 
-template <typename T>
+// T: precision of the Krylov vectors; M: storage precision of the matrix
+// and factor values the SpMV and sweeps stream (widened to T before use).
+// M = FP32 is the mixed solver (FP64 reliable updates); with T = FP32 its
+// solution is kept split as x64 + x32.
+template <typename T, typename M>
+constexpr bool is_mixed() {
+  return sizeof(M) == 4;
+}
+
+template <typename T, typename M = T>
 struct PcgArgs {
   unsigned nblk;
   size_t n;
   // matrix (all rows, sliced) and factors (forward / backward plans, sliced)
   PlanView a_plan;
   unsigned a_nsl;
-  const T *a_val, *a_dia;
+  const M *a_val, *a_dia;
   PlanView f_plan;
-  const T *f_val, *f_dia;
+  const M *f_val, *f_dia;
   PlanView b_plan;
-  const T *b_val;
+  const M *b_val;
   // forward level 0 inside the update: per matrix slice, the rows that are
   // level 0 (l0mask) and those that are also backward level 0 (isomask),
   // and their D^-1 blocks packed in the matrix-slice layout
   const unsigned *l0mask, *isomask;
-  const T *dinv0;
+  const M *dinv0;
   const unsigned *fumask;  // per forward slice: rows with no upper blocks
   // vectors
   T *r, *x, *Pr, *p;
@@ -161,7 +173,8 @@ struct PcgArgs {
   double *partials3;  // (r,Pr) partials of the fused iteration
   unsigned upd_part_off;
   PcgScal *S;
-  // mixed precision: FP64 solution, right-hand side, matrix and outer history
+  // mixed precision: FP64 solution part (FP32 vectors), right-hand side,
+  // matrix and outer history
   double *x64;
   const double *b64;
   const double *a64_val, *a64_dia;
@@ -169,8 +182,8 @@ struct PcgArgs {
 };
 
 // x = 0, partial (r,r)
-template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T> a) {
+template <typename T, typename M>
+__global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T, M> a) {
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
        i += (size_t)gridDim.x * kVecBlock) {
@@ -255,6 +268,7 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
     S->nupd = 0;
     S->upd = 0;
     S->final_ = 0;
+    S->it_upd = 0;
     S->converged = 0;
     S->nan = 0;
     S->done = 0;
@@ -293,18 +307,30 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
       S->nan = 1;
       return;
     }
-    if (t < S->delta2 * S->rmax2 || sqrt(t / S->sq_b) < S->tol || it >= S->max_nitr) {
+    if (t < S->delta2 * S->rmax2 || sqrt(t / S->sq_b) < S->tol || it >= S->max_nitr ||
+        (S->period > 0 && it - S->it_upd >= S->period)) {
       S->upd = 1;
       S->final_ = it >= S->max_nitr ? 1 : 0;
+      S->it_upd = it;
     }
     S->beta = t2 / S->rPr;
     S->rPr = t2;
   }
 }
 
-// Reliable update, row part: r_i = b_i - (A (x64 + x32))_i in FP64 (FP64
-// matrix values), rounded into r32; returns r_i^2.
-__device__ inline double residual_row(const PcgArgs<float> &a, const SliceLane &L, unsigned i) {
+// the mixed solver's FP64 solution component j (x64 + x32 for FP32 vectors)
+template <typename T, typename M>
+__device__ inline double x_full(const PcgArgs<T, M> &a, size_t j) {
+  if constexpr (sizeof(T) == 4)
+    return a.x64[j] + (double)a.x[j];
+  else
+    return (double)a.x[j];
+}
+
+// Reliable update, row part: r_i = b_i - (A x)_i in FP64 (FP64 matrix
+// values, FP64 solution), stored into r; returns r_i^2.
+template <typename T, typename M>
+__device__ inline double residual_row(const PcgArgs<T, M> &a, const SliceLane &L, unsigned i) {
   const unsigned K = a.a_plan.len[L.s * kSliceRows + L.r];
   const unsigned q0 = a.a_plan.off[L.s];
   double acc = 0.0;
@@ -312,40 +338,42 @@ __device__ inline double residual_row(const PcgArgs<float> &a, const SliceLane &
     const size_t q = q0 + k;
     const size_t j = kVS * (size_t)a.a_plan.col[q * kSliceRows + L.r];
     const Trip<double> v = load_trip(a.a64_val, q, L.lane);
-    const double x0 = a.x64[j] + (double)a.x[j];
-    const double x1 = a.x64[j + 1] + (double)a.x[j + 1];
-    const double x2 = a.x64[j + 2] + (double)a.x[j + 2];
-    acc += v.a * x0 + v.b * x1 + v.c * x2;
+    acc += v.a * x_full(a, j) + v.b * x_full(a, j + 1) + v.c * x_full(a, j + 2);
   }
   const size_t ii = kVS * (size_t)i;
-  acc += dia_row_dot(a.a64_dia, L, a.x64[ii] + (double)a.x[ii],
-                     a.x64[ii + 1] + (double)a.x[ii + 1], a.x64[ii + 2] + (double)a.x[ii + 2]);
+  acc += dia_row_dot(a.a64_dia, L, x_full(a, ii), x_full(a, ii + 1), x_full(a, ii + 2));
   const double r = a.b64[ii + L.c] - acc;
-  a.r[ii + L.c] = (float)r;
+  a.r[ii + L.c] = (T)r;
   return r * r;
 }
 
-// Ap -> Pr, partial (p, Ap).  FP32 form with a reliable update due: also
+// blocks per load round of the SpMV
+template <typename T, typename M>
+constexpr int spmv_unroll() {
+  return sizeof(M) == 4 ? 8 : 4;
+}
+
+// Ap -> Pr, partial (p, Ap).  Mixed form with a reliable update due: also
 // r = b - A x in FP64 (partials2: |r|^2), for the update kernel that
 // follows.
-template <typename T>
-__global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T> a) {
+template <typename T, typename M>
+__global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
   if (a.S->done) return;
-  const bool refine = sizeof(T) == 4 && a.S->upd;  // grid-uniform
+  const bool refine = is_mixed<T, M>() && a.S->upd;  // grid-uniform
   const SliceLane L = slice_lane(0);
   const bool slice_ok = L.s < a.a_nsl && L.c < 3;
   const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
   double part = 0.0, part2 = 0.0;
   if (i != ~0u) {
     T acc = 0;
-    acc = sliced_row_sum<T, false, unroll_spmv<T>(), kVS>(
+    acc = sliced_row_sum_m<T, M, false, spmv_unroll<T, M>(), kVS>(
         acc, (T)1, L, a.a_plan.len[L.s * kSliceRows + L.r], a.a_plan.kmax[L.s],
         a.a_plan.off[L.s], a.a_plan.col, a.a_val, a.p);
     const Trip<T> pi = load_node<T, kVS>(a.p, i);
-    acc += dia_row_dot(a.a_dia, L, pi.a, pi.b, pi.c);
+    acc += dia_row_dot_m<T, M>(a.a_dia, L, pi.a, pi.b, pi.c);
     a.Pr[kVS * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
     part = (double)a.p[kVS * (size_t)i + L.c] * (double)acc;
-    if constexpr (sizeof(T) == 4)
+    if constexpr (is_mixed<T, M>())
       if (refine) part2 = residual_row(a, L, i);
   }
   part = block_reduce_sum<kPackBlock>(part);
@@ -360,8 +388,8 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T> a) {
 // lane's new r component rv (wavefront-uniform call): Pr_i = D_i^-1 r_i, the
 // same products in the same order as the sweep.  Rows that are also
 // backward level 0 add r_i . Pr_i to srp.
-template <typename T>
-__device__ inline void fwd_level0(const PcgArgs<T> &a, const SliceLane &L, unsigned i, T rv,
+template <typename T, typename M>
+__device__ inline void fwd_level0(const PcgArgs<T, M> &a, const SliceLane &L, unsigned i, T rv,
                                   double &srp) {
   const unsigned m = a.l0mask[L.s];
   if (!m) return;
@@ -369,7 +397,7 @@ __device__ inline void fwd_level0(const PcgArgs<T> &a, const SliceLane &L, unsig
   const T t1 = __shfl(rv, (int)L.r + kSliceRows, kWave);
   const T t2 = __shfl(rv, (int)L.r + 2 * kSliceRows, kWave);
   if (i != ~0u && ((m >> L.r) & 1u)) {
-    const T z = dia_row_dot(a.dinv0, L, t0, t1, t2);
+    const T z = dia_row_dot_m<T, M>(a.dinv0, L, t0, t1, t2);
     a.Pr[kVS * (size_t)i + L.c] = z;
     if ((a.isomask[L.s] >> L.r) & 1u) srp += (double)rv * (double)z;
   }
@@ -377,15 +405,16 @@ __device__ inline void fwd_level0(const PcgArgs<T> &a, const SliceLane &L, unsig
 
 // r += -alpha Ap, x += alpha p, partial (r,r); then forward level 0.
 // One wavefront per matrix slice (rows in order, components in lanes).
-// FP32 form right after a reliable update: x32 is first folded into x64.
-template <typename T>
-__global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T> a) {
+// Mixed form with FP32 vectors, right after a reliable update: x32 is first
+// folded into x64.
+template <typename T, typename M>
+__global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   if (a.S->done) return;
   const double alpha = a.S->alpha;
   const SliceLane L = slice_lane(0);
   const bool sl = L.s < a.a_nsl;
   const unsigned i = sl && L.c < 3 ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  const bool fold = sizeof(T) == 4 && a.S->upd;  // a reliable update just ran
+  const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && a.S->upd;  // an update just ran
   double s2 = 0.0, srp = 0.0;
   T rn = 0;
   if (i != ~0u) {
@@ -428,15 +457,16 @@ __device__ inline SliceLane sweep_lane(unsigned slice0, unsigned &half, unsigned
   return L;
 }
 
-// acc (-)= sum over the row's blocks k = h, h + H, h + 2H, ... (H halves)
-template <typename T, int U, int H>
+// acc -= sum over the row's blocks k = h, h + H, h + 2H, ... (H halves)
+template <typename T, typename M, int U, int H>
 __device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, unsigned K,
                                         unsigned kmax, unsigned q0,
                                         const unsigned *__restrict__ col,
-                                        const T *__restrict__ val, const T *x) {
+                                        const M *__restrict__ val, const T *x) {
   for (unsigned k0 = h; k0 < kmax; k0 += H * U) {
     unsigned j[U];
-    Trip<T> v[U], xv[U];
+    Trip<M> v[U];
+    Trip<T> xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const unsigned k = k0 + H * u;
@@ -451,7 +481,8 @@ __device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, u
       if (k0 + H * u < K) xv[u] = load_node<T, kVS>(x, j[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (k0 + H * u < K) acc -= v[u].a * xv[u].a + v[u].b * xv[u].b + v[u].c * xv[u].c;
+      if (k0 + H * u < K)
+        acc -= (T)v[u].a * xv[u].a + (T)v[u].b * xv[u].b + (T)v[u].c * xv[u].c;
   }
   return acc;
 }
@@ -475,9 +506,9 @@ inline unsigned sweep_grid(unsigned nsl) {
 // forward level >= 1: Pr = D^-1 (r - L Pr) on the level's slices (reads r
 // directly: the reference's copy Pr = r is fused away).  PART: the level
 // holds rows with no upper blocks; their (r, Pr) is summed here.
-template <typename T, bool PART, int SPLIT, int U>
+template <typename T, typename M, bool PART, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
-k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
+k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
   unsigned half, slot;
   const SliceLane L = sweep_lane<SPLIT>(slice0, half, slot);
@@ -487,7 +518,7 @@ k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
   T t = 0;
   if (i != ~0u) {
     if (half == 0) t = a.r[kVS * (size_t)i + L.c];
-    t = sliced_row_sub_part<T, U, SPLIT>(t, L, half, a.f_plan.len[L.s * kSliceRows + L.r],
+    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, a.f_plan.len[L.s * kSliceRows + L.r],
                                          a.f_plan.kmax[L.s], a.f_plan.off[L.s], a.f_plan.col,
                                          a.f_val, a.Pr);
   }
@@ -498,7 +529,7 @@ k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
     const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
     const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
     if (i != ~0u) {
-      const T out = dia_row_dot(a.f_dia, L, t0, t1, t2);
+      const T out = dia_row_dot_m<T, M>(a.f_dia, L, t0, t1, t2);
       a.Pr[kVS * (size_t)i + L.c] = out;
       if (PART && ((a.fumask[L.s] >> L.r) & 1u))
         part = (double)a.r[kVS * (size_t)i + L.c] * (double)out;
@@ -512,9 +543,9 @@ k_pcg_fwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
 
 // backward level + partial (r, Pr) of the rows it finalises, written at
 // dst[workgroup]
-template <typename T, int SPLIT, int U>
+template <typename T, typename M, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
-k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
+k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
   unsigned half, slot;
   const SliceLane L = sweep_lane<SPLIT>(slice0, half, slot);
@@ -523,7 +554,7 @@ k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
   T t = 0;
   if (i != ~0u) {
     if (half == 0) t = a.Pr[kVS * (size_t)i + L.c];
-    t = sliced_row_sub_part<T, U, SPLIT>(t, L, half, a.b_plan.len[L.s * kSliceRows + L.r],
+    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, a.b_plan.len[L.s * kSliceRows + L.r],
                                          a.b_plan.kmax[L.s], a.b_plan.off[L.s], a.b_plan.col,
                                          a.b_val, a.Pr);
   }
@@ -538,8 +569,8 @@ k_pcg_bwd(PcgArgs<T> a, unsigned slice0, unsigned nsl, double *dst) {
 }
 
 // p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue)
-template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T> a, int mode) {
+template <typename T, typename M>
+__global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T, M> a, int mode) {
   if (a.S->done) return;
   const double beta = a.S->beta;
   for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
@@ -551,27 +582,29 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T> a, int mod
   }
 }
 
-// x64 = 0, x32 = 0, r32 = (float) b, partial |b|^2 (FP64)
-__global__ void __launch_bounds__(kVecBlock) k_mixed_init(PcgArgs<float> a) {
+// x = 0 (and x64 = 0 for FP32 vectors), r = b, partial |b|^2 (FP64)
+template <typename T, typename M>
+__global__ void __launch_bounds__(kVecBlock) k_mixed_init(PcgArgs<T, M> a) {
   double s = 0.0;
   for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
        i += (size_t)gridDim.x * kVecBlock) {
     const double v = a.b64[i];
     s += v * v;
-    a.r[i] = (float)v;
-    a.x[i] = 0.0f;
-    a.x64[i] = 0.0;
+    a.r[i] = (T)v;
+    a.x[i] = (T)0;
+    if (sizeof(T) == 4) a.x64[i] = 0.0;
   }
   s = block_reduce_sum<kVecBlock>(s);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }
 
-// x = x64 + x32 (the solution at the last reliable update, or the cap)
+// the mixed solution: x64 + x32 for FP32 vectors, x itself for FP64
+template <typename T>
 __global__ void k_mixed_out(double *__restrict__ x, const double *__restrict__ x64,
-                            const float *__restrict__ x32, size_t n) {
+                            const T *__restrict__ xv, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
-    x[i] = x64[i] + (double)x32[i];
+    x[i] = sizeof(T) == 4 ? x64[i] + (double)xv[i] : (double)xv[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -579,6 +612,7 @@ __global__ void k_mixed_out(double *__restrict__ x, const double *__restrict__ x
 
 template <typename T>
 struct PcgBufs {
+  using elem = T;
   DBuf<T> r, x, Pr, p;
   DBuf<double> partials, partials2, partials3, hist;
   DBuf<PcgScal> S;
@@ -652,10 +686,16 @@ struct PcgWork {
   PcgScal *host_status = nullptr;  // pinned, 2 entries
   PcgBufs<double> f64;
   PcgBufs<float> f32;
+  PcgBufs<double> m64;  // mixed solver with FP64 vectors
   DBuf<double> b64, x64, hist64;
   Fusion fu;
-  int shape[2] = {-1, -1};          // sweep shape per precision [fp32]; -1: autotune
-  std::vector<double> tune_ms[2];   // last autotuner timings (ms per sweep pair)
+  // per solver kind (0 FP64, 1 mixed with FP32 vectors, 2 mixed with FP64
+  // vectors): sweep shape (-1: autotune) and the last autotuner timings
+  int shape[3] = {-1, -1, -1};
+  std::vector<double> tune_ms[3];
+  // mixed solver options: Krylov vector precision, forced-update period
+  int mixed_vectors64 = 0;
+  unsigned mixed_period = 50;
   PcgWork() {
     DFM2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     DFM2H_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
@@ -667,6 +707,7 @@ struct PcgWork {
   ~PcgWork() {
     f64.drop_graph();
     f32.drop_graph();
+    m64.drop_graph();
     if (host_status) (void)hipHostFree(host_status);
     for (hipEvent_t e : {ev_in, ev_out, ev_poll[0], ev_poll[1]})
       if (e) (void)hipEventDestroy(e);
@@ -724,12 +765,12 @@ static void ensure_dinv(dfm2h_bsr A, dfm2h_ilu P, PackedVals<T> &V, hipStream_t 
   V.version = P->version;
 }
 
-template <typename T>
-static PcgArgs<T> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const PackedVals<T> &av,
-                            const PackedVals<T> &fv, const PackedVals<T> &bv,
-                            const PackedVals<T> &dv) {
+template <typename T, typename M>
+static PcgArgs<T, M> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const PackedVals<M> &av,
+                               const PackedVals<M> &fv, const PackedVals<M> &bv,
+                               const PackedVals<M> &dv) {
   const Fusion &F = P->work->fu;
-  PcgArgs<T> a{};
+  PcgArgs<T, M> a{};
   a.nblk = A->nblk;
   a.n = (size_t)A->nblk * kVS;
   a.a_plan = view(A->plan);
@@ -794,18 +835,18 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
 
 // Prologue form: every forward and backward level, (r,Pr) partials into
 // `partials` (one per backward workgroup); returns their count.
-template <typename T>
-static unsigned launch_sweeps_full(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t s) {
+template <typename T, typename M>
+static unsigned launch_sweeps_full(const PcgArgs<T, M> &a, dfm2h_ilu P, hipStream_t s) {
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
-    hipLaunchKernelGGL((k_pcg_fwd<T, false, 1, 8>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
-                       a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
+    hipLaunchKernelGGL((k_pcg_fwd<T, M, false, 1, 8>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0,
+                       s, a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
   }
   unsigned off = 0;
   for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
     const unsigned g = slice_grid(nsl);
-    hipLaunchKernelGGL((k_pcg_bwd<T, 1, 8>), dim3(g), dim3(kPackBlock), 0, s, a,
+    hipLaunchKernelGGL((k_pcg_bwd<T, M, 1, 8>), dim3(g), dim3(kPackBlock), 0, s, a,
                        P->bplan.group_ptr[l], nsl, a.partials + off);
     off += g;
   }
@@ -814,38 +855,44 @@ static unsigned launch_sweeps_full(const PcgArgs<T> &a, dfm2h_ilu P, hipStream_t
 
 // Iteration form: forward levels >= 1 then backward levels >= 1 (level 0 of
 // each is fused away), (r,Pr) partials into partials3 at layout Lo.
-template <typename T, int SPLIT, int U>
-static void launch_sweeps_shape(const PcgArgs<T> &a, dfm2h_ilu P, const SweepLayout &Lo,
+template <typename T, typename M, int SPLIT, int U>
+static void launch_sweeps_shape(const PcgArgs<T, M> &a, dfm2h_ilu P, const SweepLayout &Lo,
                                 hipStream_t s, int which) {
   for (size_t l = 1; (which & 1) && l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
     const dim3 g(split_grid(nsl, SPLIT));
     if (Lo.fwd_off[l] != ~0u)
-      hipLaunchKernelGGL((k_pcg_fwd<T, true, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
+      hipLaunchKernelGGL((k_pcg_fwd<T, M, true, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
                          P->fplan.group_ptr[l], nsl, a.partials3 + Lo.fwd_off[l]);
     else
-      hipLaunchKernelGGL((k_pcg_fwd<T, false, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
+      hipLaunchKernelGGL((k_pcg_fwd<T, M, false, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
                          P->fplan.group_ptr[l], nsl, (double *)nullptr);
   }
   for (size_t l = 1; (which & 2) && l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
-    hipLaunchKernelGGL((k_pcg_bwd<T, SPLIT, U>), dim3(split_grid(nsl, SPLIT)), dim3(kPackBlock),
+    hipLaunchKernelGGL((k_pcg_bwd<T, M, SPLIT, U>), dim3(split_grid(nsl, SPLIT)), dim3(kPackBlock),
                        0, s, a, P->bplan.group_ptr[l], nsl, a.partials3 + Lo.bwd_off[l]);
   }
 }
 
 // which: 1 forward, 2 backward, 3 both
-template <typename T>
-static void launch_sweeps_fused(const PcgArgs<T> &a, dfm2h_ilu P, int shape,
+template <typename T, typename M>
+static void launch_sweeps_fused(const PcgArgs<T, M> &a, dfm2h_ilu P, int shape,
                                 const SweepLayout &Lo, hipStream_t s, int which = 3) {
   switch (shape) {
-    case 0: launch_sweeps_shape<T, 1, 8>(a, P, Lo, s, which); break;
-    case 1: launch_sweeps_shape<T, 1, 4>(a, P, Lo, s, which); break;
-    case 2: launch_sweeps_shape<T, 2, 8>(a, P, Lo, s, which); break;
-    case 3: launch_sweeps_shape<T, 2, 4>(a, P, Lo, s, which); break;
-    case 4: launch_sweeps_shape<T, 1, 16>(a, P, Lo, s, which); break;
-    default: launch_sweeps_shape<T, 2, 16>(a, P, Lo, s, which); break;
+    case 0: launch_sweeps_shape<T, M, 1, 8>(a, P, Lo, s, which); break;
+    case 1: launch_sweeps_shape<T, M, 1, 4>(a, P, Lo, s, which); break;
+    case 2: launch_sweeps_shape<T, M, 2, 8>(a, P, Lo, s, which); break;
+    case 3: launch_sweeps_shape<T, M, 2, 4>(a, P, Lo, s, which); break;
+    case 4: launch_sweeps_shape<T, M, 1, 16>(a, P, Lo, s, which); break;
+    default: launch_sweeps_shape<T, M, 2, 16>(a, P, Lo, s, which); break;
   }
+}
+
+// solver kind: 0 FP64, 1 mixed with FP32 vectors, 2 mixed with FP64 vectors
+template <typename T, typename M>
+constexpr int solver_kind() {
+  return sizeof(M) == 8 ? 0 : (sizeof(T) == 4 ? 1 : 2);
 }
 
 template <int MODE>
@@ -858,17 +905,17 @@ static void launch_scalar(hipStream_t s, PcgScal *S, const double *pa, unsigned 
 // One FP64 iteration (lsitrsol.h:107-146): spmv, alpha, update (+ forward
 // level 0), forward levels >= 1, backward levels >= 1, stop test + beta,
 // p update.
-template <typename T>
-static void launch_iteration(const PcgArgs<T> &a, dfm2h_ilu P, int shape, const SweepLayout &Lo,
-                             hipStream_t s) {
+template <typename T, typename M>
+static void launch_iteration(const PcgArgs<T, M> &a, dfm2h_ilu P, int shape,
+                             const SweepLayout &Lo, hipStream_t s) {
   const unsigned gs = slice_grid(a.a_nsl);
   const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
-  hipLaunchKernelGGL(k_pcg_update<T>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, shape, Lo, s);
   launch_scalar<kScalConvBeta>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
-  hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
 // Captures `chunk` iterations into B's graph unless its signature matches.
@@ -908,8 +955,8 @@ static void run_chunks(PcgWork &W, PcgBufs<T> &B, unsigned nchunks, PcgScal &sta
   status = W.host_status[0];
 }
 
-template <typename T>
-static std::vector<uintptr_t> graph_sig(const PcgArgs<T> &a, dfm2h_bsr A, dfm2h_ilu P,
+template <typename T, typename M>
+static std::vector<uintptr_t> graph_sig(const PcgArgs<T, M> &a, dfm2h_bsr A, dfm2h_ilu P,
                                         unsigned chunk) {
   return {(uintptr_t)a.a_val,    (uintptr_t)a.a_dia,     (uintptr_t)a.f_val,
           (uintptr_t)a.f_dia,    (uintptr_t)a.b_val,     (uintptr_t)a.dinv0,
@@ -918,7 +965,8 @@ static std::vector<uintptr_t> graph_sig(const PcgArgs<T> &a, dfm2h_bsr A, dfm2h_
           (uintptr_t)a.a64_val,  (uintptr_t)a.a64_dia,   (uintptr_t)a.x64,
           (uintptr_t)a.b64,      (uintptr_t)a.hist64,    (uintptr_t)chunk,
           (uintptr_t)A,          (uintptr_t)A->pattern_serial, (uintptr_t)P->pattern_serial,
-          (uintptr_t)P->work->fu.serial, (uintptr_t)P->work->shape[sizeof(T) == 4 ? 1 : 0]};
+          (uintptr_t)P->work->fu.serial,
+          (uintptr_t)P->work->shape[solver_kind<T, M>()]};
 }
 
 // Autotuner of the sweep shape: times kRep forward + backward sweep pairs
@@ -926,8 +974,8 @@ static std::vector<uintptr_t> graph_sig(const PcgArgs<T> &a, dfm2h_bsr A, dfm2h_
 // the one-wavefront-per-slice shapes, whose sums run in the reference's
 // order).  The sweeps only rewrite Pr, which the prologue recomputes
 // afterwards.
-template <typename T>
-static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T> &a0,
+template <typename T, typename M>
+static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T, M> &a0,
                        std::vector<double> &ms) {
   constexpr int kRep = 20;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -937,9 +985,9 @@ static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T> &a
   int best = 0;
   for (int v = 0; v < kNumSweepShapes; ++v) {
     // FP64 keeps the reference's summation order (one wavefront per slice)
-    if (sizeof(T) == 8 && kSweepSplit[v] != 1) continue;
+    if (sizeof(M) == 8 && kSweepSplit[v] != 1) continue;
     const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[v]);
-    PcgArgs<T> a = a0;
+    PcgArgs<T, M> a = a0;
     a.upd_part_off = Lo.upd_off;
     launch_sweeps_fused(a, P, v, Lo, W.s);  // warm-up
     DFM2H_HIP(hipEventRecord(e0, W.s));
@@ -960,10 +1008,10 @@ static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T> &a
 // The sweep shape of this solve: the handle's, else the cached one for
 // (DOF bucket, precision), else measured now.  Called after the
 // prologue's first scalar step; a solve that is already done skips tuning.
-template <typename T>
+template <typename T, typename M>
 static int resolve_shape(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
-                         const PcgArgs<T> &a) {
-  const int p = sizeof(T) == 4 ? 1 : 0;
+                         const PcgArgs<T, M> &a) {
+  const int p = solver_kind<T, M>();
   if (W.shape[p] >= 0) return W.shape[p];
   const int bucket = dof_bucket(3 * (size_t)A->nblk);
   int v = tune_lookup("pcg", bucket, p, kNumSweepShapes);
@@ -972,7 +1020,7 @@ static int resolve_shape(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
     DFM2H_HIP(hipMemcpyAsync(&st, B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, W.s));
     DFM2H_HIP(hipStreamSynchronize(W.s));
     if (st.done) return 0;
-    v = tune_sweeps<T>(A, P, W, a, W.tune_ms[p]);
+    v = tune_sweeps<T, M>(A, P, W, a, W.tune_ms[p]);
     tune_store("pcg", bucket, p, v, W.tune_ms[p][v]);
   }
   W.shape[p] = v;
@@ -980,9 +1028,9 @@ static int resolve_shape(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
 }
 
 // Runs the fused PCG on B.r (holding b) into B.x; fills status.
-template <typename T>
-static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const PcgArgs<T> &a0,
-                    double tol, unsigned max_nitr, PcgScal &status) {
+template <typename T, typename M>
+static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
+                    const PcgArgs<T, M> &a0, double tol, unsigned max_nitr, PcgScal &status) {
   hipStream_t s = W.s;
   // scalars: tol and cap (a plain copy into the scalar block, not a memset)
   PcgScal init{};
@@ -990,15 +1038,15 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
   init.max_nitr = (int)max_nitr;
   DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
   const unsigned gv = vec_grid(a0.n);
-  hipLaunchKernelGGL(k_pcg_init<T>, dim3(gv), dim3(kVecBlock), 0, s, a0);
+  hipLaunchKernelGGL((k_pcg_init<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a0);
   launch_scalar<kScalInit>(s, a0.S, a0.partials, gv, nullptr, 0, a0.hist);
   const int shape = resolve_shape(A, P, W, B, a0);
   const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
-  PcgArgs<T> a = a0;
+  PcgArgs<T, M> a = a0;
   a.upd_part_off = Lo.upd_off;
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
-  hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
   const unsigned chunk = std::max(1u, std::min(16u, max_nitr));
   ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk,
@@ -1013,38 +1061,41 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const P
 // FP64, tested in FP64 by the alpha step (stop when |r| / |b| < tol); the
 // r/x update then folds x32 into x64 and the same Krylov sequence goes on
 // from the true residual.
-static void launch_mixed_iteration(const PcgArgs<float> &a, dfm2h_ilu P, int shape,
+template <typename T>
+static void launch_mixed_iteration(const PcgArgs<T, float> &a, dfm2h_ilu P, int shape,
                                    const SweepLayout &Lo, hipStream_t s) {
   const unsigned gs = slice_grid(a.a_nsl);
   const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL((k_pcg_spmv<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64);
-  hipLaunchKernelGGL(k_pcg_update<float>, dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, shape, Lo, s);
   launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
-  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 1);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
-static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<float> &B,
-                          const PcgArgs<float> &a0, double tol, double delta, unsigned max_outer,
-                          unsigned max_inner, PcgScal &status) {
+template <typename T>
+static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
+                          const PcgArgs<T, float> &a0, double tol, double delta,
+                          unsigned max_outer, unsigned max_inner, PcgScal &status) {
   hipStream_t s = W.s;
   PcgScal init{};
   init.tol = tol;
   init.max_nitr = (int)max_inner;
   init.delta2 = delta * delta;
   init.max_upd = (int)max_outer;
+  init.period = (int)W.mixed_period;
   DFM2H_HIP(hipMemcpyAsync(B.S.p, &init, sizeof(PcgScal), hipMemcpyHostToDevice, s));
   const unsigned gv = vec_grid(a0.n);
-  hipLaunchKernelGGL(k_mixed_init, dim3(gv), dim3(kVecBlock), 0, s, a0);
+  hipLaunchKernelGGL((k_mixed_init<T, float>), dim3(gv), dim3(kVecBlock), 0, s, a0);
   launch_scalar<kScalInitMixed>(s, a0.S, a0.partials, gv, nullptr, 0, a0.hist64);
   const int shape = resolve_shape(A, P, W, B, a0);
   const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
-  PcgArgs<float> a = a0;
+  PcgArgs<T, float> a = a0;
   a.upd_part_off = Lo.upd_off;
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
-  hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
   check_launch("mixed prologue");
   const unsigned chunk = std::max(1u, std::min(16u, max_inner));
   ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk,
@@ -1080,6 +1131,39 @@ static void need_solver(dfm2h_bsr A, dfm2h_ilu P) {
   DFM2H_REQUIRE(A && P, DFM2H_ERR_INVALID, "null handle");
   DFM2H_REQUIRE(A->has_pattern && P->has_pattern, DFM2H_ERR_STATE, "pattern not set");
   DFM2H_REQUIRE(A->nblk == P->nblk, DFM2H_ERR_INVALID, "matrix / preconditioner size mismatch");
+}
+
+// mixed solve with Krylov vectors of precision T; returns the status
+template <typename T>
+static PcgScal mixed_solve(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, const double *b,
+                           double *x, double tol, double delta, unsigned max_outer,
+                           unsigned max_inner) {
+  ensure_bufs(B, A, P, max_inner);
+  const size_t nn = A->nblk;
+  if (W.b64.n != kVS * nn) {
+    W.b64.alloc(kVS * nn);
+    W.x64.alloc(kVS * nn);
+    W.b64.zero();
+    W.x64.zero();
+  }
+  if (W.hist64.n < (size_t)max_outer + 2) W.hist64.alloc((size_t)max_outer + 2);
+  fence_in(W);
+  ensure_f32(A, P, W.s);
+  ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, W.s);
+  hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s,
+                     W.b64.p, b, nn);
+  PcgArgs<T, float> a = make_args<T, float>(A, P, B, A->pk32, P->f32, P->b32, W.fu.d32);
+  a.x64 = W.x64.p;
+  a.b64 = W.b64.p;
+  a.a64_val = A->pk64.val.p;
+  a.a64_dia = A->pk64.dia.p;
+  a.hist64 = W.hist64.p;
+  PcgScal st{};
+  run_pcg_mixed(A, P, W, B, a, tol, delta, max_outer, max_inner, st);
+  hipLaunchKernelGGL(k_mixed_out<T>, dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x, W.x64.p,
+                     B.x.p, 3 * nn);
+  check_launch("k_mixed_out");
+  return st;
 }
 
 extern "C" {
@@ -1154,7 +1238,8 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
     hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s, B.r.p,
                        r, nn);
     ensure_f64(A, P, W.s);
-    const PcgArgs<double> a = make_args<double>(A, P, B, A->pk64, P->f64, P->b64, W.fu.d64);
+    const PcgArgs<double, double> a =
+        make_args<double, double>(A, P, B, A->pk64, P->f64, P->b64, W.fu.d64);
     PcgScal st{};
     run_pcg(A, P, W, B, a, tol, max_nitr, st);
     hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, r,
@@ -1184,33 +1269,11 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
     DFM2H_REQUIRE(delta > 0.0 && delta < 1.0, DFM2H_ERR_INVALID, "delta must be in (0, 1)");
     DFM2H_REQUIRE(max_outer >= 1 && max_inner >= 1, DFM2H_ERR_INVALID, "caps must be >= 1");
     PcgWork &W = work_of(P);
-    PcgBufs<float> &B = W.f32;
     ensure_fusion(A, P, W);
-    ensure_bufs(B, A, P, max_inner);
-    const size_t nn = A->nblk;
-    if (W.b64.n != kVS * nn) {
-      W.b64.alloc(kVS * nn);
-      W.x64.alloc(kVS * nn);
-      W.b64.zero();
-      W.x64.zero();
-    }
-    if (W.hist64.n < (size_t)max_outer + 2) W.hist64.alloc((size_t)max_outer + 2);
-    fence_in(W);
-    ensure_f32(A, P, W.s);
-    ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, W.s);
-    hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s,
-                       W.b64.p, b, nn);
-    PcgArgs<float> a = make_args<float>(A, P, B, A->pk32, P->f32, P->b32, W.fu.d32);
-    a.x64 = W.x64.p;
-    a.b64 = W.b64.p;
-    a.a64_val = A->pk64.val.p;
-    a.a64_dia = A->pk64.dia.p;
-    a.hist64 = W.hist64.p;
-    PcgScal st{};
-    run_pcg_mixed(A, P, W, B, a, tol, delta, max_outer, max_inner, st);
-    hipLaunchKernelGGL(k_mixed_out, dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x, W.x64.p, B.x.p,
-                       3 * nn);
-    check_launch("k_mixed_out");
+    const PcgScal st =
+        W.mixed_vectors64
+            ? mixed_solve<double>(A, P, W, W.m64, b, x, tol, delta, max_outer, max_inner)
+            : mixed_solve<float>(A, P, W, W.f32, b, x, tol, delta, max_outer, max_inner);
     std::vector<double> h((size_t)st.nupd + 1);
     DFM2H_HIP(hipMemcpyAsync(h.data(), W.hist64.p, h.size() * sizeof(double),
                              hipMemcpyDeviceToHost, W.s));
@@ -1223,14 +1286,28 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
   });
 }
 
+int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_period) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null handle");
+    PcgWork &W = work_of(P);
+    W.mixed_vectors64 = fp64_vectors ? 1 : 0;
+    W.mixed_period = update_period;
+  });
+}
+
+// the solver kind a `fp32` argument names: 0 FP64, else the mixed solver
+// with its current vector precision
+static int kind_of(PcgWork &W, int fp32) { return fp32 ? (W.mixed_vectors64 ? 2 : 1) : 0; }
+
 int dfm2h_pcg_set_shape(dfm2h_ilu P, int fp32, int shape) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null handle");
     DFM2H_REQUIRE(shape >= -1 && shape < kNumSweepShapes, DFM2H_ERR_INVALID, "bad shape");
     PcgWork &W = work_of(P);
-    W.shape[fp32 ? 1 : 0] = shape;
+    W.shape[kind_of(W, fp32)] = shape;
     W.f64.drop_graph();
     W.f32.drop_graph();
+    W.m64.drop_graph();
   });
 }
 
@@ -1239,7 +1316,7 @@ int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name, do
   return guard(__func__, [&] {
     DFM2H_REQUIRE(P && shape, DFM2H_ERR_INVALID, "null argument");
     PcgWork &W = work_of(P);
-    const int p = fp32 ? 1 : 0;
+    const int p = kind_of(W, fp32);
     *shape = W.shape[p];
     if (name) *name = W.shape[p] >= 0 ? kSweepNames[W.shape[p]] : "autotune";
     for (size_t k = 0; tune_ms && k < cap && k < W.tune_ms[p].size(); ++k) tune_ms[k] = W.tune_ms[p][k];
@@ -1256,9 +1333,10 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
     // consecutive kernel classes, so every class is timed inside the real
     // dependency chain (as in the captured graph).  Convergence and
     // refinement are disabled for the duration.
-    auto run = [&](auto tag, auto &B, const auto &a0) {
+    auto run = [&](auto tag, auto mtag, auto &B, const auto &a0) {
       using T = decltype(tag);
-      const int shape = std::max(0, W.shape[sizeof(T) == 4 ? 1 : 0]);
+      using M = decltype(mtag);
+      const int shape = std::max(0, W.shape[solver_kind<T, M>()]);
       const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
       auto a = a0;
       a.upd_part_off = Lo.upd_off;
@@ -1281,14 +1359,14 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       double acc[kSeg] = {0};
       for (unsigned rep = 0; rep <= nrep; ++rep) {
         DFM2H_HIP(hipEventRecord(ev[0], W.s));
-        hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(gs), dim3(kPackBlock), 0, W.s, a);
+        hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[1], W.s));
-        if constexpr (sizeof(T) == 4)
+        if constexpr (is_mixed<T, M>())
           launch_scalar<kScalAlphaMixed>(W.s, a.S, a.partials, gs, a.partials2, gs, a.hist64);
         else
           launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[2], W.s));
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(gs), dim3(kPackBlock), 0, W.s, a);
+        hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[3], W.s));
         DFM2H_HIP(hipEventRecord(ev[4], W.s));
         DFM2H_HIP(hipEventRecord(ev[5], W.s));
@@ -1296,11 +1374,11 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         DFM2H_HIP(hipEventRecord(ev[6], W.s));
         launch_sweeps_fused(a, P, shape, Lo, W.s, 2);
         DFM2H_HIP(hipEventRecord(ev[7], W.s));
-        if constexpr (sizeof(T) == 4)
+        if constexpr (is_mixed<T, M>())
           launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
         else
           launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
-        hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(gv), dim3(kVecBlock), 0, W.s, a, 1);
+        hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, W.s, a, 1);
         DFM2H_HIP(hipEventRecord(ev[8], W.s));
         DFM2H_HIP(hipEventSynchronize(ev[8]));
         if (rep == 0) continue;  // warm-up iteration
@@ -1325,20 +1403,31 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       DFM2H_HIP(hipStreamSynchronize(W.s));
     };
     if (fp32) {
-      DFM2H_REQUIRE(W.f32.S.p && A->pk32.val.p && W.x64.p && W.fu.A == A, DFM2H_ERR_STATE,
+      DFM2H_REQUIRE(A->pk32.val.p && W.x64.p && W.fu.A == A, DFM2H_ERR_STATE,
                     "run a mixed solve first");
-      PcgArgs<float> a = make_args<float>(A, P, W.f32, A->pk32, P->f32, P->b32, W.fu.d32);
-      a.x64 = W.x64.p;
-      a.b64 = W.b64.p;
-      a.a64_val = A->pk64.val.p;
-      a.a64_dia = A->pk64.dia.p;
-      a.hist64 = W.hist64.p;
-      run(float{}, W.f32, a);
+      auto mixed_args = [&](auto &B) {
+        using T = typename std::remove_reference_t<decltype(B)>::elem;
+        PcgArgs<T, float> a = make_args<T, float>(A, P, B, A->pk32, P->f32, P->b32, W.fu.d32);
+        a.x64 = W.x64.p;
+        a.b64 = W.b64.p;
+        a.a64_val = A->pk64.val.p;
+        a.a64_dia = A->pk64.dia.p;
+        a.hist64 = W.hist64.p;
+        return a;
+      };
+      if (W.mixed_vectors64) {
+        DFM2H_REQUIRE(W.m64.S.p, DFM2H_ERR_STATE, "run a mixed solve first");
+        run(double{}, float{}, W.m64, mixed_args(W.m64));
+      } else {
+        DFM2H_REQUIRE(W.f32.S.p, DFM2H_ERR_STATE, "run a mixed solve first");
+        run(float{}, float{}, W.f32, mixed_args(W.f32));
+      }
     } else {
       DFM2H_REQUIRE(W.f64.S.p && A->pk64.val.p && W.fu.A == A, DFM2H_ERR_STATE,
                     "run an FP64 solve first");
-      PcgArgs<double> a = make_args<double>(A, P, W.f64, A->pk64, P->f64, P->b64, W.fu.d64);
-      run(double{}, W.f64, a);
+      PcgArgs<double, double> a =
+          make_args<double, double>(A, P, W.f64, A->pk64, P->f64, P->b64, W.fu.d64);
+      run(double{}, double{}, W.f64, a);
     }
   });
 }

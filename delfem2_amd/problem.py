@@ -66,7 +66,7 @@ class DeviceSystem:
         hist = S.Solve_PCG(r, x, tol, max_nitr, self.A, self.ilu)
         return x, hist, r
 
-    def solve_mixed(self, tol: float = 1e-8, delta: float = 1e-2, max_outer: int = 200,
+    def solve_mixed(self, tol: float = 1e-8, delta: float = 1e-1, max_outer: int = 200,
                     max_inner: int = 20000):
         x = torch.zeros_like(self.b)
         hist, n_inner = S.Solve_PCG_Mixed(self.b, x, tol, self.A, self.ilu, delta,
@@ -88,7 +88,7 @@ class BatchSystem(DeviceSystem):
             self.solver = S.BatchSolver(self.A, self.ilu, self.batch.node_offset)
         return ok
 
-    def solve_batch(self, tol: float = 1e-8, fp32: bool = False, delta: float = 1e-2,
+    def solve_batch(self, tol: float = 1e-8, fp32: bool = False, delta: float = 1e-1,
                     max_nitr: int = 20000):
         x = torch.zeros_like(self.b)
         it, rr, st = self.solver.solve(self.b, x, tol, fp32, delta, max_nitr)

@@ -226,6 +226,14 @@ class CPreconditionerILU:
         N.call("dfm2h_ilu_levels", self.h, C.byref(f), C.byref(b))
         return f.value, b.value
 
+    def set_mixed_options(self, fp64_vectors: bool = False, update_period: int = 50) -> None:
+        """Solve_PCG_Mixed options: Krylov vector precision (default FP32;
+        the matrix and factors are streamed in FP32 either way) and a forced
+        reliable update after `update_period` inner iterations without one
+        (0: off; default 50)."""
+        N.call("dfm2h_pcg_set_mixed_options", self.h, int(bool(fp64_vectors)),
+               int(update_period))
+
     def set_sweep_shape(self, fp32: bool, shape: int) -> None:
         """Sweep launch shape of the fused solvers (-1 = autotune)."""
         N.call("dfm2h_pcg_set_shape", self.h, int(bool(fp32)), int(shape))
@@ -312,7 +320,7 @@ def pcg_kernel_times(mat: CMatrixSparse, ilu: CPreconditionerILU, fp32: bool,
 
 
 def Solve_PCG_Mixed(b_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float,
-                    mat: CMatrixSparse, ilu: CPreconditionerILU, delta: float = 1e-2,
+                    mat: CMatrixSparse, ilu: CPreconditionerILU, delta: float = 1e-1,
                     max_outer: int = 200, max_inner: int = 20000):
     """FP32 ILU(0)-PCG inside FP64 refinement by reliable updates (no
     reference counterpart): refine whenever the FP32 residual fell by `delta`.
@@ -352,7 +360,7 @@ class BatchSolver:
         return {"nmesh": nm.value, "nslices": ns.value, "levels": (f.value, b.value)}
 
     def solve(self, b: torch.Tensor, x: torch.Tensor, tol: float, fp32: bool = False,
-              delta: float = 1e-2, max_nitr: int = 20000):
+              delta: float = 1e-1, max_nitr: int = 20000):
         """Returns (iterations, relative residuals, state) per mesh; state is 1
         converged, 0 iteration cap, -1 NaN."""
         it = np.zeros(self.nmesh, np.uint32)
@@ -374,6 +382,11 @@ class BatchSolver:
 
     def set_variant(self, fp32: bool, variant: int) -> None:
         N.call("dfm2h_batch_set_variant", self.h, int(bool(fp32)), int(variant))
+
+    def set_update_period(self, period: int) -> None:
+        """FP32 form: forced reliable update after `period` inner iterations
+        without one (0: off; default 50)."""
+        N.call("dfm2h_batch_set_update_period", self.h, int(period))
 
     def variant(self, fp32: bool):
         v = C.c_int(-1)

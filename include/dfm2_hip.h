@@ -203,6 +203,15 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
                           double *hist, size_t hist_cap, size_t *n_hist,
                           size_t *n_inner);
 
+/* Options of the mixed solver (no reference counterpart).
+ * fp64_vectors = 0 (default): r, x, p, Pr in FP32 (the solution kept as
+ * x64 + x32), dot products accumulated in FP64; = 1: the vectors in FP64,
+ * the matrix and factor values still streamed in FP32 (widened before the
+ * products).  update_period > 0 forces a reliable update after that many
+ * inner iterations without one (0: only the delta rule).  Defaults: FP32
+ * vectors, period 50 (measured on MI355X: with delta = 0.1 it keeps the
+ * inner iteration count of the 64^3 cube at the FP64 solver's). */
+int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_period);
 /* Launch shape of the fused solvers' triangular sweeps (no reference
  * counterpart): shape s = {wavefronts per 21-row slice, blocks per load
  * round} from { {1,8}, {1,4}, {2,8}, {2,4}, {1,16}, {2,16} }; with two
@@ -211,10 +220,12 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
  * shapes 0, 1, 4 keep it).  -1 (the default): the first solve of each
  * precision times every shape (FP64: shapes 0, 1, 4 only) on its own
  * sweeps and keeps the fastest, per
- * (floor(log2 ndof), precision) for the process (and in the file named by
- * DFM2H_TUNE_CACHE, if set).  get_shape: current shape, its name, and the
- * autotuner's ms per forward+backward sweep pair of every shape (0 where
- * not measured); n_shapes = 6. */
+ * (floor(log2 ndof), solver kind) for the process (and in the file named
+ * by DFM2H_TUNE_CACHE, if set).  fp32 = 0 names the FP64 solver, else the
+ * mixed solver with its current vector precision (each has its own shape).
+ * get_shape: current shape, its name, and the autotuner's ms per
+ * forward+backward sweep pair of every shape (0 where not measured);
+ * n_shapes = 6. */
 int dfm2h_pcg_set_shape(dfm2h_ilu P, int fp32, int shape);
 int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name,
                         double *tune_ms, size_t cap, size_t *n_shapes);
@@ -285,6 +296,9 @@ int dfm2h_batch_solve(dfm2h_batch B, const double *b_dev, double *x_dev,
  * names a file it is read from and appended to that file as well. */
 int dfm2h_batch_tune(dfm2h_batch B, int fp32, int *best, double *ms,
                      size_t ms_cap, size_t *n_variants);
+/* FP32 form: force a reliable update of a mesh after update_period inner
+ * iterations without one (0: only the delta rule; default 50). */
+int dfm2h_batch_set_update_period(dfm2h_batch B, unsigned update_period);
 /* variant -1 = autotune on the next solve; 0 .. n_variants-1 = fixed. */
 int dfm2h_batch_set_variant(dfm2h_batch B, int fp32, int variant);
 int dfm2h_batch_get_variant(dfm2h_batch B, int fp32, int *variant,

@@ -150,16 +150,18 @@ def test_pcg_c1_golden(gpu):
     assert np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr) <= FP64_X_TOL
 
 
-@pytest.mark.parametrize("case", ["c1_color8", "c12_sigma05_color4", "n7_sigma1_color8"])
-def test_pcg_mixed_vs_oracle(gpu, case):
+@pytest.mark.parametrize("vec64", [True, False])
+@pytest.mark.parametrize("case", ["c1", "c1_color8", "c12_sigma05_color4", "n7_sigma1_color8"])
+def test_pcg_mixed_vs_oracle(gpu, case, vec64):
     prob, _ = CASES[case]()
     C = cpu_system(prob)
     C.factor()
     xc, _ = C.solve_pcg(C.b.copy(), 1e-12, 5000)
     G = DeviceSystem(prob)
+    G.ilu.set_mixed_options(vec64)
     G.assemble()
     G.factor()
-    x, hist, n_inner = G.solve_mixed(1e-8, delta=1e-2)
+    x, hist, n_inner = G.solve_mixed(1e-8, delta=1e-1)
     assert hist[-1] / hist[0] < 1e-8
     assert n_inner > 0
     xg = x.cpu().numpy()
@@ -316,13 +318,20 @@ def test_c3_mixed_128cube_to_1e10(gpu):
     G = DeviceSystem(prob)
     assert G.assemble()
     assert G.factor()
-    xm, hm, n_inner = G.solve_mixed(1e-10, delta=1e-2, max_outer=400, max_inner=40000)
+    x64, h64, _ = G.solve(1e-12, 40000)
+    for vec64 in (True, False):
+        G.ilu.set_mixed_options(vec64)
+        c3_mixed_check(G, x64, h64, vec64)
+
+
+def c3_mixed_check(G, x64, h64, vec64):
+    xm, hm, n_inner = G.solve_mixed(1e-10, delta=1e-1, max_outer=400, max_inner=40000)
     r = G.b.clone()
     G.A.MatVec(r, -1.0, xm, 1.0)
     relres = (torch.linalg.norm(r) / torch.linalg.norm(G.b)).item()
-    x64, h64, _ = G.solve(1e-12, 40000)
     err = (torch.linalg.norm(xm - x64) / torch.linalg.norm(x64)).item()
-    print(f"[c3] mixed: {n_inner} inner / {len(hm) - 1} outer, true relres {relres:.3e}, "
+    print(f"[c3] mixed (FP{64 if vec64 else 32} vectors): {n_inner} inner / {len(hm) - 1} "
+          f"outer, true relres {relres:.3e}, "
           f"rel-L2 vs FP64(1e-12) {err:.3e}; FP64 iterations {len(h64) - 1}", flush=True)
     assert hm[-1] / hm[0] < 1e-10
     assert relres < 1.5e-10

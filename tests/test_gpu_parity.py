@@ -336,3 +336,31 @@ def c3_mixed_check(G, x64, h64, vec64):
     assert hm[-1] / hm[0] < 1e-10
     assert relres < 1.5e-10
     assert err <= MIXED_X_TOL
+
+
+def test_isolated_row(gpu):
+    """A node no element uses (clamped, with a non-zero right-hand side set
+    after the BCs): its row has neither lower nor upper blocks, so the fused
+    iteration sums its (r, Pr) share in the update kernel."""
+    prob = W.cube(4, sigma=0.5, seed=3, ordering="color4")
+    xyz = np.concatenate([prob.xyz, [0.5, 0.5, 2.0]])
+    flag = np.concatenate([prob.flag, [1, 1, 1]]).astype(np.int32)
+    p2 = W.Problem("iso", xyz, prob.tet, prob.myu, prob.lam, flag, g=prob.g)
+    rhs = np.array([1.0, -2.0, 0.5])
+    C = cpu_system(p2)
+    C.b[-3:] = rhs
+    C.factor()
+    xc, hc = C.solve_pcg(C.b.copy(), 1e-10, 5000)
+    G = DeviceSystem(p2)
+    G.assemble()
+    G.b[-3:] = torch.from_numpy(rhs).cuda()
+    G.factor()
+    x, hist, _ = G.solve(1e-10, 5000)
+    assert abs(len(hist) - len(hc)) <= 1
+    assert np.linalg.norm(x.cpu().numpy() - xc) / np.linalg.norm(xc) <= FP64_X_TOL
+    assert np.allclose(x[-3:].cpu().numpy(), rhs, rtol=0, atol=1e-12)
+    for vec64 in (False, True):
+        G.ilu.set_mixed_options(vec64)
+        xm, hm, _ = G.solve_mixed(1e-10)
+        assert hm[-1] / hm[0] < 1e-10
+        assert np.linalg.norm(xm.cpu().numpy() - xc) / np.linalg.norm(xc) <= MIXED_X_TOL

@@ -26,16 +26,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(nb: int, nnzb: int, s: int, read_y: bool) -> int:
+def spmv_bytes(nb: int, nnzb: int, s: int, read_y: bool, sv: int = 0) -> int:
     """Algorithmic bytes of one 3x3 BSR MatVec: off-diagonal blocks and their
     column indices, diagonal blocks, row starts, x read once, y written
-    (+ read once when y is scaled by beta)."""
+    (+ read once when y is scaled by beta); s = bytes per stored value, sv =
+    bytes per vector entry (default s)."""
+    sv = sv or s
     ndof = 3 * nb
-    b = nnzb * (9 * s + 4) + nb * 9 * s + (nb + 1) * 4 + 2 * ndof * s
-    return b + (ndof * s if read_y else 0)
+    b = nnzb * (9 * s + 4) + nb * 9 * s + (nb + 1) * 4 + 2 * ndof * sv
+    return b + (ndof * sv if read_y else 0)
 
 
-def kernel_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0) -> dict:
+def kernel_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0, sv: int = 0) -> dict:
     """Algorithmic bytes of one launch class of the fused PCG iteration (s =
     bytes per scalar; column indices 4 B; nnzb off-diagonal blocks, nL =
     nnzb/2 lower = upper blocks; n0 rows without lower blocks = forward level 0, computed by the
@@ -48,16 +50,18 @@ def kernel_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0) -> dict:
              of schedule, r read, Pr written
       bwd    (levels >= 1) upper blocks + indices, 8 B per row, Pr read and
              written, r read for (r, Pr)
-      pupd   p, Pr read; p written"""
+      pupd   p, Pr read; p written
+    s: bytes per stored matrix / factor value; sv: per vector entry (default s)"""
+    sv = sv or s
     ndof = 3 * nb
     nl = nnzb // 2
     nf, nbk = nb - n0, nb - nu0
     return {
-        "spmv": spmv_bytes(nb, nnzb, s, False),
-        "update": 6 * ndof * s + n0 * (9 * s + 3 * s) + 8 * ((nb + 20) // 21),
-        "fwd_sweep": nl * (9 * s + 4) + nf * 9 * s + nf * 8 + 2 * 3 * nf * s,
-        "bwd_sweep": nl * (9 * s + 4) + nbk * 8 + 3 * 3 * nbk * s,
-        "pupdate": 3 * ndof * s,
+        "spmv": spmv_bytes(nb, nnzb, s, False, sv),
+        "update": 6 * ndof * sv + n0 * (9 * s + 3 * sv) + 8 * ((nb + 20) // 21),
+        "fwd_sweep": nl * (9 * s + 4) + nf * 9 * s + nf * 8 + 2 * 3 * nf * sv,
+        "bwd_sweep": nl * (9 * s + 4) + nbk * 8 + 3 * 3 * nbk * sv,
+        "pupdate": 3 * ndof * sv,
     }
 
 
@@ -76,8 +80,8 @@ def level0_rows(psup_ind, psup, nb: int):
     return int((~has_lo).sum()), int((~has_up).sum())
 
 
-def pcg_iter_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0) -> int:
-    return sum(kernel_bytes(nb, nnzb, s, n0, nu0).values())
+def pcg_iter_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0, sv: int = 0) -> int:
+    return sum(kernel_bytes(nb, nnzb, s, n0, nu0, sv).values())
 
 
 def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
@@ -272,7 +276,7 @@ def bench_cube128(args):
         cpu = cpu_baseline(prob, it64, min(args.cpu_iters, 20))
         cpu["unit"] = "s (time-to-1e-10 = Decompose + FP64 PCG, iterations x measured s/iteration)"
     n0, nu0 = level0_rows(G.psup_ind, G.psup, nb)
-    b32 = pcg_iter_bytes(nb, nnzb, 4, n0, nu0)
+    b32 = pcg_iter_bytes(nb, nnzb, 4, n0, nu0, 4 if args.mixed_vectors == 32 else 8)
     out = {
         "metric": "time-to-1e-10 PCG at 6.44M DOF: FP32+refine vs FP64 (s)",
         "value": round(t_m, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
@@ -416,7 +420,8 @@ def main():
         else:
             step_fp64()
         ms = S.pcg_kernel_times(G.A, G.ilu, fp32, args.spmv_reps)
-        kb = kernel_bytes(nb, nnzb, s, n0, nu0)
+        sv = 8 if (not fp32 or args.mixed_vectors == 64) else 4
+        kb = kernel_bytes(nb, nnzb, s, n0, nu0, sv)
         kern[prec] = {}
         for k, b in kb.items():
             launches = nlev.get(k, 1)
@@ -432,9 +437,10 @@ def main():
     launch_ms = dk["ms_per_call"] / dk["launches_per_call"]
     launch_bytes = dk["bytes_per_call"] / dk["launches_per_call"]
     dom_gbs = launch_bytes / (launch_ms * 1e-3) / 1e9
-    kname = {"spmv": "k_pcg_spmv<float>", "fwd_sweep": "k_pcg_fwd<float>",
-             "bwd_sweep": "k_pcg_bwd<float>", "update": "k_pcg_update<float>",
-             "pupdate": "k_pcg_pupdate<float>"}[dom]
+    vt = "float, float" if args.mixed_vectors == 32 else "double, float"
+    kname = {"spmv": f"k_pcg_spmv<{vt}>", "fwd_sweep": f"k_pcg_fwd<{vt}>",
+             "bwd_sweep": f"k_pcg_bwd<{vt}>", "update": f"k_pcg_update<{vt}>",
+             "pupdate": f"k_pcg_pupdate<{vt}>"}[dom]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):

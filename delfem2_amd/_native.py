@@ -65,6 +65,7 @@ SIGNATURES = {
     "dfm2h_set_rhs_zero": [_vp, _vp, _sz, _i],
     "dfm2h_xplusay": [_vp, _sz, _vp, _d, _vp],
     "dfm2h_tetmesh_create": [_pvp, _vp, _sz, _sz],
+    "dfm2h_psup_tet_device": [_vp, _vp, _vp, _sz, _psz],
     "dfm2h_tetmesh_destroy": [_vp],
     "dfm2h_merge_linsys_solidlinear_static_tet": [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _vp,
                                                   _vp, _vp],

@@ -94,6 +94,7 @@ struct dfm2h_bsr_s {
 
 struct dfm2h_tetmesh_s {
   size_t ntet = 0, np = 0;
+  unsigned max_elsup = 0;           // most elements around one node
   dfm2h::DBuf<unsigned> tet;        // [4*ntet]
   dfm2h::DBuf<unsigned> elsup_ind;  // [np+1]
   dfm2h::DBuf<unsigned> elsup;      // element ids, ascending per node

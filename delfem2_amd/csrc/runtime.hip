@@ -68,6 +68,49 @@ void tune_store(const char *kind, int bucket, int fp32, int variant, double ms) 
 
 int dof_bucket(size_t ndof) { return ndof < 1 ? 0 : (int)std::floor(std::log2((double)ndof)); }
 
+// ---------------------------------------------------------------------------
+// one-ring pattern on the device: one thread per node gathers the nodes of
+// its surrounding elements into LDS, then emits the distinct ones other
+// than itself in ascending order (a selection over at most 3 x 32
+// candidates) -- the same rows as JArray_PSuP_MeshElem + JArray_Sort.
+// Pass 1 (psup_ind == nullptr) writes each row's count into out; pass 2
+// writes the row at out[psup_ind[i]].
+
+constexpr int kPsupMaxElem = 32;  // Kuhn-split voxel meshes: 24
+constexpr int kPsupCand = 3 * kPsupMaxElem;
+constexpr int kPsupBlock = 64;
+
+__global__ void __launch_bounds__(kPsupBlock)
+k_psup_rows(unsigned np, const unsigned *__restrict__ tet, const unsigned *__restrict__ esi,
+            const unsigned *__restrict__ es, const unsigned *__restrict__ psup_ind,
+            unsigned *__restrict__ out) {
+  __shared__ unsigned cand[kPsupBlock][kPsupCand + 1];
+  const unsigned i = blockIdx.x * kPsupBlock + threadIdx.x;
+  if (i >= np) return;
+  unsigned *c = cand[threadIdx.x];
+  unsigned nc = 0;
+  for (unsigned k = esi[i]; k < esi[i + 1]; ++k) {
+    const unsigned e = es[k];
+    for (int a = 0; a < 4; ++a) {
+      const unsigned v = tet[4 * (size_t)e + a];
+      if (v != i) c[nc++] = v;
+    }
+  }
+  unsigned cnt = 0, last = 0;
+  for (;;) {
+    unsigned best = 0xffffffffu;
+    for (unsigned q = 0; q < nc; ++q) {
+      const unsigned v = c[q];
+      if ((cnt == 0 || v > last) && v < best) best = v;
+    }
+    if (best == 0xffffffffu) break;
+    if (psup_ind) out[psup_ind[i] + cnt] = best;
+    ++cnt;
+    last = best;
+  }
+  if (!psup_ind) out[i] = cnt;
+}
+
 }  // namespace dfm2h
 
 using namespace dfm2h;
@@ -188,6 +231,7 @@ int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet, s
       std::vector<unsigned> cur(ind.begin(), ind.end() - 1);
       for (size_t e = 0; e < ntet; ++e)
         for (int k = 0; k < 4; ++k) es[cur[tet[e * 4 + k]]++] = (unsigned)e;
+      for (size_t q = 0; q < np; ++q) M->max_elsup = std::max(M->max_elsup, ind[q + 1] - ind[q]);
       M->tet.upload(tet, ntet * 4);
       M->elsup_ind.upload(ind);
       M->elsup.upload(es);
@@ -197,6 +241,48 @@ int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet, s
       throw;
     }
     *out = M;
+  });
+}
+
+int dfm2h_psup_tet_device(dfm2h_tetmesh M, unsigned *psup_ind, unsigned *psup,
+                          size_t psup_cap, size_t *npsup) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(M && psup_ind && npsup, DFM2H_ERR_INVALID, "null argument");
+    DFM2H_REQUIRE(M->max_elsup <= (unsigned)kPsupMaxElem, DFM2H_ERR_INVALID,
+                  "a node has more than 32 elements around it; use dfm2h_psup_tet");
+    const size_t np = M->np;
+    psup_ind[0] = 0;
+    if (np == 0) {
+      *npsup = 0;
+      return;
+    }
+    DBuf<unsigned> cnt;
+    cnt.alloc(np + 1);
+    const dim3 g(ceil_div(np, kPsupBlock));
+    hipLaunchKernelGGL(k_psup_rows, g, dim3(kPsupBlock), 0, stream(), (unsigned)np, M->tet.p,
+                       M->elsup_ind.p, M->elsup.p, (const unsigned *)nullptr, cnt.p);
+    check_launch("k_psup_rows");
+    DFM2H_HIP(hipMemcpyAsync(psup_ind + 1, cnt.p, np * sizeof(unsigned), hipMemcpyDeviceToHost,
+                             stream()));
+    DFM2H_HIP(hipStreamSynchronize(stream()));
+    size_t tot = 0;
+    for (size_t q = 1; q <= np; ++q) {
+      tot += psup_ind[q];
+      DFM2H_REQUIRE(tot < (1ull << 32), DFM2H_ERR_INVALID, "pattern too large for 32-bit indices");
+      psup_ind[q] = (unsigned)tot;
+    }
+    *npsup = tot;
+    if (!psup || psup_cap < tot) return;
+    DBuf<unsigned> ind, out;
+    ind.upload(psup_ind, np + 1);
+    out.alloc(std::max<size_t>(tot, 1));
+    hipLaunchKernelGGL(k_psup_rows, g, dim3(kPsupBlock), 0, stream(), (unsigned)np, M->tet.p,
+                       M->elsup_ind.p, M->elsup.p, ind.p, out.p);
+    check_launch("k_psup_rows");
+    if (tot)
+      DFM2H_HIP(hipMemcpyAsync(psup, out.p, tot * sizeof(unsigned), hipMemcpyDeviceToHost,
+                               stream()));
+    DFM2H_HIP(hipStreamSynchronize(stream()));
   });
 }
 

@@ -22,14 +22,14 @@ class DeviceSystem:
         dev = torch.device("cuda", device)
         self.dev = dev
         t0 = time.perf_counter()
-        ind, arr = S.JArray_PSuP_MeshElem(prob.tet, prob.np_)
+        self.mesh = S.TetMesh(prob.tet, prob.np_)
+        ind, arr = S.JArray_PSuP_MeshElem_Device(self.mesh)
         self.psup_ind, self.psup = ind, arr
         self.A = S.CMatrixSparse()
         self.A.Initialize(prob.np_, 3, True)
         self.A.SetPattern(ind, arr)
         self.ilu = S.CPreconditionerILU()
         self.ilu.SetPattern0(self.A)
-        self.mesh = S.TetMesh(prob.tet, prob.np_)
         self.xyz = torch.from_numpy(prob.xyz).to(dev)
         self.myu = torch.from_numpy(prob.myu).to(dev)
         self.lam = torch.from_numpy(prob.lam).to(dev)

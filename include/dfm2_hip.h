@@ -111,6 +111,14 @@ int dfm2h_xplusay(double *X_dev, size_t n, const int *flag_dev, double alpha,
  * (msh_topology_uniform.cpp:19-51) that the gather assembly walks. */
 int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet,
                          size_t np);
+/* The same pattern built on the device from the mesh handle (one thread per
+ * node over its surrounding elements; msh_topology_uniform.cpp:301-316 +
+ * jagarray.cpp:72-92, identical output).  Host outputs as dfm2h_psup_tet:
+ * psup_ind[np+1] always, psup when psup_cap >= *npsup.  Limit: at most 32
+ * elements around a node (DFM2H_ERR_INVALID otherwise; the host build has
+ * no limit). */
+int dfm2h_psup_tet_device(dfm2h_tetmesh M, unsigned *psup_ind, unsigned *psup,
+                          size_t psup_cap, size_t *npsup);
 int dfm2h_tetmesh_destroy(dfm2h_tetmesh M);
 /* fem_solidlinear.h:192-237 MergeLinSys_SolidLinear_Static_MeshTet3D with
  * EMat_SolidLinear_Static_Tet (fem_solidlinear.cpp:409-435) merged by

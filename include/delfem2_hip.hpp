@@ -11,6 +11,8 @@
  *   MergeLinSys_SolidLinear_Static_MeshTet3D  fem_solidlinear.h:192-237
  *   setRHS_Zero, XPlusAY vecxitrsol.cpp:226-296
  *   Solve_PCG            lsitrsol.h:80-148, fused on the device
+ *   Solve_CG             lsitrsol.h:36-74 (device views)
+ *   LinearSystemSolver_BlockSparseILU  ls_solver_block_sparse_ilu.h:18-103
  *
  * With these types the reference's own template
  *   delfem2::Solve_PCG(VEC&&, VEC&&, VEC&&, VEC&&, double, unsigned, const MAT&, const PREC&)
@@ -23,6 +25,8 @@
 #ifndef DELFEM2_HIP_HPP
 #define DELFEM2_HIP_HPP
 
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -200,6 +204,192 @@ inline void setRHS_Zero(double *vec_b, const int *aBCFlag, size_t n, int iflag_n
 inline void XPlusAY(double *X, size_t nDoF, const int *aBCFlag, double alpha, const double *Y) {
   check(dfm2h_xplusay(X, nDoF, aBCFlag, alpha, Y));
 }
+
+// lsitrsol.h:36-74 Solve_CG (no preconditioner) on device views, the same
+// recurrences and return value (relative residual per iteration); each Dot
+// is a host synchronisation.
+inline std::vector<double> Solve_CG(ViewAsVectorXd &&r_vec, ViewAsVectorXd &&u_vec,
+                                    ViewAsVectorXd &&Ap_vec, ViewAsVectorXd &&p_vec,
+                                    double conv_ratio_tol, unsigned int max_iteration,
+                                    const CMatrixSparse &mat) {
+  std::vector<double> conv;
+  u_vec.setZero();
+  double sq = Dot(r_vec, r_vec);
+  if (sq < 1.0e-30) return conv;
+  const double inv_sq0 = 1.0 / sq;
+  p_vec = r_vec;
+  for (unsigned int it = 0; it < max_iteration; ++it) {
+    AddMatVec(Ap_vec, 0.0, 1.0, mat, p_vec);
+    const double alpha = sq / Dot(p_vec, Ap_vec);
+    AddScaledVec(u_vec, +alpha, p_vec);
+    AddScaledVec(r_vec, -alpha, Ap_vec);
+    const double sq_new = Dot(r_vec, r_vec);
+    const double ratio = std::sqrt(sq_new * inv_sq0);
+    conv.push_back(ratio);
+    if (ratio < conv_ratio_tol) return conv;
+    const double beta = sq_new / sq;
+    sq = sq_new;
+    ScaleAndAddVec(p_vec, beta, r_vec);
+  }
+  return conv;
+}
+
+// Device buffer owned by the facade below.
+template <typename T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  ~DeviceArray() {
+    if (p) dfm2h_free(p);
+  }
+  void resize(size_t count) {
+    if (count == n) return;
+    if (p) dfm2h_free(p);
+    p = nullptr;
+    n = count;
+    if (n) {
+      void *q = nullptr;
+      check(dfm2h_malloc(&q, sizeof(T) * n));
+      p = static_cast<T *>(q);
+    }
+  }
+  void upload(const std::vector<T> &v) {
+    resize(v.size());
+    if (n) check(dfm2h_memcpy_htod(p, v.data(), sizeof(T) * n));
+  }
+  void download(std::vector<T> &v) const {
+    v.resize(n);
+    if (n) check(dfm2h_memcpy_dtoh(v.data(), p, sizeof(T) * n));
+  }
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+// ls_solver_block_sparse_ilu.h:18-103 LinearSystemSolver_BlockSparseILU, the
+// caller-level facade.  Same members and calls; 3x3 blocks (ndim == 3).
+// Element matrices are merged on the host as Merge (ls_block_sparse.h:
+// 260-310) merges them -- the same additions in call order, into host copies
+// of val_crs_ / val_dia_ -- and the solves upload those values and run on the
+// device: SetFixedBC, CopyValue, Decompose and the fused Solve_PCG
+// (Solve_PcgIlu: tol 1e-5, 1000 iterations), or Solve_CG (Solve_Cg: tol
+// 1e-4, 300 iterations).  vec_r / vec_x / dof_bcflag stay host vectors as in
+// the reference; Solve_PcgIlu also keeps its history in conv_hist (the
+// reference prints it).
+class LinearSystemSolver_BlockSparseILU {
+ public:
+  void Initialize(unsigned int nblk, unsigned int ndim, std::vector<unsigned int> &psup_ind,
+                  std::vector<unsigned int> &psup) {
+    if (ndim != 3) throw Error(DFM2H_ERR_INVALID, "only 3x3 blocks are supported");
+    matrix.Initialize(nblk, ndim, true);
+    matrix.SetPattern(psup_ind.data(), psup_ind.size(), psup.data(), psup.size());
+    col_ind_ = psup_ind;
+    row_ptr_ = psup;
+    val_crs_.assign(row_ptr_.size() * 9, 0.0);
+    val_dia_.assign(static_cast<size_t>(nblk) * 9, 0.0);
+    dof_bcflag.assign(ndof(), 0);
+    ilu_sparse.Initialize_ILUk(matrix, 0);
+  }
+
+  size_t nblk() const { return matrix.nrowblk_; }
+  size_t ndim() const { return matrix.nrowdim_; }
+  size_t ndof() const { return nblk() * ndim(); }
+
+  void BeginMerge() {
+    std::fill(val_crs_.begin(), val_crs_.end(), 0.0);
+    std::fill(val_dia_.begin(), val_dia_.end(), 0.0);
+    vec_r.assign(ndof(), 0.0);
+  }
+
+  // returns false when a block is missing from the pattern (the reference's
+  // Merge result; its facade ignores it)
+  template <int nrow, int ncol, int ndimrow, int ndimcol>
+  bool Merge(const unsigned int *aIpRow, const unsigned int *aIpCol,
+             const double emat[nrow][ncol][ndimrow][ndimcol]) {
+    static_assert(ndimrow == 3 && ndimcol == 3, "3x3 blocks only");
+    merge_buffer.resize(nblk(), ~0u);
+    for (int ir = 0; ir < nrow; ++ir) {
+      const unsigned int i = aIpRow[ir];
+      for (unsigned int q = col_ind_[i]; q < col_ind_[i + 1]; ++q) merge_buffer[row_ptr_[q]] = q;
+      bool ok = true;
+      for (int jc = 0; jc < ncol && ok; ++jc) {
+        const unsigned int j = aIpCol[jc];
+        double *out;
+        const double *in;
+        if (i == j) {
+          in = &emat[ir][ir][0][0];
+          out = &val_dia_[static_cast<size_t>(i) * 9];
+        } else {
+          if (merge_buffer[j] == ~0u) {
+            ok = false;
+            break;
+          }
+          in = &emat[ir][jc][0][0];
+          out = &val_crs_[static_cast<size_t>(merge_buffer[j]) * 9];
+        }
+        for (int k = 0; k < 9; ++k) out[k] += in[k];
+      }
+      for (unsigned int q = col_ind_[i]; q < col_ind_[i + 1]; ++q) merge_buffer[row_ptr_[q]] = ~0u;
+      if (!ok) return false;
+    }
+    return true;
+  }
+
+  void AddValueToDiagonal(unsigned int iblk, unsigned int idim, double val) {
+    val_dia_[static_cast<size_t>(iblk) * 9 + idim * 3 + idim] += val;
+  }
+
+  void Solve_Cg() {
+    prepare();
+    conv_hist = Solve_CG(ViewAsVectorXd(d_r.p, ndof()), ViewAsVectorXd(d_x.p, ndof()),
+                         ViewAsVectorXd(d_t0.p, ndof()), ViewAsVectorXd(d_t1.p, ndof()), 1.0e-4,
+                         300, matrix);
+    finish();
+  }
+
+  void Solve_PcgIlu() {
+    prepare();
+    ilu_sparse.CopyValue(matrix);
+    ilu_sparse.Decompose();
+    conv_hist = Solve_PCG(ViewAsVectorXd(d_r.p, ndof()), ViewAsVectorXd(d_x.p, ndof()),
+                          ViewAsVectorXd(d_t0.p, ndof()), ViewAsVectorXd(d_t1.p, ndof()), 1.0e-5,
+                          1000, matrix, ilu_sparse);
+    finish();
+  }
+
+ public:
+  std::vector<double> vec_r;
+  std::vector<double> vec_x;
+  std::vector<int> dof_bcflag;
+  std::vector<double> conv_hist;
+  std::vector<unsigned int> merge_buffer;
+  CMatrixSparse matrix;
+  CPreconditionerILU ilu_sparse;
+  // host copies of the pattern and of the merged values
+  std::vector<unsigned int> col_ind_, row_ptr_;
+  std::vector<double> val_crs_, val_dia_;
+
+ private:
+  // values and flags to the device, fixed BCs, rhs zeroed at fixed DOFs
+  void prepare() {
+    matrix.upload(val_crs_.data(), val_dia_.data());
+    d_flag.upload(dof_bcflag);
+    matrix.SetFixedBC(d_flag.p);
+    for (size_t i = 0; i < vec_r.size(); ++i)
+      if (dof_bcflag[i] != 0) vec_r[i] = 0.0;  // setRHS_Zero (vecxitrsol.cpp:286-296)
+    d_r.upload(vec_r);
+    d_x.resize(ndof());
+    d_t0.resize(ndof());
+    d_t1.resize(ndof());
+  }
+  void finish() {
+    d_x.download(vec_x);
+    d_r.download(vec_r);
+  }
+  DeviceArray<double> d_r, d_x, d_t0, d_t1;
+  DeviceArray<int> d_flag;
+};
 
 }  // namespace hip
 }  // namespace delfem2

@@ -22,3 +22,22 @@ def test_reference_template_on_device_types(gpu):
     assert out["template_iterations"] == 44  # the reference's count on this problem
     assert abs(out["fused_iterations"] - 44) <= 1
     assert out["x_rel_diff"] < 1e-9
+
+
+FACADE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "facade")
+
+
+def test_facade_against_reference_facade(gpu):
+    """ls_solver_block_sparse_ilu.h's LinearSystemSolver_BlockSparseILU
+    (reference, host) and its device twin driven by the same caller code:
+    merged values and rhs bit-identical; Solve_PcgIlu x rel-L2 < 1e-9;
+    Solve_Cg x rel-L2 < 1e-8 and the same iteration count (+-1)."""
+    if not os.path.exists(FACADE):
+        pytest.skip("facade binary not built (needs the reference headers at build time)")
+    r = subprocess.run([FACADE], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["same_values"] and out["same_rhs"]
+    assert out["pcg_x_rel"] < 1e-9 and out["cg_x_rel"] < 1e-8
+    assert abs(out["cg_iterations_ref"] - out["cg_iterations_dev"]) <= 1

@@ -204,13 +204,15 @@ kb_fwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
     T t = 0;
     if (valid) {
       t = a.r[kVSB * (size_t)i + L.c];
-      t = sliced_row_sum<T, true, V::UW, kVSB>(t, (T)1, L, a.F.len[s * kSliceRows + L.r],
-                                               a.F.kmax[s], a.F.off[s], a.F.col, a.f_val, a.Pr);
+      t = sliced_row_sum_m<T, T, true, V::UW, kVSB, sweep_nt<T>()>(
+          t, (T)1, L, a.F.len[s * kSliceRows + L.r], a.F.kmax[s], a.F.off[s], a.F.col, a.f_val,
+          a.Pr);
     }
     const T t0 = __shfl(t, (int)L.r, kWave);
     const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
     const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
-    if (valid) a.Pr[kVSB * (size_t)i + L.c] = dia_row_dot(a.f_dia, L, t0, t1, t2);
+    if (valid)
+      a.Pr[kVSB * (size_t)i + L.c] = dia_row_dot_m<T, T, sweep_nt<T>()>(a.f_dia, L, t0, t1, t2);
   }
 }
 
@@ -229,8 +231,9 @@ kb_bwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
     if (i != ~0u) {
       const size_t e = kVSB * (size_t)i + L.c;
       T t = a.Pr[e];
-      t = sliced_row_sum<T, true, V::UW, kVSB>(t, (T)1, L, a.B.len[s * kSliceRows + L.r],
-                                               a.B.kmax[s], a.B.off[s], a.B.col, a.b_val, a.Pr);
+      t = sliced_row_sum_m<T, T, true, V::UW, kVSB, sweep_nt<T>()>(
+          t, (T)1, L, a.B.len[s * kSliceRows + L.r], a.B.kmax[s], a.B.off[s], a.B.col, a.b_val,
+          a.Pr);
       a.Pr[e] = t;
       part = (double)a.r[e] * (double)t;
     }

@@ -163,14 +163,30 @@ struct alignas(4 * sizeof(T)) Quad {
   T a, b, c, d;
 };
 
-// the three values of lane `lane`'s block row in block-slot q
-template <typename T>
+// the three values of lane `lane`'s block row in block-slot q.  NT: loads
+// marked non-temporal (streamed values that should not displace the
+// gathered vector lines from the caches; measured: FP32 sweeps -14%, FP64
+// kernels and the FP32 SpMV slower, so only the FP32 sweeps use it)
+template <typename T, bool NT = false>
 __device__ inline Trip<T> load_trip(const T *__restrict__ base, size_t q, unsigned lane) {
   if (planar_vals<T>()) {
     const T *p = base + q * kSliceVals + lane;
+    if (NT)
+      return Trip<T>{__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 63),
+                     __builtin_nontemporal_load(p + 126)};
     return Trip<T>{p[0], p[63], p[126]};
   }
-  return *reinterpret_cast<const Trip<T> *>(base + (q * 63 + lane) * 3);
+  const T *p = base + (q * 63 + lane) * 3;
+  if (NT)
+    return Trip<T>{__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1),
+                   __builtin_nontemporal_load(p + 2)};
+  return *reinterpret_cast<const Trip<T> *>(p);
+}
+
+// non-temporal value loads for a sweep in storage precision M
+template <typename M>
+constexpr bool sweep_nt() {
+  return sizeof(M) == 4;
 }
 
 // node j of a vector with node stride VS
@@ -190,7 +206,7 @@ __device__ inline Trip<T> load_node(const T *x, size_t j) {
 // memory round trip), then the U gathers of x (a second one), then the U
 // updates in order.  Values are stored in precision M and widened to the
 // vector precision T before the products (M = T: no conversion).
-template <typename T, typename M, bool SUB, int U, int VS>
+template <typename T, typename M, bool SUB, int U, int VS, bool NT = false>
 __device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigned K,
                                      unsigned kmax, unsigned q0, const unsigned *__restrict__ col,
                                      const M *__restrict__ val, const T *x) {
@@ -203,7 +219,7 @@ __device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigne
       if (k0 + u < K) {
         const size_t q = q0 + k0 + u;
         j[u] = col[q * kSliceRows + L.r];
-        v[u] = load_trip(val, q, L.lane);
+        v[u] = load_trip<M, NT>(val, q, L.lane);
       }
     }
 #pragma unroll
@@ -231,10 +247,10 @@ __device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned 
 }
 
 // this lane's row of a stored 3x3 block (precision M) times (x0, x1, x2)
-template <typename T, typename M>
+template <typename T, typename M, bool NT = false>
 __device__ inline T dia_row_dot_m(const M *__restrict__ dia, const SliceLane &L, T x0, T x1,
                                   T x2) {
-  const Trip<M> d = load_trip(dia, L.s, L.lane);
+  const Trip<M> d = load_trip<M, NT>(dia, L.s, L.lane);
   return (T)d.a * x0 + (T)d.b * x1 + (T)d.c * x2;
 }
 

@@ -474,7 +474,7 @@ __device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, u
       if (k < K) {
         const size_t q = q0 + k;
         j[u] = col[q * kSliceRows + L.r];
-        v[u] = load_trip(val, q, L.lane);
+        v[u] = load_trip<M, sweep_nt<M>()>(val, q, L.lane);
       }
     }
 #pragma unroll
@@ -530,7 +530,7 @@ k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
     const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
     const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
     if (i != ~0u) {
-      const T out = dia_row_dot_m<T, M>(a.f_dia, L, t0, t1, t2);
+      const T out = dia_row_dot_m<T, M, sweep_nt<M>()>(a.f_dia, L, t0, t1, t2);
       a.Pr[kVS * (size_t)i + L.c] = out;
       if (PART && ((a.fumask[L.s] >> L.r) & 1u))
         part = (double)a.r[kVS * (size_t)i + L.c] * (double)out;

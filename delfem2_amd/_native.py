@@ -72,6 +72,8 @@ SIGNATURES = {
     "dfm2h_ilu_create": [_pvp],
     "dfm2h_ilu_destroy": [_vp],
     "dfm2h_ilu_set_pattern0": [_vp, _vp],
+    "dfm2h_ilu_set_pattern_k": [_vp, _vp, _i],
+    "dfm2h_ilu_info": [_vp, _psz, _psz, _vp],
     "dfm2h_ilu_copy_value": [_vp, _vp],
     "dfm2h_ilu_decompose": [_vp, _pi, _pi],
     "dfm2h_ilu_solve_precond": [_vp, _vp],

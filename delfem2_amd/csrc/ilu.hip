@@ -174,78 +174,169 @@ int dfm2h_ilu_destroy(dfm2h_ilu P) {
   return guard(__func__, [&] { delete P; });
 }
 
+// Everything after the ILU pattern (colind / rowptr / diaind on the host) is
+// known: device copies, level schedules, sliced sweep plans, the A -> ILU
+// slot map.
+static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
+  const Pattern &pa = A->pat;
+  P->ncrs = P->colind[P->nblk];
+  P->colind_d.upload(P->colind);
+  P->rowptr_d.upload(P->rowptr);
+  P->diaind_d.upload(P->diaind);
+  P->vcrs.alloc(P->ncrs * 9);
+  P->vdia.alloc((size_t)P->nblk * 9);
+  std::vector<unsigned> rows;
+  {
+    std::vector<unsigned> s0(P->nblk), s1(P->nblk);
+    build_levels(P->nblk, P->colind, P->rowptr, P->diaind, true, P->levptr_f, rows);
+    P->levrows_f.upload(rows);
+    std::vector<std::vector<unsigned>> groups;
+    for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l)
+      groups.emplace_back(rows.begin() + P->levptr_f[l], rows.begin() + P->levptr_f[l + 1]);
+    for (unsigned i = 0; i < P->nblk; ++i) {
+      s0[i] = P->colind[i];
+      s1[i] = P->diaind[i];
+    }
+    P->fplan.build(groups, s0, s1, P->rowptr);
+    {
+      std::vector<unsigned> um;
+      P->flev_u0.assign(groups.size(), 0);
+      for (size_t l = 0; l < groups.size(); ++l)
+        for (size_t b = 0; b < groups[l].size(); b += kSliceRows) {
+          unsigned m = 0;
+          for (size_t k = b; k < std::min(groups[l].size(), b + kSliceRows); ++k) {
+            const unsigned i = groups[l][k];
+            if (P->diaind[i] == P->colind[i + 1]) m |= 1u << (k - b);
+          }
+          um.push_back(m);
+          if (m) P->flev_u0[l] = 1;
+        }
+      P->fumask.upload(um);
+    }
+    build_levels(P->nblk, P->colind, P->rowptr, P->diaind, false, P->levptr_b, rows);
+    groups.clear();
+    for (size_t l = 0; l + 1 < P->levptr_b.size(); ++l)
+      groups.emplace_back(rows.begin() + P->levptr_b[l], rows.begin() + P->levptr_b[l + 1]);
+    for (unsigned i = 0; i < P->nblk; ++i) {
+      s0[i] = P->diaind[i];
+      s1[i] = P->colind[i + 1];
+    }
+    P->bplan.build(groups, s0, s1, P->rowptr);
+  }
+  const std::vector<int> a2l = build_a2l(*P, pa);
+  P->a2l_d.upload(a2l);
+  P->mapped_for = A;
+  P->nsing_d.alloc(1);
+  P->has_pattern = true;
+  P->pattern_serial++;
+  P->decomposed = false;
+  P->version++;
+  DFM2H_HIP(hipStreamSynchronize(stream()));
+}
+
+// first column > i in the sorted row (or the row end)
+static void fill_diaind(dfm2h_ilu P) {
+  P->diaind.assign(P->nblk, 0);
+  for (unsigned i = 0; i < P->nblk; ++i) {
+    P->diaind[i] = P->colind[i + 1];
+    for (unsigned s = P->colind[i]; s < P->colind[i + 1]; ++s)
+      if (P->rowptr[s] > i) {
+        P->diaind[i] = s;
+        break;
+      }
+  }
+}
+
+// Symbolic ILU(k) (ls_ilu_block_sparse.cpp:849-999), lev_fill >= 1 or -1
+// (every fill).  Row i starts from A's sorted columns at level 0; its
+// entries k < i are taken in ascending order (fills added on the way
+// included), skipping those with level + 1 > lev_fill; each upper entry j
+// of row k (level + 1 <= lev_fill), j != i, not yet in row i, is inserted
+// with level max(lev_ik, lev_kj) + 1.  An entry found again keeps its
+// first level, as in the reference.
+static void symbolic_iluk(dfm2h_ilu P, const Pattern &pa, int lev_fill) {
+  const unsigned n = pa.nblk;
+  std::vector<unsigned> col, lev;  // all finished rows
+  std::vector<unsigned> dia(n);
+  P->colind.assign(n + 1, 0);
+  std::vector<std::pair<unsigned, unsigned>> row;  // (column, level), sorted
+  auto allowed = [&](unsigned l) { return lev_fill < 0 || (int)l + 1 <= lev_fill; };
+  for (unsigned i = 0; i < n; ++i) {
+    row.clear();
+    for (unsigned s = pa.colind[i]; s < pa.colind[i + 1]; ++s) row.emplace_back(pa.rowptr[s], 0u);
+    std::sort(row.begin(), row.end());
+    for (size_t t = 0; t < row.size(); ++t) {
+      const unsigned k = row[t].first, lik = row[t].second;
+      if (!allowed(lik)) continue;
+      if (k >= i) break;
+      for (unsigned kj = dia[k]; kj < P->colind[k + 1]; ++kj) {
+        const unsigned j = col[kj], lkj = lev[kj];
+        if (!allowed(lkj) || j == i) continue;
+        auto it = std::lower_bound(row.begin() + t + 1, row.end(), std::make_pair(j, 0u),
+                                   [](const auto &a, const auto &b) { return a.first < b.first; });
+        if (it != row.end() && it->first == j) continue;
+        row.insert(it, std::make_pair(j, std::max(lik, lkj) + 1));
+      }
+    }
+    const size_t base = col.size();
+    for (const auto &e : row) {
+      col.push_back(e.first);
+      lev.push_back(e.second);
+    }
+    DFM2H_REQUIRE(col.size() < (1ull << 32), DFM2H_ERR_INVALID, "ILU(k) pattern too large");
+    P->colind[i + 1] = (unsigned)col.size();
+    dia[i] = P->colind[i + 1];
+    for (size_t t = 0; t < row.size(); ++t)
+      if (row[t].first > i) {
+        dia[i] = (unsigned)(base + t);
+        break;
+      }
+  }
+  P->rowptr = std::move(col);
+  P->diaind = std::move(dia);
+}
+
+// ls_ilu_block_sparse.cpp:1057-1090: A's pattern, rows sorted
+static void pattern0(dfm2h_ilu P, dfm2h_bsr A) {
+  const Pattern &pa = A->pat;
+  P->nblk = pa.nblk;
+  P->colind = pa.colind;
+  P->rowptr = pa.rowptr;
+  for (unsigned i = 0; i < P->nblk; ++i)
+    std::sort(P->rowptr.begin() + P->colind[i], P->rowptr.begin() + P->colind[i + 1]);
+  fill_diaind(P);
+}
+
 int dfm2h_ilu_set_pattern0(dfm2h_ilu P, dfm2h_bsr A) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(P && A, DFM2H_ERR_INVALID, "null handle");
     DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_STATE, "matrix has no pattern");
-    const Pattern &pa = A->pat;
-    P->nblk = pa.nblk;
-    P->ncrs = pa.ncrs;
-    P->colind = pa.colind;
-    P->rowptr = pa.rowptr;
-    for (unsigned i = 0; i < P->nblk; ++i)
-      std::sort(P->rowptr.begin() + P->colind[i], P->rowptr.begin() + P->colind[i + 1]);
-    P->diaind.assign(P->nblk, 0);
-    for (unsigned i = 0; i < P->nblk; ++i) {
-      P->diaind[i] = P->colind[i + 1];
-      for (unsigned s = P->colind[i]; s < P->colind[i + 1]; ++s)
-        if (P->rowptr[s] > i) {
-          P->diaind[i] = s;
-          break;
-        }
+    pattern0(P, A);
+    finish_pattern(P, A);
+  });
+}
+
+int dfm2h_ilu_info(dfm2h_ilu P, size_t *nblk, size_t *ncrs, unsigned *colind) {
+  return guard(__func__, [&] {
+    need_ilu(P);
+    if (nblk) *nblk = P->nblk;
+    if (ncrs) *ncrs = P->ncrs;
+    if (colind) std::copy(P->colind.begin(), P->colind.end(), colind);
+  });
+}
+
+int dfm2h_ilu_set_pattern_k(dfm2h_ilu P, dfm2h_bsr A, int lev_fill) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P && A, DFM2H_ERR_INVALID, "null handle");
+    DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_STATE, "matrix has no pattern");
+    DFM2H_REQUIRE(lev_fill >= -1, DFM2H_ERR_INVALID, "lev_fill must be >= -1");
+    if (lev_fill == 0) {
+      pattern0(P, A);
+    } else {
+      P->nblk = A->pat.nblk;
+      symbolic_iluk(P, A->pat, lev_fill);
     }
-    P->colind_d.upload(P->colind);
-    P->rowptr_d.upload(P->rowptr);
-    P->diaind_d.upload(P->diaind);
-    P->vcrs.alloc(P->ncrs * 9);
-    P->vdia.alloc((size_t)P->nblk * 9);
-    std::vector<unsigned> rows;
-    {
-      std::vector<unsigned> s0(P->nblk), s1(P->nblk);
-      build_levels(P->nblk, P->colind, P->rowptr, P->diaind, true, P->levptr_f, rows);
-      P->levrows_f.upload(rows);
-      std::vector<std::vector<unsigned>> groups;
-      for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l)
-        groups.emplace_back(rows.begin() + P->levptr_f[l], rows.begin() + P->levptr_f[l + 1]);
-      for (unsigned i = 0; i < P->nblk; ++i) {
-        s0[i] = P->colind[i];
-        s1[i] = P->diaind[i];
-      }
-      P->fplan.build(groups, s0, s1, P->rowptr);
-      {
-        std::vector<unsigned> um;
-        P->flev_u0.assign(groups.size(), 0);
-        for (size_t l = 0; l < groups.size(); ++l)
-          for (size_t b = 0; b < groups[l].size(); b += kSliceRows) {
-            unsigned m = 0;
-            for (size_t k = b; k < std::min(groups[l].size(), b + kSliceRows); ++k) {
-              const unsigned i = groups[l][k];
-              if (P->diaind[i] == P->colind[i + 1]) m |= 1u << (k - b);
-            }
-            um.push_back(m);
-            if (m) P->flev_u0[l] = 1;
-          }
-        P->fumask.upload(um);
-      }
-      build_levels(P->nblk, P->colind, P->rowptr, P->diaind, false, P->levptr_b, rows);
-      groups.clear();
-      for (size_t l = 0; l + 1 < P->levptr_b.size(); ++l)
-        groups.emplace_back(rows.begin() + P->levptr_b[l], rows.begin() + P->levptr_b[l + 1]);
-      for (unsigned i = 0; i < P->nblk; ++i) {
-        s0[i] = P->diaind[i];
-        s1[i] = P->colind[i + 1];
-      }
-      P->bplan.build(groups, s0, s1, P->rowptr);
-    }
-    const std::vector<int> a2l = build_a2l(*P, pa);
-    P->a2l_d.upload(a2l);
-    P->mapped_for = A;
-    P->nsing_d.alloc(1);
-    P->has_pattern = true;
-    P->pattern_serial++;
-    P->decomposed = false;
-    P->version++;
-    DFM2H_HIP(hipStreamSynchronize(stream()));
+    finish_pattern(P, A);
   });
 }
 

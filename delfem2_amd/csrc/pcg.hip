@@ -47,6 +47,8 @@ struct PcgScal {
   int final_;     // iteration cap reached: stop after this update
   int it_upd;     // inner iteration of the last reliable update
   int period;     // force an update after this many inner iterations (0: off)
+  int restart;    // the search direction restarts from the true residual
+  int pad3;
 };
 
 constexpr int kVecBlock = 256;
@@ -171,6 +173,7 @@ struct PcgArgs {
   double *partials, *hist;
   double *partials2;  // (r,r) partials of the update kernel
   double *partials3;  // (r,Pr) partials of the fused iteration
+  double *partials4;  // mixed: |r_true - r_recursive|^2 partials of a reliable update
   unsigned upd_part_off;
   PcgScal *S;
   // mixed precision: FP64 solution part (FP32 vectors), right-hand side,
@@ -209,11 +212,13 @@ enum ScalMode {
                         // iteration; then beta
 };
 
-// pa/na: first partial set, pb/nb: second (ConvBeta, AlphaMixed, BetaMixed)
+// pa/na: first partial set, pb/nb: second (ConvBeta, AlphaMixed, BetaMixed),
+// pc/nc: third (AlphaMixed)
 template <int MODE>
 __global__ void __launch_bounds__(kScalBlock)
 k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
-             const double *__restrict__ pb, unsigned nb, double *hist) {
+             const double *__restrict__ pb, unsigned nb, const double *__restrict__ pc,
+             unsigned nc, double *hist) {
   if (MODE != kScalInit && MODE != kScalInitMixed && S->done) return;
   const bool upd = S->upd != 0;  // block-uniform
   const double t = reduce_partials<kScalBlock>(pa, na);
@@ -221,6 +226,7 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
                      (MODE == kScalAlphaMixed && upd))
                         ? reduce_partials<kScalBlock>(pb, nb)
                         : 0.0;
+  const double t3 = (MODE == kScalAlphaMixed && upd) ? reduce_partials<kScalBlock>(pc, nc) : 0.0;
   if (threadIdx.x != 0) return;
   if (MODE == kScalInit) {  // |r0|^2: history[0]; early return when < 1e-30
     hist[0] = sqrt(t);
@@ -269,6 +275,7 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
     S->upd = 0;
     S->final_ = 0;
     S->it_upd = 0;
+    S->restart = 0;
     S->converged = 0;
     S->nan = 0;
     S->done = 0;
@@ -296,7 +303,15 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
         S->done = 1;
       }
     }
-    S->alpha = S->rPr / t;
+    if (upd && (!(t > 0.0) || t3 > 1.0e-2 * t2)) {
+      // the recursive residual the current direction was built from is more
+      // than 10% off the true one (or reached zero, p = 0): no step now, and
+      // the next direction restarts from the true residual
+      S->alpha = 0.0;
+      S->restart = 1;
+    } else {
+      S->alpha = S->rPr / t;
+    }
   } else {  // kScalBetaMixed: t: FP32 |r|^2 after the update, t2: (r, Pr)
     const int it = S->it + 1;
     S->it = it;
@@ -313,7 +328,8 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
       S->final_ = it >= S->max_nitr ? 1 : 0;
       S->it_upd = it;
     }
-    S->beta = t2 / S->rPr;
+    S->beta = S->restart ? 0.0 : t2 / S->rPr;
+    S->restart = 0;
     S->rPr = t2;
   }
 }
@@ -328,9 +344,11 @@ __device__ inline double x_full(const PcgArgs<T, M> &a, size_t j) {
 }
 
 // Reliable update, row part: r_i = b_i - (A x)_i in FP64 (FP64 matrix
-// values, FP64 solution), stored into r; returns r_i^2.
+// values, FP64 solution), stored into r; returns r_i^2 and adds the squared
+// change of r_i (true minus recursive) to drift.
 template <typename T, typename M>
-__device__ inline double residual_row(const PcgArgs<T, M> &a, const SliceLane &L, unsigned i) {
+__device__ inline double residual_row(const PcgArgs<T, M> &a, const SliceLane &L, unsigned i,
+                                      double &drift) {
   const unsigned K = a.a_plan.len[L.s * kSliceRows + L.r];
   const unsigned q0 = a.a_plan.off[L.s];
   double acc = 0.0;
@@ -343,6 +361,8 @@ __device__ inline double residual_row(const PcgArgs<T, M> &a, const SliceLane &L
   const size_t ii = kVS * (size_t)i;
   acc += dia_row_dot(a.a64_dia, L, x_full(a, ii), x_full(a, ii + 1), x_full(a, ii + 2));
   const double r = a.b64[ii + L.c] - acc;
+  const double e = r - (double)a.r[ii + L.c];
+  drift += e * e;
   a.r[ii + L.c] = (T)r;
   return r * r;
 }
@@ -363,7 +383,7 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
   const SliceLane L = slice_lane(0);
   const bool slice_ok = L.s < a.a_nsl && L.c < 3;
   const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  double part = 0.0, part2 = 0.0;
+  double part = 0.0, part2 = 0.0, part4 = 0.0;
   if (i != ~0u) {
     T acc = 0;
     acc = sliced_row_sum_m<T, M, false, spmv_unroll<T, M>(), kVS>(
@@ -374,13 +394,19 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
     a.Pr[kVS * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
     part = (double)a.p[kVS * (size_t)i + L.c] * (double)acc;
     if constexpr (is_mixed<T, M>())
-      if (refine) part2 = residual_row(a, L, i);
+      if (refine) part2 = residual_row(a, L, i, part4);
   }
   part = block_reduce_sum<kPackBlock>(part);
-  if (refine) part2 = block_reduce_sum<kPackBlock>(part2);
+  if (refine) {
+    part2 = block_reduce_sum<kPackBlock>(part2);
+    part4 = block_reduce_sum<kPackBlock>(part4);
+  }
   if (threadIdx.x == 0) {
     a.partials[blockIdx.x] = part;
-    if (refine) a.partials2[blockIdx.x] = part2;
+    if (refine) {
+      a.partials2[blockIdx.x] = part2;
+      a.partials4[blockIdx.x] = part4;
+    }
   }
 }
 
@@ -615,7 +641,7 @@ template <typename T>
 struct PcgBufs {
   using elem = T;
   DBuf<T> r, x, Pr, p;
-  DBuf<double> partials, partials2, partials3, hist;
+  DBuf<double> partials, partials2, partials3, partials4, hist;
   DBuf<PcgScal> S;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -800,6 +826,7 @@ static PcgArgs<T, M> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Pa
   a.partials = B.partials.p;
   a.partials2 = B.partials2.p;
   a.partials3 = B.partials3.p;
+  a.partials4 = B.partials4.p;
   a.upd_part_off = 0;  // set per sweep shape
   a.hist = B.hist.p;
   a.S = B.S.p;
@@ -831,6 +858,7 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
   const size_t np = std::max<size_t>({gs, (size_t)vec_grid(n), (size_t)bwd_partial_total(P), 1});
   if (B.partials.n < np) B.partials.alloc(np);
   if (B.partials2.n < gs) B.partials2.alloc(gs);
+  if (B.partials4.n < gs) B.partials4.alloc(gs);
   const size_t n3 = std::max<size_t>(sweep_layout(A, P, 2).n3, 1);  // the larger layout
   if (B.partials3.n < n3) B.partials3.alloc(n3);
   if (B.hist.n < (size_t)max_nitr + 2) B.hist.alloc((size_t)max_nitr + 2);
@@ -904,9 +932,10 @@ constexpr int solver_kind() {
 
 template <int MODE>
 static void launch_scalar(hipStream_t s, PcgScal *S, const double *pa, unsigned na,
-                          const double *pb, unsigned nb, double *hist) {
-  hipLaunchKernelGGL(k_pcg_scalar<MODE>, dim3(1), dim3(kScalBlock), 0, s, S, pa, na, pb, nb,
-                     hist);
+                          const double *pb, unsigned nb, double *hist,
+                          const double *pc = nullptr, unsigned nc = 0) {
+  hipLaunchKernelGGL(k_pcg_scalar<MODE>, dim3(1), dim3(kScalBlock), 0, s, S, pa, na, pb, nb, pc,
+                     nc, hist);
 }
 
 // One FP64 iteration (lsitrsol.h:107-146): spmv, alpha, update (+ forward
@@ -968,7 +997,7 @@ static std::vector<uintptr_t> graph_sig(const PcgArgs<T, M> &a, dfm2h_bsr A, dfm
   return {(uintptr_t)a.a_val,    (uintptr_t)a.a_dia,     (uintptr_t)a.f_val,
           (uintptr_t)a.f_dia,    (uintptr_t)a.b_val,     (uintptr_t)a.dinv0,
           (uintptr_t)a.r,        (uintptr_t)a.partials,  (uintptr_t)a.partials2,
-          (uintptr_t)a.partials3, (uintptr_t)a.hist,     (uintptr_t)a.S,
+          (uintptr_t)a.partials3, (uintptr_t)a.partials4, (uintptr_t)a.hist, (uintptr_t)a.S,
           (uintptr_t)a.a64_val,  (uintptr_t)a.a64_dia,   (uintptr_t)a.x64,
           (uintptr_t)a.b64,      (uintptr_t)a.hist64,    (uintptr_t)chunk,
           (uintptr_t)A,          (uintptr_t)A->pattern_serial, (uintptr_t)P->pattern_serial,
@@ -1075,7 +1104,8 @@ static void launch_mixed_iteration(const PcgArgs<T, float> &a, dfm2h_ilu P, int 
   const unsigned gs = slice_grid(a.a_nsl);
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL((k_pcg_spmv<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
-  launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64);
+  launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64,
+                                 a.partials4, gs);
   hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, shape, Lo, s);
   launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
@@ -1370,7 +1400,8 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[1], W.s));
         if constexpr (is_mixed<T, M>())
-          launch_scalar<kScalAlphaMixed>(W.s, a.S, a.partials, gs, a.partials2, gs, a.hist64);
+          launch_scalar<kScalAlphaMixed>(W.s, a.S, a.partials, gs, a.partials2, gs, a.hist64,
+                                         a.partials4, gs);
         else
           launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[2], W.s));

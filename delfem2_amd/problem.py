@@ -16,7 +16,7 @@ from .workload import Problem
 class DeviceSystem:
     """Pattern, matrix, ILU(0) and device-resident inputs of one Problem."""
 
-    def __init__(self, prob: Problem, device: int = 0):
+    def __init__(self, prob: Problem, device: int = 0, fill_level: int = 0):
         S.init(device)
         self.prob = prob
         dev = torch.device("cuda", device)
@@ -29,7 +29,10 @@ class DeviceSystem:
         self.A.Initialize(prob.np_, 3, True)
         self.A.SetPattern(ind, arr)
         self.ilu = S.CPreconditionerILU()
-        self.ilu.SetPattern0(self.A)
+        if fill_level == 0:
+            self.ilu.SetPattern0(self.A)
+        else:
+            self.ilu.Initialize_ILUk(self.A, fill_level)
         self.xyz = torch.from_numpy(prob.xyz).to(dev)
         self.myu = torch.from_numpy(prob.myu).to(dev)
         self.lam = torch.from_numpy(prob.lam).to(dev)

@@ -208,9 +208,22 @@ class CPreconditionerILU:
         self.ncrs = m.ncrs
 
     def Initialize_ILUk(self, m: CMatrixSparse, fill_level: int) -> None:
-        if fill_level != 0:
-            raise NotImplementedError("only ILU(0) (fill_level=0) is implemented")
-        self.SetPattern0(m)
+        """ls_ilu_block_sparse.cpp:849-999: ILU(k) pattern (0: SetPattern0,
+        -1: every fill)."""
+        N.call("dfm2h_ilu_set_pattern_k", self.h, m.h, int(fill_level))
+        self.nblk = m.nrowblk_
+        self.ncrs = self._ncrs()
+
+    def _ncrs(self) -> int:
+        n = C.c_size_t(0)
+        N.call("dfm2h_ilu_info", self.h, None, C.byref(n), None)
+        return n.value
+
+    def colind(self) -> np.ndarray:
+        """Row starts of the ILU pattern (A's for ILU(0))."""
+        ci = np.zeros(self.nblk + 1, np.uint32)
+        N.call("dfm2h_ilu_info", self.h, None, None, _hptr(ci))
+        return ci
 
     def CopyValue(self, m: CMatrixSparse) -> None:
         N.call("dfm2h_ilu_copy_value", self.h, m.h)

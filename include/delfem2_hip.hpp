@@ -4,7 +4,7 @@
  *
  * Namespace delfem2::hip mirrors the reference's names and signatures:
  *   CMatrixSparse        ls_block_sparse.h:24-204   (3x3 blocks, double)
- *   CPreconditionerILU   ls_ilu_block_sparse.h:21-60 (ILU(0))
+ *   CPreconditionerILU   ls_ilu_block_sparse.h:21-60 (ILU(0), ILU(k))
  *   ViewAsVectorX<REAL>  view_vectorx.h:26-55 (a non-owning view of DEVICE memory)
  *   AddScaledVec / ScaleAndAddVec / AddMatVec / SolvePrecond / Dot
  *                        view_vectorx.h:61-109 (found by ADL)
@@ -106,8 +106,7 @@ class CPreconditionerILU {
   }
   void SetPattern0(const CMatrixSparse &m) { check(dfm2h_ilu_set_pattern0(h, m.h)); }
   void Initialize_ILUk(const CMatrixSparse &m, int fill_level) {
-    if (fill_level != 0) throw Error(DFM2H_ERR_INVALID, "only ILU(0) is implemented");
-    SetPattern0(m);
+    check(dfm2h_ilu_set_pattern_k(h, m.h, fill_level));
   }
   void CopyValue(const CMatrixSparse &m) { check(dfm2h_ilu_copy_value(h, m.h)); }
   bool Decompose() {

@@ -143,6 +143,13 @@ int dfm2h_ilu_destroy(dfm2h_ilu P);
  * finds the first upper slot of each row, and schedules the rows in
  * dependency levels for the parallel factorisation and sweeps. */
 int dfm2h_ilu_set_pattern0(dfm2h_ilu P, dfm2h_bsr A);
+/* ls_ilu_block_sparse.cpp:849-999 Initialize_ILUk: the ILU(k) pattern of A
+ * (lev_fill 0: SetPattern0; -1: every fill).  The numeric CopyValue /
+ * Decompose / sweeps and the fused solvers work on it unchanged. */
+int dfm2h_ilu_set_pattern_k(dfm2h_ilu P, dfm2h_bsr A, int lev_fill);
+/* block rows, stored off-diagonal blocks, and (nullable, nblk+1 entries)
+ * the row starts of the ILU pattern (colInd; A's own for ILU(0)). */
+int dfm2h_ilu_info(dfm2h_ilu P, size_t *nblk, size_t *ncrs, unsigned *colind);
 /* ls_ilu_block_sparse.cpp:1009-1047 CopyValue(m). */
 int dfm2h_ilu_copy_value(dfm2h_ilu P, dfm2h_bsr A);
 /* ls_ilu_block_sparse.cpp:152-420 Decompose() (3x3 branch 286-356).  Rows

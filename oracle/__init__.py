@@ -59,6 +59,9 @@ def lib():
         L.orc_matvec.argtypes = [C.c_uint, _u32p, _u32p, _f64p, _f64p, _f64p, C.c_double,
                                  _f64p, C.c_double]
         L.orc_ilu_set_pattern0.argtypes = [C.c_uint, _u32p, _u32p, _u32p, _u32p]
+        L.orc_ilu_pattern_k.argtypes = [C.c_uint, _u32p, _u32p, C.c_int, _u32p, _vp, _vp,
+                                        C.c_size_t]
+        L.orc_ilu_pattern_k.restype = C.c_size_t
         L.orc_ilu_copy_value.argtypes = [C.c_uint, _u32p, _u32p, _f64p, _f64p, _u32p, _u32p,
                                          _f64p, _f64p]
         L.orc_ilu_decompose.argtypes = [C.c_uint, _u32p, _u32p, _u32p, _f64p, _f64p,
@@ -110,6 +113,8 @@ def ref():
         R.ref_matvec.argtypes = [_vp, _f64p, C.c_double, _f64p, C.c_double]
         R.ref_ilu_factor.argtypes = [_vp]
         R.ref_ilu_factor.restype = C.c_int
+        R.ref_ilu_init_k.argtypes = [_vp, C.c_int, _vp]
+        R.ref_ilu_init_k.restype = C.c_size_t
         R.ref_ilu_get.argtypes = [_vp, _vp, _vp, _vp, _vp]
         R.ref_ilu_forward.argtypes = [_vp, _f64p]
         R.ref_ilu_backward.argtypes = [_vp, _f64p]
@@ -159,9 +164,34 @@ class System:
         lib().orc_ilu_set_pattern0(self.nblk, self.colind, self.rowptr, self.l_rowptr,
                                    self.l_diaind)
         self.l_rowptr = self.l_rowptr[:ncrs].copy() if ncrs else self.l_rowptr
+        self.l_colind = self.colind  # the ILU(0) pattern has A's rows
         self.l_vcrs = np.zeros(ncrs * 9, np.float64)
         self.l_vdia = np.zeros(self.nblk * 9, np.float64)
         self.n_singular = 0
+
+    def set_fill_level(self, lev_fill: int) -> None:
+        """Initialize_ILUk (ls_ilu_block_sparse.cpp:849-999): the ILU(k)
+        pattern (lev_fill 0: A's pattern; -1: every fill)."""
+        if lev_fill == 0:
+            ncrs = int(self.colind[-1])
+            self.l_rowptr = np.zeros(max(ncrs, 1), np.uint32)
+            self.l_diaind = np.zeros(self.nblk, np.uint32)
+            lib().orc_ilu_set_pattern0(self.nblk, self.colind, self.rowptr, self.l_rowptr,
+                                       self.l_diaind)
+            self.l_rowptr = self.l_rowptr[:ncrs].copy() if ncrs else self.l_rowptr
+            self.l_colind = self.colind
+        else:
+            ci = np.zeros(self.nblk + 1, np.uint32)
+            n = lib().orc_ilu_pattern_k(self.nblk, self.colind, self.rowptr, int(lev_fill), ci,
+                                        None, None, 0)
+            rp = np.zeros(max(n, 1), np.uint32)
+            di = np.zeros(self.nblk, np.uint32)
+            lib().orc_ilu_pattern_k(self.nblk, self.colind, self.rowptr, int(lev_fill), ci,
+                                    rp.ctypes.data, di.ctypes.data, rp.size)
+            self.l_colind, self.l_rowptr, self.l_diaind = ci, rp[:n].copy(), di
+            ncrs = n
+        self.l_vcrs = np.zeros(ncrs * 9, np.float64)
+        self.l_vdia = np.zeros(self.nblk * 9, np.float64)
 
     @property
     def ndof(self) -> int:
@@ -186,20 +216,20 @@ class System:
                          float(alpha), x, float(beta))
 
     def factor(self) -> bool:
-        lib().orc_ilu_copy_value(self.nblk, self.colind, self.l_rowptr, self.l_vcrs, self.l_vdia,
-                                 self.colind, self.rowptr, self.vcrs, self.vdia)
+        lib().orc_ilu_copy_value(self.nblk, self.l_colind, self.l_rowptr, self.l_vcrs,
+                                 self.l_vdia, self.colind, self.rowptr, self.vcrs, self.vdia)
         ns = C.c_int(0)
-        ok = lib().orc_ilu_decompose(self.nblk, self.colind, self.l_rowptr, self.l_diaind,
+        ok = lib().orc_ilu_decompose(self.nblk, self.l_colind, self.l_rowptr, self.l_diaind,
                                      self.l_vcrs, self.l_vdia, C.byref(ns))
         self.n_singular = ns.value
         return bool(ok)
 
     def forward(self, vec):
-        lib().orc_ilu_forward(self.nblk, self.colind, self.l_rowptr, self.l_diaind, self.l_vcrs,
-                              self.l_vdia, vec)
+        lib().orc_ilu_forward(self.nblk, self.l_colind, self.l_rowptr, self.l_diaind,
+                              self.l_vcrs, self.l_vdia, vec)
 
     def backward(self, vec):
-        lib().orc_ilu_backward(self.nblk, self.colind, self.l_rowptr, self.l_diaind,
+        lib().orc_ilu_backward(self.nblk, self.l_colind, self.l_rowptr, self.l_diaind,
                                self.l_vcrs, vec)
 
     def solve_precond(self, vec):
@@ -212,7 +242,7 @@ class System:
         cap = max_nitr + 2
         hist = np.zeros(cap, np.float64)
         n = lib().orc_solve_pcg(r, x, float(tol), int(max_nitr), self.nblk, self.colind,
-                                self.rowptr, self.vcrs, self.vdia, self.colind, self.l_rowptr,
+                                self.rowptr, self.vcrs, self.vdia, self.l_colind, self.l_rowptr,
                                 self.l_diaind, self.l_vcrs, self.l_vdia, hist, cap)
         return x, hist[:n].copy()
 
@@ -273,8 +303,15 @@ class RefSystem:
     def factor(self) -> bool:
         return bool(ref().ref_ilu_factor(self.h))
 
+    def set_fill_level(self, lev_fill: int):
+        """Initialize_ILUk on the reference; returns the ILU colind."""
+        ci = np.zeros(self.nblk + 1, np.uint32)
+        self.l_ncrs = int(ref().ref_ilu_init_k(self.h, int(lev_fill), ci.ctypes.data))
+        self.l_colind = ci
+        return ci
+
     def ilu(self):
-        ncrs = int(self.colind[-1])
+        ncrs = getattr(self, "l_ncrs", int(self.colind[-1]))
         rp = np.zeros(max(ncrs, 1), np.uint32)
         di = np.zeros(self.nblk, np.uint32)
         vc = np.zeros(max(ncrs * 9, 1))

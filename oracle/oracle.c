@@ -537,3 +537,81 @@ void orc_adjoint_grad_tet(double *grad, const double *mu1_e,
     grad[iel] = -s;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* ls_ilu_block_sparse.cpp:849-999 Initialize_ILUk, symbolic phase for
+ * lev_fill >= 1 (or -1: every fill).  Row by row: the row's entries start as
+ * A's columns (sorted, level 0); the entries k < i are visited in ascending
+ * order, including fills added meanwhile; a k whose level + 1 exceeds
+ * lev_fill is skipped; for every entry j of row k's upper part (levels
+ * likewise limited), j != i, not yet in row i: add (j, max(lev_ik, lev_kj) +
+ * 1).  An entry already present keeps its level (the reference's rule).
+ * Output: ILU colind[nblk+1]; rowptr / diaind (first column > i) filled when
+ * cap >= the returned entry count.  Empty rows are allowed here. */
+typedef struct { unsigned col, lev; } orc_cl;
+
+static int orc_cl_cmp(const void *a, const void *b) {
+  const unsigned x = ((const orc_cl *)a)->col, y = ((const orc_cl *)b)->col;
+  return x < y ? -1 : x > y;
+}
+
+size_t orc_ilu_pattern_k(unsigned nblk, const unsigned *a_colind, const unsigned *a_rowptr,
+                         int lev_fill, unsigned *colind, unsigned *rowptr, unsigned *diaind,
+                         size_t cap) {
+  size_t ncap = (size_t)a_colind[nblk] * 2 + 16, nent = 0;
+  orc_cl *ent = (orc_cl *)malloc(sizeof(orc_cl) * ncap);
+  unsigned *dia = (unsigned *)malloc(sizeof(unsigned) * (nblk + 1));
+  size_t rcap = 64;
+  orc_cl *row = (orc_cl *)malloc(sizeof(orc_cl) * rcap);
+  colind[0] = 0;
+  for (unsigned i = 0; i < nblk; ++i) {
+    size_t n = 0;
+    for (unsigned s = a_colind[i]; s < a_colind[i + 1]; ++s) {
+      if (n == rcap) { rcap *= 2; row = (orc_cl *)realloc(row, sizeof(orc_cl) * rcap); }
+      row[n].col = a_rowptr[s];
+      row[n].lev = 0;
+      ++n;
+    }
+    qsort(row, n, sizeof(orc_cl), orc_cl_cmp);
+    for (size_t t = 0; t < n; ++t) {
+      const unsigned k = row[t].col, lik = row[t].lev;
+      if (lev_fill != -1 && (int)lik + 1 > lev_fill) continue;
+      if (k >= i) break;
+      for (size_t kj = dia[k]; kj < colind[k + 1]; ++kj) {
+        const unsigned lkj = ent[kj].lev, j = ent[kj].col;
+        if (lev_fill != -1 && (int)lkj + 1 > lev_fill) continue;
+        if (j == i) continue;
+        /* position of j among row[t+1 .. n) (j > k) */
+        size_t lo = t + 1, hi = n;
+        while (lo < hi) {
+          const size_t mid = (lo + hi) / 2;
+          if (row[mid].col < j) lo = mid + 1; else hi = mid;
+        }
+        if (lo < n && row[lo].col == j) continue;
+        if (n == rcap) { rcap *= 2; row = (orc_cl *)realloc(row, sizeof(orc_cl) * rcap); }
+        memmove(row + lo + 1, row + lo, sizeof(orc_cl) * (n - lo));
+        row[lo].col = j;
+        row[lo].lev = (lik > lkj ? lik : lkj) + 1;
+        ++n;
+      }
+    }
+    if (nent + n > ncap) {
+      while (nent + n > ncap) ncap *= 2;
+      ent = (orc_cl *)realloc(ent, sizeof(orc_cl) * ncap);
+    }
+    memcpy(ent + nent, row, sizeof(orc_cl) * n);
+    colind[i + 1] = (unsigned)(nent + n);
+    dia[i] = colind[i + 1];
+    for (size_t t = 0; t < n; ++t)
+      if (row[t].col > i) { dia[i] = (unsigned)(nent + t); break; }
+    nent += n;
+  }
+  if (rowptr && cap >= nent) {
+    for (size_t t = 0; t < nent; ++t) rowptr[t] = ent[t].col;
+    if (diaind) memcpy(diaind, dia, sizeof(unsigned) * nblk);
+  }
+  free(ent);
+  free(dia);
+  free(row);
+  return nent;
+}

@@ -82,6 +82,9 @@ void orc_ilu_set_pattern0(unsigned nblk, const unsigned *colind,
                           unsigned *diaind);
 
 /* ls_ilu_block_sparse.cpp:1009-1047 CopyValue. */
+size_t orc_ilu_pattern_k(unsigned nblk, const unsigned *a_colind, const unsigned *a_rowptr,
+                         int lev_fill, unsigned *colind, unsigned *rowptr, unsigned *diaind,
+                         size_t cap);
 void orc_ilu_copy_value(unsigned nblk, const unsigned *ilu_colind,
                         const unsigned *ilu_rowptr, double *ilu_vcrs,
                         double *ilu_vdia, const unsigned *a_colind,

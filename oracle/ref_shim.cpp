@@ -149,6 +149,14 @@ void ref_matvec(void *h, double *y, double alpha, const double *x, double beta) 
   static_cast<RefSys *>(h)->A.MatVec(y, alpha, x, beta);
 }
 
+// ls_ilu_block_sparse.cpp:849-999 (lev_fill 0: SetPattern0)
+size_t ref_ilu_init_k(void *h, int lev_fill, unsigned *colind) {
+  auto *s = static_cast<RefSys *>(h);
+  s->ilu.Initialize_ILUk(s->A, lev_fill);
+  if (colind) std::memcpy(colind, s->ilu.colInd.data(), sizeof(unsigned) * s->ilu.colInd.size());
+  return s->ilu.rowPtr.size();
+}
+
 // ls_ilu_block_sparse.cpp:1009-1047 then :152-420
 int ref_ilu_factor(void *h) {
   auto *s = static_cast<RefSys *>(h);

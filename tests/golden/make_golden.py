@@ -32,8 +32,9 @@ def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000):
-    """Full reference pipeline on one problem; returns a dict of outputs."""
+def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000, fill=0):
+    """Full reference pipeline on one problem (ILU(fill): Initialize_ILUk);
+    returns a dict of outputs."""
     R = oracle.ref()
     ind = np.zeros(prob.np_ + 1, np.uint32)
     n = R.ref_psup(prob.tet, prob.ne, prob.np_, ind, None)
@@ -54,6 +55,7 @@ def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000):
     xv = W.uniform(777, prob.ndof) - 0.5
     yv = W.uniform(778, prob.ndof) - 0.5
     sysr.matvec(yv, 1.5, xv, -0.25)
+    ci = sysr.set_fill_level(fill) if fill != 0 else ind
     ok = sysr.factor()
     rp, di, lc, ld = sysr.ilu()
     pre = bb.copy()
@@ -62,7 +64,7 @@ def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000):
     x, hist = sysr.solve_pcg(r, tol, maxit)
     return dict(psup_ind=ind, psup=psup, merge_status=st, vcrs_asm=vcrs_asm, vdia_asm=vdia_asm,
                 b_asm=b_asm, vcrs=vcrs, vdia=vdia, b=bb, matvec_x=xv, matvec_y=yv,
-                ilu_ok=ok, ilu_rowptr=rp, ilu_diaind=di, ilu_vcrs=lc, ilu_vdia=ld,
+                ilu_ok=ok, ilu_colind=ci, ilu_rowptr=rp, ilu_diaind=di, ilu_vcrs=lc, ilu_vdia=ld,
                 precond_b=pre, x=x, hist=hist, r_final=r)
 
 
@@ -161,6 +163,15 @@ def main():
     oc = ref_case(pc)
     meta["cases"]["cantilever844"] = summary(oc)
     arrays["cant_x"] = oc["x"]
+
+    # --- ILU(k) (Initialize_ILUk, ls_ilu_block_sparse.cpp:849-999): fill levels 1, 2 and
+    # every fill (-1, an exact LU: one PCG iteration)
+    for name, prob, fill in (("iluk1_c6_sigma05", W.cube(6, sigma=0.5, seed=12), 1),
+                             ("iluk2_c6_sigma05_color4",
+                              W.cube(6, sigma=0.5, seed=12, ordering="color4"), 2),
+                             ("iluk_all_n3", W.cube(3, sigma=0.5, seed=99), -1)):
+        meta["cases"][name] = summary(ref_case(prob, fill=fill))
+        meta["cases"][name]["fill"] = fill
 
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **arrays)
     with open(os.path.join(HERE, "golden.json"), "w") as f:

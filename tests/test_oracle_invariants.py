@@ -243,3 +243,53 @@ def test_decompose_singular_count():
     S.vcrs[:] = 0.0
     assert not S.factor()
     assert S.n_singular == 11  # stops at the 11th, as the reference
+
+
+@pytest.mark.parametrize("fill", [1, 2])
+def test_iluk_defining_property(fill):
+    # ILU(k): the pattern contains A's; (L + D)(I + U) == A on every position of
+    # the ILU(k) pattern (A is zero at the fill positions); the sweeps invert
+    # those factors
+    p = W.cube(3, sigma=0.5, seed=5)
+    S, b = assemble(p)
+    S.set_fill_level(fill)
+    assert S.factor()
+    nb = S.nblk
+    for i in range(nb):
+        a_cols = set(S.rowptr[S.colind[i]:S.colind[i + 1]].tolist())
+        l_cols = S.l_rowptr[S.l_colind[i]:S.l_colind[i + 1]]
+        assert a_cols <= set(l_cols.tolist()) and np.all(np.diff(l_cols.astype(int)) > 0)
+    A = bsr_to_csr(nb, S.colind, S.rowptr, S.vcrs, S.vdia).toarray()
+    L = np.zeros_like(A); U = np.zeros_like(A); D = np.zeros_like(A)
+    lc = S.l_vcrs.reshape(-1, 3, 3)
+    ld = S.l_vdia.reshape(-1, 3, 3)
+    mask = np.zeros((nb, nb), bool)
+    for i in range(nb):
+        D[3 * i:3 * i + 3, 3 * i:3 * i + 3] = np.linalg.inv(ld[i])
+        mask[i, i] = True
+        for s in range(S.l_colind[i], S.l_colind[i + 1]):
+            j = S.l_rowptr[s]
+            mask[i, j] = True
+            tgt = L if j < i else U
+            tgt[3 * i:3 * i + 3, 3 * j:3 * j + 3] = lc[s]
+    M = (L + D) @ (np.eye(3 * nb) + U)
+    maskf = np.kron(mask, np.ones((3, 3), bool))
+    assert np.allclose(M[maskf], A[maskf], rtol=1e-11, atol=1e-12 * np.abs(A).max())
+    y = W.uniform(9, 3 * nb) - 0.5
+    v = (L + D) @ y
+    S.forward(v)
+    assert np.allclose(v, y, rtol=1e-10, atol=1e-12)
+
+
+def test_iluk_full_fill_is_exact_lu():
+    # every fill (-1): the factorisation is a complete block LU; PCG converges
+    # in one iteration to scipy's sparse direct solve
+    import scipy.sparse.linalg as spla
+    p = W.cube(3, sigma=0.5, seed=5)
+    S, b = assemble(p)
+    S.set_fill_level(-1)
+    assert S.factor()
+    x, hist = S.solve_pcg(b.copy(), 1e-12, 50)
+    assert hist.size - 1 == 1
+    A = bsr_to_csr(S.nblk, S.colind, S.rowptr, S.vcrs, S.vdia).tocsc()
+    assert np.allclose(x, spla.spsolve(A, b), rtol=1e-9, atol=1e-14)

@@ -471,7 +471,7 @@ struct dfm2h_batch_s {
   DBuf<MeshScal> M;
   DBuf<BatchGlobal> G;
   int variant[2] = {-1, -1};  // [fp32] chosen launch shape, -1 = autotune
-  unsigned period = 50;       // FP32 form: forced reliable-update period
+  unsigned period = 25;       // FP32 form: forced reliable-update period
   hipStream_t s = nullptr;
   hipEvent_t ev_in = nullptr, ev_poll[2] = {nullptr, nullptr}, ev_t0 = nullptr, ev_t1 = nullptr;
   BatchGlobal *host_g = nullptr;  // pinned, 2 entries

@@ -409,7 +409,7 @@ class BatchSolver:
 
     def set_update_period(self, period: int) -> None:
         """FP32 form: forced reliable update after `period` inner iterations
-        without one (0: off; default 50)."""
+        without one (0: off; default 25)."""
         N.call("dfm2h_batch_set_update_period", self.h, int(period))
 
     def variant(self, fp32: bool):

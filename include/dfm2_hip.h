@@ -312,7 +312,7 @@ int dfm2h_batch_solve(dfm2h_batch B, const double *b_dev, double *x_dev,
 int dfm2h_batch_tune(dfm2h_batch B, int fp32, int *best, double *ms,
                      size_t ms_cap, size_t *n_variants);
 /* FP32 form: force a reliable update of a mesh after update_period inner
- * iterations without one (0: only the delta rule; default 50). */
+ * iterations without one (0: only the delta rule; default 25). */
 int dfm2h_batch_set_update_period(dfm2h_batch B, unsigned update_period);
 /* variant -1 = autotune on the next solve; 0 .. n_variants-1 = fixed. */
 int dfm2h_batch_set_variant(dfm2h_batch B, int fp32, int variant);

@@ -36,7 +36,7 @@ constexpr unsigned kChunk = 16;    // iterations per captured graph
 
 struct MeshScal {
   double rPr, alpha, beta, inv_sq0, sq_b, rmax2, res;
-  int it, done, converged, nan, upd, nupd, final_, it_upd;
+  int it, done, converged, nan, upd, nupd, final_, it_upd, restart, pad;
 };
 
 struct BatchGlobal {
@@ -52,17 +52,26 @@ struct BPlan {
 
 template <typename T>
 struct BArgs {
-  unsigned nmesh, nlev_b;
+  unsigned nmesh, nlev_b, nlev_f;
+  int any_iso;
   BPlan A, F, B;
   const T *a_val, *a_dia, *f_val, *f_dia, *b_val;
   const double *a64_val, *a64_dia;
+  // fused forward level 0 / backward level 0 (matrix-slice masks, D^-1 of
+  // the level-0 rows in matrix-slice layout; per forward slice: rows with no
+  // upper blocks; per forward level: whether it has such rows)
+  const unsigned *l0mask, *isomask, *fumask;
+  const T *dinv0;
+  const unsigned char *flev_u0;
   T *r, *x, *Pr, *p;
   double *x64;
   const double *b64;
-  double *partA, *partA2, *partB;
+  // per-slice partials: partA (p,Ap) / |b|^2, partA2 |r|^2, partB and partF
+  // and partI (r,Pr), partE reliable-update drift
+  double *partA, *partA2, *partB, *partF, *partI, *partE;
   MeshScal *M;
   BatchGlobal *G;
-  const unsigned *a_mptr, *b_gptr;
+  const unsigned *a_mptr, *b_gptr, *f_gptr;
 };
 
 // Launch-shape variants the autotuner chooses from:
@@ -113,7 +122,13 @@ inline unsigned grid_of(unsigned nsl) {
 
 // ---------------------------------------------------------------------------
 // slice kernels (every one walks the slices of one plan; a wavefront is
-// uniform in slice, hence in mesh)
+// uniform in slice, hence in mesh).  The iteration has the fused form of the
+// single-system solver: forward level 0 (rows with no lower blocks) is
+// computed by the r/x update from the row's new r components; backward level
+// 0 (rows with no upper blocks) has no work, its (r, Pr) share is summed by
+// the forward level that finalises those rows (partF) or, for rows with
+// neither lower nor upper blocks, by the update (partI); the FP32 form's
+// reliable update runs inside the next SpMV.
 
 // r = b, x = 0 (and x64 = 0), partial |b|^2
 template <typename T, class V>
@@ -138,17 +153,46 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_init(BArgs<T> a, unsigned n
   }
 }
 
-// Ap -> Pr, partial (p, Ap)
+// FP32 form, reliable update of row i: r_i = b_i - (A (x64 + x32))_i in FP64
+// (FP64 values), stored into r; returns r_i^2 and adds (r_true - r_rec)_i^2
+// to drift.
+template <class V>
+__device__ inline double batch_residual_row(const BArgs<float> &a, const SliceLane &L,
+                                            unsigned s, unsigned i, double &drift) {
+  const unsigned K = a.A.len[s * kSliceRows + L.r];
+  const unsigned q0 = a.A.off[s];
+  double acc = 0.0;
+  for (unsigned k = 0; k < K; ++k) {
+    const size_t q = q0 + k;
+    const size_t j = kVSB * (size_t)a.A.col[q * kSliceRows + L.r];
+    const Trip<double> v = load_trip(a.a64_val, q, L.lane);
+    acc += v.a * (a.x64[j] + (double)a.x[j]) + v.b * (a.x64[j + 1] + (double)a.x[j + 1]) +
+           v.c * (a.x64[j + 2] + (double)a.x[j + 2]);
+  }
+  const size_t ii = kVSB * (size_t)i;
+  acc += dia_row_dot(a.a64_dia, L, a.x64[ii] + (double)a.x[ii],
+                     a.x64[ii + 1] + (double)a.x[ii + 1], a.x64[ii + 2] + (double)a.x[ii + 2]);
+  const double r = a.b64[ii + L.c] - acc;
+  const double e = r - (double)a.r[ii + L.c];
+  drift += e * e;
+  a.r[ii + L.c] = (float)r;
+  return r * r;
+}
+
+// Ap -> Pr, partial (p, Ap); FP32 form with the mesh's reliable update due:
+// also r = b - A x (partA2: |r|^2, partE: drift)
 template <typename T, class V>
 __global__ void __launch_bounds__(V::WPB * kWave) kb_spmv(BArgs<T> a, unsigned nsl) {
   const unsigned s0 = wave_first<V>(0);
   for (int k = 0; k < V::SPW; ++k) {
     const unsigned s = s0 + k;
     if (s >= nsl) return;
-    if (a.M[a.A.mesh[s]].done) continue;
+    const MeshScal *S = a.M + a.A.mesh[s];
+    if (S->done) continue;
+    const bool refine = sizeof(T) == 4 && S->upd;  // wavefront-uniform
     const SliceLane L = lane_of<V>(s);
     const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
-    double part = 0.0;
+    double part = 0.0, part2 = 0.0, part3 = 0.0;
     if (i != ~0u) {
       T acc = sliced_row_sum<T, false, V::template US<T>(), kVSB>(
           (T)0, (T)1, L, a.A.len[s * kSliceRows + L.r], a.A.kmax[s], a.A.off[s], a.A.col,
@@ -158,13 +202,25 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_spmv(BArgs<T> a, unsigned n
       const size_t e = kVSB * (size_t)i + L.c;
       a.Pr[e] = acc;
       part = (double)a.p[e] * (double)acc;
+      if constexpr (sizeof(T) == 4)
+        if (refine) part2 = batch_residual_row<V>(a, L, s, i, part3);
     }
     part = wave_reduce_sum(part);
     if (L.lane == 0) a.partA[s] = part;
+    if (refine) {
+      part2 = wave_reduce_sum(part2);
+      part3 = wave_reduce_sum(part3);
+      if (L.lane == 0) {
+        a.partA2[s] = part2;
+        a.partE[s] = part3;
+      }
+    }
   }
 }
 
-// r -= alpha Ap, x += alpha p, partial |r|^2
+// r -= alpha Ap, x += alpha p (FP32 form after a reliable update: x64 +=
+// x32, x32 = alpha p), partial |r|^2; then forward level 0 of the slice's
+// level-0 rows (and (r, Pr) of its isolated rows)
 template <typename T, class V>
 __global__ void __launch_bounds__(V::WPB * kWave) kb_update(BArgs<T> a, unsigned nsl) {
   const unsigned s0 = wave_first<V>(0);
@@ -174,23 +230,46 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_update(BArgs<T> a, unsigned
     const MeshScal *S = a.M + a.A.mesh[s];
     if (S->done) continue;
     const double alpha = S->alpha;
+    const bool fold = sizeof(T) == 4 && S->upd;
     const SliceLane L = lane_of<V>(s);
     const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
-    double part = 0.0;
+    double part = 0.0, srp = 0.0;
+    T rn = 0;
     if (i != ~0u) {
       const size_t e = kVSB * (size_t)i + L.c;
-      const T rn = (T)((double)a.r[e] + (-alpha) * (double)a.Pr[e]);
+      rn = (T)((double)a.r[e] + (-alpha) * (double)a.Pr[e]);
       a.r[e] = rn;
-      a.x[e] = (T)((double)a.x[e] + alpha * (double)a.p[e]);
+      if (fold) {
+        a.x64[e] += (double)a.x[e];
+        a.x[e] = (T)(alpha * (double)a.p[e]);
+      } else {
+        a.x[e] = (T)((double)a.x[e] + alpha * (double)a.p[e]);
+      }
       part = (double)rn * (double)rn;
+    }
+    const unsigned m0 = a.l0mask[s];
+    if (m0) {
+      const T t0 = __shfl(rn, (int)L.r, kWave);
+      const T t1 = __shfl(rn, (int)L.r + kSliceRows, kWave);
+      const T t2 = __shfl(rn, (int)L.r + 2 * kSliceRows, kWave);
+      if (i != ~0u && ((m0 >> L.r) & 1u)) {
+        const T z = dia_row_dot(a.dinv0, L, t0, t1, t2);
+        a.Pr[kVSB * (size_t)i + L.c] = z;
+        if ((a.isomask[s] >> L.r) & 1u) srp = (double)rn * (double)z;
+      }
     }
     part = wave_reduce_sum(part);
     if (L.lane == 0) a.partA2[s] = part;
+    if (a.any_iso) {
+      srp = wave_reduce_sum(srp);
+      if (L.lane == 0) a.partI[s] = srp;
+    }
   }
 }
 
-// forward level: Pr = D^-1 (r - L Pr)
-template <typename T, class V>
+// forward level: Pr = D^-1 (r - L Pr); PART: partial (r, Pr) of the rows
+// with no upper blocks (partF)
+template <typename T, class V, bool PART>
 __global__ void __launch_bounds__(V::WPB * kWave)
 kb_fwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
   const unsigned s0 = wave_first<V>(slice0);
@@ -211,8 +290,17 @@ kb_fwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
     const T t0 = __shfl(t, (int)L.r, kWave);
     const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
     const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
-    if (valid)
-      a.Pr[kVSB * (size_t)i + L.c] = dia_row_dot_m<T, T, sweep_nt<T>()>(a.f_dia, L, t0, t1, t2);
+    double part = 0.0;
+    if (valid) {
+      const T out = dia_row_dot_m<T, T, sweep_nt<T>()>(a.f_dia, L, t0, t1, t2);
+      a.Pr[kVSB * (size_t)i + L.c] = out;
+      if (PART && ((a.fumask[s] >> L.r) & 1u))
+        part = (double)a.r[kVSB * (size_t)i + L.c] * (double)out;
+    }
+    if (PART) {
+      part = wave_reduce_sum(part);
+      if (L.lane == 0) a.partF[s] = part;
+    }
   }
 }
 
@@ -261,50 +349,13 @@ kb_pupdate(BArgs<T> a, unsigned nsl, int mode) {
   }
 }
 
-// reliable update, part 1 (meshes due one): x64 += x32, x32 = 0
-template <class V>
-__global__ void __launch_bounds__(V::WPB * kWave) kb_fold(BArgs<float> a, unsigned nsl) {
-  const unsigned s0 = wave_first<V>(0);
-  for (int k = 0; k < V::SPW; ++k) {
-    const unsigned s = s0 + k;
-    if (s >= nsl) return;
-    const MeshScal *S = a.M + a.A.mesh[s];
-    if (S->done || !S->upd) continue;
-    const SliceLane L = lane_of<V>(s);
-    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
-    if (i == ~0u) continue;
-    const size_t e = kVSB * (size_t)i + L.c;
-    a.x64[e] += (double)a.x[e];
-    a.x[e] = 0.0f;
-  }
-}
-
-// reliable update, part 2: r = b - A x64 in FP64, rounded into r32, partial |r|^2
-template <class V>
-__global__ void __launch_bounds__(V::WPB * kWave) kb_residual(BArgs<float> a, unsigned nsl) {
-  const unsigned s0 = wave_first<V>(0);
-  for (int k = 0; k < V::SPW; ++k) {
-    const unsigned s = s0 + k;
-    if (s >= nsl) return;
-    const MeshScal *S = a.M + a.A.mesh[s];
-    if (S->done || !S->upd) continue;
-    const SliceLane L = lane_of<V>(s);
-    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
-    double part = 0.0;
-    if (i != ~0u) {
-      double acc = sliced_row_sum<double, false, V::template US<double>(), kVSB>(
-          0.0, 1.0, L, a.A.len[s * kSliceRows + L.r], a.A.kmax[s], a.A.off[s], a.A.col,
-          a.a64_val, a.x64);
-      const Trip<double> xi = load_node<double, kVSB>(a.x64, i);
-      acc += dia_row_dot(a.a64_dia, L, xi.a, xi.b, xi.c);
-      const size_t e = kVSB * (size_t)i + L.c;
-      const double r = a.b64[e] - acc;
-      a.r[e] = (float)r;
-      part = r * r;
-    }
-    part = wave_reduce_sum(part);
-    if (L.lane == 0) a.partA[s] = part;
-  }
+// the FP32 form's solution: x64 + x32 (a mesh that stops at a reliable
+// update keeps its last x32 part unfolded)
+__global__ void kb_mixed_out(double *__restrict__ x, const double *__restrict__ x64,
+                             const float *__restrict__ x32, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = x64[i] + (double)x32[i];
 }
 
 __global__ void kb_fill(double *y, double v, size_t n) {
@@ -320,17 +371,23 @@ __global__ void kb_fill(double *y, double v, size_t n) {
 enum BMode {
   kBInit = 0,         // |r0|^2 (FP64 form)
   kBAlpha = 1,        // alpha = rPr / pAp
-  kBConv = 2,         // FP64 |r|^2: stop test
-  kBRPr0 = 3,         // first (r, Pr)
-  kBBeta = 4,         // beta = rPr1 / rPr
+  kBConvBeta = 2,     // FP64 |r|^2: stop test; then beta
+  kBRPr0 = 3,         // first (r, Pr) (prologue: every backward level)
   kBInitMixed = 5,    // |b|^2
-  kBConvMixed = 6,    // FP32 |r|^2: decide a reliable update
-  kBReliable = 7      // FP64 |b - A x|^2: stop test
+  kBAlphaMixed = 6,   // reliable update (FP64 |b - A x|^2, drift) when due; then alpha
+  kBBetaMixed = 7     // FP32 |r|^2: decide the next reliable update; then beta
 };
 
 __device__ inline void mark_done(MeshScal *S, BatchGlobal *G) {
   S->done = 1;
   atomicAdd(&G->ndone, 1u);
+}
+
+// lane-strided sum of part[q] over [q0, q1)
+__device__ inline double lane_sum(const double *part, unsigned q0, unsigned q1, unsigned lane) {
+  double t = 0.0;
+  for (unsigned q = q0 + lane; q < q1; q += kWave) t += part[q];
+  return t;
 }
 
 template <int MODE>
@@ -343,18 +400,36 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
   MeshScal *S = a.M + m;
   constexpr bool kInit = MODE == kBInit || MODE == kBInitMixed;
   if (!kInit && S->done) return;
-  if (MODE == kBReliable && !S->upd) return;
-  double t = 0.0;
-  if (MODE == kBRPr0 || MODE == kBBeta) {
+  const bool upd = S->upd != 0;
+  const unsigned a0 = a.a_mptr[m], a1 = a.a_mptr[m + 1];
+  double t = 0.0, t2 = 0.0, t3 = 0.0;
+  if (MODE == kBRPr0) {
     for (unsigned l = 0; l < a.nlev_b; ++l) {
       const unsigned g = l * a.nmesh + m;
-      for (unsigned q = a.b_gptr[g] + lane; q < a.b_gptr[g + 1]; q += kWave) t += a.partB[q];
+      t += lane_sum(a.partB, a.b_gptr[g], a.b_gptr[g + 1], lane);
     }
+  } else if (MODE == kBConvBeta || MODE == kBBetaMixed) {
+    t = lane_sum(a.partA2, a0, a1, lane);
+    for (unsigned l = 1; l < a.nlev_b; ++l) {
+      const unsigned g = l * a.nmesh + m;
+      t2 += lane_sum(a.partB, a.b_gptr[g], a.b_gptr[g + 1], lane);
+    }
+    for (unsigned l = 1; l < a.nlev_f; ++l) {
+      if (!a.flev_u0[l]) continue;
+      const unsigned g = l * a.nmesh + m;
+      t2 += lane_sum(a.partF, a.f_gptr[g], a.f_gptr[g + 1], lane);
+    }
+    if (a.any_iso) t2 += lane_sum(a.partI, a0, a1, lane);
   } else {
-    const double *src = (MODE == kBConv || MODE == kBConvMixed) ? a.partA2 : a.partA;
-    for (unsigned q = a.a_mptr[m] + lane; q < a.a_mptr[m + 1]; q += kWave) t += src[q];
+    t = lane_sum(a.partA, a0, a1, lane);
+    if (MODE == kBAlphaMixed && upd) {
+      t2 = lane_sum(a.partA2, a0, a1, lane);
+      t3 = lane_sum(a.partE, a0, a1, lane);
+    }
   }
   t = wave_reduce_sum(t);
+  t2 = wave_reduce_sum(t2);
+  t3 = wave_reduce_sum(t3);
   if (lane != 0) return;
   BatchGlobal *G = a.G;
   const bool bad = !(t == t) || t > 1.0e308;
@@ -364,6 +439,7 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
     S->upd = 0;
     S->final_ = 0;
     S->it_upd = 0;
+    S->restart = 0;
     S->converged = 0;
     S->nan = 0;
     S->done = 0;
@@ -383,10 +459,7 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
     S->alpha = S->rPr / t;
   } else if (MODE == kBRPr0) {
     S->rPr = t;
-  } else if (MODE == kBBeta) {
-    S->beta = t / S->rPr;
-    S->rPr = t;
-  } else if (MODE == kBConv) {
+  } else if (MODE == kBConvBeta) {
     const int it = S->it + 1;
     S->it = it;
     S->res = sqrt(t * S->inv_sq0);
@@ -398,34 +471,50 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
       mark_done(S, G);
     } else if (it >= G->max_nitr) {
       mark_done(S, G);
+    } else {
+      S->beta = t2 / S->rPr;
+      S->rPr = t2;
     }
-  } else if (MODE == kBConvMixed) {
+  } else if (MODE == kBAlphaMixed) {
+    if (upd) {
+      const int k = S->nupd + 1;
+      S->nupd = k;
+      S->rmax2 = t2;
+      S->res = sqrt(t2 / S->sq_b);
+      if (S->res < G->tol) {
+        S->converged = 1;
+        mark_done(S, G);
+      } else if (!(t2 == t2) || t2 > 1.0e308) {
+        S->nan = 1;
+        mark_done(S, G);
+      } else if (S->final_ || k >= G->max_upd) {
+        mark_done(S, G);
+      }
+    }
+    if (upd && (!(t > 0.0) || t3 > 1.0e-2 * t2)) {  // drifted: restart the direction
+      S->alpha = 0.0;
+      S->restart = 1;
+    } else {
+      S->alpha = S->rPr / t;
+    }
+  } else {  // kBBetaMixed
     const int it = S->it + 1;
     S->it = it;
+    S->upd = 0;
     if (bad) {
       S->nan = 1;
       mark_done(S, G);
-    } else if (t < G->delta2 * S->rmax2 || sqrt(t / S->sq_b) < G->tol || it >= G->max_nitr ||
-               (G->period > 0 && it - S->it_upd >= G->period)) {
+      return;
+    }
+    if (t < G->delta2 * S->rmax2 || sqrt(t / S->sq_b) < G->tol || it >= G->max_nitr ||
+        (G->period > 0 && it - S->it_upd >= G->period)) {
       S->upd = 1;
       S->final_ = it >= G->max_nitr ? 1 : 0;
       S->it_upd = it;
     }
-  } else {  // kBReliable
-    const int k = S->nupd + 1;
-    S->nupd = k;
-    S->upd = 0;
-    S->rmax2 = t;
-    S->res = sqrt(t / S->sq_b);
-    if (S->res < G->tol) {
-      S->converged = 1;
-      mark_done(S, G);
-    } else if (bad) {
-      S->nan = 1;
-      mark_done(S, G);
-    } else if (S->final_ || k >= G->max_upd) {
-      mark_done(S, G);
-    }
+    S->beta = S->restart ? 0.0 : t2 / S->rPr;
+    S->restart = 0;
+    S->rPr = t2;
   }
 }
 
@@ -465,9 +554,16 @@ struct dfm2h_batch_s {
   unsigned nlev_f = 0, nlev_b = 0;
   PackedVals<double> a64, f64, b64v;
   PackedVals<float> a32, f32, b32;
+  // fused level-0 data (see BArgs)
+  DBuf<unsigned> l0mask, isomask, row_l0, fumask, f_gptr;
+  DBuf<unsigned char> flev_u0;
+  std::vector<unsigned char> flev_u0_host;
+  bool any_iso = false;
+  PackedVals<double> d64;
+  PackedVals<float> d32;
   BatchBufs<double> s64;
   BatchBufs<float> s32;
-  DBuf<double> x64, b64, partA, partA2, partB;
+  DBuf<double> x64, b64, partA, partA2, partB, partF, partI, partE;
   DBuf<MeshScal> M;
   DBuf<BatchGlobal> G;
   int variant[2] = {-1, -1};  // [fp32] chosen launch shape, -1 = autotune
@@ -518,10 +614,21 @@ std::vector<std::vector<unsigned>> level_mesh_groups(const std::vector<unsigned>
 
 template <typename T>
 BArgs<T> make_bargs(dfm2h_batch B, BatchBufs<T> &S, const PackedVals<T> &av,
-                    const PackedVals<T> &fv, const PackedVals<T> &bv) {
+                    const PackedVals<T> &fv, const PackedVals<T> &bv, const PackedVals<T> &dv) {
   BArgs<T> a{};
   a.nmesh = B->nmesh;
   a.nlev_b = B->nlev_b;
+  a.nlev_f = B->nlev_f;
+  a.any_iso = B->any_iso ? 1 : 0;
+  a.l0mask = B->l0mask.p;
+  a.isomask = B->isomask.p;
+  a.fumask = B->fumask.p;
+  a.dinv0 = dv.dia.p;
+  a.flev_u0 = B->flev_u0.p;
+  a.partF = B->partF.p;
+  a.partI = B->partI.p;
+  a.partE = B->partE.p;
+  a.f_gptr = B->f_gptr.p;
   a.A = BPlan{B->aplan.off.p, B->aplan.kmax.p, B->aplan.row.p, B->aplan.len.p, B->aplan.col.p,
               B->a_mesh.p};
   a.F = BPlan{B->fplan.off.p, B->fplan.kmax.p, B->fplan.row.p, B->fplan.len.p, B->fplan.col.p,
@@ -557,9 +664,16 @@ BArgs<float> ctl_view(const BArgs<T> &a) {
   BArgs<float> c{};
   c.nmesh = a.nmesh;
   c.nlev_b = a.nlev_b;
+  c.nlev_f = a.nlev_f;
+  c.any_iso = a.any_iso;
+  c.flev_u0 = a.flev_u0;
+  c.f_gptr = a.f_gptr;
   c.partA = a.partA;
   c.partA2 = a.partA2;
   c.partB = a.partB;
+  c.partF = a.partF;
+  c.partI = a.partI;
+  c.partE = a.partE;
   c.M = a.M;
   c.G = a.G;
   c.a_mptr = a.a_mptr;
@@ -573,17 +687,24 @@ void launch_scalar(const BArgs<T> &a, hipStream_t s) {
                      dim3(kMeshPerBlock * kWave), 0, s, ctl_view(a));
 }
 
+// first_level = 0: every level (prologue: (r,Pr) partials of every
+// backward level in partB); 1: the fused iteration (levels >= 1, forward
+// levels with rows that have no upper blocks write partF)
 template <typename T, class V>
-void launch_sweeps(const BArgs<T> &a, dfm2h_batch B, hipStream_t s) {
+void launch_sweeps(const BArgs<T> &a, dfm2h_batch B, hipStream_t s, unsigned first_level) {
   const unsigned nm = B->nmesh;
-  for (unsigned l = 0; l < B->nlev_f; ++l) {
+  for (unsigned l = first_level; l < B->nlev_f; ++l) {
     const unsigned s0 = B->fplan.group_ptr[(size_t)l * nm];
     const unsigned nsl = B->fplan.group_ptr[(size_t)(l + 1) * nm] - s0;
     if (!nsl) continue;
-    hipLaunchKernelGGL((kb_fwd<T, V>), dim3(grid_of<V>(nsl)), dim3(V::WPB * kWave), 0, s, a, s0,
-                       nsl);
+    if (first_level > 0 && B->flev_u0_host[l])
+      hipLaunchKernelGGL((kb_fwd<T, V, true>), dim3(grid_of<V>(nsl)), dim3(V::WPB * kWave), 0,
+                         s, a, s0, nsl);
+    else
+      hipLaunchKernelGGL((kb_fwd<T, V, false>), dim3(grid_of<V>(nsl)), dim3(V::WPB * kWave), 0,
+                         s, a, s0, nsl);
   }
-  for (unsigned l = 0; l < B->nlev_b; ++l) {
+  for (unsigned l = first_level; l < B->nlev_b; ++l) {
     const unsigned s0 = B->bplan.group_ptr[(size_t)l * nm];
     const unsigned nsl = B->bplan.group_ptr[(size_t)(l + 1) * nm] - s0;
     if (!nsl) continue;
@@ -601,28 +722,29 @@ void launch_prologue(const BArgs<T> &a, dfm2h_batch B, hipStream_t s) {
     launch_scalar<kBInitMixed>(a, s);
   else
     launch_scalar<kBInit>(a, s);
-  launch_sweeps<T, V>(a, B, s);
+  launch_sweeps<T, V>(a, B, s, 0);
   launch_scalar<kBRPr0>(a, s);
   hipLaunchKernelGGL((kb_pupdate<T, V>), ga, bs, 0, s, a, na, 0);
 }
 
+// spmv (+ reliable update) -> alpha -> update (+ forward level 0) -> forward
+// levels >= 1 -> backward levels >= 1 -> stop test / update decision + beta
+// -> p update
 template <typename T, class V>
 void launch_iteration(const BArgs<T> &a, dfm2h_batch B, hipStream_t s) {
   const unsigned na = B->aplan.nslices;
   const dim3 ga(grid_of<V>(na)), bs(V::WPB * kWave);
   hipLaunchKernelGGL((kb_spmv<T, V>), ga, bs, 0, s, a, na);
-  launch_scalar<kBAlpha>(a, s);
+  if (sizeof(T) == 4)
+    launch_scalar<kBAlphaMixed>(a, s);
+  else
+    launch_scalar<kBAlpha>(a, s);
   hipLaunchKernelGGL((kb_update<T, V>), ga, bs, 0, s, a, na);
-  if constexpr (sizeof(T) == 4) {
-    launch_scalar<kBConvMixed>(a, s);
-    hipLaunchKernelGGL((kb_fold<V>), ga, bs, 0, s, a, na);
-    hipLaunchKernelGGL((kb_residual<V>), ga, bs, 0, s, a, na);
-    launch_scalar<kBReliable>(a, s);
-  } else {
-    launch_scalar<kBConv>(a, s);
-  }
-  launch_sweeps<T, V>(a, B, s);
-  launch_scalar<kBBeta>(a, s);
+  launch_sweeps<T, V>(a, B, s, 1);
+  if (sizeof(T) == 4)
+    launch_scalar<kBBetaMixed>(a, s);
+  else
+    launch_scalar<kBConvBeta>(a, s);
   hipLaunchKernelGGL((kb_pupdate<T, V>), ga, bs, 0, s, a, na, 1);
 }
 
@@ -657,12 +779,25 @@ int batch_bucket(dfm2h_batch B) { return dof_bucket(3 * (size_t)B->nblk); }
 
 // values packed for the batch plans, from the current A and P
 template <typename T>
-void ensure_batch_vals(dfm2h_batch B, PackedVals<T> &av, PackedVals<T> &fv, PackedVals<T> &bv) {
+void ensure_batch_vals(dfm2h_batch B, PackedVals<T> &av, PackedVals<T> &fv, PackedVals<T> &bv,
+                       PackedVals<T> &dv) {
   dfm2h_bsr A = B->A;
   dfm2h_ilu P = B->P;
   ensure_packed(B->aplan, av, A->vcrs.p, A->vdia.p, A->version, B->s);
   ensure_packed(B->fplan, fv, P->vcrs.p, P->vdia.p, P->version, B->s);
   ensure_packed(B->bplan, bv, P->vcrs.p, nullptr, P->version, B->s);
+  // D^-1 of the forward-level-0 rows in the matrix-slice layout
+  if (dv.version != P->version || !dv.dia.p) {
+    const unsigned nsl = B->aplan.nslices;
+    dv.dia.alloc(std::max<size_t>((size_t)nsl * kSliceVals, 1));
+    if (nsl) {
+      const size_t n = (size_t)nsl * kSliceVals;
+      hipLaunchKernelGGL(k_pack_dia<T>, dim3(ceil_div(n, 256)), dim3(256), 0, B->s, nsl,
+                         B->row_l0.p, P->vdia.p, dv.dia.p);
+      check_launch("k_pack_dia");
+    }
+    dv.version = P->version;
+  }
 }
 
 template <typename T>
@@ -697,7 +832,7 @@ void run_batch(dfm2h_batch B, BatchBufs<T> &S, const BArgs<T> &a, int v, double 
                                 (uintptr_t)a.a_dia,  (uintptr_t)a.f_val,  (uintptr_t)a.f_dia,
                                 (uintptr_t)a.b_val,  (uintptr_t)a.a64_val, (uintptr_t)a.a64_dia,
                                 (uintptr_t)a.r,      (uintptr_t)a.x64,    (uintptr_t)a.b64,
-                                (uintptr_t)a.partA,  (uintptr_t)a.M};
+                                (uintptr_t)a.partA,  (uintptr_t)a.M,       (uintptr_t)a.dinv0};
   if (!S.exec || S.sig != sig) {
     S.drop_graph();
     DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -730,16 +865,16 @@ BArgs<T> prepare(dfm2h_batch B, BatchBufs<T> &S);
 template <>
 BArgs<double> prepare<double>(dfm2h_batch B, BatchBufs<double> &S) {
   ensure_batch_bufs(B, S);
-  ensure_batch_vals(B, B->a64, B->f64, B->b64v);
-  return make_bargs<double>(B, S, B->a64, B->f64, B->b64v);
+  ensure_batch_vals(B, B->a64, B->f64, B->b64v, B->d64);
+  return make_bargs<double>(B, S, B->a64, B->f64, B->b64v, B->d64);
 }
 
 template <>
 BArgs<float> prepare<float>(dfm2h_batch B, BatchBufs<float> &S) {
   ensure_batch_bufs(B, S);
-  ensure_batch_vals(B, B->a32, B->f32, B->b32);
+  ensure_batch_vals(B, B->a32, B->f32, B->b32, B->d32);
   ensure_packed(B->aplan, B->a64, B->A->vcrs.p, B->A->vdia.p, B->A->version, B->s);
-  return make_bargs<float>(B, S, B->a32, B->f32, B->b32);
+  return make_bargs<float>(B, S, B->a32, B->f32, B->b32, B->d32);
 }
 
 // Times a short fixed-length solve (tolerance 0, `nit` iterations per
@@ -833,6 +968,22 @@ int dfm2h_batch_create(dfm2h_batch *out, dfm2h_bsr A, dfm2h_ilu P, const unsigne
       B->aplan.build(groups, s0, s1, A->pat.rowptr);
       B->a_mesh.upload(slice_mesh(B->aplan, B->nmesh, false));
       B->a_mptr.upload(B->aplan.group_ptr);
+      // forward-level-0 rows (no lower blocks) and isolated rows per matrix slice
+      std::vector<unsigned> l0(B->aplan.nslices, 0), iso(B->aplan.nslices, 0);
+      std::vector<unsigned> rl0(std::max<size_t>((size_t)B->aplan.nslices * kSliceRows, 1), ~0u);
+      for (size_t m = 0; m < nmesh; ++m)
+        for (unsigned q = B->aplan.group_ptr[m]; q < B->aplan.group_ptr[m + 1]; ++q)
+          for (unsigned r = 0; r < (unsigned)kSliceRows; ++r) {
+            const size_t i = off[m] + (size_t)(q - B->aplan.group_ptr[m]) * kSliceRows + r;
+            if (i >= off[m + 1] || P->diaind[i] != P->colind[i]) continue;
+            l0[q] |= 1u << r;
+            rl0[(size_t)q * kSliceRows + r] = (unsigned)i;
+            if (P->diaind[i] == P->colind[i + 1]) iso[q] |= 1u << r;
+          }
+      B->any_iso = std::any_of(iso.begin(), iso.end(), [](unsigned v) { return v != 0; });
+      B->l0mask.upload(l0);
+      B->isomask.upload(iso);
+      B->row_l0.upload(rl0);
     }
     // sweep plans: one group per (level, mesh), level-major
     {
@@ -843,8 +994,25 @@ int dfm2h_batch_create(dfm2h_batch *out, dfm2h_bsr A, dfm2h_ilu P, const unsigne
         s0[i] = P->colind[i];
         s1[i] = P->diaind[i];
       }
-      B->fplan.build(level_mesh_groups(levptr, rows, off), s0, s1, P->rowptr);
+      const auto fgroups = level_mesh_groups(levptr, rows, off);
+      B->fplan.build(fgroups, s0, s1, P->rowptr);
       B->f_mesh.upload(slice_mesh(B->fplan, B->nmesh, true));
+      B->f_gptr.upload(B->fplan.group_ptr);
+      // per forward slice: rows with no upper blocks; per level: any such row
+      std::vector<unsigned> um;
+      B->flev_u0_host.assign(B->nlev_f, 0);
+      for (size_t g = 0; g < fgroups.size(); ++g)
+        for (size_t b = 0; b < fgroups[g].size(); b += kSliceRows) {
+          unsigned msk = 0;
+          for (size_t k = b; k < std::min(fgroups[g].size(), b + kSliceRows); ++k) {
+            const unsigned i = fgroups[g][k];
+            if (P->diaind[i] == P->colind[i + 1]) msk |= 1u << (k - b);
+          }
+          um.push_back(msk);
+          if (msk) B->flev_u0_host[g / nmesh] = 1;
+        }
+      B->fumask.upload(um);
+      B->flev_u0.upload(B->flev_u0_host);
       build_levels(nblk, P->colind, P->rowptr, P->diaind, false, levptr, rows);
       B->nlev_b = (unsigned)levptr.size() - 1;
       for (unsigned i = 0; i < nblk; ++i) {
@@ -861,6 +1029,9 @@ int dfm2h_batch_create(dfm2h_batch *out, dfm2h_bsr A, dfm2h_ilu P, const unsigne
     B->partA.alloc(std::max<size_t>(B->aplan.nslices, 1));
     B->partA2.alloc(std::max<size_t>(B->aplan.nslices, 1));
     B->partB.alloc(std::max<size_t>(B->bplan.nslices, 1));
+    B->partF.alloc(std::max<size_t>(B->fplan.nslices, 1));
+    B->partI.alloc(std::max<size_t>(B->aplan.nslices, 1));
+    B->partE.alloc(std::max<size_t>(B->aplan.nslices, 1));
     B->M.alloc(nmesh);
     B->G.alloc(1);
     DFM2H_HIP(hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking));
@@ -956,8 +1127,11 @@ int dfm2h_batch_solve(dfm2h_batch B, const double *b, double *x, double tol, int
       if (v < 0) v = autotune<float>(B, B->s32, nullptr, 0);
       const BArgs<float> a = prepare<float>(B, B->s32);
       run_batch<float>(B, B->s32, a, v, tol, delta, max_nitr);
-      if (n)
-        DFM2H_HIP(hipMemcpyAsync(x, B->x64.p, n * sizeof(double), hipMemcpyDeviceToDevice, B->s));
+      if (n) {
+        hipLaunchKernelGGL(kb_mixed_out, dim3(std::min<size_t>(2048, ceil_div(n, 256))),
+                           dim3(256), 0, B->s, x, B->x64.p, B->s32.x.p, n);
+        check_launch("kb_mixed_out");
+      }
     } else {
       if (v < 0) v = autotune<double>(B, B->s64, nullptr, 0);
       const BArgs<double> a = prepare<double>(B, B->s64);

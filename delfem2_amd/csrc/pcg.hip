@@ -175,6 +175,7 @@ struct PcgArgs {
   double *partials3;  // (r,Pr) partials of the fused iteration
   double *partials4;  // mixed: |r_true - r_recursive|^2 partials of a reliable update
   unsigned upd_part_off;
+  unsigned xcd;  // workgroups of the slice kernels numbered per XCD (wg_id)
   PcgScal *S;
   // mixed precision: FP64 solution part (FP32 vectors), right-hand side,
   // matrix and outer history
@@ -380,7 +381,7 @@ template <typename T, typename M>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
   if (a.S->done) return;
   const bool refine = is_mixed<T, M>() && a.S->upd;  // grid-uniform
-  const SliceLane L = slice_lane(0);
+  const SliceLane L = slice_lane(0, a.xcd);
   const bool slice_ok = L.s < a.a_nsl && L.c < 3;
   const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
   double part = 0.0, part2 = 0.0, part4 = 0.0;
@@ -402,10 +403,11 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
     part4 = block_reduce_sum<kPackBlock>(part4);
   }
   if (threadIdx.x == 0) {
-    a.partials[blockIdx.x] = part;
+    const unsigned w = wg_id(a.xcd);
+    a.partials[w] = part;
     if (refine) {
-      a.partials2[blockIdx.x] = part2;
-      a.partials4[blockIdx.x] = part4;
+      a.partials2[w] = part2;
+      a.partials4[w] = part4;
     }
   }
 }
@@ -437,7 +439,7 @@ template <typename T, typename M>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   if (a.S->done) return;
   const double alpha = a.S->alpha;
-  const SliceLane L = slice_lane(0);
+  const SliceLane L = slice_lane(0, a.xcd);
   const bool sl = L.s < a.a_nsl;
   const unsigned i = sl && L.c < 3 ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
   const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && a.S->upd;  // an update just ran
@@ -460,8 +462,9 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   const bool iso = a.upd_part_off != ~0u;  // rows with neither lower nor upper blocks exist
   if (iso) srp = block_reduce_sum<kPackBlock>(srp);
   if (threadIdx.x == 0) {
-    a.partials2[blockIdx.x] = s2;
-    if (iso) a.partials3[a.upd_part_off + blockIdx.x] = srp;
+    const unsigned w = wg_id(a.xcd);
+    a.partials2[w] = s2;
+    if (iso) a.partials3[a.upd_part_off + w] = srp;
   }
 }
 
@@ -471,14 +474,15 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
 // per load round.  With SPLIT = 1 the per-component sums run in stored
 // block order, exactly as the reference's loops.
 template <int SPLIT>
-__device__ inline SliceLane sweep_lane(unsigned slice0, unsigned &half, unsigned &slot) {
+__device__ inline SliceLane sweep_lane(unsigned slice0, bool xcd, unsigned &half,
+                                       unsigned &slot) {
   constexpr unsigned SPB = kPackBlock / kWave / SPLIT;  // slices per workgroup
   const unsigned w = threadIdx.x / kWave;
   half = SPLIT == 1 ? 0u : w / SPB;
   slot = w % SPB;
   SliceLane L;
   L.lane = threadIdx.x & (kWave - 1);
-  L.s = slice0 + blockIdx.x * SPB + slot;
+  L.s = slice0 + wg_id(xcd) * SPB + slot;
   L.c = L.lane / kSliceRows;
   L.r = L.lane - L.c * kSliceRows;
   return L;
@@ -538,7 +542,7 @@ __global__ void __launch_bounds__(kPackBlock)
 k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
   unsigned half, slot;
-  const SliceLane L = sweep_lane<SPLIT>(slice0, half, slot);
+  const SliceLane L = sweep_lane<SPLIT>(slice0, a.xcd, half, slot);
   const bool sl = L.s < slice0 + nsl;
   if (SPLIT == 1 && !PART && !sl) return;  // whole wavefront
   const unsigned i = sl && L.c < 3 ? a.f_plan.row[L.s * kSliceRows + L.r] : ~0u;
@@ -564,7 +568,7 @@ k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
   }
   if (PART) {
     part = block_reduce_sum<kPackBlock>(part);
-    if (threadIdx.x == 0) dst[blockIdx.x] = part;
+    if (threadIdx.x == 0) dst[wg_id(a.xcd)] = part;
   }
 }
 
@@ -575,7 +579,7 @@ __global__ void __launch_bounds__(kPackBlock)
 k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
   if (a.S->done) return;
   unsigned half, slot;
-  const SliceLane L = sweep_lane<SPLIT>(slice0, half, slot);
+  const SliceLane L = sweep_lane<SPLIT>(slice0, a.xcd, half, slot);
   const bool sl = L.s < slice0 + nsl;
   const unsigned i = sl && L.c < 3 ? a.b_plan.row[L.s * kSliceRows + L.r] : ~0u;
   T t = 0;
@@ -592,7 +596,7 @@ k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
     part = (double)a.r[kVS * (size_t)i + L.c] * (double)t;
   }
   part = block_reduce_sum<kPackBlock>(part);
-  if (threadIdx.x == 0) dst[blockIdx.x] = part;
+  if (threadIdx.x == 0) dst[wg_id(a.xcd)] = part;
 }
 
 // p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue)
@@ -679,13 +683,19 @@ static unsigned split_grid(unsigned nsl, int split) {
   return std::max(1u, ceil_div(nsl, kPackBlock / kWave / split));
 }
 
-// Sweep shapes of the fused iteration: {wavefronts per slice, blocks per
-// load round}.  Shape 0 keeps the reference's summation order.
-constexpr int kNumSweepShapes = 6;
-constexpr int kSweepSplit[kNumSweepShapes] = {1, 1, 2, 2, 1, 2};
-constexpr int kSweepU[kNumSweepShapes] = {8, 4, 8, 4, 16, 16};
-const char *const kSweepNames[kNumSweepShapes] = {"split1_u8", "split1_u4", "split2_u8",
-                                                  "split2_u4", "split1_u16", "split2_u16"};
+// Launch shapes of the fused iteration: sweep shape {wavefronts per slice,
+// blocks per load round} (shapes 0..5) and, for shapes 6..11, the same with
+// the slice kernels' workgroups numbered per XCD.  Shape 0 keeps the
+// reference's summation order; the XCD numbering changes no sum.
+constexpr int kNumSweepBase = 6;
+constexpr int kNumSweepShapes = 2 * kNumSweepBase;
+constexpr int kSweepSplit[kNumSweepShapes] = {1, 1, 2, 2, 1, 2, 1, 1, 2, 2, 1, 2};
+constexpr int kSweepU[kNumSweepShapes] = {8, 4, 8, 4, 16, 16, 8, 4, 8, 4, 16, 16};
+const char *const kSweepNames[kNumSweepShapes] = {
+    "split1_u8",     "split1_u4",     "split2_u8",     "split2_u4",
+    "split1_u16",    "split2_u16",    "split1_u8_xcd", "split1_u4_xcd",
+    "split2_u8_xcd", "split2_u4_xcd", "split1_u16_xcd", "split2_u16_xcd"};
+inline unsigned shape_xcd(int shape) { return shape >= kNumSweepBase ? 1u : 0u; }
 
 struct PcgWork {
   hipStream_t s = nullptr;
@@ -828,6 +838,7 @@ static PcgArgs<T, M> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Pa
   a.partials3 = B.partials3.p;
   a.partials4 = B.partials4.p;
   a.upd_part_off = 0;  // set per sweep shape
+  a.xcd = 0;           // set per launch shape
   a.hist = B.hist.p;
   a.S = B.S.p;
   return a;
@@ -914,7 +925,7 @@ static void launch_sweeps_shape(const PcgArgs<T, M> &a, dfm2h_ilu P, const Sweep
 template <typename T, typename M>
 static void launch_sweeps_fused(const PcgArgs<T, M> &a, dfm2h_ilu P, int shape,
                                 const SweepLayout &Lo, hipStream_t s, int which = 3) {
-  switch (shape) {
+  switch (shape % kNumSweepBase) {
     case 0: launch_sweeps_shape<T, M, 1, 8>(a, P, Lo, s, which); break;
     case 1: launch_sweeps_shape<T, M, 1, 4>(a, P, Lo, s, which); break;
     case 2: launch_sweeps_shape<T, M, 2, 8>(a, P, Lo, s, which); break;
@@ -1005,11 +1016,12 @@ static std::vector<uintptr_t> graph_sig(const PcgArgs<T, M> &a, dfm2h_bsr A, dfm
           (uintptr_t)P->work->shape[solver_kind<T, M>()]};
 }
 
-// Autotuner of the sweep shape: times kRep forward + backward sweep pairs
-// of every shape on the current vectors and returns the fastest (FP64: of
-// the one-wavefront-per-slice shapes, whose sums run in the reference's
-// order).  The sweeps only rewrite Pr, which the prologue recomputes
-// afterwards.
+// Autotuner of the launch shape: times kRep rounds of one SpMV plus one
+// forward + backward sweep pair for every shape on the current vectors and
+// returns the fastest (FP64: of the one-wavefront-per-slice shapes, whose
+// sums run in the reference's order).  The SpMV reads r in place of p (p is
+// not set yet) and writes Pr and the partials; the sweeps only rewrite Pr;
+// the prologue recomputes both afterwards.
 template <typename T, typename M>
 static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T, M> &a0,
                        std::vector<double> &ms) {
@@ -1018,6 +1030,7 @@ static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T, M>
   DFM2H_HIP(hipEventCreate(&e0));
   DFM2H_HIP(hipEventCreate(&e1));
   ms.assign(kNumSweepShapes, 0.0);
+  const unsigned gs = slice_grid(a0.a_nsl);
   int best = 0;
   for (int v = 0; v < kNumSweepShapes; ++v) {
     // FP64 keeps the reference's summation order (one wavefront per slice)
@@ -1025,9 +1038,15 @@ static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T, M>
     const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[v]);
     PcgArgs<T, M> a = a0;
     a.upd_part_off = Lo.upd_off;
-    launch_sweeps_fused(a, P, v, Lo, W.s);  // warm-up
+    a.xcd = shape_xcd(v);
+    a.p = a0.r;
+    auto round = [&] {
+      hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(gs), dim3(kPackBlock), 0, W.s, a);
+      launch_sweeps_fused(a, P, v, Lo, W.s);
+    };
+    round();  // warm-up
     DFM2H_HIP(hipEventRecord(e0, W.s));
-    for (int k = 0; k < kRep; ++k) launch_sweeps_fused(a, P, v, Lo, W.s);
+    for (int k = 0; k < kRep; ++k) round();
     DFM2H_HIP(hipEventRecord(e1, W.s));
     DFM2H_HIP(hipEventSynchronize(e1));
     float t = 0.f;
@@ -1036,7 +1055,7 @@ static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T, M>
     // another shape must beat the default (shape 0) by 3% to replace it
     if (ms[v] < (best == 0 ? 0.97 : 1.0) * ms[best]) best = v;
   }
-  check_launch("sweep autotuner");
+  check_launch("launch-shape autotuner");
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return best;
@@ -1081,6 +1100,7 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
   PcgArgs<T, M> a = a0;
   a.upd_part_off = Lo.upd_off;
+  a.xcd = shape_xcd(shape);
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
@@ -1131,6 +1151,7 @@ static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
   PcgArgs<T, float> a = a0;
   a.upd_part_off = Lo.upd_off;
+  a.xcd = shape_xcd(shape);
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
@@ -1378,6 +1399,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
       auto a = a0;
       a.upd_part_off = Lo.upd_off;
+      a.xcd = shape_xcd(shape);
       PcgScal st{};
       DFM2H_HIP(hipMemcpyAsync(&st, B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, W.s));
       DFM2H_HIP(hipStreamSynchronize(W.s));

@@ -227,20 +227,22 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
  * vectors, period 50 (measured on MI355X: with delta = 0.1 it keeps the
  * inner iteration count of the 64^3 cube at the FP64 solver's). */
 int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_period);
-/* Launch shape of the fused solvers' triangular sweeps (no reference
- * counterpart): shape s = {wavefronts per 21-row slice, blocks per load
- * round} from { {1,8}, {1,4}, {2,8}, {2,4}, {1,16}, {2,16} }; with two
+/* Launch shape of the fused solvers (no reference counterpart): shapes
+ * 0..5 set the triangular sweeps' {wavefronts per 21-row slice, blocks per
+ * load round} from { {1,8}, {1,4}, {2,8}, {2,4}, {1,16}, {2,16} }; with two
  * wavefronts per slice each takes every other block of a row and the two
  * sums are added (a different rounding order from the reference's loops;
- * shapes 0, 1, 4 keep it).  -1 (the default): the first solve of each
- * precision times every shape (FP64: shapes 0, 1, 4 only) on its own
- * sweeps and keeps the fastest, per
- * (floor(log2 ndof), solver kind) for the process (and in the file named
- * by DFM2H_TUNE_CACHE, if set).  fp32 = 0 names the FP64 solver, else the
- * mixed solver with its current vector precision (each has its own shape).
- * get_shape: current shape, its name, and the autotuner's ms per
- * forward+backward sweep pair of every shape (0 where not measured);
- * n_shapes = 6. */
+ * shapes 0, 1, 4 keep it).  Shapes 6..11 are shapes 0..5 with the slice
+ * kernels' workgroups numbered so that each XCD (own L2) runs one contiguous
+ * run of rows; that changes no sum, so shape s + 6 gives the bits of shape
+ * s.  -1 (the default): the first solve of each precision times one SpMV
+ * plus one sweep pair for every shape (FP64: shapes 0, 1, 4, 6, 7, 10 only)
+ * and keeps the fastest, per (floor(log2 ndof), solver kind) for the
+ * process (and in the file named by DFM2H_TUNE_CACHE, if set).  fp32 = 0
+ * names the FP64 solver, else the mixed solver with its current vector
+ * precision (each has its own shape).  get_shape: current shape, its name,
+ * and the autotuner's ms per round of every shape (0 where not measured);
+ * n_shapes = 12. */
 int dfm2h_pcg_set_shape(dfm2h_ilu P, int fp32, int shape);
 int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name,
                         double *tune_ms, size_t cap, size_t *n_shapes);

@@ -265,9 +265,11 @@ def test_assembly_deterministic_and_c2_properties(gpu):
 
 
 def test_sweep_shapes_and_autotuner(gpu):
-    """Every sweep shape of the fused solvers gives the oracle's solution
-    (FP64 shapes 0/1/4 keep the summation order: same iterations as the
-    default); the autotuner fills its timings and picks an allowed shape."""
+    """Every launch shape of the fused solvers gives the oracle's solution
+    (FP64 sweep shapes 0/1/4 keep the summation order: same iterations as the
+    default); shapes 6..11 (workgroups numbered per XCD) give exactly the
+    bits of shapes 0..5; the autotuner fills its timings and picks an
+    allowed shape."""
     prob = W.cube(12, sigma=0.5, seed=11, ordering="color4")
     C = cpu_system(prob)
     C.factor()
@@ -282,23 +284,30 @@ def test_sweep_shapes_and_autotuner(gpu):
     xm0, _, _ = G.solve_mixed(1e-8)
     for fp32 in (False, True):
         v, name, ms = G.ilu.sweep_shape(fp32)
-        assert ms.size == 6 and 0 <= v < 6, (v, name)
+        assert ms.size == 12 and 0 <= v < 12, (v, name)
+        assert ms[v] > 0
         if not fp32:
-            assert v in (0, 1, 4)
+            assert v % 6 in (0, 1, 4)
     ref64 = None
-    for shape in range(6):
+    base = {}
+    for shape in range(12):
         for fp32 in (False, True):
             G.ilu.set_sweep_shape(fp32, shape)
         x, hist, _ = G.solve(1e-8, 5000)
         assert abs(len(hist) - len(hc)) <= 1
         assert np.linalg.norm(x.cpu().numpy() - xc) / np.linalg.norm(xc) <= FP64_X_TOL
-        if shape in (0, 1, 4):  # same summation order: same bits
+        if shape % 6 in (0, 1, 4):  # same summation order: same bits
             if ref64 is None:
                 ref64 = x.clone()
             assert torch.equal(x, ref64)
         xm, hm, _ = G.solve_mixed(1e-8)
         assert hm[-1] / hm[0] < 1e-8
         assert np.linalg.norm(xm.cpu().numpy() - xe) / np.linalg.norm(xe) <= MIXED_X_TOL
+        if shape < 6:
+            base[shape] = (x.clone(), xm.clone())
+        else:
+            assert torch.equal(x, base[shape - 6][0]), shape
+            assert torch.equal(xm, base[shape - 6][1]), shape
     assert torch.equal(x0, ref64)
     for fp32 in (False, True):
         G.ilu.set_sweep_shape(fp32, -1)

@@ -285,9 +285,9 @@ kb_fwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
     T t = 0;
     if (valid) {
       t = a.r[kVSB * (size_t)i + L.c];
+      const unsigned K = a.F.len[s * kSliceRows + L.r];
       t = sliced_row_sum_m<T, T, true, V::UW, kVSB, sweep_nt<T>()>(
-          t, (T)1, L, a.F.len[s * kSliceRows + L.r], a.F.kmax[s], a.F.off[s], a.F.col, a.f_val,
-          a.Pr);
+          t, (T)1, L, K, K, a.F.kmax[s], a.F.off[s], a.F.col, a.f_val, a.Pr);
     }
     const T t0 = __shfl(t, (int)L.r, kWave);
     const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
@@ -321,9 +321,9 @@ kb_bwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
     if (i != ~0u) {
       const size_t e = kVSB * (size_t)i + L.c;
       T t = a.Pr[e];
+      const unsigned K = a.B.len[s * kSliceRows + L.r];
       t = sliced_row_sum_m<T, T, true, V::UW, kVSB, sweep_nt<T>()>(
-          t, (T)1, L, a.B.len[s * kSliceRows + L.r], a.B.kmax[s], a.B.off[s], a.B.col, a.b_val,
-          a.Pr);
+          t, (T)1, L, K, K, a.B.kmax[s], a.B.off[s], a.B.col, a.b_val, a.Pr);
       a.Pr[e] = t;
       part = (double)a.r[e] * (double)t;
     }
@@ -385,10 +385,20 @@ __device__ inline void mark_done(MeshScal *S, BatchGlobal *G) {
   atomicAdd(&G->ndone, 1u);
 }
 
-// lane-strided sum of part[q] over [q0, q1)
+// lane-strided sum of part[q] over [q0, q1), in increasing q, with the
+// loads of U strides issued before their adds (each load round is a full
+// memory latency: the partials were just written by other XCDs)
 __device__ inline double lane_sum(const double *part, unsigned q0, unsigned q1, unsigned lane) {
+  constexpr unsigned U = 4;
   double t = 0.0;
-  for (unsigned q = q0 + lane; q < q1; q += kWave) t += part[q];
+  for (unsigned q = q0 + lane; q < q1; q += U * kWave) {
+    double v[U];
+#pragma unroll
+    for (unsigned u = 0; u < U; ++u) v[u] = q + u * kWave < q1 ? part[q + u * kWave] : 0.0;
+#pragma unroll
+    for (unsigned u = 0; u < U; ++u)
+      if (q + u * kWave < q1) t += v[u];
+  }
   return t;
 }
 

@@ -225,7 +225,7 @@ int dfm2h_bsr_set_pattern(dfm2h_bsr A, const unsigned *colind, size_t ncolind,
         s0[i] = colind[i];
         s1[i] = colind[i + 1];
       }
-      A->plan.build({all}, s0, s1, P.rowptr);
+      A->plan.build({all}, s0, s1, P.rowptr, true);
     }
     A->has_pattern = true;
     A->pattern_serial++;

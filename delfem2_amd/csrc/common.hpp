@@ -157,6 +157,57 @@ __device__ inline double reduce_partials(const double *partials, unsigned n) {
   return block_reduce_sum<BS>(s);
 }
 
+// The same sums for NS partial sets at once (set k: p[k][0..n[k]), n[k] = 0
+// for an unused set), each formed in exactly reduce_partials' order, with
+// the loads of U strides of every set issued before their adds: the
+// partials were just written by other XCDs, so every load round is a full
+// memory latency, and one round replaces up to U x NS of them.
+template <int BS, int NS>
+__device__ inline void reduce_partial_sets(const double *const (&p)[NS], const unsigned (&n)[NS],
+                                           double (&t)[NS]) {
+  constexpr int U = 8;
+  constexpr int NW = BS / kWave;
+  double s[NS];
+  unsigned nmax = 0;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    s[k] = 0.0;
+    nmax = n[k] > nmax ? n[k] : nmax;
+  }
+  for (unsigned base = 0; base < nmax; base += BS * U) {
+    double v[NS][U];
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const unsigned i = base + u * BS + threadIdx.x;
+        v[k][u] = i < n[k] ? p[k][i] : 0.0;
+      }
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (base + u * BS + threadIdx.x < n[k]) s[k] += v[k][u];
+  }
+  __shared__ double sm[NS][NW];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const double w = wave_reduce_sum(s[k]);
+    if (lane == 0) sm[k][wid] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    double a = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) a += sm[k][w];
+    t[k] = a;
+  }
+  __syncthreads();
+}
+
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace dfm2h

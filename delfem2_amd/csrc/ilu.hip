@@ -197,7 +197,7 @@ static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
       s0[i] = P->colind[i];
       s1[i] = P->diaind[i];
     }
-    P->fplan.build(groups, s0, s1, P->rowptr);
+    P->fplan.build(groups, s0, s1, P->rowptr, true);
     {
       std::vector<unsigned> um;
       P->flev_u0.assign(groups.size(), 0);
@@ -221,7 +221,7 @@ static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
       s0[i] = P->diaind[i];
       s1[i] = P->colind[i + 1];
     }
-    P->bplan.build(groups, s0, s1, P->rowptr);
+    P->bplan.build(groups, s0, s1, P->rowptr, true);
   }
   const std::vector<int> a2l = build_a2l(*P, pa);
   P->a2l_d.upload(a2l);

@@ -39,6 +39,17 @@ constexpr int kSliceVals = 9 * kSliceRows;  // 189
 constexpr int kPackBlock = 256;              // 4 wavefronts = 4 slices per workgroup
 constexpr int kSlicesPerBlock = kPackBlock / kWave;
 
+// A group whose slices all hold the same number of block-slots K (the
+// group's longest row; shorter rows padded with zero blocks of column 0) and
+// whose rows are consecutive: a kernel computes a slice's first block-slot
+// and its rows from these instead of loading them, which takes one memory
+// round trip off the chain of dependent loads of every row.
+struct GroupView {
+  unsigned uni;    // 1: the group has this shape (else use the tables)
+  unsigned q0, K;  // first block-slot of the group's first slice, slots per slice
+  unsigned r0, nrows;
+};
+
 // Host-built index structure of a plan; values are packed per precision.
 struct PackedPlan {
   unsigned nslices = 0;
@@ -48,20 +59,49 @@ struct PackedPlan {
   DBuf<unsigned> row, len;                // per slice row (row = ~0u for padding)
   DBuf<unsigned> col;                     // per block-slot x slice row
   DBuf<int> src;                          // CRS slot feeding (q, r), -1 for padding
+  std::vector<GroupView> gview;           // per group
   bool built = false;
 
-  // groups: list of row lists; ranges: CRS slot range [s0, s1) of each row's blocks
+  // groups: list of row lists; ranges: CRS slot range [s0, s1) of each row's
+  // blocks.  uniform: give a group of consecutive rows the GroupView shape
+  // when padding every slice to the longest row adds at most 10% slots.
   void build(const std::vector<std::vector<unsigned>> &groups, const std::vector<unsigned> &s0,
-             const std::vector<unsigned> &s1, const std::vector<unsigned> &crs_col) {
+             const std::vector<unsigned> &s1, const std::vector<unsigned> &crs_col,
+             bool uniform = false) {
     std::vector<unsigned> h_off, h_kmax, h_row, h_len, h_col;
     std::vector<int> h_src;
     group_ptr.assign(1, 0);
+    gview.clear();
     size_t q = 0;
     for (const auto &g : groups) {
+      GroupView gv{0u, (unsigned)q, 0u, 0u, (unsigned)g.size()};
+      if (uniform && !g.empty()) {
+        bool consecutive = true;
+        unsigned K = 0;
+        size_t tight = 0;
+        for (size_t k = 0; k < g.size(); ++k) {
+          consecutive = consecutive && g[k] == g[0] + k;
+          K = std::max(K, s1[g[k]] - s0[g[k]]);
+        }
+        for (size_t b = 0; b < g.size(); b += kSliceRows) {
+          unsigned km = 0;
+          for (size_t k = b; k < std::min(g.size(), b + kSliceRows); ++k)
+            km = std::max(km, s1[g[k]] - s0[g[k]]);
+          tight += km;
+        }
+        const size_t nsl = (g.size() + kSliceRows - 1) / kSliceRows;
+        if (consecutive && (double)K * nsl <= 1.1 * (double)tight) {
+          gv.uni = 1;
+          gv.K = K;
+          gv.r0 = g[0];
+        }
+      }
+      gview.push_back(gv);
       for (size_t b = 0; b < g.size(); b += kSliceRows) {
         const size_t e = std::min(g.size(), b + kSliceRows);
         unsigned km = 0;
         for (size_t k = b; k < e; ++k) km = std::max(km, s1[g[k]] - s0[g[k]]);
+        if (gv.uni) km = gv.K;
         h_off.push_back((unsigned)q);
         h_kmax.push_back(km);
         for (int rr = 0; rr < kSliceRows; ++rr) {
@@ -223,9 +263,12 @@ __device__ inline Trip<T> load_node(const T *x, size_t j) {
 // U blocks per round: their column indices and values are loaded first (one
 // memory round trip), then the U gathers of x (a second one), then the U
 // updates in order.  Values are stored in precision M and widened to the
-// vector precision T before the products (M = T: no conversion).
+// vector precision T before the products (M = T: no conversion).  The row
+// has K blocks; slots below KL >= K are loaded (KL = K, or the slice's slot
+// count when padding slots hold zero blocks of column 0: then no load waits
+// for K).
 template <typename T, typename M, bool SUB, int U, int VS, bool NT = false>
-__device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigned K,
+__device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigned K, unsigned KL,
                                      unsigned kmax, unsigned q0, const unsigned *__restrict__ col,
                                      const M *__restrict__ val, const T *x) {
   for (unsigned k0 = 0; k0 < kmax; k0 += U) {
@@ -234,7 +277,7 @@ __device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigne
     Trip<T> xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (k0 + u < K) {
+      if (k0 + u < KL) {
         const size_t q = q0 + k0 + u;
         j[u] = col[q * kSliceRows + L.r];
         v[u] = load_trip<M, NT>(val, q, L.lane);
@@ -242,7 +285,7 @@ __device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigne
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (k0 + u < K) xv[u] = load_node<T, VS>(x, j[u]);
+      if (k0 + u < KL) xv[u] = load_node<T, VS>(x, j[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (k0 + u < K) {
@@ -261,7 +304,7 @@ template <typename T, bool SUB, int U, int VS>
 __device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned K, unsigned kmax,
                                    unsigned q0, const unsigned *__restrict__ col,
                                    const T *__restrict__ val, const T *x) {
-  return sliced_row_sum_m<T, T, SUB, U, VS>(acc, alpha, L, K, kmax, q0, col, val, x);
+  return sliced_row_sum_m<T, T, SUB, U, VS>(acc, alpha, L, K, K, kmax, q0, col, val, x);
 }
 
 // this lane's row of a stored 3x3 block (precision M) times (x0, x1, x2)
@@ -283,6 +326,44 @@ struct PlanView {
 
 inline PlanView view(const PackedPlan &P) {
   return PlanView{P.off.p, P.kmax.p, P.row.p, P.len.p, P.col.p};
+}
+
+// Lane L's row in a slice of a group whose first slice is slice0 (sl: the
+// slice exists), and the slice's slot range: computed for a GroupView-shaped
+// group, loaded from the tables otherwise.  i = ~0u: no row (idle lane,
+// padding row).  kl: slots the row sum may load without waiting for len.
+struct RowSlots {
+  unsigned i, len, kl, kmax, off;
+};
+
+__device__ inline unsigned row_of(const PlanView &P, const GroupView &g, const SliceLane &L,
+                                  unsigned slice0, bool sl) {
+  if (!sl || L.c >= 3) return ~0u;
+  if (!g.uni) return P.row[L.s * kSliceRows + L.r];
+  const unsigned rr = (L.s - slice0) * kSliceRows + L.r;
+  return rr < g.nrows ? g.r0 + rr : ~0u;
+}
+
+__device__ inline RowSlots row_slots(const PlanView &P, const GroupView &g, const SliceLane &L,
+                                     unsigned slice0, bool sl) {
+  RowSlots R{~0u, 0u, 0u, 0u, 0u};
+  if (g.uni) {
+    const unsigned ls = L.s - slice0, rr = ls * kSliceRows + L.r;
+    if (sl && L.c < 3 && rr < g.nrows) {
+      R.i = g.r0 + rr;
+      R.len = P.len[L.s * kSliceRows + L.r];
+      R.kl = g.K;
+      R.kmax = g.K;
+      R.off = g.q0 + ls * g.K;
+    }
+  } else if (sl && L.c < 3) {
+    R.i = P.row[L.s * kSliceRows + L.r];
+    R.len = P.len[L.s * kSliceRows + L.r];  // 0 for a padding row
+    R.kl = R.len;
+    R.kmax = P.kmax[L.s];
+    R.off = P.off[L.s];
+  }
+  return R;
 }
 
 // blocks per load round (measured on MI355X, 64^3 cube, 4-colour order:

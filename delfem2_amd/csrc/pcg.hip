@@ -156,6 +156,7 @@ struct PcgArgs {
   size_t n;
   // matrix (all rows, sliced) and factors (forward / backward plans, sliced)
   PlanView a_plan;
+  GroupView a_gv;
   unsigned a_nsl;
   const M *a_val, *a_dia;
   PlanView f_plan;
@@ -222,12 +223,13 @@ k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
              unsigned nc, double *hist) {
   if (MODE != kScalInit && MODE != kScalInitMixed && S->done) return;
   const bool upd = S->upd != 0;  // block-uniform
-  const double t = reduce_partials<kScalBlock>(pa, na);
-  const double t2 = (MODE == kScalConvBeta || MODE == kScalBetaMixed ||
-                     (MODE == kScalAlphaMixed && upd))
-                        ? reduce_partials<kScalBlock>(pb, nb)
-                        : 0.0;
-  const double t3 = (MODE == kScalAlphaMixed && upd) ? reduce_partials<kScalBlock>(pc, nc) : 0.0;
+  const bool two = MODE == kScalConvBeta || MODE == kScalBetaMixed || (MODE == kScalAlphaMixed && upd);
+  const bool three = MODE == kScalAlphaMixed && upd;
+  const double *const ps[3] = {pa, pb, pc};
+  const unsigned ns[3] = {na, two ? nb : 0u, three ? nc : 0u};
+  double ts[3];
+  reduce_partial_sets<kScalBlock, 3>(ps, ns, ts);
+  const double t = ts[0], t2 = ts[1], t3 = ts[2];
   if (threadIdx.x != 0) return;
   if (MODE == kScalInit) {  // |r0|^2: history[0]; early return when < 1e-30
     hist[0] = sqrt(t);
@@ -382,18 +384,19 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
   if (a.S->done) return;
   const bool refine = is_mixed<T, M>() && a.S->upd;  // grid-uniform
   const SliceLane L = slice_lane(0, a.xcd);
-  const bool slice_ok = L.s < a.a_nsl && L.c < 3;
-  const unsigned i = slice_ok ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  const RowSlots R = row_slots(a.a_plan, a.a_gv, L, 0, L.s < a.a_nsl);
+  const unsigned i = R.i;
   double part = 0.0, part2 = 0.0, part4 = 0.0;
   if (i != ~0u) {
+    // the row's own node and diagonal block first: off the gathers' chain
+    const Trip<T> pi = load_node<T, kVS>(a.p, i);
+    const Trip<M> d = load_trip<M>(a.a_dia, L.s, L.lane);
     T acc = 0;
     acc = sliced_row_sum_m<T, M, false, spmv_unroll<T, M>(), kVS>(
-        acc, (T)1, L, a.a_plan.len[L.s * kSliceRows + L.r], a.a_plan.kmax[L.s],
-        a.a_plan.off[L.s], a.a_plan.col, a.a_val, a.p);
-    const Trip<T> pi = load_node<T, kVS>(a.p, i);
-    acc += dia_row_dot_m<T, M>(a.a_dia, L, pi.a, pi.b, pi.c);
+        acc, (T)1, L, R.len, R.kl, R.kmax, R.off, a.a_plan.col, a.a_val, a.p);
+    acc += (T)d.a * pi.a + (T)d.b * pi.b + (T)d.c * pi.c;  // dia_row_dot_m's products
     a.Pr[kVS * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
-    part = (double)a.p[kVS * (size_t)i + L.c] * (double)acc;
+    part = (double)(L.c == 0 ? pi.a : L.c == 1 ? pi.b : pi.c) * (double)acc;
     if constexpr (is_mixed<T, M>())
       if (refine) part2 = residual_row(a, L, i, part4);
   }
@@ -441,7 +444,7 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   const double alpha = a.S->alpha;
   const SliceLane L = slice_lane(0, a.xcd);
   const bool sl = L.s < a.a_nsl;
-  const unsigned i = sl && L.c < 3 ? a.a_plan.row[L.s * kSliceRows + L.r] : ~0u;
+  const unsigned i = row_of(a.a_plan, a.a_gv, L, 0, sl);
   const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && a.S->upd;  // an update just ran
   double s2 = 0.0, srp = 0.0;
   T rn = 0;
@@ -488,10 +491,11 @@ __device__ inline SliceLane sweep_lane(unsigned slice0, bool xcd, unsigned &half
   return L;
 }
 
-// acc -= sum over the row's blocks k = h, h + H, h + 2H, ... (H halves)
+// acc -= sum over the row's K blocks k = h, h + H, h + 2H, ... (H halves);
+// slots below KL >= K are loaded (see sliced_row_sum_m)
 template <typename T, typename M, int U, int H>
 __device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, unsigned K,
-                                        unsigned kmax, unsigned q0,
+                                        unsigned KL, unsigned kmax, unsigned q0,
                                         const unsigned *__restrict__ col,
                                         const M *__restrict__ val, const T *x) {
   for (unsigned k0 = h; k0 < kmax; k0 += H * U) {
@@ -501,7 +505,7 @@ __device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, u
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const unsigned k = k0 + H * u;
-      if (k < K) {
+      if (k < KL) {
         const size_t q = q0 + k;
         j[u] = col[q * kSliceRows + L.r];
         v[u] = load_trip<M, sweep_nt<M>()>(val, q, L.lane);
@@ -509,7 +513,7 @@ __device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, u
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (k0 + H * u < K) xv[u] = load_node<T, kVS>(x, j[u]);
+      if (k0 + H * u < KL) xv[u] = load_node<T, kVS>(x, j[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (k0 + H * u < K)
@@ -539,18 +543,23 @@ inline unsigned sweep_grid(unsigned nsl) {
 // holds rows with no upper blocks; their (r, Pr) is summed here.
 template <typename T, typename M, bool PART, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
-k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
+k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *dst) {
   if (a.S->done) return;
   unsigned half, slot;
   const SliceLane L = sweep_lane<SPLIT>(slice0, a.xcd, half, slot);
   const bool sl = L.s < slice0 + nsl;
   if (SPLIT == 1 && !PART && !sl) return;  // whole wavefront
-  const unsigned i = sl && L.c < 3 ? a.f_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  T t = 0;
+  const RowSlots R = row_slots(a.f_plan, gv, L, slice0, sl);
+  const unsigned i = R.i;
+  T t = 0, rv = 0;
+  Trip<M> d{};
   if (i != ~0u) {
-    if (half == 0) t = a.r[kVS * (size_t)i + L.c];
-    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, a.f_plan.len[L.s * kSliceRows + L.r],
-                                         a.f_plan.kmax[L.s], a.f_plan.off[L.s], a.f_plan.col,
+    if (half == 0) {  // r_i and the D_i^-1 row first: off the gathers' chain
+      rv = a.r[kVS * (size_t)i + L.c];
+      d = load_trip<M, sweep_nt<M>()>(a.f_dia, L.s, L.lane);
+      t = rv;
+    }
+    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, R.len, R.kl, R.kmax, R.off, a.f_plan.col,
                                          a.f_val, a.Pr);
   }
   t = join_halves<T, SPLIT>(t, half, slot, L.lane);
@@ -560,10 +569,9 @@ k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
     const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
     const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
     if (i != ~0u) {
-      const T out = dia_row_dot_m<T, M, sweep_nt<M>()>(a.f_dia, L, t0, t1, t2);
+      const T out = (T)d.a * t0 + (T)d.b * t1 + (T)d.c * t2;  // dia_row_dot_m's products
       a.Pr[kVS * (size_t)i + L.c] = out;
-      if (PART && ((a.fumask[L.s] >> L.r) & 1u))
-        part = (double)a.r[kVS * (size_t)i + L.c] * (double)out;
+      if (PART && ((a.fumask[L.s] >> L.r) & 1u)) part = (double)rv * (double)out;
     }
   }
   if (PART) {
@@ -576,24 +584,27 @@ k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
 // dst[workgroup]
 template <typename T, typename M, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
-k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, double *dst) {
+k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *dst) {
   if (a.S->done) return;
   unsigned half, slot;
   const SliceLane L = sweep_lane<SPLIT>(slice0, a.xcd, half, slot);
   const bool sl = L.s < slice0 + nsl;
-  const unsigned i = sl && L.c < 3 ? a.b_plan.row[L.s * kSliceRows + L.r] : ~0u;
-  T t = 0;
+  const RowSlots R = row_slots(a.b_plan, gv, L, slice0, sl);
+  const unsigned i = R.i;
+  T t = 0, rv = 0;
   if (i != ~0u) {
-    if (half == 0) t = a.Pr[kVS * (size_t)i + L.c];
-    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, a.b_plan.len[L.s * kSliceRows + L.r],
-                                         a.b_plan.kmax[L.s], a.b_plan.off[L.s], a.b_plan.col,
+    if (half == 0) {
+      t = a.Pr[kVS * (size_t)i + L.c];
+      rv = a.r[kVS * (size_t)i + L.c];
+    }
+    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, R.len, R.kl, R.kmax, R.off, a.b_plan.col,
                                          a.b_val, a.Pr);
   }
   t = join_halves<T, SPLIT>(t, half, slot, L.lane);
   double part = 0.0;
   if (half == 0 && i != ~0u) {
     a.Pr[kVS * (size_t)i + L.c] = t;
-    part = (double)a.r[kVS * (size_t)i + L.c] * (double)t;
+    part = (double)rv * (double)t;
   }
   part = block_reduce_sum<kPackBlock>(part);
   if (threadIdx.x == 0) dst[wg_id(a.xcd)] = part;
@@ -817,6 +828,7 @@ static PcgArgs<T, M> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Pa
   a.nblk = A->nblk;
   a.n = (size_t)A->nblk * kVS;
   a.a_plan = view(A->plan);
+  a.a_gv = A->plan.gview.empty() ? GroupView{0u, 0u, 0u, 0u, 0u} : A->plan.gview[0];
   a.a_nsl = A->plan.nslices;
   a.a_val = av.val.p;
   a.a_dia = av.dia.p;
@@ -886,14 +898,14 @@ static unsigned launch_sweeps_full(const PcgArgs<T, M> &a, dfm2h_ilu P, hipStrea
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
     hipLaunchKernelGGL((k_pcg_fwd<T, M, false, 1, 8>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0,
-                       s, a, P->fplan.group_ptr[l], nsl, (double *)nullptr);
+                       s, a, P->fplan.group_ptr[l], nsl, P->fplan.gview[l], (double *)nullptr);
   }
   unsigned off = 0;
   for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
     const unsigned g = slice_grid(nsl);
     hipLaunchKernelGGL((k_pcg_bwd<T, M, 1, 8>), dim3(g), dim3(kPackBlock), 0, s, a,
-                       P->bplan.group_ptr[l], nsl, a.partials + off);
+                       P->bplan.group_ptr[l], nsl, P->bplan.gview[l], a.partials + off);
     off += g;
   }
   return off;
@@ -909,15 +921,17 @@ static void launch_sweeps_shape(const PcgArgs<T, M> &a, dfm2h_ilu P, const Sweep
     const dim3 g(split_grid(nsl, SPLIT));
     if (Lo.fwd_off[l] != ~0u)
       hipLaunchKernelGGL((k_pcg_fwd<T, M, true, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
-                         P->fplan.group_ptr[l], nsl, a.partials3 + Lo.fwd_off[l]);
+                         P->fplan.group_ptr[l], nsl, P->fplan.gview[l],
+                         a.partials3 + Lo.fwd_off[l]);
     else
       hipLaunchKernelGGL((k_pcg_fwd<T, M, false, SPLIT, U>), g, dim3(kPackBlock), 0, s, a,
-                         P->fplan.group_ptr[l], nsl, (double *)nullptr);
+                         P->fplan.group_ptr[l], nsl, P->fplan.gview[l], (double *)nullptr);
   }
   for (size_t l = 1; (which & 2) && l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
     hipLaunchKernelGGL((k_pcg_bwd<T, M, SPLIT, U>), dim3(split_grid(nsl, SPLIT)), dim3(kPackBlock),
-                       0, s, a, P->bplan.group_ptr[l], nsl, a.partials3 + Lo.bwd_off[l]);
+                       0, s, a, P->bplan.group_ptr[l], nsl, P->bplan.gview[l],
+                       a.partials3 + Lo.bwd_off[l]);
   }
 }
 

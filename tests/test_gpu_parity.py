@@ -285,7 +285,6 @@ def test_sweep_shapes_and_autotuner(gpu):
     for fp32 in (False, True):
         v, name, ms = G.ilu.sweep_shape(fp32)
         assert ms.size == 12 and 0 <= v < 12, (v, name)
-        assert ms[v] > 0
         if not fp32:
             assert v % 6 in (0, 1, 4)
     ref64 = None

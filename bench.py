@@ -224,6 +224,103 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_newmark(args):
+    """Implicit Newmark-beta time stepping of the 64^3 cube (823,875 DOF, the
+    headline cube's mesh, material and z=0 clamp; gravity (0, 0, -3)): one
+    step = assembly of M + beta dt^2 K and the Newmark residual, fixed BCs,
+    ILU(0) CopyValue + Decompose, FP64 Solve_PCG to 1e-4, and the u / v / a
+    updates -- the step of 618_Fem3Tet/main.cpp:361-416 with its gamma = 1,
+    beta = 0.5, dt = 0.03 and tolerance.  Value: seconds per step (steps
+    from rest; the state evolves over the timed steps)."""
+    import numpy as np
+    import torch
+
+    from delfem2_amd import workload as W
+    from delfem2_amd.problem import DynamicSystem
+
+    dt, gamma, beta, tol = 0.03, 1.0, 0.5, 1e-4
+    print(f"[bench] building the {args.n}^3 Newmark problem", file=sys.stderr, flush=True)
+    prob = W.cube(args.n, sigma=args.sigma, seed=20261203, ordering=args.ordering)
+    prob.g = np.array([0.0, 0.0, -3.0])
+    G = DynamicSystem(prob, "newmark", dt=dt, gamma_newmark=gamma, beta_newmark=beta)
+    for _ in range(args.warmup):
+        G.step(tol, 1000)
+    torch.cuda.synchronize()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        its.append(G.step(tol, 1000))
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    # one more step, phase by phase (synchronised: for the breakdown only)
+    ph = {}
+    import delfem2_amd.solver as S
+
+    def timed(name, fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ph[name] = round((time.perf_counter() - t) * 1e3, 3)
+        return r
+
+    timed("assembly_bc_ms", G.assemble)
+    timed("ilu_copy_decompose_ms", G.factor)
+    x = torch.zeros_like(G.b)
+    hist = timed("pcg_ms", lambda: S.Solve_PCG(G.b, x, tol, 1000, G.A, G.ilu))
+    ph["pcg_iterations"] = len(hist) - 1
+
+    def upd():
+        S.XPlusAYBZCW(G.u, G.flag, dt, G.v, 0.5 * dt * dt, G.a, dt * dt * beta, x)
+        S.XPlusAYBZ(G.v, G.flag, dt * gamma, x, dt, G.a)
+        S.XPlusAY(G.a, G.flag, 1.0, x)
+    timed("update_ms", upd)
+    nb, nnzb = prob.np_, G.A.ncrs
+    # dominant phase against HBM: its algorithmic bytes over its time
+    phase_bytes = {
+        # A and b written once, coordinates / tets / Lame pairs / u, v, a read once
+        "assembly_bc_ms": (nnzb + nb) * 72 + nb * 24 + prob.ne * 32 + nb * 24 * 4,
+        # A values read, factors written (CopyValue + Decompose)
+        "ilu_copy_decompose_ms": 2 * (nnzb + nb) * 72,
+        # FP64 PCG iterations
+        "pcg_ms": pcg_iter_bytes(nb, nnzb, 8) * max(ph["pcg_iterations"], 1),
+    }
+    dom = max(phase_bytes, key=lambda k: ph[k])
+    ach = phase_bytes[dom] / (ph[dom] * 1e-3) / 1e9
+    cpu = None
+    if args.cpu_iters > 0:
+        import oracle
+        print("[bench] CPU baseline sample (one Newmark step)", file=sys.stderr, flush=True)
+        kind = "reference" if oracle.ref_available() else "port"
+        run = oracle.DynamicRun(prob, "newmark", dt=dt, gamma_newmark=gamma, beta_newmark=beta,
+                                use_ref=kind == "reference")
+        tc = time.perf_counter()
+        it_c = run.step(tol, 1000)
+        tc = time.perf_counter() - tc
+        cpu = {"value": round(tc, 3), "unit": "s/step", "cores": 1, "kind": kind,
+               "sample": f"one Newmark step from rest of the same {args.n}^3 problem "
+                         f"({it_c} PCG iterations to {tol})"}
+    out = {
+        "metric": "Newmark-beta implicit time step of the 823k-DOF tet cube (s/step)",
+        "value": round(t_step, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded voxel tet cube, lognormal E)",
+        "config": {"workload": f"Newmark-beta step, tet cube {args.n}^3 hex x6 tets, "
+                               f"{prob.ndof} DOF, E~lognormal(0,{args.sigma}), nu=0.3, rho=1, "
+                               f"g=(0,0,-3), dt={dt}, gamma={gamma}, beta={beta}, "
+                               f"ILU(0)-PCG to {tol}, {args.ordering} order",
+                   "ndof": prob.ndof},
+        "pcg_iterations_per_step": its,
+        "phases": ph,
+        "roofline": {"bound": "hbm", "phase": dom, "achieved": round(ach, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_phase": int(phase_bytes[dom])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_cube128(args):
     """128^3 hexes x 6 tets (6,440,067 DOF), E lognormal(0, 1.0), nu ~ U[0.25,
     0.35], 4-colour order: FP32 inner + FP64 reliable updates to relative
@@ -306,10 +403,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cube", choices=["cube", "cube128", "batch512"],
+    ap.add_argument("--workload", default="cube",
+                    choices=["cube", "cube128", "batch512", "newmark64"],
                     help="cube: the 823,875-DOF single solve to 1e-8 (headline); cube128: "
                          "the 6,440,067-DOF mixed-precision solve to 1e-10; batch512: 512 "
-                         "independent meshes in one batched solve")
+                         "independent meshes in one batched solve; newmark64: implicit "
+                         "Newmark-beta time steps of the 823,875-DOF cube")
     ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
     ap.add_argument("--sigma", type=float, default=0.5)
     ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
@@ -350,6 +449,9 @@ def main():
         return
     if args.workload == "cube128":
         bench_cube128(args)
+        return
+    if args.workload == "newmark64":
+        bench_newmark(args)
         return
 
     import torch

@@ -69,6 +69,12 @@ SIGNATURES = {
     "dfm2h_tetmesh_destroy": [_vp],
     "dfm2h_merge_linsys_solidlinear_static_tet": [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _vp,
                                                   _vp, _vp],
+    "dfm2h_merge_linsys_solidlinear_newmark_tet": [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _d,
+                                                   _d, _d, _vp, _vp, _vp, _vp, _vp],
+    "dfm2h_merge_linsys_solidlinear_beuler_tet": [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _d,
+                                                  _vp, _vp, _vp, _vp],
+    "dfm2h_xplusaybz": [_vp, _sz, _vp, _d, _vp, _d, _vp],
+    "dfm2h_xplusaybzcw": [_vp, _sz, _vp, _d, _vp, _d, _vp, _d, _vp],
     "dfm2h_ilu_create": [_pvp],
     "dfm2h_ilu_destroy": [_vp],
     "dfm2h_ilu_set_pattern0": [_vp, _vp],

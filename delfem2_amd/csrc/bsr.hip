@@ -17,7 +17,41 @@ struct AsmArgs {
   const uint4 *tet;
   const unsigned *elsup_ind, *elsup;
   int *missing;  // set when an element edge has no slot in the pattern
+  // time-stepping drivers: step, Newmark beta, velocity and acceleration
+  double dt, beta_nb;
+  const double *velo, *acc;
 };
+
+// Element kinds of the gather assembly:
+//   static    fem_solidlinear.h:192-237 (EMat_SolidLinear_Static_Tet, additive form)
+//   newmark   fem_solidlinear.h:331-379 with fem_solidlinear.cpp:437-490
+//             (is_initial_iter = true): M_e + beta dt^2 K_e,
+//             eres = f - K_e u - M_e a - dt K_e v - (0.5 dt^2) K_e a
+//   beuler    fem_solidlinear.h:381-443: K_e + rho vol / 4 / dt^2 on the
+//             diagonal entries of the diagonal blocks, eres = (f - K_e (u + dt v)) / dt
+enum AsmKind { kAsmStatic = 0, kAsmNewmark = 1, kAsmBEuler = 2 };
+
+__device__ inline void load_node3(double out[3], const double *v, unsigned n) {
+  out[0] = v ? v[(size_t)n * 3 + 0] : 0.0;
+  out[1] = v ? v[(size_t)n * 3 + 1] : 0.0;
+  out[2] = v ? v[(size_t)n * 3 + 2] : 0.0;
+}
+
+// R_li += scale * (E_li,jn v_jn) for one node pair (AddEmatEvecScale3<4>'s
+// inner step, femutil.h:252-268)
+__device__ inline void add_block_vec(double er[3], const double e[9], const double v[3],
+                                     double scale) {
+  er[0] += scale * (e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
+  er[1] += scale * (e[3] * v[0] + e[4] * v[1] + e[5] * v[2]);
+  er[2] += scale * (e[6] * v[0] + e[7] * v[1] + e[8] * v[2]);
+}
+
+// consistent mass block (li, lj) of SetEmatConsistentMassTet (femutil.cpp:260-280)
+__device__ inline void mass_block(double m[9], double dm, bool same) {
+  const double d = same ? dm + dm : dm;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) m[k] = (k % 4 == 0) ? d : 0.0;
+}
 
 // Gather assembly, one thread per destination block: off-diagonal slot s
 // (t < ncrs) or the diagonal block and right-hand side of row t-ncrs.  Each
@@ -25,6 +59,7 @@ struct AsmArgs {
 // and adds their contribution to its block; the additions therefore happen
 // in exactly the order of the reference's sequential element loop with
 // Merge<4,4,3,3>, and no two threads write the same address (no atomics).
+template <int KIND>
 __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool diag = t >= A.ncrs;
@@ -59,10 +94,82 @@ __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
     pick_row(di, G, li);
     pick_row(dj, G, lj);
     double blk[9];
-    emat_block(blk, di, dj, w, myu, lam);
+    if (KIND == kAsmStatic) {
+      emat_block(blk, di, dj, w, myu, lam);
+    } else if (KIND == kAsmNewmark) {  // (&emat)[k] = eM[k] + (beta dt dt) eK[k]
+      double kb[9], mb[9];
+      emat_block<true>(kb, di, dj, w, myu, lam);
+      mass_block(mb, A.rho * G.vol * 0.05, li == lj);
+      const double s = A.beta_nb * A.dt * A.dt;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) blk[k] = mb[k] + s * kb[k];
+    } else {  // backward Euler: K_e, lumped mass on the diagonal block
+      emat_block<true>(blk, di, dj, w, myu, lam);
+      if (li == lj) {
+        blk[0] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
+        blk[4] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
+        blk[8] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) acc[k] += blk[k];
-    if (diag) {
+    if (diag && KIND == kAsmNewmark) {
+      const double vol = G.vol;
+      double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
+                      vol * A.rho * A.g2 * 0.25};
+      const unsigned id[4] = {n.x, n.y, n.z, n.w};
+      double kb[9], mb[9], x[3];
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {  // - K_e u
+        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+        load_node3(x, A.disp, id[jn]);
+        add_block_vec(er, kb, x, -1.);
+      }
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {  // - M_e a
+        mass_block(mb, A.rho * vol * 0.05, jn == li);
+        load_node3(x, A.acc, id[jn]);
+        add_block_vec(er, mb, x, -1.);
+      }
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {  // - dt K_e v
+        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+        load_node3(x, A.velo, id[jn]);
+        add_block_vec(er, kb, x, -A.dt);
+      }
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {  // - (0.5 dt^2) K_e a
+        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+        load_node3(x, A.acc, id[jn]);
+        add_block_vec(er, kb, x, -0.5 * A.dt * A.dt);
+      }
+      bacc[0] += er[0];
+      bacc[1] += er[1];
+      bacc[2] += er[2];
+    }
+    if (diag && KIND == kAsmBEuler) {
+      const double vol = G.vol;
+      double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
+                      vol * A.rho * A.g2 * 0.25};
+      const unsigned id[4] = {n.x, n.y, n.z, n.w};
+      double kb[9], u[3], v[3];
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {  // - K_e (u + dt v), three subtractions per pair
+        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+        load_node3(u, A.disp, id[jn]);
+        load_node3(v, A.velo, id[jn]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          er[a] -= kb[3 * a + 0] * (u[0] + A.dt * v[0]);
+          er[a] -= kb[3 * a + 1] * (u[1] + A.dt * v[1]);
+          er[a] -= kb[3 * a + 2] * (u[2] + A.dt * v[2]);
+        }
+      }
+      bacc[0] += er[0] / A.dt;
+      bacc[1] += er[1] / A.dt;
+      bacc[2] += er[2] / A.dt;
+    }
+    if (diag && KIND == kAsmStatic) {
       // eres = vol*rho*g*0.25 - K_e u_e (fem_solidlinear.h:214-221 and
       // AddEmatEvecScale3 femutil.h:252-268, scale -1), then vec_b += eres
       const double vol = G.vol;
@@ -87,6 +194,9 @@ __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
       bacc[0] += er0;
       bacc[1] += er1;
       bacc[2] += er2;
+    }
+    if (diag) {
+      const unsigned id[4] = {n.x, n.y, n.z, n.w};
       // every neighbour of i in this element must own a slot in row i
       const unsigned r0 = A.colind[i], r1 = A.colind[i + 1];
 #pragma unroll
@@ -162,6 +272,72 @@ using namespace dfm2h;
 static void need_pattern(dfm2h_bsr A) {
   DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
   DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_STATE, "SetPattern has not been called");
+}
+
+// Shared host side of the gather assembly: checks, argument block, launch,
+// missing-slot report.  `a` carries the kind's extra fields.
+static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu_e,
+                         const double *lambda_e, double myu, double lambda, double rho,
+                         const double *g, const double *xyz, dfm2h_tetmesh M, AsmArgs a) {
+  need_pattern(A);
+  DFM2H_REQUIRE(M, DFM2H_ERR_INVALID, "null mesh handle");
+  DFM2H_REQUIRE(vec_b && g && xyz, DFM2H_ERR_INVALID, "null argument");
+  DFM2H_REQUIRE(M->np == A->nblk, DFM2H_ERR_INVALID, "mesh node count != matrix block rows");
+  const size_t n = A->pat.ncrs + A->nblk;
+  if (!n) return;
+  DBuf<int> missing;
+  missing.alloc(1);
+  missing.zero();
+  a.nblk = A->nblk;
+  a.ncrs = A->pat.ncrs;
+  a.colind = A->pat.colind_d.p;
+  a.rowptr = A->pat.rowptr_d.p;
+  a.rowof = A->pat.rowof_d.p;
+  a.vcrs = A->vcrs.p;
+  a.vdia = A->vdia.p;
+  a.vec_b = vec_b;
+  a.myu_e = myu_e;
+  a.lambda_e = lambda_e;
+  a.myu = myu;
+  a.lambda = lambda;
+  a.rho = rho;
+  a.g0 = g[0];
+  a.g1 = g[1];
+  a.g2 = g[2];
+  a.xyz = xyz;
+  a.tet = reinterpret_cast<const uint4 *>(M->tet.p);
+  a.elsup_ind = M->elsup_ind.p;
+  a.elsup = M->elsup.p;
+  a.missing = missing.p;
+  const dim3 grid(ceil_div(n, 256)), block(256);
+  if (kind == kAsmNewmark)
+    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmNewmark>, grid, block, 0, stream(), a);
+  else if (kind == kAsmBEuler)
+    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmBEuler>, grid, block, 0, stream(), a);
+  else
+    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmStatic>, grid, block, 0, stream(), a);
+  check_launch("k_assemble_tet_gather");
+  int h_missing = 0;
+  DFM2H_HIP(hipMemcpyAsync(&h_missing, missing.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  DFM2H_HIP(hipStreamSynchronize(stream()));
+  A->version++;
+  DFM2H_REQUIRE(!h_missing, DFM2H_ERR_PATTERN,
+                "an element edge has no block in the matrix pattern");
+}
+
+// XPlusAYBZ (vecxitrsol.cpp:312-325) and, with W, XPlusAYBZCW (:327-342):
+// X += alpha Y + beta Z (+ gamma W) on the free entries, same association.
+template <bool CW>
+__global__ void k_xplusaybzcw(double *X, size_t n, const int *__restrict__ flag, double alpha,
+                              const double *__restrict__ Y, double beta,
+                              const double *__restrict__ Z, double gamma,
+                              const double *__restrict__ W) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || flag[i] != 0) return;
+  if (CW)
+    X[i] += alpha * Y[i] + beta * Z[i] + gamma * W[i];
+  else
+    X[i] += alpha * Y[i] + beta * Z[i];
 }
 
 extern "C" {
@@ -331,46 +507,66 @@ int dfm2h_merge_linsys_solidlinear_static_tet(dfm2h_bsr A, double *vec_b, const 
                                               double rho, const double *g, const double *xyz,
                                               dfm2h_tetmesh M, const double *disp) {
   return guard(__func__, [&] {
-    need_pattern(A);
-    DFM2H_REQUIRE(M, DFM2H_ERR_INVALID, "null mesh handle");
-    DFM2H_REQUIRE(vec_b && g && xyz, DFM2H_ERR_INVALID, "null argument");
-    DFM2H_REQUIRE(M->np == A->nblk, DFM2H_ERR_INVALID, "mesh node count != matrix block rows");
-    const size_t n = A->pat.ncrs + A->nblk;
-    if (!n) return;
-    DBuf<int> missing;
-    missing.alloc(1);
-    missing.zero();
-    AsmArgs a;
-    a.nblk = A->nblk;
-    a.ncrs = A->pat.ncrs;
-    a.colind = A->pat.colind_d.p;
-    a.rowptr = A->pat.rowptr_d.p;
-    a.rowof = A->pat.rowof_d.p;
-    a.vcrs = A->vcrs.p;
-    a.vdia = A->vdia.p;
-    a.vec_b = vec_b;
-    a.myu_e = myu_e;
-    a.lambda_e = lambda_e;
-    a.myu = myu;
-    a.lambda = lambda;
-    a.rho = rho;
-    a.g0 = g[0];
-    a.g1 = g[1];
-    a.g2 = g[2];
-    a.xyz = xyz;
+    AsmArgs a{};
     a.disp = disp;
-    a.tet = reinterpret_cast<const uint4 *>(M->tet.p);
-    a.elsup_ind = M->elsup_ind.p;
-    a.elsup = M->elsup.p;
-    a.missing = missing.p;
-    hipLaunchKernelGGL(k_assemble_tet_gather, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), a);
-    check_launch("k_assemble_tet_gather");
-    int h_missing = 0;
-    DFM2H_HIP(hipMemcpyAsync(&h_missing, missing.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
-    DFM2H_HIP(hipStreamSynchronize(stream()));
-    A->version++;
-    DFM2H_REQUIRE(!h_missing, DFM2H_ERR_PATTERN,
-                  "an element edge has no block in the matrix pattern");
+    run_assembly(kAsmStatic, A, vec_b, myu_e, lambda_e, myu, lambda, rho, g, xyz, M, a);
+  });
+}
+
+int dfm2h_merge_linsys_solidlinear_newmark_tet(dfm2h_bsr A, double *vec_b, const double *myu_e,
+                                               const double *lambda_e, double myu, double lambda,
+                                               double rho, const double *g, double dt,
+                                               double gamma_newmark, double beta_newmark,
+                                               const double *xyz, dfm2h_tetmesh M,
+                                               const double *disp, const double *velo,
+                                               const double *acc) {
+  (void)gamma_newmark;  // unused by the element matrix (fem_solidlinear.cpp:447)
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(dt > 0.0, DFM2H_ERR_INVALID, "time step must be positive");
+    AsmArgs a{};
+    a.disp = disp;
+    a.velo = velo;
+    a.acc = acc;
+    a.dt = dt;
+    a.beta_nb = beta_newmark;
+    run_assembly(kAsmNewmark, A, vec_b, myu_e, lambda_e, myu, lambda, rho, g, xyz, M, a);
+  });
+}
+
+int dfm2h_merge_linsys_solidlinear_beuler_tet(dfm2h_bsr A, double *vec_b, const double *myu_e,
+                                              const double *lambda_e, double myu, double lambda,
+                                              double rho, const double *g, double dt,
+                                              const double *xyz, dfm2h_tetmesh M,
+                                              const double *disp, const double *velo) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(dt > 0.0, DFM2H_ERR_INVALID, "time step must be positive");
+    AsmArgs a{};
+    a.disp = disp;
+    a.velo = velo;
+    a.dt = dt;
+    run_assembly(kAsmBEuler, A, vec_b, myu_e, lambda_e, myu, lambda, rho, g, xyz, M, a);
+  });
+}
+
+int dfm2h_xplusaybz(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                    double beta, const double *Z) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE((X && flag && Y && Z) || n == 0, DFM2H_ERR_INVALID, "null vector");
+    if (!n) return;
+    hipLaunchKernelGGL(k_xplusaybzcw<false>, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), X,
+                       n, flag, alpha, Y, beta, Z, 0.0, (const double *)nullptr);
+    check_launch("k_xplusaybz");
+  });
+}
+
+int dfm2h_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                      double beta, const double *Z, double gamma, const double *W) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE((X && flag && Y && Z && W) || n == 0, DFM2H_ERR_INVALID, "null vector");
+    if (!n) return;
+    hipLaunchKernelGGL(k_xplusaybzcw<true>, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), X,
+                       n, flag, alpha, Y, beta, Z, gamma, W);
+    check_launch("k_xplusaybzcw");
   });
 }
 

@@ -43,12 +43,19 @@ __device__ inline void tet_geometry(TetGeo &g, const double P[4][3]) {
 }
 
 // emat[li][lj][a][b] built exactly as AddEmat_LinearSolid3 does on a zeroed
-// emat: 0 + w*(lambda*di_a*dj_b + myu*dj_a*di_b), plus w*myu*(di.dj) when a==b.
-template <int a, int b>
+// emat: 0 + w*(lambda*di_a*dj_b + myu*dj_a*di_b), plus w*myu*(di.dj) when a==b
+// (SET: assigned as SetEmat_LinearSolid3 does, fem_solidlinear.h:129-153,
+// which keeps the sign of an exact zero).
+template <int a, int b, bool SET = false>
 __device__ inline double emat_entry(const double *di, const double *dj, double w, double myu,
                                     double lam) {
-  double v = 0.0;
-  v += w * (lam * di[a] * dj[b] + myu * dj[a] * di[b]);
+  double v;
+  if (SET) {
+    v = w * (lam * di[a] * dj[b] + myu * dj[a] * di[b]);
+  } else {
+    v = 0.0;
+    v += w * (lam * di[a] * dj[b] + myu * dj[a] * di[b]);
+  }
   if (a == b) {
     const double dt = di[0] * dj[0] + di[1] * dj[1] + di[2] * dj[2];
     v += w * myu * dt;
@@ -56,17 +63,18 @@ __device__ inline double emat_entry(const double *di, const double *dj, double w
   return v;
 }
 
+template <bool SET = false>
 __device__ inline void emat_block(double blk[9], const double *di, const double *dj, double w,
                                   double myu, double lam) {
-  blk[0] = emat_entry<0, 0>(di, dj, w, myu, lam);
-  blk[1] = emat_entry<0, 1>(di, dj, w, myu, lam);
-  blk[2] = emat_entry<0, 2>(di, dj, w, myu, lam);
-  blk[3] = emat_entry<1, 0>(di, dj, w, myu, lam);
-  blk[4] = emat_entry<1, 1>(di, dj, w, myu, lam);
-  blk[5] = emat_entry<1, 2>(di, dj, w, myu, lam);
-  blk[6] = emat_entry<2, 0>(di, dj, w, myu, lam);
-  blk[7] = emat_entry<2, 1>(di, dj, w, myu, lam);
-  blk[8] = emat_entry<2, 2>(di, dj, w, myu, lam);
+  blk[0] = emat_entry<0, 0, SET>(di, dj, w, myu, lam);
+  blk[1] = emat_entry<0, 1, SET>(di, dj, w, myu, lam);
+  blk[2] = emat_entry<0, 2, SET>(di, dj, w, myu, lam);
+  blk[3] = emat_entry<1, 0, SET>(di, dj, w, myu, lam);
+  blk[4] = emat_entry<1, 1, SET>(di, dj, w, myu, lam);
+  blk[5] = emat_entry<1, 2, SET>(di, dj, w, myu, lam);
+  blk[6] = emat_entry<2, 0, SET>(di, dj, w, myu, lam);
+  blk[7] = emat_entry<2, 1, SET>(di, dj, w, myu, lam);
+  blk[8] = emat_entry<2, 2, SET>(di, dj, w, myu, lam);
 }
 
 __device__ inline int local_index(uint4 n, unsigned v) {

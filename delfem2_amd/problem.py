@@ -96,3 +96,74 @@ class BatchSystem(DeviceSystem):
         x = torch.zeros_like(self.b)
         it, rr, st = self.solver.solve(self.b, x, tol, fp32, delta, max_nitr)
         return x, it, rr, st
+
+
+class DynamicSystem(DeviceSystem):
+    """Implicit time stepping of a Problem on the device (body force only).
+
+    ``scheme="newmark"``: the step of
+    examples_oldgl_glfw/618_Fem3Tet/main.cpp:361-416 --
+    MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D (M + beta dt^2 K and the
+    Newmark residual), SetFixedBC, setRHS_Zero, ILU CopyValue + Decompose,
+    Solve_PCG for the acceleration increment x, then
+    u += dt v + (0.5 dt^2) a + (dt^2 beta) x, v += (dt gamma) x + dt a,
+    a += x on the free DOFs (XPlusAYBZCW / XPlusAYBZ / XPlusAY).
+
+    ``scheme="beuler"``: the step of
+    examples_oldgl_glfw/603_Fem3DStiffwarp/main.cpp:135-178 --
+    MergeLinSys_SolidLinear_BEuler_MeshTet3D (K + lumped M / dt^2 and
+    (f - K (u + dt v)) / dt), the same BC / ILU steps, a solve for the
+    velocity increment dv, then u += dt dv + dt v, v += dv.  The example
+    solves with Solve_PBiCGStab; the matrix is symmetric positive definite,
+    so this driver uses the fused Solve_PCG (the solution agrees to the
+    solver tolerance, the iterates do not).
+    """
+
+    def __init__(self, prob: Problem, scheme: str = "newmark", dt: float = 0.03,
+                 gamma_newmark: float = 0.6, beta_newmark: float = 0.36, device: int = 0,
+                 fill_level: int = 0):
+        if scheme not in ("newmark", "beuler"):
+            raise ValueError(f"unknown scheme {scheme!r}")
+        super().__init__(prob, device, fill_level)
+        self.scheme = scheme
+        self.dt = float(dt)
+        self.gamma = float(gamma_newmark)
+        self.beta = float(beta_newmark)
+        self.u = torch.zeros_like(self.b)
+        self.v = torch.zeros_like(self.b)
+        self.a = torch.zeros_like(self.b)
+
+    def assemble(self, disp=None) -> bool:
+        self.A.setZero()
+        self.b.zero_()
+        p = self.prob
+        if self.scheme == "newmark":
+            ok = S.MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(
+                self.A, self.b, self.myu, self.lam, p.rho, p.g, self.dt, self.gamma, self.beta,
+                self.xyz, self.mesh, self.u, self.v, self.a)
+        else:
+            ok = S.MergeLinSys_SolidLinear_BEuler_MeshTet3D(
+                self.A, self.b, self.myu, self.lam, p.rho, p.g, self.dt, self.xyz, self.mesh,
+                self.u, self.v)
+        self.A.SetFixedBC(self.flag)
+        S.setRHS_Zero(self.b, self.flag, 0)
+        return ok
+
+    def step(self, tol: float = 1e-4, max_nitr: int = 1000) -> int:
+        """One time step; returns the PCG iteration count."""
+        if not self.assemble():
+            raise RuntimeError("an element edge is missing from the matrix pattern")
+        if not self.factor():
+            raise RuntimeError("ILU decomposition failed")
+        x = torch.zeros_like(self.b)
+        hist = S.Solve_PCG(self.b, x, tol, max_nitr, self.A, self.ilu)
+        dt = self.dt
+        if self.scheme == "newmark":
+            S.XPlusAYBZCW(self.u, self.flag, dt, self.v, 0.5 * dt * dt, self.a,
+                          dt * dt * self.beta, x)
+            S.XPlusAYBZ(self.v, self.flag, dt * self.gamma, x, dt, self.a)
+            S.XPlusAY(self.a, self.flag, 1.0, x)
+        else:
+            S.XPlusAYBZ(self.u, self.flag, dt, x, dt, self.v)
+            S.XPlusAY(self.v, self.flag, 1.0, x)
+        return len(hist) - 1

@@ -6,6 +6,9 @@ Same names and argument meaning as the reference C++ API so that callers
 * ``CMatrixSparse``      -- ls_block_sparse.h:24-204 (3x3 blocks, diagonal kept apart)
 * ``CPreconditionerILU`` -- ls_ilu_block_sparse.h:21-60
 * ``MergeLinSys_SolidLinear_Static_MeshTet3D`` -- fem_solidlinear.h:192-237
+* ``MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D`` / ``..._BEuler_...`` --
+  fem_solidlinear.h:331-443
+* ``XPlusAYBZ / XPlusAYBZCW`` -- vecxitrsol.cpp:312-342
 * ``Solve_PCG``          -- lsitrsol.h:80-148 (fused on the device)
 * ``Dot / AddScaledVec / ScaleAndAddVec / setRHS_Zero / XPlusAY`` --
   view_vectorx.h:61-109, vecxitrsol.cpp:226-296
@@ -192,6 +195,55 @@ def MergeLinSys_SolidLinear_Static_MeshTet3D(mat_A: CMatrixSparse, vec_b: torch.
     return True
 
 
+def _lame(myu, lam):
+    if isinstance(myu, torch.Tensor):
+        return _dptr(myu), _dptr(lam), 0.0, 0.0
+    return None, None, float(myu), float(lam)
+
+
+def _opt(t):
+    return None if t is None else _dptr(t)
+
+
+def MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(mat_A: CMatrixSparse, vec_b: torch.Tensor,
+                                                  myu, lam, rho: float, g, dt: float,
+                                                  gamma_newmark: float, beta_newmark: float,
+                                                  aXYZ: torch.Tensor, mesh: TetMesh,
+                                                  aVal: torch.Tensor, aVelo: torch.Tensor,
+                                                  aAcc: torch.Tensor) -> bool:
+    """fem_solidlinear.h:331-379 on the device (is_initial_iter = true, as the
+    reference driver passes): M + beta dt^2 K into A, the Newmark residual into
+    vec_b.  ``myu``/``lam`` scalars or per-element tensors."""
+    g = np.ascontiguousarray(np.asarray(g, np.float64).reshape(3))
+    pm, pl, sm, sl = _lame(myu, lam)
+    st = N.load().dfm2h_merge_linsys_solidlinear_newmark_tet(
+        mat_A.h, _dptr(vec_b), pm, pl, sm, sl, float(rho), _hptr(g), float(dt),
+        float(gamma_newmark), float(beta_newmark), _dptr(aXYZ), mesh.h, _opt(aVal), _opt(aVelo),
+        _opt(aAcc))
+    if st == N.ERR_PATTERN:
+        return False
+    N.check(st)
+    return True
+
+
+def MergeLinSys_SolidLinear_BEuler_MeshTet3D(mat_A: CMatrixSparse, vec_b: torch.Tensor, myu,
+                                             lam, rho: float, g, dt: float,
+                                             aXYZ: torch.Tensor, mesh: TetMesh,
+                                             aDisp: torch.Tensor,
+                                             aVelo: torch.Tensor) -> bool:
+    """fem_solidlinear.h:381-443 on the device: K + lumped M / dt^2 into A,
+    (f - K (u + dt v)) / dt into vec_b."""
+    g = np.ascontiguousarray(np.asarray(g, np.float64).reshape(3))
+    pm, pl, sm, sl = _lame(myu, lam)
+    st = N.load().dfm2h_merge_linsys_solidlinear_beuler_tet(
+        mat_A.h, _dptr(vec_b), pm, pl, sm, sl, float(rho), _hptr(g), float(dt), _dptr(aXYZ),
+        mesh.h, _opt(aDisp), _opt(aVelo))
+    if st == N.ERR_PATTERN:
+        return False
+    N.check(st)
+    return True
+
+
 class CPreconditionerILU:
     """Block ILU(0) with level-scheduled factorisation and sweeps."""
 
@@ -318,6 +370,20 @@ def setRHS_Zero(vec_b: torch.Tensor, aBCFlag: torch.Tensor, iflag_nonzero: int =
 def XPlusAY(X: torch.Tensor, aBCFlag: torch.Tensor, alpha: float, Y: torch.Tensor) -> None:
     N.call("dfm2h_xplusay", _dptr(X), X.numel(), _dptr(aBCFlag, torch.int32), float(alpha),
            _dptr(Y))
+
+
+def XPlusAYBZ(X: torch.Tensor, aBCFlag: torch.Tensor, alpha: float, Y: torch.Tensor,
+              beta: float, Z: torch.Tensor) -> None:
+    """vecxitrsol.cpp:312-325"""
+    N.call("dfm2h_xplusaybz", _dptr(X), X.numel(), _dptr(aBCFlag, torch.int32), float(alpha),
+           _dptr(Y), float(beta), _dptr(Z))
+
+
+def XPlusAYBZCW(X: torch.Tensor, aBCFlag: torch.Tensor, alpha: float, Y: torch.Tensor,
+                beta: float, Z: torch.Tensor, gamma: float, W: torch.Tensor) -> None:
+    """vecxitrsol.cpp:327-342"""
+    N.call("dfm2h_xplusaybzcw", _dptr(X), X.numel(), _dptr(aBCFlag, torch.int32), float(alpha),
+           _dptr(Y), float(beta), _dptr(Z), float(gamma), _dptr(W))
 
 
 def Solve_PCG(r_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float, max_nitr: int,

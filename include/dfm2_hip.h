@@ -134,6 +134,37 @@ int dfm2h_merge_linsys_solidlinear_static_tet(
     const double *lambda_e_dev, double myu, double lambda, double rho,
     const double *g, const double *xyz_dev, dfm2h_tetmesh M,
     const double *disp_dev);
+/* fem_solidlinear.h:331-379 MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D:
+ * per element (ascending) EMat_SolidLinear_NewmarkBeta_MeshTet3D
+ * (fem_solidlinear.cpp:437-490) with is_initial_iter = true, as the driver
+ * passes: M_e + beta dt^2 K_e into A (consistent mass, femutil.cpp:260-280)
+ * and vol rho g / 4 - K_e u - M_e a - dt K_e v - (0.5 dt^2) K_e a into
+ * vec_b; bit-identical to the reference.  gamma_newmark is unused by the
+ * element (as in the reference).  disp / velo / acc_dev [3*np] (NULL: zero);
+ * other arguments as for the static assembly. */
+int dfm2h_merge_linsys_solidlinear_newmark_tet(
+    dfm2h_bsr A, double *vec_b_dev, const double *myu_e_dev,
+    const double *lambda_e_dev, double myu, double lambda, double rho,
+    const double *g, double dt, double gamma_newmark, double beta_newmark,
+    const double *xyz_dev, dfm2h_tetmesh M, const double *disp_dev,
+    const double *velo_dev, const double *acc_dev);
+/* fem_solidlinear.h:381-443 MergeLinSys_SolidLinear_BEuler_MeshTet3D:
+ * K_e plus rho vol / 4 / dt^2 on the diagonal entries of the diagonal
+ * blocks into A, (vol rho g / 4 - K_e (u + dt v)) / dt into vec_b;
+ * bit-identical to the reference. */
+int dfm2h_merge_linsys_solidlinear_beuler_tet(
+    dfm2h_bsr A, double *vec_b_dev, const double *myu_e_dev,
+    const double *lambda_e_dev, double myu, double lambda, double rho,
+    const double *g, double dt, const double *xyz_dev, dfm2h_tetmesh M,
+    const double *disp_dev, const double *velo_dev);
+/* vecxitrsol.cpp:312-325 XPlusAYBZ: X += alpha*Y + beta*Z where flag == 0. */
+int dfm2h_xplusaybz(double *X_dev, size_t n, const int *flag_dev, double alpha,
+                    const double *Y_dev, double beta, const double *Z_dev);
+/* vecxitrsol.cpp:327-342 XPlusAYBZCW: X += alpha*Y + beta*Z + gamma*W where
+ * flag == 0. */
+int dfm2h_xplusaybzcw(double *X_dev, size_t n, const int *flag_dev, double alpha,
+                      const double *Y_dev, double beta, const double *Z_dev,
+                      double gamma, const double *W_dev);
 
 /* ---- CPreconditionerILU<double> (ls_ilu_block_sparse.h) ------------------- */
 int dfm2h_ilu_create(dfm2h_ilu *out);

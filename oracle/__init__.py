@@ -76,6 +76,19 @@ def lib():
                                     C.c_size_t]
         L.orc_solve_pcg.restype = C.c_size_t
         L.orc_xplusay.argtypes = [_f64p, C.c_size_t, _i32p, C.c_double, _f64p]
+        L.orc_assemble_tet_newmark.argtypes = [C.c_uint, _u32p, _u32p, _f64p, _f64p, _f64p,
+                                               _f64p, _f64p, C.c_double, _f64p, C.c_double,
+                                               C.c_double, _f64p, _u32p, C.c_size_t, _f64p,
+                                               _f64p, _f64p]
+        L.orc_assemble_tet_newmark.restype = C.c_int
+        L.orc_assemble_tet_beuler.argtypes = [C.c_uint, _u32p, _u32p, _f64p, _f64p, _f64p,
+                                              _f64p, _f64p, C.c_double, _f64p, C.c_double,
+                                              _f64p, _u32p, C.c_size_t, _f64p, _f64p]
+        L.orc_assemble_tet_beuler.restype = C.c_int
+        L.orc_xplusaybz.argtypes = [_f64p, C.c_size_t, _i32p, C.c_double, _f64p, C.c_double,
+                                    _f64p]
+        L.orc_xplusaybzcw.argtypes = [_f64p, C.c_size_t, _i32p, C.c_double, _f64p, C.c_double,
+                                      _f64p, C.c_double, _f64p]
         L.orc_adjoint_grad_tet.argtypes = [_f64p, _f64p, _f64p, _f64p, _u32p, C.c_size_t,
                                            _f64p, _f64p]
         _lib = L
@@ -122,6 +135,21 @@ def ref():
         R.ref_solve_pcg.argtypes = [_vp, _f64p, _f64p, C.c_double, C.c_uint, _f64p, C.c_size_t]
         R.ref_solve_pcg.restype = C.c_size_t
         R.ref_xplusay.argtypes = [_f64p, C.c_size_t, _i32p, C.c_double, _f64p]
+        R.ref_assemble_newmark.argtypes = [_vp, _f64p, C.c_double, C.c_double, C.c_double,
+                                           _f64p, C.c_double, C.c_double, C.c_double, _f64p,
+                                           C.c_size_t, _u32p, C.c_size_t, _f64p, _f64p, _f64p]
+        R.ref_assemble_newmark_per_element.argtypes = [_vp, _f64p, _f64p, _f64p, C.c_double,
+                                                       _f64p, C.c_double, C.c_double,
+                                                       C.c_double, _f64p, _u32p, C.c_size_t,
+                                                       _f64p, _f64p, _f64p]
+        R.ref_assemble_newmark_per_element.restype = C.c_int
+        R.ref_assemble_beuler.argtypes = [_vp, _f64p, C.c_double, C.c_double, C.c_double,
+                                          _f64p, C.c_double, _f64p, C.c_size_t, _u32p,
+                                          C.c_size_t, _f64p, _f64p]
+        R.ref_xplusaybz.argtypes = [_f64p, C.c_size_t, _i32p, C.c_double, _f64p, C.c_double,
+                                    _f64p]
+        R.ref_xplusaybzcw.argtypes = [_f64p, C.c_size_t, _i32p, C.c_double, _f64p, C.c_double,
+                                      _f64p, C.c_double, _f64p]
         _ref = R
     return _ref
 
@@ -206,6 +234,22 @@ class System:
             np.ascontiguousarray(tet, np.uint32), tet.size // 4,
             None if d is None else d.ctypes.data)
         return st
+
+    def assemble_newmark(self, b, myu_e, lambda_e, rho, g, dt, beta, xyz, tet, u, v, a):
+        """fem_solidlinear.h:331-379 (per-element Lame pair)."""
+        f = lambda x: np.ascontiguousarray(x, np.float64)
+        return lib().orc_assemble_tet_newmark(
+            self.nblk, self.colind, self.rowptr, self.vcrs, self.vdia, b, f(myu_e), f(lambda_e),
+            float(rho), f(g), float(dt), float(beta), f(xyz), np.ascontiguousarray(tet, np.uint32),
+            tet.size // 4, f(u), f(v), f(a))
+
+    def assemble_beuler(self, b, myu_e, lambda_e, rho, g, dt, xyz, tet, u, v):
+        """fem_solidlinear.h:381-443 (per-element Lame pair)."""
+        f = lambda x: np.ascontiguousarray(x, np.float64)
+        return lib().orc_assemble_tet_beuler(
+            self.nblk, self.colind, self.rowptr, self.vcrs, self.vdia, b, f(myu_e), f(lambda_e),
+            float(rho), f(g), float(dt), f(xyz), np.ascontiguousarray(tet, np.uint32),
+            tet.size // 4, f(u), f(v))
 
     def set_fixed_bc(self, flag):
         lib().orc_set_fixed_bc(self.nblk, self.colind, self.rowptr, self.vcrs, self.vdia,
@@ -294,6 +338,26 @@ class RefSystem:
                                    np.ascontiguousarray(tet, np.uint32), tet.size // 4,
                                    np.ascontiguousarray(disp, np.float64))
 
+    def assemble_newmark_uniform(self, b, myu, lam, rho, g, dt, gamma, beta, xyz, tet, u, v, a):
+        f = lambda x: np.ascontiguousarray(x, np.float64)
+        ref().ref_assemble_newmark(self.h, b, float(myu), float(lam), float(rho), f(g),
+                                   float(dt), float(gamma), float(beta), f(xyz), xyz.size // 3,
+                                   np.ascontiguousarray(tet, np.uint32), tet.size // 4, f(u),
+                                   f(v), f(a))
+
+    def assemble_newmark(self, b, myu_e, lambda_e, rho, g, dt, gamma, beta, xyz, tet, u, v, a):
+        f = lambda x: np.ascontiguousarray(x, np.float64)
+        return ref().ref_assemble_newmark_per_element(
+            self.h, b, f(myu_e), f(lambda_e), float(rho), f(g), float(dt), float(gamma),
+            float(beta), f(xyz), np.ascontiguousarray(tet, np.uint32), tet.size // 4, f(u),
+            f(v), f(a))
+
+    def assemble_beuler_uniform(self, b, myu, lam, rho, g, dt, xyz, tet, u, v):
+        f = lambda x: np.ascontiguousarray(x, np.float64)
+        ref().ref_assemble_beuler(self.h, b, float(myu), float(lam), float(rho), f(g), float(dt),
+                                  f(xyz), xyz.size // 3, np.ascontiguousarray(tet, np.uint32),
+                                  tet.size // 4, f(u), f(v))
+
     def set_fixed_bc(self, flag):
         ref().ref_set_fixed_bc(self.h, np.ascontiguousarray(flag, np.int32))
 
@@ -344,6 +408,16 @@ def xplusay(X, flag, alpha, Y):
     lib().orc_xplusay(X, X.size, np.ascontiguousarray(flag, np.int32), float(alpha), Y)
 
 
+def xplusaybz(X, flag, alpha, Y, beta, Z):
+    lib().orc_xplusaybz(X, X.size, np.ascontiguousarray(flag, np.int32), float(alpha), Y,
+                        float(beta), Z)
+
+
+def xplusaybzcw(X, flag, alpha, Y, beta, Z, gamma, W):
+    lib().orc_xplusaybzcw(X, X.size, np.ascontiguousarray(flag, np.int32), float(alpha), Y,
+                          float(beta), Z, float(gamma), W)
+
+
 def adjoint_grad(mu1_e, lambda1_e, xyz, tet, u, lam):
     ne = tet.size // 4
     g = np.zeros(ne)
@@ -354,3 +428,82 @@ def adjoint_grad(mu1_e, lambda1_e, xyz, tet, u, lam):
                                np.ascontiguousarray(u, np.float64),
                                np.ascontiguousarray(lam, np.float64))
     return g
+
+
+class DynamicRun:
+    """CPU time stepping with the same call sequence as the device driver
+    (delfem2_amd.problem.DynamicSystem): the restatement (``use_ref=False``)
+    or the compiled reference (``use_ref=True``; backward Euler needs a
+    uniform Lame pair there, the reference driver's signature)."""
+
+    def __init__(self, prob, scheme="newmark", dt=0.03, gamma_newmark=0.6, beta_newmark=0.36,
+                 use_ref=False):
+        ind, ps = psup(prob.tet, prob.np_)
+        self.sys = RefSystem(ind, ps) if use_ref else System(ind, ps)
+        self.use_ref = use_ref
+        self.prob = prob
+        self.scheme = scheme
+        self.dt, self.gamma, self.beta = float(dt), float(gamma_newmark), float(beta_newmark)
+        n = prob.ndof
+        self.u, self.v, self.a = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.b = np.zeros(n)
+
+    def assemble(self):
+        p, S = self.prob, self.sys
+        self.b = np.zeros(p.ndof)
+        if self.use_ref:
+            vc, vd = S.matrix()
+            S.set_matrix(np.zeros_like(vc), np.zeros_like(vd))
+        else:
+            S.vcrs[:] = 0.0
+            S.vdia[:] = 0.0
+        if self.scheme == "newmark":
+            if self.use_ref:
+                st = S.assemble_newmark(self.b, p.myu, p.lam, p.rho, p.g, self.dt, self.gamma,
+                                        self.beta, p.xyz, p.tet, self.u, self.v, self.a)
+            else:
+                st = S.assemble_newmark(self.b, p.myu, p.lam, p.rho, p.g, self.dt, self.beta,
+                                        p.xyz, p.tet, self.u, self.v, self.a)
+        else:
+            if self.use_ref:
+                assert np.all(p.myu == p.myu[0]) and np.all(p.lam == p.lam[0])
+                S.assemble_beuler_uniform(self.b, p.myu[0], p.lam[0], p.rho, p.g, self.dt,
+                                          p.xyz, p.tet, self.u, self.v)
+                st = 0
+            else:
+                st = S.assemble_beuler(self.b, p.myu, p.lam, p.rho, p.g, self.dt, p.xyz, p.tet,
+                                       self.u, self.v)
+        S.set_fixed_bc(p.flag)
+        if self.use_ref:
+            ref().ref_set_rhs_zero(self.b, np.ascontiguousarray(p.flag, np.int32), self.b.size, 0)
+        else:
+            set_rhs_zero(self.b, p.flag, 0)
+        return st
+
+    def step(self, tol=1e-4, max_nitr=1000):
+        """One step; returns the PCG iteration count."""
+        assert self.assemble() == 0
+        assert self.sys.factor()
+        x, hist = self.sys.solve_pcg(self.b, tol, max_nitr)
+        dt, fl = self.dt, np.ascontiguousarray(self.prob.flag, np.int32)
+        n = self.u.size
+        if self.use_ref:
+            R = ref()
+            if self.scheme == "newmark":
+                R.ref_xplusaybzcw(self.u, n, fl, dt, self.v, 0.5 * dt * dt, self.a,
+                                  dt * dt * self.beta, x)
+                R.ref_xplusaybz(self.v, n, fl, dt * self.gamma, x, dt, self.a)
+                R.ref_xplusay(self.a, n, fl, 1.0, x)
+            else:
+                R.ref_xplusaybz(self.u, n, fl, dt, x, dt, self.v)
+                R.ref_xplusay(self.v, n, fl, 1.0, x)
+        else:
+            if self.scheme == "newmark":
+                xplusaybzcw(self.u, fl, dt, self.v, 0.5 * dt * dt, self.a, dt * dt * self.beta,
+                            x)
+                xplusaybz(self.v, fl, dt * self.gamma, x, dt, self.a)
+                xplusay(self.a, fl, 1.0, x)
+            else:
+                xplusaybz(self.u, fl, dt, x, dt, self.v)
+                xplusay(self.v, fl, 1.0, x)
+        return len(hist) - 1

@@ -615,3 +615,182 @@ size_t orc_ilu_pattern_k(unsigned nblk, const unsigned *a_colind, const unsigned
   free(row);
   return nent;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Time-stepping drivers. */
+
+/* fem_solidlinear.h:129-153 SetEmat_LinearSolid3<4>: assigned, so an exact
+ * zero keeps its sign (the additive form on a zeroed emat would turn -0 into
+ * +0). */
+static void set_emat_ls3(double e[4][4][3][3], double lambda, double myu, const double d[4][3],
+                         double w) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          e[i][j][a][b] = w * (lambda * d[i][a] * d[j][b] + myu * d[j][a] * d[i][b]);
+      const double dtmp1 = d[i][0] * d[j][0] + d[i][1] * d[j][1] + d[i][2] * d[j][2];
+      e[i][j][0][0] += w * myu * dtmp1;
+      e[i][j][1][1] += w * myu * dtmp1;
+      e[i][j][2][2] += w * myu * dtmp1;
+    }
+}
+
+/* femutil.cpp:260-280 SetEmatConsistentMassTet */
+static void set_emat_mass_tet(double m[4][4][3][3], double w0) {
+  const double dtmp1 = w0 * 0.05;
+  for (int i = 0; i < 4; ++i) {
+    for (int j = 0; j < 4; ++j)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) m[i][j][a][b] = a == b ? dtmp1 : 0.0;
+    m[i][i][0][0] += dtmp1;
+    m[i][i][1][1] += dtmp1;
+    m[i][i][2][2] += dtmp1;
+  }
+}
+
+/* femutil.h:252-268 AddEmatEvecScale3<4> */
+static void add_emat_evec_scale3(double R[4][3], const double e[4][4][3][3],
+                                 const double v[4][3], double scale) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int a = 0; a < 3; ++a)
+        R[i][a] += scale * (e[i][j][a][0] * v[j][0] + e[i][j][a][1] * v[j][1] +
+                            e[i][j][a][2] * v[j][2]);
+}
+
+/* ls_block_sparse.h:260-310 Merge<4,4,3,3>: row i's column -> slot map in buf
+ * (all UINT_MAX on entry and exit); returns 1 on a missing block. */
+static int merge_tet(const unsigned *ip, const double e[4][4][3][3], const unsigned *colind,
+                     const unsigned *rowptr, double *vcrs, double *vdia, unsigned *buf) {
+  int status = 0;
+  for (int ir = 0; ir < 4; ++ir) {
+    const unsigned i = ip[ir];
+    for (unsigned s = colind[i]; s < colind[i + 1]; ++s) buf[rowptr[s]] = s;
+    for (int jc = 0; jc < 4; ++jc) {
+      const unsigned j = ip[jc];
+      double *out;
+      if (i == j) {
+        out = vdia + (size_t)i * 9;
+      } else {
+        if (buf[j] == UINT_MAX) { status = 1; continue; }
+        out = vcrs + (size_t)buf[j] * 9;
+      }
+      const double *in = &e[ir][jc][0][0];
+      for (int k = 0; k < 9; ++k) out[k] += in[k];
+    }
+    for (unsigned s = colind[i]; s < colind[i + 1]; ++s) buf[rowptr[s]] = UINT_MAX;
+  }
+  return status;
+}
+
+static void fetch43(double out[4][3], const unsigned *ip, const double *v) {
+  for (int k = 0; k < 4; ++k)
+    for (int a = 0; a < 3; ++a) out[k][a] = v[ip[k] * 3 + a];
+}
+
+int orc_assemble_tet_newmark(unsigned nblk, const unsigned *colind, const unsigned *rowptr,
+                             double *vcrs, double *vdia, double *vec_b, const double *myu_e,
+                             const double *lambda_e, double rho, const double *g, double dt,
+                             double beta_newmark, const double *xyz, const unsigned *tet,
+                             size_t ntet, const double *disp, const double *velo,
+                             const double *acc) {
+  unsigned *buf = (unsigned *)malloc(sizeof(unsigned) * (nblk + 1));
+  for (unsigned i = 0; i < nblk; ++i) buf[i] = UINT_MAX;
+  int status = 0;
+  for (size_t iel = 0; iel < ntet; ++iel) {
+    const unsigned *ip = tet + iel * 4;
+    double P[4][3], U[4][3], V[4][3], A[4][3];
+    fetch43(P, ip, xyz);
+    fetch43(U, ip, disp);
+    fetch43(V, ip, velo);
+    fetch43(A, ip, acc);
+    /* fem_solidlinear.cpp:437-490 */
+    const double vol = orc_tet_volume(P[0], P[1], P[2], P[3]);
+    double d[4][3];
+    orc_tet_dldx(d, P[0], P[1], P[2], P[3]);
+    double eK[4][4][3][3], eM[4][4][3][3], emat[4][4][3][3], eres[4][3];
+    set_emat_ls3(eK, lambda_e[iel], myu_e[iel], (const double(*)[3])d, vol);
+    set_emat_mass_tet(eM, rho * vol);
+    const double dtmp1 = beta_newmark * dt * dt;
+    for (int k = 0; k < 144; ++k)
+      (&emat[0][0][0][0])[k] = (&eM[0][0][0][0])[k] + dtmp1 * (&eK[0][0][0][0])[k];
+    for (int k = 0; k < 4; ++k) {
+      eres[k][0] = vol * rho * g[0] * 0.25;
+      eres[k][1] = vol * rho * g[1] * 0.25;
+      eres[k][2] = vol * rho * g[2] * 0.25;
+    }
+    add_emat_evec_scale3(eres, (const double(*)[4][3][3])eK, (const double(*)[3])U, -1.);
+    add_emat_evec_scale3(eres, (const double(*)[4][3][3])eM, (const double(*)[3])A, -1.);
+    add_emat_evec_scale3(eres, (const double(*)[4][3][3])eK, (const double(*)[3])V, -dt);
+    add_emat_evec_scale3(eres, (const double(*)[4][3][3])eK, (const double(*)[3])A,
+                         -0.5 * dt * dt);
+    for (int k = 0; k < 4; ++k) {
+      vec_b[ip[k] * 3 + 0] += eres[k][0];
+      vec_b[ip[k] * 3 + 1] += eres[k][1];
+      vec_b[ip[k] * 3 + 2] += eres[k][2];
+    }
+    status |= merge_tet(ip, (const double(*)[4][3][3])emat, colind, rowptr, vcrs, vdia, buf);
+  }
+  free(buf);
+  return status;
+}
+
+int orc_assemble_tet_beuler(unsigned nblk, const unsigned *colind, const unsigned *rowptr,
+                            double *vcrs, double *vdia, double *vec_b, const double *myu_e,
+                            const double *lambda_e, double rho, const double *g, double dt,
+                            const double *xyz, const unsigned *tet, size_t ntet,
+                            const double *disp, const double *velo) {
+  unsigned *buf = (unsigned *)malloc(sizeof(unsigned) * (nblk + 1));
+  for (unsigned i = 0; i < nblk; ++i) buf[i] = UINT_MAX;
+  int status = 0;
+  for (size_t iel = 0; iel < ntet; ++iel) {
+    const unsigned *ip = tet + iel * 4;
+    double P[4][3], u[4][3], v[4][3];
+    fetch43(P, ip, xyz);
+    const double vol = orc_tet_volume(P[0], P[1], P[2], P[3]);
+    double d[4][3], emat[4][4][3][3], eres[4][3];
+    orc_tet_dldx(d, P[0], P[1], P[2], P[3]);
+    set_emat_ls3(emat, lambda_e[iel], myu_e[iel], (const double(*)[3])d, vol);
+    fetch43(u, ip, disp);
+    fetch43(v, ip, velo);
+    for (int ino = 0; ino < 4; ++ino)
+      for (int a = 0; a < 3; ++a) {
+        eres[ino][a] = vol * rho * g[a] * 0.25;
+        for (int jno = 0; jno < 4; ++jno) {
+          eres[ino][a] -= emat[ino][jno][a][0] * (u[jno][0] + dt * v[jno][0]);
+          eres[ino][a] -= emat[ino][jno][a][1] * (u[jno][1] + dt * v[jno][1]);
+          eres[ino][a] -= emat[ino][jno][a][2] * (u[jno][2] + dt * v[jno][2]);
+        }
+      }
+    for (int ino = 0; ino < 4; ++ino) {
+      emat[ino][ino][0][0] += rho * vol * 0.25 / (dt * dt);
+      emat[ino][ino][1][1] += rho * vol * 0.25 / (dt * dt);
+      emat[ino][ino][2][2] += rho * vol * 0.25 / (dt * dt);
+    }
+    for (int k = 0; k < 4; ++k) {
+      vec_b[ip[k] * 3 + 0] += eres[k][0] / dt;
+      vec_b[ip[k] * 3 + 1] += eres[k][1] / dt;
+      vec_b[ip[k] * 3 + 2] += eres[k][2] / dt;
+    }
+    status |= merge_tet(ip, (const double(*)[4][3][3])emat, colind, rowptr, vcrs, vdia, buf);
+  }
+  free(buf);
+  return status;
+}
+
+void orc_xplusaybz(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                   double beta, const double *Z) {
+  for (size_t i = 0; i < n; ++i) {
+    if (flag[i] != 0) continue;
+    X[i] += alpha * Y[i] + beta * Z[i];
+  }
+}
+
+void orc_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                     double beta, const double *Z, double gamma, const double *W) {
+  for (size_t i = 0; i < n; ++i) {
+    if (flag[i] != 0) continue;
+    X[i] += alpha * Y[i] + beta * Z[i] + gamma * W[i];
+  }
+}

@@ -136,6 +136,42 @@ void orc_adjoint_grad_tet(double *grad, const double *mu1_e,
                           const unsigned *tet, size_t ntet, const double *u,
                           const double *lam);
 
+/* ---- time-stepping drivers (fem_solidlinear.h:331-443) ------------------ */
+
+/* fem_solidlinear.h:331-379 MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D,
+ * element by element (ascending) through fem_solidlinear.cpp:437-490
+ * EMat_SolidLinear_NewmarkBeta_MeshTet3D with is_initial_iter = true (what
+ * the driver passes): emat = M_e + beta dt^2 K_e (SetEmat_LinearSolid3<4>,
+ * fem_solidlinear.h:129-153; SetEmatConsistentMassTet, femutil.cpp:260-280,
+ * with w0 = rho vol), eres = vol rho g / 4 - K_e u - M_e a - dt K_e v -
+ * (0.5 dt^2) K_e a (AddEmatEvecScale3<4>, femutil.h:252-268, in that order);
+ * merged with Merge<4,4,3,3>.  Per-element (myu_e, lambda_e); additive.
+ * Returns 0, or 1 when a block is missing from the pattern. */
+int orc_assemble_tet_newmark(unsigned nblk, const unsigned *colind, const unsigned *rowptr,
+                             double *vcrs, double *vdia, double *vec_b, const double *myu_e,
+                             const double *lambda_e, double rho, const double *g, double dt,
+                             double beta_newmark, const double *xyz, const unsigned *tet,
+                             size_t ntet, const double *disp, const double *velo,
+                             const double *acc);
+
+/* fem_solidlinear.h:381-443 MergeLinSys_SolidLinear_BEuler_MeshTet3D:
+ * K_e (SetEmat_LinearSolid3<4>), eres = vol rho g / 4 - K_e (u + dt v)
+ * (three subtractions per node pair), then rho vol / 4 / dt^2 on the diagonal
+ * entries of the diagonal blocks; vec_b += eres / dt.  Per-element
+ * (myu_e, lambda_e); additive.  Returns 0, or 1 on a missing block. */
+int orc_assemble_tet_beuler(unsigned nblk, const unsigned *colind, const unsigned *rowptr,
+                            double *vcrs, double *vdia, double *vec_b, const double *myu_e,
+                            const double *lambda_e, double rho, const double *g, double dt,
+                            const double *xyz, const unsigned *tet, size_t ntet,
+                            const double *disp, const double *velo);
+
+/* vecxitrsol.cpp:312-325 XPlusAYBZ: X += alpha Y + beta Z where flag == 0. */
+void orc_xplusaybz(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                   double beta, const double *Z);
+/* vecxitrsol.cpp:327-342 XPlusAYBZCW: X += alpha Y + beta Z + gamma W. */
+void orc_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                     double beta, const double *Z, double gamma, const double *W);
+
 #ifdef __cplusplus
 }
 #endif

@@ -196,4 +196,77 @@ void ref_xplusay(double *X, size_t n, const int *flag, double alpha, const doubl
   std::memcpy(X, vx.data(), sizeof(double) * n);
 }
 
+// fem_solidlinear.h:331-379: the reference's Newmark-beta driver itself
+// (uniform myu, lambda).
+void ref_assemble_newmark(void *h, double *vec_b, double myu, double lambda, double rho,
+                          const double *g, double dt, double gamma, double beta,
+                          const double *xyz, size_t np, const unsigned *tet, size_t ntet,
+                          const double *disp, const double *velo, const double *acc) {
+  auto *s = static_cast<RefSys *>(h);
+  const double gg[3] = {g[0], g[1], g[2]};
+  dfm2::MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(s->A, vec_b, myu, lambda, rho, gg, dt,
+                                                      gamma, beta, xyz, np, tet, ntet, disp,
+                                                      velo, acc);
+}
+
+// The same element loop with per-element (myu, lambda): FetchData,
+// EMat_SolidLinear_NewmarkBeta_MeshTet3D (fem_solidlinear.cpp:437-490,
+// is_initial_iter = true) and Merge<4,4,3,3> in ascending element order.
+int ref_assemble_newmark_per_element(void *h, double *vec_b, const double *myu_e,
+                                     const double *lambda_e, double rho, const double *g,
+                                     double dt, double gamma, double beta, const double *xyz,
+                                     const unsigned *tet, size_t ntet, const double *disp,
+                                     const double *velo, const double *acc) {
+  auto *s = static_cast<RefSys *>(h);
+  s->buf.assign(s->A.nrowblk_, UINT_MAX);
+  int status = 0;
+  for (size_t iel = 0; iel < ntet; ++iel) {
+    const unsigned *aIP = tet + iel * 4;
+    double P[4][3], U[4][3], V[4][3], Ac[4][3];
+    dfm2::FetchData<4, 3>(P, aIP, xyz);
+    dfm2::FetchData<4, 3>(U, aIP, disp);
+    dfm2::FetchData<4, 3>(V, aIP, velo);
+    dfm2::FetchData<4, 3>(Ac, aIP, acc);
+    double eres[4][3], emat[4][4][3][3];
+    dfm2::EMat_SolidLinear_NewmarkBeta_MeshTet3D(eres, emat, myu_e[iel], lambda_e[iel], rho,
+                                                 g[0], g[1], g[2], dt, gamma, beta, U, V, Ac,
+                                                 P, true);
+    for (int ino = 0; ino < 4; ++ino) {
+      const unsigned ip = aIP[ino];
+      vec_b[ip * 3 + 0] += eres[ino][0];
+      vec_b[ip * 3 + 1] += eres[ino][1];
+      vec_b[ip * 3 + 2] += eres[ino][2];
+    }
+    if (!dfm2::Merge<4, 4, 3, 3, double>(s->A, aIP, aIP, emat, s->buf)) status = 1;
+  }
+  return status;
+}
+
+// fem_solidlinear.h:381-443: the reference's backward-Euler driver itself.
+void ref_assemble_beuler(void *h, double *vec_b, double myu, double lambda, double rho,
+                         const double *g, double dt, const double *xyz, size_t np,
+                         const unsigned *tet, size_t ntet, const double *disp,
+                         const double *velo) {
+  auto *s = static_cast<RefSys *>(h);
+  dfm2::MergeLinSys_SolidLinear_BEuler_MeshTet3D(s->A, vec_b, myu, lambda, rho, g, dt, xyz,
+                                                 static_cast<unsigned>(np), tet,
+                                                 static_cast<unsigned>(ntet), disp, velo);
+}
+
+// vecxitrsol.cpp:312-325 and :327-342
+void ref_xplusaybz(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                   double beta, const double *Z) {
+  std::vector<double> vx(X, X + n), vy(Y, Y + n), vz(Z, Z + n);
+  std::vector<int> vf(flag, flag + n);
+  dfm2::XPlusAYBZ(vx, n, vf, alpha, vy, beta, vz);
+  std::memcpy(X, vx.data(), sizeof(double) * n);
+}
+void ref_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const double *Y,
+                     double beta, const double *Z, double gamma, const double *W) {
+  std::vector<double> vx(X, X + n), vy(Y, Y + n), vz(Z, Z + n), vw(W, W + n);
+  std::vector<int> vf(flag, flag + n);
+  dfm2::XPlusAYBZCW(vx, n, vf, alpha, vy, beta, vz, gamma, vw);
+  std::memcpy(X, vx.data(), sizeof(double) * n);
+}
+
 }  // extern "C"

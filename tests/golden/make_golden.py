@@ -173,6 +173,37 @@ def main():
         meta["cases"][name] = summary(ref_case(prob, fill=fill))
         meta["cases"][name]["fill"] = fill
 
+    # --- time-stepping drivers (fem_solidlinear.h:331-443): assembly from a
+    # seeded state, then three steps of the device driver's call sequence
+    # (uniform Lame pair: the reference drivers' signature), 4^3 cube
+    for scheme in ("newmark", "beuler"):
+        pd = W.cube(4, seed=5)
+        pd.g = np.array([0.0, -0.5, -3.0])
+        run = oracle.DynamicRun(pd, scheme, dt=0.03, gamma_newmark=0.6, beta_newmark=0.36,
+                                use_ref=True)
+        run.u = (W.uniform(61, pd.ndof) - 0.5) * 1e-2
+        run.v = (W.uniform(62, pd.ndof) - 0.5) * 1e-1
+        run.a = (W.uniform(63, pd.ndof) - 0.5)
+        b = np.zeros(pd.ndof)
+        run.sys.set_matrix(*[np.zeros_like(m) for m in run.sys.matrix()])
+        if scheme == "newmark":
+            run.sys.assemble_newmark_uniform(b, pd.myu[0], pd.lam[0], pd.rho, pd.g, 0.03, 0.6,
+                                             0.36, pd.xyz, pd.tet, run.u, run.v, run.a)
+        else:
+            run.sys.assemble_beuler_uniform(b, pd.myu[0], pd.lam[0], pd.rho, pd.g, 0.03,
+                                            pd.xyz, pd.tet, run.u, run.v)
+        vc, vd = run.sys.matrix()
+        case = {"assembly": {"vcrs": sha(vc), "vdia": sha(vd), "b": sha(b)},
+                "seeds": [61, 62, 63], "dt": 0.03, "gamma": 0.6, "beta": 0.36,
+                "g": pd.g.tolist()}
+        run.u[:] = 0.0
+        run.v[:] = 0.0
+        run.a[:] = 0.0
+        its = [run.step(1e-8, 1000) for _ in range(3)]
+        case["steps"] = {"tol": 1e-8, "iterations": its, "u": sha(run.u), "v": sha(run.v),
+                         "a": sha(run.a), "u_maxabs": float(np.abs(run.u).max())}
+        meta["cases"][f"dyn_{scheme}_c4"] = case
+
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **arrays)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1)

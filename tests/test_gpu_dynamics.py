@@ -1,0 +1,117 @@
+"""Time-stepping drivers on the device through the C ABI, against the
+oracle (itself pinned to the reference: tests/test_oracle_dynamics.py).
+
+Bit-exact: Newmark-beta and backward-Euler assembly (A and b, per-element
+Lame pairs, nonzero u / v / a), XPlusAYBZ and XPlusAYBZCW.  Tolerance: the
+stepped state (the fused PCG reorders its dot products): per step the PCG
+iteration count within 1 of the oracle's, and after five steps u, v, a
+within rel-L2 1e-7 of the oracle's stepping at tolerance 1e-10.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from delfem2_amd import solver as S
+from delfem2_amd import workload as W
+from delfem2_amd.problem import DynamicSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda").to(dtype)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("scheme", ["newmark", "beuler"])
+@pytest.mark.parametrize("order", ["natural", "color4"])
+def test_dynamic_assembly_bit_exact(gpu, scheme, order):
+    p = W.cube(5, sigma=0.5, seed=31, ordering=order)
+    print(f"[dyn asm] {scheme} {order} ndof={p.ndof}", flush=True)
+    u, v, a = ((W.uniform(k, p.ndof) - 0.5) * s for k, s in ((41, 1e-2), (42, 1e-1), (43, 1.0)))
+    g = np.array([0.1, -0.4, -2.0])
+    rho, dt, beta, gamma = 1.2, 0.025, 0.3, 0.55
+    ind, ps = oracle.psup(p.tet, p.np_)
+    C = oracle.System(ind, ps)
+    bc = np.zeros(p.ndof)
+    G = DynamicSystem(p, scheme)
+    G.A.setZero()
+    bd = torch.zeros(p.ndof, dtype=torch.float64, device="cuda")
+    if scheme == "newmark":
+        assert C.assemble_newmark(bc, p.myu, p.lam, rho, g, dt, beta, p.xyz, p.tet, u, v, a) == 0
+        assert S.MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(
+            G.A, bd, G.myu, G.lam, rho, g, dt, gamma, beta, G.xyz, G.mesh, dev(u), dev(v), dev(a))
+    else:
+        assert C.assemble_beuler(bc, p.myu, p.lam, rho, g, dt, p.xyz, p.tet, u, v) == 0
+        assert S.MergeLinSys_SolidLinear_BEuler_MeshTet3D(G.A, bd, G.myu, G.lam, rho, g, dt,
+                                                          G.xyz, G.mesh, dev(u), dev(v))
+    vc, vd = G.A.download()
+    assert np.array_equal(bits(bd.cpu().numpy()), bits(bc))
+    assert np.array_equal(bits(vc), bits(C.vcrs))
+    assert np.array_equal(bits(vd), bits(C.vdia))
+    # scalar Lame pair (the reference signature) = per-element arrays of that pair
+    q = W.cube(3, seed=2, ordering=order)
+    H = DynamicSystem(q, scheme)
+    for per_element in (True, False):
+        H.A.setZero()
+        b = torch.zeros(q.ndof, dtype=torch.float64, device="cuda")
+        m = H.myu if per_element else float(q.myu[0])
+        l_ = H.lam if per_element else float(q.lam[0])
+        z = torch.zeros_like(b)
+        if scheme == "newmark":
+            S.MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(H.A, b, m, l_, rho, g, dt, gamma, beta,
+                                                            H.xyz, H.mesh, z, z, z)
+        else:
+            S.MergeLinSys_SolidLinear_BEuler_MeshTet3D(H.A, b, m, l_, rho, g, dt, H.xyz, H.mesh,
+                                                       z, z)
+        out = (b.cpu().numpy(),) + tuple(H.A.download())
+        if per_element:
+            ref = out
+        else:
+            assert all(np.array_equal(bits(x), bits(y)) for x, y in zip(out, ref))
+
+
+def test_xplus_bit_exact(gpu):
+    S.init(0)
+    n = 1001
+    X = W.uniform(1, n) - 0.5
+    Y, Z, Wv = (W.uniform(k, n) - 0.5 for k in (2, 3, 4))
+    flag = (np.arange(n) % 5 == 0).astype(np.int32)
+    Xc = X.copy()
+    oracle.xplusaybzcw(Xc, flag, 0.3, Y, -1.7, Z, 2.5, Wv)
+    Xd = dev(X)
+    S.XPlusAYBZCW(Xd, dev(flag, torch.int32), 0.3, dev(Y), -1.7, dev(Z), 2.5, dev(Wv))
+    assert np.array_equal(bits(Xd.cpu().numpy()), bits(Xc))
+    Xc = X.copy()
+    oracle.xplusaybz(Xc, flag, -0.6, Y, 1.1, Z)
+    Xd = dev(X)
+    S.XPlusAYBZ(Xd, dev(flag, torch.int32), -0.6, dev(Y), 1.1, dev(Z))
+    assert np.array_equal(bits(Xd.cpu().numpy()), bits(Xc))
+    S.XPlusAYBZ(Xd[:0], dev(flag, torch.int32)[:0], 1.0, Xd[:0], 1.0, Xd[:0])  # n = 0
+
+
+@pytest.mark.parametrize("scheme", ["newmark", "beuler"])
+def test_stepping_vs_oracle(gpu, scheme):
+    p = W.cube(8, sigma=0.5, seed=9, ordering="color4")
+    p.g = np.array([0.0, -0.5, -3.0])
+    print(f"[dyn steps] {scheme} ndof={p.ndof}", flush=True)
+    G = DynamicSystem(p, scheme, dt=0.03, gamma_newmark=0.6, beta_newmark=0.36)
+    C = oracle.DynamicRun(p, scheme, dt=0.03, gamma_newmark=0.6, beta_newmark=0.36)
+    for k in range(5):
+        i_d = G.step(1e-10, 2000)
+        i_c = C.step(1e-10, 2000)
+        assert abs(i_d - i_c) <= 1, (k, i_d, i_c)
+    for dvec, cvec in ((G.u, C.u), (G.v, C.v), (G.a, C.a)):
+        assert rel(dvec.cpu().numpy(), cvec) <= 1e-7
+    assert np.abs(C.u).max() > 1e-4  # the body moved
+    # fixed DOFs never move
+    fixed = p.flag != 0
+    assert np.all(G.u.cpu().numpy()[fixed] == 0.0)

@@ -9,7 +9,10 @@
  *   AddScaledVec / ScaleAndAddVec / AddMatVec / SolvePrecond / Dot
  *                        view_vectorx.h:61-109 (found by ADL)
  *   MergeLinSys_SolidLinear_Static_MeshTet3D  fem_solidlinear.h:192-237
+ *   MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D fem_solidlinear.h:331-379
+ *   MergeLinSys_SolidLinear_BEuler_MeshTet3D  fem_solidlinear.h:381-443
  *   setRHS_Zero, XPlusAY vecxitrsol.cpp:226-296
+ *   XPlusAYBZ, XPlusAYBZCW vecxitrsol.cpp:312-342
  *   Solve_PCG            lsitrsol.h:80-148, fused on the device
  *   Solve_CG             lsitrsol.h:36-74 (device views)
  *   LinearSystemSolver_BlockSparseILU  ls_solver_block_sparse_ilu.h:18-103
@@ -202,6 +205,46 @@ inline void setRHS_Zero(double *vec_b, const int *aBCFlag, size_t n, int iflag_n
 
 inline void XPlusAY(double *X, size_t nDoF, const int *aBCFlag, double alpha, const double *Y) {
   check(dfm2h_xplusay(X, nDoF, aBCFlag, alpha, Y));
+}
+
+// fem_solidlinear.h:331-379 (the reference's argument order; device arrays;
+// the mesh handle in place of nXYZ / aTet / nTet)
+inline bool MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(
+    CMatrixSparse &mat_A, double *vec_b, const double myu, const double lambda, const double rho,
+    const double g[3], const double dt_timestep, const double gamma_newmark,
+    const double beta_newmark, const double *aXYZ, const TetMesh &mesh, const double *aVal,
+    const double *aVelo, const double *aAcc, const double *myu_e = nullptr,
+    const double *lambda_e = nullptr) {
+  const int st = dfm2h_merge_linsys_solidlinear_newmark_tet(
+      mat_A.h, vec_b, myu_e, lambda_e, myu, lambda, rho, g, dt_timestep, gamma_newmark,
+      beta_newmark, aXYZ, mesh.h, aVal, aVelo, aAcc);
+  if (st == DFM2H_ERR_PATTERN) return false;
+  check(st);
+  return true;
+}
+
+// fem_solidlinear.h:381-443
+inline bool MergeLinSys_SolidLinear_BEuler_MeshTet3D(
+    CMatrixSparse &mat_A, double *vec_b, const double myu, const double lambda, const double rho,
+    const double *g, const double dt, const double *aXYZ, const TetMesh &mesh,
+    const double *aDisp, const double *aVelo, const double *myu_e = nullptr,
+    const double *lambda_e = nullptr) {
+  const int st = dfm2h_merge_linsys_solidlinear_beuler_tet(mat_A.h, vec_b, myu_e, lambda_e, myu,
+                                                           lambda, rho, g, dt, aXYZ, mesh.h,
+                                                           aDisp, aVelo);
+  if (st == DFM2H_ERR_PATTERN) return false;
+  check(st);
+  return true;
+}
+
+inline void XPlusAYBZ(double *X, size_t nDoF, const int *aBCFlag, double alpha, const double *Y,
+                      double beta, const double *Z) {
+  check(dfm2h_xplusaybz(X, nDoF, aBCFlag, alpha, Y, beta, Z));
+}
+
+inline void XPlusAYBZCW(double *X, size_t nDoF, const int *aBCFlag, double alpha, const double *Y,
+                        double beta, const double *Z, double gamma, const double *W) {
+  check(dfm2h_xplusaybzcw(X, nDoF, aBCFlag, alpha, Y, beta, Z, gamma, W));
 }
 
 // lsitrsol.h:36-74 Solve_CG (no preconditioner) on device views, the same

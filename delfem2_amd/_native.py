@@ -73,7 +73,11 @@ SIGNATURES = {
                                                    _d, _d, _vp, _vp, _vp, _vp, _vp],
     "dfm2h_merge_linsys_solidlinear_beuler_tet": [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _d,
                                                   _vp, _vp, _vp, _vp],
+    "dfm2h_merge_linsys_solidstiffwarp_beuler_tet": [_vp, _vp, _vp, _vp, _d, _d, _d, _vp,
+                                                     _d, _vp, _vp, _vp, _vp, _vp],
+    "dfm2h_rotation_at_points": [_vp, _vp, _vp, _vp],
     "dfm2h_xplusaybz": [_vp, _sz, _vp, _d, _vp, _d, _vp],
+    "dfm2h_solve_pbicgstab": [_vp, _vp, _vp, _vp, _d, _u, _vp, _sz, _psz],
     "dfm2h_xplusaybzcw": [_vp, _sz, _vp, _d, _vp, _d, _vp, _d, _vp],
     "dfm2h_ilu_create": [_pvp],
     "dfm2h_ilu_destroy": [_vp],

@@ -17,9 +17,10 @@ struct AsmArgs {
   const uint4 *tet;
   const unsigned *elsup_ind, *elsup;
   int *missing;  // set when an element edge has no slot in the pattern
-  // time-stepping drivers: step, Newmark beta, velocity and acceleration
+  // time-stepping drivers: step, Newmark beta, velocity and acceleration,
+  // node rotations (stiffness warping, 9 per node, row-major)
   double dt, beta_nb;
-  const double *velo, *acc;
+  const double *velo, *acc, *rot;
 };
 
 // Element kinds of the gather assembly:
@@ -29,7 +30,27 @@ struct AsmArgs {
 //             eres = f - K_e u - M_e a - dt K_e v - (0.5 dt^2) K_e a
 //   beuler    fem_solidlinear.h:381-443: K_e + rho vol / 4 / dt^2 on the
 //             diagonal entries of the diagonal blocks, eres = (f - K_e (u + dt v)) / dt
-enum AsmKind { kAsmStatic = 0, kAsmNewmark = 1, kAsmBEuler = 2 };
+//   stiffwarp fem_solidlinear.h:445-530: blocks R_i K_e,ij R_i^T (R_i: rotation
+//             of the row node), eres_i = (f - sum_j E_ij (P_j + u_j + dt v_j - R_i P_j)) / dt,
+//             the same lumped mass
+enum AsmKind { kAsmStatic = 0, kAsmNewmark = 1, kAsmBEuler = 2, kAsmStiffwarp = 3 };
+
+// E = R K R^T as femutil.cpp:107-116 MatMatTrans3 (K R^T) then :96-105 MatMat3
+__device__ inline void rotate_block(double E[9], const double K[9], const double R[9]) {
+  double t[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      t[i * 3 + j] = K[i * 3 + 0] * R[j * 3 + 0] + K[i * 3 + 1] * R[j * 3 + 1] +
+                     K[i * 3 + 2] * R[j * 3 + 2];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      E[i * 3 + j] = R[i * 3 + 0] * t[0 * 3 + j] + R[i * 3 + 1] * t[1 * 3 + j] +
+                     R[i * 3 + 2] * t[2 * 3 + j];
+}
 
 __device__ inline void load_node3(double out[3], const double *v, unsigned n) {
   out[0] = v ? v[(size_t)n * 3 + 0] : 0.0;
@@ -103,6 +124,17 @@ __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
       const double s = A.beta_nb * A.dt * A.dt;
 #pragma unroll
       for (int k = 0; k < 9; ++k) blk[k] = mb[k] + s * kb[k];
+    } else if (KIND == kAsmStiffwarp) {  // R_i K_e R_i^T, lumped mass on the diagonal block
+      double kb[9], Ri[9];
+      emat_block<true>(kb, di, dj, w, myu, lam);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Ri[k] = A.rot[(size_t)i * 9 + k];
+      rotate_block(blk, kb, Ri);
+      if (li == lj) {
+        blk[0] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
+        blk[4] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
+        blk[8] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
+      }
     } else {  // backward Euler: K_e, lumped mass on the diagonal block
       emat_block<true>(blk, di, dj, w, myu, lam);
       if (li == lj) {
@@ -163,6 +195,39 @@ __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
           er[a] -= kb[3 * a + 0] * (u[0] + A.dt * v[0]);
           er[a] -= kb[3 * a + 1] * (u[1] + A.dt * v[1]);
           er[a] -= kb[3 * a + 2] * (u[2] + A.dt * v[2]);
+        }
+      }
+      bacc[0] += er[0] / A.dt;
+      bacc[1] += er[1] / A.dt;
+      bacc[2] += er[2] / A.dt;
+    }
+    if (diag && KIND == kAsmStiffwarp) {
+      const double vol = G.vol;
+      double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
+                      vol * A.rho * A.g2 * 0.25};
+      const unsigned id[4] = {n.x, n.y, n.z, n.w};
+      double Ri[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Ri[k] = A.rot[(size_t)i * 9 + k];
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {
+        double kb[9], E[9], u[3], v[3];
+        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+        rotate_block(E, kb, Ri);
+        load_node3(u, A.disp, id[jn]);
+        load_node3(v, A.velo, id[jn]);
+        const double *Pj = P[jn];
+        const double Pj1[3] = {Ri[0] * Pj[0] + Ri[1] * Pj[1] + Ri[2] * Pj[2],
+                               Ri[3] * Pj[0] + Ri[4] * Pj[1] + Ri[5] * Pj[2],
+                               Ri[6] * Pj[0] + Ri[7] * Pj[1] + Ri[8] * Pj[2]};
+        const double uj1[3] = {Pj[0] + u[0] + A.dt * v[0] - Pj1[0],
+                               Pj[1] + u[1] + A.dt * v[1] - Pj1[1],
+                               Pj[2] + u[2] + A.dt * v[2] - Pj1[2]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          er[a] -= E[3 * a + 0] * uj1[0];
+          er[a] -= E[3 * a + 1] * uj1[1];
+          er[a] -= E[3 * a + 2] * uj1[2];
         }
       }
       bacc[0] += er[0] / A.dt;
@@ -312,6 +377,8 @@ static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu
   const dim3 grid(ceil_div(n, 256)), block(256);
   if (kind == kAsmNewmark)
     hipLaunchKernelGGL(k_assemble_tet_gather<kAsmNewmark>, grid, block, 0, stream(), a);
+  else if (kind == kAsmStiffwarp)
+    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmStiffwarp>, grid, block, 0, stream(), a);
   else if (kind == kAsmBEuler)
     hipLaunchKernelGGL(k_assemble_tet_gather<kAsmBEuler>, grid, block, 0, stream(), a);
   else
@@ -545,6 +612,24 @@ int dfm2h_merge_linsys_solidlinear_beuler_tet(dfm2h_bsr A, double *vec_b, const 
     a.velo = velo;
     a.dt = dt;
     run_assembly(kAsmBEuler, A, vec_b, myu_e, lambda_e, myu, lambda, rho, g, xyz, M, a);
+  });
+}
+
+int dfm2h_merge_linsys_solidstiffwarp_beuler_tet(dfm2h_bsr A, double *vec_b,
+                                                  const double *myu_e, const double *lambda_e,
+                                                  double myu, double lambda, double rho,
+                                                  const double *g, double dt, const double *xyz,
+                                                  dfm2h_tetmesh M, const double *disp,
+                                                  const double *velo, const double *rot) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(dt > 0.0, DFM2H_ERR_INVALID, "time step must be positive");
+    DFM2H_REQUIRE(rot, DFM2H_ERR_INVALID, "null rotation array");
+    AsmArgs a{};
+    a.disp = disp;
+    a.velo = velo;
+    a.rot = rot;
+    a.dt = dt;
+    run_assembly(kAsmStiffwarp, A, vec_b, myu_e, lambda_e, myu, lambda, rho, g, xyz, M, a);
   });
 }
 

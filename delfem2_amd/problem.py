@@ -117,12 +117,18 @@ class DynamicSystem(DeviceSystem):
     solves with Solve_PBiCGStab; the matrix is symmetric positive definite,
     so this driver uses the fused Solve_PCG (the solution agrees to the
     solver tolerance, the iterates do not).
+
+    ``scheme="stiffwarp"``: the step of
+    examples_oldgl_glfw/603_Fem3DStiffwarp/main.cpp:180-227 -- node
+    rotations (RotationAtMeshPoints), MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D
+    (R_i K_e R_i^T blocks, not symmetric), BCs, ILU(0), Solve_PBiCGStab for
+    dv, then the backward-Euler updates.
     """
 
     def __init__(self, prob: Problem, scheme: str = "newmark", dt: float = 0.03,
                  gamma_newmark: float = 0.6, beta_newmark: float = 0.36, device: int = 0,
                  fill_level: int = 0):
-        if scheme not in ("newmark", "beuler"):
+        if scheme not in ("newmark", "beuler", "stiffwarp"):
             raise ValueError(f"unknown scheme {scheme!r}")
         super().__init__(prob, device, fill_level)
         self.scheme = scheme
@@ -132,6 +138,7 @@ class DynamicSystem(DeviceSystem):
         self.u = torch.zeros_like(self.b)
         self.v = torch.zeros_like(self.b)
         self.a = torch.zeros_like(self.b)
+        self.R = torch.zeros(prob.np_ * 9, dtype=torch.float64, device=self.dev)
 
     def assemble(self, disp=None) -> bool:
         self.A.setZero()
@@ -141,6 +148,11 @@ class DynamicSystem(DeviceSystem):
             ok = S.MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(
                 self.A, self.b, self.myu, self.lam, p.rho, p.g, self.dt, self.gamma, self.beta,
                 self.xyz, self.mesh, self.u, self.v, self.a)
+        elif self.scheme == "stiffwarp":
+            S.RotationAtMeshPoints(self.A, self.xyz, self.u, self.R)
+            ok = S.MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D(
+                self.A, self.b, self.myu, self.lam, p.rho, p.g, self.dt, self.xyz, self.mesh,
+                self.u, self.v, self.R)
         else:
             ok = S.MergeLinSys_SolidLinear_BEuler_MeshTet3D(
                 self.A, self.b, self.myu, self.lam, p.rho, p.g, self.dt, self.xyz, self.mesh,
@@ -156,7 +168,10 @@ class DynamicSystem(DeviceSystem):
         if not self.factor():
             raise RuntimeError("ILU decomposition failed")
         x = torch.zeros_like(self.b)
-        hist = S.Solve_PCG(self.b, x, tol, max_nitr, self.A, self.ilu)
+        if self.scheme == "stiffwarp":  # history without a start entry
+            hist = [0.0] + S.Solve_PBiCGStab(self.b, x, tol, max_nitr, self.A, self.ilu)
+        else:
+            hist = S.Solve_PCG(self.b, x, tol, max_nitr, self.A, self.ilu)
         dt = self.dt
         if self.scheme == "newmark":
             S.XPlusAYBZCW(self.u, self.flag, dt, self.v, 0.5 * dt * dt, self.a,

@@ -6,8 +6,10 @@ Same names and argument meaning as the reference C++ API so that callers
 * ``CMatrixSparse``      -- ls_block_sparse.h:24-204 (3x3 blocks, diagonal kept apart)
 * ``CPreconditionerILU`` -- ls_ilu_block_sparse.h:21-60
 * ``MergeLinSys_SolidLinear_Static_MeshTet3D`` -- fem_solidlinear.h:192-237
-* ``MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D`` / ``..._BEuler_...`` --
-  fem_solidlinear.h:331-443
+* ``MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D`` / ``..._BEuler_...`` /
+  ``MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D`` -- fem_solidlinear.h:331-530
+* ``RotationAtMeshPoints`` -- 603_Fem3DStiffwarp/main.cpp:80-106
+* ``Solve_PBiCGStab`` -- vecxitrsol.h:350-437
 * ``XPlusAYBZ / XPlusAYBZCW`` -- vecxitrsol.cpp:312-342
 * ``Solve_PCG``          -- lsitrsol.h:80-148 (fused on the device)
 * ``Dot / AddScaledVec / ScaleAndAddVec / setRHS_Zero / XPlusAY`` --
@@ -370,6 +372,42 @@ def setRHS_Zero(vec_b: torch.Tensor, aBCFlag: torch.Tensor, iflag_nonzero: int =
 def XPlusAY(X: torch.Tensor, aBCFlag: torch.Tensor, alpha: float, Y: torch.Tensor) -> None:
     N.call("dfm2h_xplusay", _dptr(X), X.numel(), _dptr(aBCFlag, torch.int32), float(alpha),
            _dptr(Y))
+
+
+def RotationAtMeshPoints(mat_A: CMatrixSparse, aXYZ: torch.Tensor,
+                         aDisp: torch.Tensor | None, aR: torch.Tensor) -> None:
+    """603_Fem3DStiffwarp/main.cpp:80-106 over the matrix's one-ring
+    pattern: aR[9*np] receives each node's polar-decomposition rotation."""
+    N.call("dfm2h_rotation_at_points", mat_A.h, _dptr(aXYZ), _opt(aDisp), _dptr(aR))
+
+
+def MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D(mat_A: CMatrixSparse, vec_b: torch.Tensor,
+                                                myu, lam, rho: float, g, dt: float,
+                                                aXYZ: torch.Tensor, mesh: TetMesh,
+                                                aDisp: torch.Tensor, aVelo: torch.Tensor,
+                                                aR: torch.Tensor) -> bool:
+    """fem_solidlinear.h:445-530 on the device (rotations aR[9*np])."""
+    g = np.ascontiguousarray(np.asarray(g, np.float64).reshape(3))
+    pm, pl, sm, sl = _lame(myu, lam)
+    st = N.load().dfm2h_merge_linsys_solidstiffwarp_beuler_tet(
+        mat_A.h, _dptr(vec_b), pm, pl, sm, sl, float(rho), _hptr(g), float(dt), _dptr(aXYZ),
+        mesh.h, _opt(aDisp), _opt(aVelo), _dptr(aR))
+    if st == N.ERR_PATTERN:
+        return False
+    N.check(st)
+    return True
+
+
+def Solve_PBiCGStab(r_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float,
+                    max_niter: int, mat: CMatrixSparse, ilu: CPreconditionerILU) -> list[float]:
+    """vecxitrsol.h:350-437 on device vectors: r_vec holds b and receives the
+    residual, x_vec the solution; returns the relative-residual history."""
+    cap = int(max_niter) + 2
+    hist = np.zeros(cap)
+    nh = C.c_size_t(0)
+    N.call("dfm2h_solve_pbicgstab", mat.h, ilu.h, _dptr(r_vec), _dptr(x_vec),
+           float(conv_ratio_tol), int(max_niter), _hptr(hist), cap, C.byref(nh))
+    return hist[: nh.value].tolist()
 
 
 def XPlusAYBZ(X: torch.Tensor, aBCFlag: torch.Tensor, alpha: float, Y: torch.Tensor,

@@ -11,6 +11,8 @@
  *   MergeLinSys_SolidLinear_Static_MeshTet3D  fem_solidlinear.h:192-237
  *   MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D fem_solidlinear.h:331-379
  *   MergeLinSys_SolidLinear_BEuler_MeshTet3D  fem_solidlinear.h:381-443
+ *   MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D fem_solidlinear.h:445-530
+ *   Solve_PBiCGStab      vecxitrsol.h:350-437 (device arrays)
  *   setRHS_Zero, XPlusAY vecxitrsol.cpp:226-296
  *   XPlusAYBZ, XPlusAYBZCW vecxitrsol.cpp:312-342
  *   Solve_PCG            lsitrsol.h:80-148, fused on the device
@@ -235,6 +237,31 @@ inline bool MergeLinSys_SolidLinear_BEuler_MeshTet3D(
   if (st == DFM2H_ERR_PATTERN) return false;
   check(st);
   return true;
+}
+
+// fem_solidlinear.h:445-530 (aR: device rotations, 9 per node)
+inline bool MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D(
+    CMatrixSparse &mat_A, double *vec_b, const double myu, const double lambda, const double rho,
+    const double g[3], const double dt, const double *aXYZ, const TetMesh &mesh,
+    const double *aDisp, const double *aVelo, const double *aR, const double *myu_e = nullptr,
+    const double *lambda_e = nullptr) {
+  const int st = dfm2h_merge_linsys_solidstiffwarp_beuler_tet(
+      mat_A.h, vec_b, myu_e, lambda_e, myu, lambda, rho, g, dt, aXYZ, mesh.h, aDisp, aVelo, aR);
+  if (st == DFM2H_ERR_PATTERN) return false;
+  check(st);
+  return true;
+}
+
+// vecxitrsol.h:350-437 on device arrays (r_vec: b in, residual out)
+inline std::vector<double> Solve_PBiCGStab(double *r_vec, double *x_vec, double conv_ratio_tol,
+                                           unsigned int max_niter, const CMatrixSparse &mat,
+                                           const CPreconditionerILU &ilu) {
+  std::vector<double> hist(max_niter + 2);
+  size_t nh = 0;
+  check(dfm2h_solve_pbicgstab(mat.h, ilu.h, r_vec, x_vec, conv_ratio_tol, max_niter,
+                              hist.data(), hist.size(), &nh));
+  hist.resize(std::min(nh, hist.size()));
+  return hist;
 }
 
 inline void XPlusAYBZ(double *X, size_t nDoF, const int *aBCFlag, double alpha, const double *Y,

@@ -157,6 +157,25 @@ int dfm2h_merge_linsys_solidlinear_beuler_tet(
     const double *lambda_e_dev, double myu, double lambda, double rho,
     const double *g, double dt, const double *xyz_dev, dfm2h_tetmesh M,
     const double *disp_dev, const double *velo_dev);
+/* fem_solidlinear.h:445-530 MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D:
+ * blocks R_i K_e,ij R_i^T (R_i the rotation of the row node, rot_dev[9*np]
+ * row-major, e.g. from dfm2h_rotation_at_points) plus rho vol / 4 / dt^2 on
+ * the diagonal entries of the diagonal blocks; vec_b += (vol rho g / 4 -
+ * sum_j E_ij (P_j + u_j + dt v_j - R_i P_j)) / dt.  Bit-identical to the
+ * reference for the same rotations.  The matrix is not symmetric: solve with
+ * dfm2h_solve_pbicgstab. */
+int dfm2h_merge_linsys_solidstiffwarp_beuler_tet(
+    dfm2h_bsr A, double *vec_b_dev, const double *myu_e_dev,
+    const double *lambda_e_dev, double myu, double lambda, double rho,
+    const double *g, double dt, const double *xyz_dev, dfm2h_tetmesh M,
+    const double *disp_dev, const double *velo_dev, const double *rot_dev);
+/* 603_Fem3DStiffwarp/main.cpp:80-106 RotationAtMeshPoints: per node, the
+ * polar-decomposition rotation (svd3.cpp:291-305 GetRotPolarDecomp, 100
+ * Jacobi sweeps) of sum_j (p_j - p_i)(P_j - P_i)^T over the node's one-ring
+ * (A's pattern), p = P + u.  rot_dev[9*nblk] row-major; disp_dev NULL =
+ * zero.  Equal to the reference to rounding (device trigonometry). */
+int dfm2h_rotation_at_points(dfm2h_bsr A, const double *xyz_dev, const double *disp_dev,
+                             double *rot_dev);
 /* vecxitrsol.cpp:312-325 XPlusAYBZ: X += alpha*Y + beta*Z where flag == 0. */
 int dfm2h_xplusaybz(double *X_dev, size_t n, const int *flag_dev, double alpha,
                     const double *Y_dev, double beta, const double *Z_dev);
@@ -215,6 +234,18 @@ int dfm2h_vec_scale_and_add(double *y_dev, double beta, const double *x_dev,
 /* view_vectorx.h:44-47 deep-copy operator= and :49-51 setZero. */
 int dfm2h_vec_copy(double *dst_dev, const double *src_dev, size_t n);
 int dfm2h_vec_set_zero(double *x_dev, size_t n);
+
+/* ---- Solve_PBiCGStab (vecxitrsol.h:350-437) ------------------------------ */
+/* ILU-preconditioned BiCGStab for non-symmetric systems (the stiffness-warped
+ * backward-Euler matrix): x_dev set to zero, r_dev holds b on entry and the
+ * final residual on return.  hist receives the relative residual after each
+ * iteration (the reference's history: no entry for the start; a single
+ * entry |b| when |b|^2 < 1e-60); at most max_niter - 1 iterations, as there.
+ * The reference's recurrences operation for operation on the device; the
+ * dot products (deterministic tree order) are read by the host. */
+int dfm2h_solve_pbicgstab(dfm2h_bsr A, dfm2h_ilu P, double *r_dev, double *x_dev,
+                          double tol, unsigned max_niter, double *hist, size_t hist_cap,
+                          size_t *n_hist);
 
 /* ---- fused Solve_PCG (lsitrsol.h:80-148) ---------------------------------- */
 /* Solve_PCG(r, x, Pr, p, tol, max_nitr, A, ilu) with the preconditioner P

@@ -794,3 +794,327 @@ void orc_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const d
     X[i] += alpha * Y[i] + beta * Z[i] + gamma * W[i];
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Stiffness warping (603_Fem3DStiffwarp) and its solver. */
+
+/* vecxitrsol.h:350-437 Solve_PBiCGStab with the ILU preconditioner; DotX /
+ * Dot sequential (vecxitrsol.cpp:20-28, 61-68), AXPY y += a x
+ * (vecxitrsol.cpp:153-159).  Returns the history length. */
+size_t orc_solve_pbicgstab(double *r, double *x, double tol, unsigned max_niter,
+                           unsigned nblk, const unsigned *a_colind, const unsigned *a_rowptr,
+                           const double *a_vcrs, const double *a_vdia,
+                           const unsigned *l_colind, const unsigned *l_rowptr,
+                           const unsigned *l_diaind, const double *l_vcrs,
+                           const double *l_vdia, double *hist, size_t hist_cap) {
+  const size_t n = (size_t)nblk * 3;
+  size_t nh = 0;
+#define PUSH(v) do { if (nh < hist_cap) hist[nh] = (v); ++nh; } while (0)
+  for (size_t i = 0; i < n; ++i) x[i] = 0.0;
+  const double sq_ini = orc_dot(r, r, n);
+  if (sq_ini < 1.0e-60) {
+    PUSH(sqrt(sq_ini));
+    return nh;
+  }
+  const double sq_inv_ini = 1.0 / sq_ini;
+  double *s = (double *)calloc(n + 1, sizeof(double));
+  double *Ms = (double *)calloc(n + 1, sizeof(double));
+  double *AMs = (double *)calloc(n + 1, sizeof(double));
+  double *Mp = (double *)calloc(n + 1, sizeof(double));
+  double *AMp = (double *)calloc(n + 1, sizeof(double));
+  double *r0 = (double *)calloc(n + 1, sizeof(double));
+  double *p = (double *)calloc(n + 1, sizeof(double));
+  memcpy(r0, r, sizeof(double) * n);
+  memcpy(p, r, sizeof(double) * n);
+  for (unsigned it = 1; it < max_niter; ++it) {
+    memcpy(Mp, p, sizeof(double) * n);
+    orc_ilu_forward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, l_vdia, Mp);
+    orc_ilu_backward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, Mp);
+    const double r_r2 = orc_dot(r, r0, n);
+    orc_matvec(nblk, a_colind, a_rowptr, a_vcrs, a_vdia, AMp, 1.0, Mp, 0.0);
+    const double alpha = r_r2 / orc_dot(AMp, r0, n);
+    memcpy(s, r, sizeof(double) * n);
+    for (size_t i = 0; i < n; ++i) s[i] += -alpha * AMp[i];
+    memcpy(Ms, s, sizeof(double) * n);
+    orc_ilu_forward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, l_vdia, Ms);
+    orc_ilu_backward(nblk, l_colind, l_rowptr, l_diaind, l_vcrs, Ms);
+    orc_matvec(nblk, a_colind, a_rowptr, a_vcrs, a_vdia, AMs, 1.0, Ms, 0.0);
+    const double denominator = orc_dot(AMs, AMs, n);
+    const double numerator = orc_dot(s, AMs, n);
+    const double omega = numerator / denominator;
+    for (size_t i = 0; i < n; ++i) x[i] += alpha * Mp[i];
+    for (size_t i = 0; i < n; ++i) x[i] += omega * Ms[i];
+    memcpy(r, s, sizeof(double) * n);
+    for (size_t i = 0; i < n; ++i) r[i] += -omega * AMs[i];
+    const double conv = sqrt(orc_dot(r, r, n) * sq_inv_ini);
+    PUSH(conv);
+    if (conv < tol) break;
+    const double beta = orc_dot(r, r0, n) * alpha / (r_r2 * omega);
+    for (size_t i = 0; i < n; ++i) p[i] *= beta;
+    for (size_t i = 0; i < n; ++i) p[i] += 1.0 * r[i];
+    for (size_t i = 0; i < n; ++i) p[i] += -beta * omega * AMp[i];
+  }
+#undef PUSH
+  free(s); free(Ms); free(AMs); free(Mp); free(AMp); free(r0); free(p);
+  return nh;
+}
+
+/* svd3.cpp:114-194 EigenSym3: cyclic-by-largest Jacobi rotations on the
+ * Frobenius-normalised symmetric matrix sm = {m00, m11, m22, m12, m20, m01}. */
+static int eigen_sym3(double u[9], double l[3], const double sm[6], int nitr) {
+  u[0] = u[4] = u[8] = 1.0;
+  u[1] = u[2] = u[3] = u[5] = u[6] = u[7] = 0.0;
+  l[0] = l[1] = l[2] = 0.0;
+  const double dnrm = sm[0] * sm[0] + sm[1] * sm[1] + sm[2] * sm[2] +
+                      2 * (sm[3] * sm[3] + sm[4] * sm[4] + sm[5] * sm[5]);
+  if (dnrm < 1.0e-30) return 0;
+  const double scale = sqrt(dnrm);
+  const double invscl = 1.0 / scale;
+  double sms[6] = {sm[0] * invscl, sm[1] * invscl, sm[2] * invscl,
+                   sm[3] * invscl, sm[4] * invscl, sm[5] * invscl};
+  for (int itr = 0; itr < nitr; itr++) {
+    const double m[6] = {sms[0], sms[1], sms[2], sms[3], sms[4], sms[5]};
+    const double v[9] = {u[0], u[1], u[2], u[3], u[4], u[5], u[6], u[7], u[8]};
+    const double a12 = fabs(sms[3]), a20 = fabs(sms[4]), a01 = fabs(sms[5]);
+    if (a12 >= a20 && a12 >= a01) {
+      const double t = 0.5 * atan2(2 * m[3], m[2] - m[1]);
+      const double ct = cos(t), st = sin(t);
+      sms[1] = ct * ct * m[1] + st * st * m[2] - 2 * st * ct * m[3];
+      sms[2] = ct * ct * m[2] + st * st * m[1] + 2 * st * ct * m[3];
+      sms[3] = 0;
+      sms[4] = st * m[5] + ct * m[4];
+      sms[5] = ct * m[5] - st * m[4];
+      u[1] = +ct * v[1] - st * v[2];
+      u[2] = +st * v[1] + ct * v[2];
+      u[4] = +ct * v[4] - st * v[5];
+      u[5] = +st * v[4] + ct * v[5];
+      u[7] = +ct * v[7] - st * v[8];
+      u[8] = +st * v[7] + ct * v[8];
+    } else if (a20 >= a01 && a20 >= a12) {
+      const double t = 0.5 * atan2(2 * m[4], m[2] - m[0]);
+      const double ct = cos(t), st = sin(t);
+      sms[0] = ct * ct * m[0] + st * st * m[2] - 2 * st * ct * m[4];
+      sms[2] = ct * ct * m[2] + st * st * m[0] + 2 * st * ct * m[4];
+      sms[3] = st * m[5] + ct * m[3];
+      sms[4] = 0;
+      sms[5] = ct * m[5] - st * m[3];
+      u[0] = +ct * v[0] - st * v[2];
+      u[2] = +st * v[0] + ct * v[2];
+      u[3] = +ct * v[3] - st * v[5];
+      u[5] = +st * v[3] + ct * v[5];
+      u[6] = +ct * v[6] - st * v[8];
+      u[8] = +st * v[6] + ct * v[8];
+    } else {
+      const double t = 0.5 * atan2(2 * m[5], m[1] - m[0]);
+      const double ct = cos(t), st = sin(t);
+      sms[0] = ct * ct * m[0] + st * st * m[1] - 2 * st * ct * m[5];
+      sms[1] = ct * ct * m[1] + st * st * m[0] + 2 * st * ct * m[5];
+      sms[3] = st * m[4] + ct * m[3];
+      sms[4] = ct * m[4] - st * m[3];
+      sms[5] = 0;
+      u[0] = +ct * v[0] - st * v[1];
+      u[1] = +st * v[0] + ct * v[1];
+      u[3] = +ct * v[3] - st * v[4];
+      u[4] = +st * v[3] + ct * v[4];
+      u[6] = +ct * v[6] - st * v[7];
+      u[7] = +st * v[6] + ct * v[7];
+    }
+  }
+  l[0] = scale * sms[0];
+  l[1] = scale * sms[1];
+  l[2] = scale * sms[2];
+  return 1;
+}
+
+/* svd3.cpp:60-108 SortEigen3: descending eigenvalues, columns of V with them */
+static void sort_eigen3(double G[3], double V[9]) {
+  double t;
+  if (G[1] > G[0]) {
+    t = G[0]; G[0] = G[1]; G[1] = t;
+    t = V[0]; V[0] = V[1]; V[1] = t;
+    t = V[3]; V[3] = V[4]; V[4] = t;
+    t = V[6]; V[6] = V[7]; V[7] = t;
+  }
+  if (G[2] > G[1]) {
+    t = G[1]; G[1] = G[2]; G[2] = t;
+    t = V[1]; V[1] = V[2]; V[2] = t;
+    t = V[4]; V[4] = V[5]; V[5] = t;
+    t = V[7]; V[7] = V[8]; V[8] = t;
+  }
+  if (G[1] > G[0]) {
+    t = G[0]; G[0] = G[1]; G[1] = t;
+    t = V[0]; V[0] = V[1]; V[1] = t;
+    t = V[3]; V[3] = V[4]; V[4] = t;
+    t = V[6]; V[6] = V[7]; V[7] = t;
+  }
+}
+
+static void normalize3(double v[3]) {
+  const double len = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= len;
+  v[1] /= len;
+  v[2] /= len;
+}
+
+/* svd3.cpp:196-289 Svd3 (m = U G V^T; only U and V are used here) and
+ * svd3.cpp:291-305 GetRotPolarDecomp: R = U V^T. */
+void orc_rot_polar_decomp(double R[9], const double m[9], int nitr) {
+  double U[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, V[9], lv[3];
+  const double mtm[6] = {m[0] * m[0] + m[3] * m[3] + m[6] * m[6],
+                         m[1] * m[1] + m[4] * m[4] + m[7] * m[7],
+                         m[2] * m[2] + m[5] * m[5] + m[8] * m[8],
+                         m[1] * m[2] + m[4] * m[5] + m[7] * m[8],
+                         m[2] * m[0] + m[5] * m[3] + m[8] * m[6],
+                         m[0] * m[1] + m[3] * m[4] + m[6] * m[7]};
+  eigen_sym3(V, lv, mtm, nitr);
+  sort_eigen3(lv, V);
+  double u0[3] = {m[0] * V[0] + m[1] * V[3] + m[2] * V[6], m[3] * V[0] + m[4] * V[3] + m[5] * V[6],
+                  m[6] * V[0] + m[7] * V[3] + m[8] * V[6]};
+  double u1[3] = {m[0] * V[1] + m[1] * V[4] + m[2] * V[7], m[3] * V[1] + m[4] * V[4] + m[5] * V[7],
+                  m[6] * V[1] + m[7] * V[4] + m[8] * V[7]};
+  double u2[3] = {m[0] * V[2] + m[1] * V[5] + m[2] * V[8], m[3] * V[2] + m[4] * V[5] + m[5] * V[8],
+                  m[6] * V[2] + m[7] * V[5] + m[8] * V[8]};
+  const double det = +V[0] * V[4] * V[8] + V[3] * V[7] * V[2] + V[6] * V[1] * V[5] -
+                     V[0] * V[7] * V[5] - V[6] * V[4] * V[2] - V[3] * V[1] * V[8];
+  if (det < 0) {  /* right-handed V (the U entries flipped here are overwritten below) */
+    V[2] *= -1;
+    V[5] *= -1;
+    V[8] *= -1;
+  }
+  const double sql0 = u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2];
+  if (sql0 > 1.0e-20) {
+    normalize3(u0);
+    const double sql1 = u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2];
+    if (sql1 < 1.0e-20) {
+      u1[0] = 1.0 - fabs(u0[0]);
+      u1[1] = 1.0 - fabs(u0[1]);
+      u1[2] = 1.0 - fabs(u0[2]);
+    } else {
+      normalize3(u1);
+    }
+    const double d01 = u0[0] * u1[0] + u0[1] * u1[1] + u0[2] * u1[2];
+    u1[0] -= d01 * u0[0];
+    u1[1] -= d01 * u0[1];
+    u1[2] -= d01 * u0[2];
+    normalize3(u1);
+    double s2[3];
+    s2[0] = u0[1] * u1[2] - u1[1] * u0[2];
+    s2[1] = u0[2] * u1[0] - u1[2] * u0[0];
+    s2[2] = u0[0] * u1[1] - u1[0] * u0[1];
+    u2[0] = s2[0];
+    u2[1] = s2[1];
+    u2[2] = s2[2];
+  } else {
+    u0[0] = 1; u0[1] = 0; u0[2] = 0;
+    u1[0] = 0; u1[1] = 1; u1[2] = 0;
+    u2[0] = 0; u2[1] = 0; u2[2] = 1;
+  }
+  U[0] = u0[0]; U[1] = u1[0]; U[2] = u2[0];
+  U[3] = u0[1]; U[4] = u1[1]; U[5] = u2[1];
+  U[6] = u0[2]; U[7] = u1[2]; U[8] = u2[2];
+  /* svd3.cpp:46-58 MatMatT3: R = U V^T */
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      R[i * 3 + j] = U[i * 3 + 0] * V[j * 3 + 0] + U[i * 3 + 1] * V[j * 3 + 1] +
+                     U[i * 3 + 2] * V[j * 3 + 2];
+}
+
+/* examples_oldgl_glfw/603_Fem3DStiffwarp/main.cpp:80-106 RotationAtMeshPoints:
+ * per node, A = sum over the one-ring of (p_j - p_i) (P_j - P_i)^T with
+ * p = P + u, then R_i = GetRotPolarDecomp(A, 100). */
+void orc_rotation_at_points(double *R, const double *xyz, const double *disp,
+                            const unsigned *psup_ind, const unsigned *psup, size_t np) {
+  for (size_t ip = 0; ip < np; ++ip) {
+    const double Pi[3] = {xyz[ip * 3 + 0], xyz[ip * 3 + 1], xyz[ip * 3 + 2]};
+    const double pi[3] = {xyz[ip * 3 + 0] + disp[ip * 3 + 0], xyz[ip * 3 + 1] + disp[ip * 3 + 1],
+                          xyz[ip * 3 + 2] + disp[ip * 3 + 2]};
+    double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned jj = psup_ind[ip]; jj < psup_ind[ip + 1]; ++jj) {
+      const unsigned jp = psup[jj];
+      const double Pj[3] = {xyz[jp * 3 + 0], xyz[jp * 3 + 1], xyz[jp * 3 + 2]};
+      const double pj[3] = {xyz[jp * 3 + 0] + disp[jp * 3 + 0], xyz[jp * 3 + 1] + disp[jp * 3 + 1],
+                            xyz[jp * 3 + 2] + disp[jp * 3 + 2]};
+      const double a[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+      const double b[3] = {Pj[0] - Pi[0], Pj[1] - Pi[1], Pj[2] - Pi[2]};
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i * 3 + j] += a[i] * b[j];
+    }
+    orc_rot_polar_decomp(R + ip * 9, A, 100);
+  }
+}
+
+/* fem_solidlinear.h:445-530 MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D:
+ * emat_ij = R_i K_e,ij R_i^T (femutil.cpp:107-116 MatMatTrans3, then
+ * :96-105 MatMat3), eres_i = vol rho g / 4 - sum_j emat_ij (P_j + u_j +
+ * dt v_j - R_i P_j) (three subtractions per pair, femutil.cpp:88-94
+ * MatVec3), lumped rho vol / 4 / dt^2 on the diagonal, vec_b += eres / dt. */
+int orc_assemble_tet_stiffwarp(unsigned nblk, const unsigned *colind, const unsigned *rowptr,
+                               double *vcrs, double *vdia, double *vec_b, const double *myu_e,
+                               const double *lambda_e, double rho, const double *g, double dt,
+                               const double *xyz, const unsigned *tet, size_t ntet,
+                               const double *disp, const double *velo, const double *R) {
+  unsigned *buf = (unsigned *)malloc(sizeof(unsigned) * (nblk + 1));
+  for (unsigned i = 0; i < nblk; ++i) buf[i] = UINT_MAX;
+  int status = 0;
+  for (size_t iel = 0; iel < ntet; ++iel) {
+    const unsigned *ip = tet + iel * 4;
+    double P[4][3], u0[4][3], v0[4][3];
+    fetch43(P, ip, xyz);
+    const double vol = orc_tet_volume(P[0], P[1], P[2], P[3]);
+    double d[4][3], emat0[4][4][3][3], emat[4][4][3][3], eres[4][3];
+    orc_tet_dldx(d, P[0], P[1], P[2], P[3]);
+    set_emat_ls3(emat0, lambda_e[iel], myu_e[iel], (const double(*)[3])d, vol);
+    for (int ino = 0; ino < 4; ++ino) {
+      const double *Mi = R + (size_t)ip[ino] * 9;
+      for (int jno = 0; jno < 4; ++jno) {
+        const double *K = &emat0[ino][jno][0][0];
+        double mtmp[9];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            mtmp[i * 3 + j] = K[i * 3 + 0] * Mi[j * 3 + 0] + K[i * 3 + 1] * Mi[j * 3 + 1] +
+                              K[i * 3 + 2] * Mi[j * 3 + 2];
+        double *E = &emat[ino][jno][0][0];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            E[i * 3 + j] = Mi[i * 3 + 0] * mtmp[0 * 3 + j] + Mi[i * 3 + 1] * mtmp[1 * 3 + j] +
+                           Mi[i * 3 + 2] * mtmp[2 * 3 + j];
+      }
+    }
+    for (int k = 0; k < 4; ++k) {
+      eres[k][0] = vol * rho * g[0] * 0.25;
+      eres[k][1] = vol * rho * g[1] * 0.25;
+      eres[k][2] = vol * rho * g[2] * 0.25;
+    }
+    fetch43(u0, ip, disp);
+    fetch43(v0, ip, velo);
+    for (int ino = 0; ino < 4; ++ino) {
+      const double *Mi = R + (size_t)ip[ino] * 9;
+      for (int idim = 0; idim < 3; ++idim)
+        for (int jno = 0; jno < 4; ++jno) {
+          double Pj1[3];
+          Pj1[0] = Mi[0] * P[jno][0] + Mi[1] * P[jno][1] + Mi[2] * P[jno][2];
+          Pj1[1] = Mi[3] * P[jno][0] + Mi[4] * P[jno][1] + Mi[5] * P[jno][2];
+          Pj1[2] = Mi[6] * P[jno][0] + Mi[7] * P[jno][1] + Mi[8] * P[jno][2];
+          const double uj1[3] = {P[jno][0] + u0[jno][0] + dt * v0[jno][0] - Pj1[0],
+                                 P[jno][1] + u0[jno][1] + dt * v0[jno][1] - Pj1[1],
+                                 P[jno][2] + u0[jno][2] + dt * v0[jno][2] - Pj1[2]};
+          eres[ino][idim] -= emat[ino][jno][idim][0] * uj1[0];
+          eres[ino][idim] -= emat[ino][jno][idim][1] * uj1[1];
+          eres[ino][idim] -= emat[ino][jno][idim][2] * uj1[2];
+        }
+    }
+    for (int ino = 0; ino < 4; ++ino) {
+      emat[ino][ino][0][0] += rho * vol * 0.25 / (dt * dt);
+      emat[ino][ino][1][1] += rho * vol * 0.25 / (dt * dt);
+      emat[ino][ino][2][2] += rho * vol * 0.25 / (dt * dt);
+    }
+    for (int k = 0; k < 4; ++k) {
+      vec_b[ip[k] * 3 + 0] += eres[k][0] / dt;
+      vec_b[ip[k] * 3 + 1] += eres[k][1] / dt;
+      vec_b[ip[k] * 3 + 2] += eres[k][2] / dt;
+    }
+    status |= merge_tet(ip, (const double(*)[4][3][3])emat, colind, rowptr, vcrs, vdia, buf);
+  }
+  free(buf);
+  return status;
+}

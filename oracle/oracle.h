@@ -172,6 +172,35 @@ void orc_xplusaybz(double *X, size_t n, const int *flag, double alpha, const dou
 void orc_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const double *Y,
                      double beta, const double *Z, double gamma, const double *W);
 
+/* ---- stiffness warping (603_Fem3DStiffwarp) --------------------------------- */
+
+/* vecxitrsol.h:350-437 Solve_PBiCGStab with the ILU(0) factors; history =
+ * relative residual after each iteration (or |b| alone when |b|^2 < 1e-60).
+ * Returns the history length. */
+size_t orc_solve_pbicgstab(double *r, double *x, double tol, unsigned max_niter,
+                           unsigned nblk, const unsigned *a_colind, const unsigned *a_rowptr,
+                           const double *a_vcrs, const double *a_vdia,
+                           const unsigned *l_colind, const unsigned *l_rowptr,
+                           const unsigned *l_diaind, const double *l_vcrs,
+                           const double *l_vdia, double *hist, size_t hist_cap);
+
+/* svd3.cpp:114-305 GetRotPolarDecomp (EigenSym3 Jacobi sweeps on m^T m,
+ * SortEigen3, Svd3's U, R = U V^T).  m, R row-major 3x3. */
+void orc_rot_polar_decomp(double R[9], const double m[9], int nitr);
+
+/* 603_Fem3DStiffwarp/main.cpp:80-106 RotationAtMeshPoints: R[9*np]. */
+void orc_rotation_at_points(double *R, const double *xyz, const double *disp,
+                            const unsigned *psup_ind, const unsigned *psup, size_t np);
+
+/* fem_solidlinear.h:445-530 MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D
+ * (per-element Lame pair; node rotations R[9*np]).  Returns 0, or 1 on a
+ * missing block. */
+int orc_assemble_tet_stiffwarp(unsigned nblk, const unsigned *colind, const unsigned *rowptr,
+                               double *vcrs, double *vdia, double *vec_b, const double *myu_e,
+                               const double *lambda_e, double rho, const double *g, double dt,
+                               const double *xyz, const unsigned *tet, size_t ntet,
+                               const double *disp, const double *velo, const double *R);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,10 @@
 #include "delfem2/msh_topology_uniform.h"
 #include "delfem2/vecxitrsol.h"
 #include "delfem2/view_vectorx.h"
+#include "delfem2/svd3.h"
+#include "delfem2/mat3.h"
+#include "delfem2/mat3_funcs.h"
+#include "delfem2/vec3.h"
 
 namespace dfm2 = delfem2;
 
@@ -267,6 +271,55 @@ void ref_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const d
   std::vector<int> vf(flag, flag + n);
   dfm2::XPlusAYBZCW(vx, n, vf, alpha, vy, beta, vz, gamma, vw);
   std::memcpy(X, vx.data(), sizeof(double) * n);
+}
+
+// svd3.cpp:291-305
+void ref_rot_polar_decomp(double *R, const double *m, int nitr) {
+  dfm2::GetRotPolarDecomp(R, m, nitr);
+}
+
+// 603_Fem3DStiffwarp/main.cpp:80-106 (an example-file function: the same
+// steps written here over the reference's CVec3d / CMat3d /
+// Mat3_OuterProduct / GetRotPolarDecomp)
+void ref_rotation_at_points(double *R, const double *xyz, const double *disp,
+                            const unsigned *psup_ind, const unsigned *psup, size_t np) {
+  for (unsigned int ip = 0; ip < np; ++ip) {
+    dfm2::CVec3d Pi(xyz[ip * 3 + 0], xyz[ip * 3 + 1], xyz[ip * 3 + 2]);
+    dfm2::CVec3d pi(xyz[ip * 3 + 0] + disp[ip * 3 + 0], xyz[ip * 3 + 1] + disp[ip * 3 + 1],
+                    xyz[ip * 3 + 2] + disp[ip * 3 + 2]);
+    dfm2::CMat3d A;
+    A.setZero();
+    for (unsigned int jjp = psup_ind[ip]; jjp < psup_ind[ip + 1]; ++jjp) {
+      const unsigned int jp = psup[jjp];
+      dfm2::CVec3d Pj(xyz[jp * 3 + 0], xyz[jp * 3 + 1], xyz[jp * 3 + 2]);
+      dfm2::CVec3d pj(xyz[jp * 3 + 0] + disp[jp * 3 + 0], xyz[jp * 3 + 1] + disp[jp * 3 + 1],
+                      xyz[jp * 3 + 2] + disp[jp * 3 + 2]);
+      A += dfm2::Mat3_OuterProduct(pj - pi, Pj - Pi);
+    }
+    dfm2::GetRotPolarDecomp(R + ip * 9, A.p_, 100);
+  }
+}
+
+// fem_solidlinear.h:445-530: the reference's stiffness-warping driver itself.
+void ref_assemble_stiffwarp(void *h, double *vec_b, double myu, double lambda, double rho,
+                            const double *g, double dt, const double *xyz, size_t np,
+                            const unsigned *tet, size_t ntet, const double *disp,
+                            const double *velo, const double *R) {
+  auto *s = static_cast<RefSys *>(h);
+  const std::vector<double> aR(R, R + np * 9);
+  const double gg[3] = {g[0], g[1], g[2]};
+  dfm2::MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D(s->A, vec_b, myu, lambda, rho, gg, dt, xyz,
+                                                    static_cast<unsigned>(np), tet,
+                                                    static_cast<unsigned>(ntet), disp, velo, aR);
+}
+
+// vecxitrsol.h:350-437
+size_t ref_solve_pbicgstab(void *h, double *r, double *x, double tol, unsigned max_niter,
+                           double *hist, size_t hist_cap) {
+  auto *s = static_cast<RefSys *>(h);
+  std::vector<double> hv = dfm2::Solve_PBiCGStab(r, x, tol, max_niter, s->A, s->ilu);
+  for (size_t i = 0; i < hv.size() && i < hist_cap; ++i) hist[i] = hv[i];
+  return hv.size();
 }
 
 }  // extern "C"

@@ -176,7 +176,7 @@ def main():
     # --- time-stepping drivers (fem_solidlinear.h:331-443): assembly from a
     # seeded state, then three steps of the device driver's call sequence
     # (uniform Lame pair: the reference drivers' signature), 4^3 cube
-    for scheme in ("newmark", "beuler"):
+    for scheme in ("newmark", "beuler", "stiffwarp"):
         pd = W.cube(4, seed=5)
         pd.g = np.array([0.0, -0.5, -3.0])
         run = oracle.DynamicRun(pd, scheme, dt=0.03, gamma_newmark=0.6, beta_newmark=0.36,
@@ -189,9 +189,13 @@ def main():
         if scheme == "newmark":
             run.sys.assemble_newmark_uniform(b, pd.myu[0], pd.lam[0], pd.rho, pd.g, 0.03, 0.6,
                                              0.36, pd.xyz, pd.tet, run.u, run.v, run.a)
-        else:
+        elif scheme == "beuler":
             run.sys.assemble_beuler_uniform(b, pd.myu[0], pd.lam[0], pd.rho, pd.g, 0.03,
                                             pd.xyz, pd.tet, run.u, run.v)
+        else:
+            rot = oracle.rotation_at_points(pd.xyz, run.u, run.psup_ind, run.psup, use_ref=True)
+            run.sys.assemble_stiffwarp_uniform(b, pd.myu[0], pd.lam[0], pd.rho, pd.g, 0.03,
+                                               pd.xyz, pd.tet, run.u, run.v, rot)
         vc, vd = run.sys.matrix()
         case = {"assembly": {"vcrs": sha(vc), "vdia": sha(vd), "b": sha(b)},
                 "seeds": [61, 62, 63], "dt": 0.03, "gamma": 0.6, "beta": 0.36,
@@ -202,6 +206,8 @@ def main():
         its = [run.step(1e-8, 1000) for _ in range(3)]
         case["steps"] = {"tol": 1e-8, "iterations": its, "u": sha(run.u), "v": sha(run.v),
                          "a": sha(run.a), "u_maxabs": float(np.abs(run.u).max())}
+        if scheme == "stiffwarp":
+            case["assembly"]["rot"] = sha(rot)
         meta["cases"][f"dyn_{scheme}_c4"] = case
 
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **arrays)

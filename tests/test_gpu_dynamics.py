@@ -1,11 +1,12 @@
 """Time-stepping drivers on the device through the C ABI, against the
 oracle (itself pinned to the reference: tests/test_oracle_dynamics.py).
 
-Bit-exact: Newmark-beta and backward-Euler assembly (A and b, per-element
-Lame pairs, nonzero u / v / a), XPlusAYBZ and XPlusAYBZCW.  Tolerance: the
-stepped state (the fused PCG reorders its dot products): per step the PCG
-iteration count within 1 of the oracle's, and after five steps u, v, a
-within rel-L2 1e-7 of the oracle's stepping at tolerance 1e-10.
+Bit-exact: Newmark-beta, backward-Euler and stiffness-warped assembly (A and
+b, per-element Lame pairs, nonzero u / v / a; warped with the oracle's
+rotations), XPlusAYBZ and XPlusAYBZCW.  Tolerance: node rotations (device
+trigonometry) within 1e-12; BiCGStab and the stepped state (dot products in
+another order): iteration counts within 1 of the oracle's, x / u / v / a
+within rel-L2 1e-7 at solver tolerance 1e-10.
 """
 import numpy as np
 import pytest
@@ -115,3 +116,105 @@ def test_stepping_vs_oracle(gpu, scheme):
     # fixed DOFs never move
     fixed = p.flag != 0
     assert np.all(G.u.cpu().numpy()[fixed] == 0.0)
+
+
+def _rot(axis, ang):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+def test_rotations_vs_oracle(gpu):
+    """Device RotationAtMeshPoints vs the oracle (host libm trigonometry):
+    agreement to 1e-12; a rigid rotation Q gives Q at every node."""
+    p = W.cube(6, sigma=0.5, seed=3, ordering="color4")
+    print(f"[rot] ndof={p.ndof}", flush=True)
+    G = DynamicSystem(p, "stiffwarp")
+    X = p.xyz.reshape(-1, 3)
+    for u in ((W.uniform(71, p.ndof) - 0.5) * 0.2,
+              (((X @ _rot([0.2, 1.0, -0.3], 1.1).T) - X)).reshape(-1)):
+        Rd = torch.zeros(p.np_ * 9, dtype=torch.float64, device="cuda")
+        S.RotationAtMeshPoints(G.A, G.xyz, dev(u), Rd)
+        Rc = oracle.rotation_at_points(p.xyz, u, G.psup_ind, G.psup)
+        assert np.abs(Rd.cpu().numpy() - Rc).max() < 1e-12
+    assert np.abs(Rc.reshape(-1, 3, 3) - _rot([0.2, 1.0, -0.3], 1.1)[None]).max() < 1e-12
+
+
+@pytest.mark.parametrize("order", ["natural", "color4"])
+def test_stiffwarp_assembly_bit_exact(gpu, order):
+    """The warped assembly with the oracle's rotations uploaded: A and b
+    bit-exact (the rotations themselves agree to rounding, above)."""
+    p = W.cube(5, sigma=0.5, seed=33, ordering=order)
+    u, v = ((W.uniform(k, p.ndof) - 0.5) * s for k, s in ((44, 5e-2), (45, 1e-1)))
+    g = np.array([0.1, -0.4, -2.0])
+    rho, dt = 1.2, 0.025
+    ind, ps = oracle.psup(p.tet, p.np_)
+    R = oracle.rotation_at_points(p.xyz, u, ind, ps)
+    C = oracle.System(ind, ps)
+    bc = np.zeros(p.ndof)
+    assert C.assemble_stiffwarp(bc, p.myu, p.lam, rho, g, dt, p.xyz, p.tet, u, v, R) == 0
+    G = DynamicSystem(p, "stiffwarp")
+    G.A.setZero()
+    bd = torch.zeros(p.ndof, dtype=torch.float64, device="cuda")
+    assert S.MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D(G.A, bd, G.myu, G.lam, rho, g, dt, G.xyz,
+                                                         G.mesh, dev(u), dev(v), dev(R))
+    vc, vd = G.A.download()
+    assert np.array_equal(bits(bd.cpu().numpy()), bits(bc))
+    assert np.array_equal(bits(vc), bits(C.vcrs))
+    assert np.array_equal(bits(vd), bits(C.vdia))
+
+
+def test_bicgstab_vs_oracle(gpu):
+    """Device Solve_PBiCGStab on the warped (non-symmetric) system vs the
+    oracle's: iteration counts within 1, x rel-L2 <= 1e-7, true residual."""
+    p = W.cube(8, sigma=0.5, seed=6, ordering="color4")
+    X = p.xyz.reshape(-1, 3)
+    u = (((X @ _rot([1.0, 0.2, -0.4], 0.9).T) - X) * 0.5).reshape(-1)
+    v = (W.uniform(9, p.ndof) - 0.5) * 0.1
+    g = np.array([0.0, 0.0, -3.0])
+    ind, ps = oracle.psup(p.tet, p.np_)
+    R = oracle.rotation_at_points(p.xyz, u, ind, ps)
+    C = oracle.System(ind, ps)
+    b = np.zeros(p.ndof)
+    C.assemble_stiffwarp(b, p.myu, p.lam, 1.0, g, 0.03, p.xyz, p.tet, u, v, R)
+    C.set_fixed_bc(p.flag)
+    oracle.set_rhs_zero(b, p.flag)
+    assert C.factor()
+    xc, hc = C.solve_pbicgstab(b.copy(), 1e-10, 1000)
+    G = DynamicSystem(p, "stiffwarp")
+    G.A.setZero()
+    bd = torch.zeros(p.ndof, dtype=torch.float64, device="cuda")
+    S.MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D(G.A, bd, G.myu, G.lam, 1.0, g, 0.03, G.xyz,
+                                                  G.mesh, dev(u), dev(v), dev(R))
+    G.A.SetFixedBC(G.flag)
+    S.setRHS_Zero(bd, G.flag, 0)
+    assert np.array_equal(bits(bd.cpu().numpy()), bits(b))
+    assert G.factor()
+    r = bd.clone()
+    x = torch.zeros_like(bd)
+    hd = S.Solve_PBiCGStab(r, x, 1e-10, 1000, G.A, G.ilu)
+    print(f"[bicgstab] iterations device {len(hd)} oracle {len(hc)}", flush=True)
+    assert abs(len(hd) - len(hc)) <= 1 and hd[-1] < 1e-10
+    assert rel(x.cpu().numpy(), xc) <= 1e-7
+    res = bd.clone()
+    G.A.MatVec(res, -1.0, x, 1.0)
+    assert torch.linalg.norm(res).item() < 1e-9 * torch.linalg.norm(bd).item()
+    # zero right-hand side: x = 0 and the history is |b| alone
+    z = torch.zeros_like(bd)
+    hz = S.Solve_PBiCGStab(z, x, 1e-10, 1000, G.A, G.ilu)
+    assert hz == [0.0] and torch.count_nonzero(x).item() == 0
+
+
+def test_stiffwarp_stepping_vs_oracle(gpu):
+    p = W.cube(6, sigma=0.5, seed=9, ordering="color4")
+    p.g = np.array([0.0, -2.0, -3.0])
+    print(f"[stiffwarp steps] ndof={p.ndof}", flush=True)
+    G = DynamicSystem(p, "stiffwarp", dt=0.03)
+    Cr = oracle.DynamicRun(p, "stiffwarp", dt=0.03)
+    for k in range(5):
+        i_d = G.step(1e-10, 1000)
+        i_c = Cr.step(1e-10, 1000)
+        assert abs(i_d - i_c) <= 1, (k, i_d, i_c)
+    for dvec, cvec in ((G.u, Cr.u), (G.v, Cr.v)):
+        assert rel(dvec.cpu().numpy(), cvec) <= 1e-7
+    assert np.abs(Cr.u).max() > 1e-4

@@ -40,7 +40,7 @@ def golden_problem(case):
     return p
 
 
-@pytest.mark.parametrize("scheme", ["newmark", "beuler"])
+@pytest.mark.parametrize("scheme", ["newmark", "beuler", "stiffwarp"])
 def test_assembly_and_steps_golden(scheme):
     case = GOLD[f"dyn_{scheme}_c4"]
     p = golden_problem(case)
@@ -55,9 +55,14 @@ def test_assembly_and_steps_golden(scheme):
     if scheme == "newmark":
         assert C.assemble_newmark(b, p.myu, p.lam, p.rho, p.g, case["dt"], case["beta"], p.xyz,
                                   p.tet, u, v, a) == 0
-    else:
+    elif scheme == "beuler":
         assert C.assemble_beuler(b, p.myu, p.lam, p.rho, p.g, case["dt"], p.xyz, p.tet, u,
                                  v) == 0
+    else:
+        rot = oracle.rotation_at_points(p.xyz, u, run.psup_ind, run.psup)
+        assert sha(rot) == case["assembly"]["rot"]
+        assert C.assemble_stiffwarp(b, p.myu, p.lam, p.rho, p.g, case["dt"], p.xyz, p.tet, u, v,
+                                    rot) == 0
     assert sha(C.vcrs) == case["assembly"]["vcrs"]
     assert sha(C.vdia) == case["assembly"]["vdia"]
     assert sha(b) == case["assembly"]["b"]
@@ -200,3 +205,99 @@ def test_newmark_step_solves_its_system():
     dt = 0.03
     assert np.allclose(run.v[free], (dt * 0.6) * x[free], rtol=1e-14)
     assert np.allclose(run.u[free], ((dt * dt) * 0.36) * x[free], rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# stiffness warping (603_Fem3DStiffwarp): rotations, assembly, BiCGStab
+
+
+def _rot(axis, ang):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+
+
+def test_polar_decomposition_closed_form():
+    """m = Q S (S symmetric positive definite) has rotation Q; mirrored and
+    near-singular inputs still give proper rotations."""
+    rng = np.random.default_rng(3)
+    for k in range(50):
+        Q = _rot(rng.standard_normal(3), rng.uniform(-3, 3))
+        B = rng.standard_normal((3, 3))
+        Sm = B @ B.T + 0.1 * np.eye(3)
+        R = oracle.rot_polar_decomp((Q @ Sm).reshape(9)).reshape(3, 3)
+        assert np.abs(R - Q).max() < 1e-10, k
+        if oracle.ref_available():
+            Rr = oracle.rot_polar_decomp((Q @ Sm).reshape(9), use_ref=True)
+            assert np.array_equal(bits(R.reshape(9)), bits(Rr))
+    R = oracle.rot_polar_decomp(np.diag([1.0, 1.0, -1.0]).reshape(9)).reshape(3, 3)
+    assert abs(np.linalg.det(R) - 1) < 1e-12 and np.abs(R @ R.T - np.eye(3)).max() < 1e-12
+
+
+def test_rotations_and_warped_residual_of_a_rigid_rotation():
+    """u = (Q - I) X rotates the body rigidly: every node's rotation is Q,
+    and with g = 0, v = 0 the warped elastic residual vanishes (a transposed
+    rotation or a dropped R_i P_j term would not)."""
+    p = W.cube(3, sigma=0.5, seed=4)
+    Q = _rot([0.3, -1.0, 0.5], 0.7)
+    X = p.xyz.reshape(-1, 3)
+    u = ((X @ Q.T) - X).reshape(-1)
+    ind, ps = oracle.psup(p.tet, p.np_)
+    R = oracle.rotation_at_points(p.xyz, u, ind, ps).reshape(-1, 3, 3)
+    assert np.abs(R - Q[None]).max() < 1e-12
+    C = _system(p)
+    b = np.zeros(p.ndof)
+    assert C.assemble_stiffwarp(b, p.myu, p.lam, 1.0, np.zeros(3), 0.03, p.xyz, p.tet, u,
+                                np.zeros(p.ndof), R.reshape(-1)) == 0
+    assert np.abs(b).max() < 1e-10
+    # the same displacement without warping (backward Euler) is not force-free
+    C2, b2 = _system(p), np.zeros(p.ndof)
+    C2.assemble_beuler(b2, p.myu, p.lam, 1.0, np.zeros(3), 0.03, p.xyz, p.tet, u,
+                       np.zeros(p.ndof))
+    assert np.abs(b2).max() > 1e-3
+    # R = I: the warped blocks are the backward-Euler ones
+    C3, C4 = _system(p), _system(p)
+    b3, b4 = np.zeros(p.ndof), np.zeros(p.ndof)
+    eye = np.tile(np.eye(3).reshape(9), p.np_)
+    v = W.uniform(5, p.ndof) - 0.5
+    C3.assemble_stiffwarp(b3, p.myu, p.lam, 1.1, [0, 0, -1.0], 0.02, p.xyz, p.tet, u * 0.1, v, eye)
+    C4.assemble_beuler(b4, p.myu, p.lam, 1.1, [0, 0, -1.0], 0.02, p.xyz, p.tet, u * 0.1, v)
+    assert np.allclose(C3.vcrs, C4.vcrs, rtol=1e-14, atol=1e-14 * np.abs(C4.vcrs).max())
+    assert np.allclose(b3, b4, rtol=1e-12, atol=1e-12 * np.abs(b4).max())
+
+
+def test_bicgstab_nonsymmetric_system():
+    """BiCGStab on the warped (non-symmetric) matrix: converges, and the
+    restatement is the reference's bit for bit (when built here)."""
+    p = W.cube(4, sigma=0.5, seed=6)
+    Q = _rot([1.0, 0.2, -0.4], 0.9)
+    X = p.xyz.reshape(-1, 3)
+    u = (((X @ Q.T) - X) * 0.5).reshape(-1)
+    ind, ps = oracle.psup(p.tet, p.np_)
+    R = oracle.rotation_at_points(p.xyz, u, ind, ps)
+    v = (W.uniform(9, p.ndof) - 0.5) * 0.1
+    g = np.array([0.0, 0.0, -3.0])
+    C = oracle.System(ind, ps)
+    b = np.zeros(p.ndof)
+    C.assemble_stiffwarp(b, p.myu, p.lam, 1.0, g, 0.03, p.xyz, p.tet, u, v, R)
+    C.set_fixed_bc(p.flag)
+    oracle.set_rhs_zero(b, p.flag)
+    assert C.factor()
+    y = np.zeros(p.ndof)
+    C.matvec(y, 1.0, v, 0.0)
+    yt = np.zeros(p.ndof)
+    C.matvec(yt, 1.0, v[::-1].copy(), 0.0)
+    # not symmetric: <A v, w> != <v, A w>
+    assert abs(y @ v[::-1] - v @ yt) > 1e-8 * abs(y @ v[::-1])
+    r = b.copy()
+    x, hist = C.solve_pbicgstab(r, 1e-10, 500)
+    assert hist[-1] < 1e-10 and len(hist) < 100
+    res = b.copy()
+    C.matvec(res, -1.0, x, 1.0)
+    assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(b)
+    if oracle.ref_available():
+        Rf = oracle.RefSystem(ind, ps)
+        Rf.set_matrix(C.vcrs, C.vdia)
+        assert Rf.factor()
+        x2, h2 = Rf.solve_pbicgstab(b.copy(), 1e-10, 500)
+        assert np.array_equal(bits(x), bits(x2)) and np.array_equal(bits(hist), bits(h2))

@@ -108,6 +108,24 @@ __device__ inline SliceLane lane_of(unsigned s) {
   return L;
 }
 
+// A slice's mesh, lane L's row, the row's block count and the slice's slot
+// range, all loaded in one round (no load waits for another; a padding row
+// has row ~0u and length 0).
+struct RowMeta {
+  unsigned mesh, i, len, kmax, off;
+};
+
+template <class P>
+__device__ inline RowMeta row_meta(const P &pl, const SliceLane &L, unsigned s) {
+  RowMeta R;
+  R.mesh = pl.mesh[s];
+  R.i = L.c < 3 ? pl.row[s * kSliceRows + L.r] : ~0u;
+  R.len = L.c < 3 ? pl.len[s * kSliceRows + L.r] : 0u;
+  R.kmax = pl.kmax[s];
+  R.off = pl.off[s];
+  return R;
+}
+
 // slices [slice0, slice0 + nsl): wavefront w of the grid takes SPW
 // consecutive ones.  (Numbering the workgroups per XCD, as the
 // single-system solver can, measured 3% slower here: the meshes converge at
@@ -189,21 +207,23 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_spmv(BArgs<T> a, unsigned n
   for (int k = 0; k < V::SPW; ++k) {
     const unsigned s = s0 + k;
     if (s >= nsl) return;
-    const MeshScal *S = a.M + a.A.mesh[s];
+    const SliceLane L = lane_of<V>(s);
+    const RowMeta R = row_meta(a.A, L, s);
+    const MeshScal *S = a.M + R.mesh;
     if (S->done) continue;
     const bool refine = sizeof(T) == 4 && S->upd;  // wavefront-uniform
-    const SliceLane L = lane_of<V>(s);
-    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    const unsigned i = R.i;
     double part = 0.0, part2 = 0.0, part3 = 0.0;
     if (i != ~0u) {
-      T acc = sliced_row_sum<T, false, V::template US<T>(), kVSB>(
-          (T)0, (T)1, L, a.A.len[s * kSliceRows + L.r], a.A.kmax[s], a.A.off[s], a.A.col,
-          a.a_val, a.p);
+      // the row's own node and diagonal block first: off the gathers' chain
       const Trip<T> pi = load_node<T, kVSB>(a.p, i);
-      acc += dia_row_dot(a.a_dia, L, pi.a, pi.b, pi.c);
+      const Trip<T> d = load_trip<T>(a.a_dia, s, L.lane);
+      T acc = sliced_row_sum<T, false, V::template US<T>(), kVSB>(
+          (T)0, (T)1, L, R.len, R.kmax, R.off, a.A.col, a.a_val, a.p);
+      acc += (T)d.a * pi.a + (T)d.b * pi.b + (T)d.c * pi.c;  // dia_row_dot's products
       const size_t e = kVSB * (size_t)i + L.c;
       a.Pr[e] = acc;
-      part = (double)a.p[e] * (double)acc;
+      part = (double)(L.c == 0 ? pi.a : L.c == 1 ? pi.b : pi.c) * (double)acc;
       if constexpr (sizeof(T) == 4)
         if (refine) part2 = batch_residual_row<V>(a, L, s, i, part3);
     }
@@ -229,12 +249,13 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_update(BArgs<T> a, unsigned
   for (int k = 0; k < V::SPW; ++k) {
     const unsigned s = s0 + k;
     if (s >= nsl) return;
-    const MeshScal *S = a.M + a.A.mesh[s];
+    const SliceLane L = lane_of<V>(s);
+    const unsigned mesh = a.A.mesh[s];
+    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
+    const MeshScal *S = a.M + mesh;
     if (S->done) continue;
     const double alpha = S->alpha;
     const bool fold = sizeof(T) == 4 && S->upd;
-    const SliceLane L = lane_of<V>(s);
-    const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
     double part = 0.0, srp = 0.0;
     T rn = 0;
     if (i != ~0u) {
@@ -278,26 +299,27 @@ kb_fwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
   for (int k = 0; k < V::SPW; ++k) {
     const unsigned s = s0 + k;
     if (s >= slice0 + nsl) return;
-    if (a.M[a.F.mesh[s]].done) continue;
     const SliceLane L = lane_of<V>(s);
-    const unsigned i = L.c < 3 ? a.F.row[s * kSliceRows + L.r] : ~0u;
+    const RowMeta R = row_meta(a.F, L, s);
+    if (a.M[R.mesh].done) continue;
+    const unsigned i = R.i;
     const bool valid = i != ~0u;
-    T t = 0;
-    if (valid) {
-      t = a.r[kVSB * (size_t)i + L.c];
-      const unsigned K = a.F.len[s * kSliceRows + L.r];
+    T t = 0, rv = 0;
+    Trip<T> d{};
+    if (valid) {  // r_i and the D_i^-1 row first: off the gathers' chain
+      rv = a.r[kVSB * (size_t)i + L.c];
+      d = load_trip<T, sweep_nt<T>()>(a.f_dia, s, L.lane);
       t = sliced_row_sum_m<T, T, true, V::UW, kVSB, sweep_nt<T>()>(
-          t, (T)1, L, K, K, a.F.kmax[s], a.F.off[s], a.F.col, a.f_val, a.Pr);
+          rv, (T)1, L, R.len, R.len, R.kmax, R.off, a.F.col, a.f_val, a.Pr);
     }
     const T t0 = __shfl(t, (int)L.r, kWave);
     const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
     const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
     double part = 0.0;
     if (valid) {
-      const T out = dia_row_dot_m<T, T, sweep_nt<T>()>(a.f_dia, L, t0, t1, t2);
+      const T out = (T)d.a * t0 + (T)d.b * t1 + (T)d.c * t2;  // dia_row_dot_m's products
       a.Pr[kVSB * (size_t)i + L.c] = out;
-      if (PART && ((a.fumask[s] >> L.r) & 1u))
-        part = (double)a.r[kVSB * (size_t)i + L.c] * (double)out;
+      if (PART && ((a.fumask[s] >> L.r) & 1u)) part = (double)rv * (double)out;
     }
     if (PART) {
       part = wave_reduce_sum(part);
@@ -314,18 +336,19 @@ kb_bwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
   for (int k = 0; k < V::SPW; ++k) {
     const unsigned s = s0 + k;
     if (s >= slice0 + nsl) return;
-    if (a.M[a.B.mesh[s]].done) continue;
     const SliceLane L = lane_of<V>(s);
-    const unsigned i = L.c < 3 ? a.B.row[s * kSliceRows + L.r] : ~0u;
+    const RowMeta R = row_meta(a.B, L, s);
+    if (a.M[R.mesh].done) continue;
+    const unsigned i = R.i;
     double part = 0.0;
     if (i != ~0u) {
       const size_t e = kVSB * (size_t)i + L.c;
+      const T rv = a.r[e];
       T t = a.Pr[e];
-      const unsigned K = a.B.len[s * kSliceRows + L.r];
       t = sliced_row_sum_m<T, T, true, V::UW, kVSB, sweep_nt<T>()>(
-          t, (T)1, L, K, K, a.B.kmax[s], a.B.off[s], a.B.col, a.b_val, a.Pr);
+          t, (T)1, L, R.len, R.len, R.kmax, R.off, a.B.col, a.b_val, a.Pr);
       a.Pr[e] = t;
-      part = (double)a.r[e] * (double)t;
+      part = (double)rv * (double)t;
     }
     part = wave_reduce_sum(part);
     if (L.lane == 0) a.partB[s] = part;

@@ -300,6 +300,74 @@ __device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigne
   return acc;
 }
 
+// One load round of a row sum: column indices and value triples of the
+// slots k0, k0 + H, ..., k0 + (U - 1) H that lie below KL.
+template <typename M, int U>
+struct SlotRound {
+  unsigned j[U];
+  Trip<M> v[U];
+};
+
+template <typename M, int U, int H, bool NT = false>
+__device__ inline void load_slot_round(SlotRound<M, U> &R, const SliceLane &L, unsigned k0,
+                                       unsigned KL, unsigned q0,
+                                       const unsigned *__restrict__ col,
+                                       const M *__restrict__ val) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const unsigned k = k0 + H * u;
+    if (k < KL) {
+      const size_t q = q0 + k;
+      R.j[u] = col[q * kSliceRows + L.r];
+      R.v[u] = load_trip<M, NT>(val, q, L.lane);
+    }
+  }
+}
+
+// Gathers and products of one loaded round (slots k0 + H u), accumulated
+// into acc as sliced_row_sum_m does.
+template <typename T, typename M, bool SUB, int U, int H, int VS>
+__device__ inline T slot_round_sum(T acc, T alpha, const SlotRound<M, U> &R, unsigned k0,
+                                   unsigned K, unsigned KL, const T *x) {
+  Trip<T> xv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (k0 + H * u < KL) xv[u] = load_node<T, VS>(x, R.j[u]);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (k0 + H * u < K) {
+      const T d = (T)R.v[u].a * xv[u].a + (T)R.v[u].b * xv[u].b + (T)R.v[u].c * xv[u].c;
+      if (SUB)
+        acc -= d;
+      else
+        acc += alpha * d;
+    }
+  }
+  return acc;
+}
+
+// sliced_row_sum_m over the slots h, h + H, h + 2H, ... (part h of a row
+// split into H interleaved parts; H = 1: the whole row), whose first round
+// the caller has already requested with load_slot_round(R0, L, h, ...) --
+// so it can issue those loads before anything they need not wait for.  Same
+// products, same order as sliced_row_sum_m.  Later rounds load into fresh
+// registers (a round carried through the loop makes the compiler wait for
+// every outstanding load before each reload).
+template <typename T, typename M, bool SUB, int U, int H, int VS, bool NT = false>
+__device__ inline T sliced_row_sum_pre(T acc, T alpha, const SlotRound<M, U> &R0,
+                                       const SliceLane &L, unsigned h, unsigned K, unsigned KL,
+                                       unsigned kmax, unsigned q0,
+                                       const unsigned *__restrict__ col,
+                                       const M *__restrict__ val, const T *x) {
+  if (h < kmax) acc = slot_round_sum<T, M, SUB, U, H, VS>(acc, alpha, R0, h, K, KL, x);
+  for (unsigned k0 = h + H * U; k0 < kmax; k0 += H * U) {
+    SlotRound<M, U> R;
+    load_slot_round<M, U, H, NT>(R, L, k0, KL, q0, col, val);
+    acc = slot_round_sum<T, M, SUB, U, H, VS>(acc, alpha, R, k0, K, KL, x);
+  }
+  return acc;
+}
+
 template <typename T, bool SUB, int U, int VS>
 __device__ inline T sliced_row_sum(T acc, T alpha, const SliceLane &L, unsigned K, unsigned kmax,
                                    unsigned q0, const unsigned *__restrict__ col,

@@ -491,37 +491,6 @@ __device__ inline SliceLane sweep_lane(unsigned slice0, bool xcd, unsigned &half
   return L;
 }
 
-// acc -= sum over the row's K blocks k = h, h + H, h + 2H, ... (H halves);
-// slots below KL >= K are loaded (see sliced_row_sum_m)
-template <typename T, typename M, int U, int H>
-__device__ inline T sliced_row_sub_part(T acc, const SliceLane &L, unsigned h, unsigned K,
-                                        unsigned KL, unsigned kmax, unsigned q0,
-                                        const unsigned *__restrict__ col,
-                                        const M *__restrict__ val, const T *x) {
-  for (unsigned k0 = h; k0 < kmax; k0 += H * U) {
-    unsigned j[U];
-    Trip<M> v[U];
-    Trip<T> xv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const unsigned k = k0 + H * u;
-      if (k < KL) {
-        const size_t q = q0 + k;
-        j[u] = col[q * kSliceRows + L.r];
-        v[u] = load_trip<M, sweep_nt<M>()>(val, q, L.lane);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (k0 + H * u < KL) xv[u] = load_node<T, kVS>(x, j[u]);
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (k0 + H * u < K)
-        acc -= (T)v[u].a * xv[u].a + (T)v[u].b * xv[u].b + (T)v[u].c * xv[u].c;
-  }
-  return acc;
-}
-
 // the halves' sums meet in half 0
 template <typename T, int SPLIT>
 __device__ inline T join_halves(T t, unsigned half, unsigned slot, unsigned lane) {
@@ -544,24 +513,31 @@ inline unsigned sweep_grid(unsigned nsl) {
 template <typename T, typename M, bool PART, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
 k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *dst) {
-  if (a.S->done) return;
+  constexpr bool NT = sweep_nt<M>();
   unsigned half, slot;
   const SliceLane L = sweep_lane<SPLIT>(slice0, a.xcd, half, slot);
   const bool sl = L.s < slice0 + nsl;
   if (SPLIT == 1 && !PART && !sl) return;  // whole wavefront
   const RowSlots R = row_slots(a.f_plan, gv, L, slice0, sl);
   const unsigned i = R.i;
-  T t = 0, rv = 0;
+  // r_i, the D_i^-1 row and the first round of blocks are requested before
+  // the state is read: off the gathers' chain, and none depends on it
+  T rv = 0;
   Trip<M> d{};
+  SlotRound<M, U> R0;
   if (i != ~0u) {
-    if (half == 0) {  // r_i and the D_i^-1 row first: off the gathers' chain
+    if (half == 0) {
       rv = a.r[kVS * (size_t)i + L.c];
-      d = load_trip<M, sweep_nt<M>()>(a.f_dia, L.s, L.lane);
-      t = rv;
+      d = load_trip<M, NT>(a.f_dia, L.s, L.lane);
     }
-    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, R.len, R.kl, R.kmax, R.off, a.f_plan.col,
-                                         a.f_val, a.Pr);
+    load_slot_round<M, U, SPLIT, NT>(R0, L, half, R.kl, R.off, a.f_plan.col, a.f_val);
   }
+  if (a.S->done) return;
+  T t = rv;
+  if (i != ~0u)
+    t = sliced_row_sum_pre<T, M, true, U, SPLIT, kVS, NT>(t, (T)1, R0, L, half, R.len, R.kl,
+                                                          R.kmax, R.off, a.f_plan.col, a.f_val,
+                                                          a.Pr);
   t = join_halves<T, SPLIT>(t, half, slot, L.lane);
   double part = 0.0;
   if (half == 0 && sl) {
@@ -585,21 +561,27 @@ k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *
 template <typename T, typename M, int SPLIT, int U>
 __global__ void __launch_bounds__(kPackBlock)
 k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *dst) {
-  if (a.S->done) return;
+  constexpr bool NT = sweep_nt<M>();
   unsigned half, slot;
   const SliceLane L = sweep_lane<SPLIT>(slice0, a.xcd, half, slot);
   const bool sl = L.s < slice0 + nsl;
   const RowSlots R = row_slots(a.b_plan, gv, L, slice0, sl);
   const unsigned i = R.i;
+  // the row's own entries and its first round of blocks before the state
   T t = 0, rv = 0;
+  SlotRound<M, U> R0;
   if (i != ~0u) {
     if (half == 0) {
       t = a.Pr[kVS * (size_t)i + L.c];
       rv = a.r[kVS * (size_t)i + L.c];
     }
-    t = sliced_row_sub_part<T, M, U, SPLIT>(t, L, half, R.len, R.kl, R.kmax, R.off, a.b_plan.col,
-                                         a.b_val, a.Pr);
+    load_slot_round<M, U, SPLIT, NT>(R0, L, half, R.kl, R.off, a.b_plan.col, a.b_val);
   }
+  if (a.S->done) return;
+  if (i != ~0u)
+    t = sliced_row_sum_pre<T, M, true, U, SPLIT, kVS, NT>(t, (T)1, R0, L, half, R.len, R.kl,
+                                                          R.kmax, R.off, a.b_plan.col, a.b_val,
+                                                          a.Pr);
   t = join_halves<T, SPLIT>(t, half, slot, L.lane);
   double part = 0.0;
   if (half == 0 && i != ~0u) {

@@ -321,6 +321,32 @@ def bench_newmark(args):
     print(json.dumps(out), flush=True)
 
 
+def kernel_table(G, S, nb, nnzb, n0, nu0, step_mixed, step_fp64, args) -> dict:
+    """Per-kernel times of the fused iterations (HIP events on the solver
+    stream, right after a solve of the same precision: same buffers, same
+    kernels), with each kernel class's algorithmic bytes and GB/s."""
+    nlev_f, nlev_b = G.ilu.levels()
+    nlev = {"fwd_sweep": max(nlev_f - 1, 1), "bwd_sweep": max(nlev_b - 1, 1)}
+    kern = {}
+    for prec, fp32, s in (("fp32", True, 4), ("fp64", False, 8)):
+        if fp32:
+            step_mixed()
+        else:
+            step_fp64()
+        ms = S.pcg_kernel_times(G.A, G.ilu, fp32, args.spmv_reps)
+        sv = 8 if (not fp32 or args.mixed_vectors == 64) else 4
+        kb = kernel_bytes(nb, nnzb, s, n0, nu0, sv)
+        kern[prec] = {}
+        for k, b in kb.items():
+            launches = nlev.get(k, 1)
+            kern[prec][k] = {"ms_per_call": round(ms[k], 5), "launches_per_call": launches,
+                             "bytes_per_call": b,
+                             "gbs": round(b / (ms[k] * 1e-3) / 1e9, 1)}
+        for k in ("alpha_step",):
+            kern[prec][k + "_ms"] = round(ms[k], 5)
+    return kern
+
+
 def bench_cube128(args):
     """128^3 hexes x 6 tets (6,440,067 DOF), E lognormal(0, 1.0), nu ~ U[0.25,
     0.35], 4-colour order: FP32 inner + FP64 reliable updates to relative
@@ -329,6 +355,7 @@ def bench_cube128(args):
     FP64 solve to 1e-12 is reported."""
     import torch
 
+    from delfem2_amd import solver as S
     from delfem2_amd import workload as W
     from delfem2_amd.problem import DeviceSystem
 
@@ -374,6 +401,7 @@ def bench_cube128(args):
         cpu["unit"] = "s (time-to-1e-10 = Decompose + FP64 PCG, iterations x measured s/iteration)"
     n0, nu0 = level0_rows(G.psup_ind, G.psup, nb)
     b32 = pcg_iter_bytes(nb, nnzb, 4, n0, nu0, 4 if args.mixed_vectors == 32 else 8)
+    kern = kernel_table(G, S, nb, nnzb, n0, nu0, lambda: step(True), lambda: step(False), args)
     out = {
         "metric": "time-to-1e-10 PCG at 6.44M DOF: FP32+refine vs FP64 (s)",
         "value": round(t_m, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
@@ -393,6 +421,7 @@ def bench_cube128(args):
                   "speedup_vs_fp64": round(t_64 / t_m, 3)},
         "fp64": {"time_s": round(t_64, 5), "iterations": it64, "x_rel_err_vs_fp64_1e-12": err64,
                  "dof_iter_per_s": round(prob.ndof * it64 / t_64, 1)},
+        "kernels": kern,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -512,26 +541,8 @@ def main():
 
     # ---- per-kernel times of the fused iterations: HIP events on the solver
     # stream, right after the timed solves (same buffers, same kernels) ----
-    nlev_f, nlev_b = G.ilu.levels()
-    nlev = {"fwd_sweep": max(nlev_f - 1, 1), "bwd_sweep": max(nlev_b - 1, 1)}
     n0, nu0 = level0_rows(G.psup_ind, G.psup, nb)
-    kern = {}
-    for prec, fp32, s in (("fp32", True, 4), ("fp64", False, 8)):
-        if fp32:
-            step_mixed()
-        else:
-            step_fp64()
-        ms = S.pcg_kernel_times(G.A, G.ilu, fp32, args.spmv_reps)
-        sv = 8 if (not fp32 or args.mixed_vectors == 64) else 4
-        kb = kernel_bytes(nb, nnzb, s, n0, nu0, sv)
-        kern[prec] = {}
-        for k, b in kb.items():
-            launches = nlev.get(k, 1)
-            kern[prec][k] = {"ms_per_call": round(ms[k], 5), "launches_per_call": launches,
-                             "bytes_per_call": b,
-                             "gbs": round(b / (ms[k] * 1e-3) / 1e9, 1)}
-        for k in ("alpha_step",):
-            kern[prec][k + "_ms"] = round(ms[k], 5)
+    kern = kernel_table(G, S, nb, nnzb, n0, nu0, step_mixed, step_fp64, args)
     # dominant kernel of the headline (FP32 + refinement) solve: largest time per iteration
     dom = max(("spmv", "fwd_sweep", "bwd_sweep", "update", "pupdate"),
               key=lambda k: kern["fp32"][k]["ms_per_call"])

@@ -115,6 +115,7 @@ struct dfm2h_ilu_s {
   std::vector<unsigned> levptr_f, levptr_b;
   dfm2h::DBuf<unsigned> levrows_f;
   dfm2h::DBuf<int> nsing_d;
+  unsigned dec_lanes = 0;  // lanes per row of the factorisation kernel (0: one thread per row)
   // sliced sweeps: forward plan (lower blocks + D^-1), backward plan (upper blocks)
   dfm2h::PackedPlan fplan, bplan;
   // per forward slice: bit r set when slice row r has no upper blocks (its

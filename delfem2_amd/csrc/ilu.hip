@@ -114,6 +114,137 @@ k_ilu_decompose_level(const unsigned *__restrict__ rows, unsigned nrows,
   }
 }
 
+// The same factorisation of one level with one lane per block of a row: G
+// lanes per row (G a power of two, more than the longest row's block count,
+// at most 64).  Lane t of row i's group holds block t of the row in CRS
+// order with the diagonal between the lower and the upper part: t < nL the
+// lower slot i0 + t, t == nL the diagonal, t > nL the upper slot i0 + t - 1.
+// Step s takes the row's s-th lower block A_ik -- final by then, since every
+// update it receives comes from a smaller k, i.e. an earlier step -- from
+// lane s, and every lane whose column j lies in U(k) applies
+// A_ij -= A_ik U_kj.  For each destination the updates come in ascending k
+// with the products and association of k_ilu_decompose_level, and D^-1 and
+// the U scaling are formed with its expressions, so both kernels give the
+// same bits; this one keeps a row's dependent loads to about two memory
+// round trips per lower block instead of one per (k, j) pair.
+template <int G>
+__global__ void __launch_bounds__(64)
+k_ilu_decompose_lanes(const unsigned *__restrict__ rows, unsigned nrows,
+                      const unsigned *__restrict__ colind, const unsigned *__restrict__ rowptr,
+                      const unsigned *__restrict__ diaind, double *vcrs, double *vdia,
+                      int *nsing) {
+  static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "G: power of two <= 64");
+  const unsigned lane = threadIdx.x;  // blockDim.x == 64: one wavefront
+  const unsigned gb = lane & ~(G - 1u), t = lane & (G - 1u);
+  const unsigned ri = blockIdx.x * (64u / G) + lane / G;
+  const bool live = ri < nrows;
+  const unsigned i = live ? rows[ri] : 0u;
+  unsigned i0 = 0, i1 = 0, idia = 0;
+  if (live) {
+    i0 = colind[i];
+    i1 = colind[i + 1];
+    idia = diaind[i];
+  }
+  const unsigned nL = idia - i0;
+  const bool has = live && t < i1 - i0 + 1;
+  const bool dia = has && t == nL;
+  const size_t slot = t < nL ? (size_t)i0 + t : (size_t)i0 + t - 1;
+  const unsigned j = !has ? ~0u : dia ? i : rowptr[slot];
+  double *const dst = dia ? vdia + (size_t)i * 9 : vcrs + slot * 9;
+  double v[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) v[q] = has ? dst[q] : 0.0;
+  // upper part [kd, k1) of row k for this lane's lower block k
+  unsigned kd = 0, k1 = 0;
+  if (has && t < nL) {
+    kd = diaind[j];
+    k1 = colind[j + 1];
+  }
+  unsigned smax = live ? nL : 0u;  // steps: the wavefront's longest lower part
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) smax = max(smax, (unsigned)__shfl_xor((int)smax, off, 64));
+  // row k's upper columns, one per lane of the group (requested a step ahead)
+  auto upper_col = [&](unsigned s, unsigned &skd) -> unsigned {
+    skd = (unsigned)__shfl((int)kd, (int)(gb + s), 64);
+    const unsigned sk1 = (unsigned)__shfl((int)k1, (int)(gb + s), 64);
+    return (live && s < nL && skd + t < sk1) ? rowptr[skd + t] : ~0u;
+  };
+  unsigned kd_next = 0;
+  unsigned c_next = smax > 0 ? upper_col(0, kd_next) : ~0u;
+  for (unsigned s = 0; s < smax; ++s) {
+    const unsigned c = c_next, skd = kd_next;
+    if (s + 1 < smax) c_next = upper_col(s + 1, kd_next);
+    double m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = __shfl(v[q], (int)(gb + s), 64);
+    unsigned kj = ~0u;
+#pragma unroll
+    for (unsigned u = 0; u < G; ++u) {
+      const unsigned cu = (unsigned)__shfl((int)c, (int)(gb + u), 64);
+      if (cu == j && j != ~0u) kj = skd + u;
+    }
+    if (kj != ~0u) {  // only where s < nL: c is ~0u otherwise
+      const double *vkj = vcrs + (size_t)kj * 9;
+      double u[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) u[q] = vkj[q];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        v[a * 3 + 0] -= m[a * 3 + 0] * u[0] + m[a * 3 + 1] * u[3] + m[a * 3 + 2] * u[6];
+        v[a * 3 + 1] -= m[a * 3 + 0] * u[1] + m[a * 3 + 1] * u[4] + m[a * 3 + 2] * u[7];
+        v[a * 3 + 2] -= m[a * 3 + 0] * u[2] + m[a * 3 + 1] * u[5] + m[a * 3 + 2] * u[8];
+      }
+    }
+  }
+  if (dia) {
+    if (fabs(det3(v)) > 1.0e-30) {  // CalcInvMat3 (ls_ilu_block_sparse.cpp:84-103)
+      const double inv_det = 1.0 / det3(v);
+      double a[9];
+      a[0] = inv_det * (v[4] * v[8] - v[5] * v[7]);
+      a[1] = inv_det * (v[2] * v[7] - v[1] * v[8]);
+      a[2] = inv_det * (v[1] * v[5] - v[2] * v[4]);
+      a[3] = inv_det * (v[5] * v[6] - v[3] * v[8]);
+      a[4] = inv_det * (v[0] * v[8] - v[2] * v[6]);
+      a[5] = inv_det * (v[2] * v[3] - v[0] * v[5]);
+      a[6] = inv_det * (v[3] * v[7] - v[4] * v[6]);
+      a[7] = inv_det * (v[1] * v[6] - v[0] * v[7]);
+      a[8] = inv_det * (v[0] * v[4] - v[1] * v[3]);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) v[q] = a[q];
+    } else {
+      atomicAdd(nsing, 1);
+    }
+  }
+  double d[9];  // D_i^-1 (or D_i itself when singular), from the diagonal lane
+#pragma unroll
+  for (int q = 0; q < 9; ++q) d[q] = __shfl(v[q], (int)(gb + (nL & (G - 1u))), 64);
+  if (has && t > nL) {
+    double tb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tb[q] = v[q];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      v[a * 3 + 0] = d[a * 3 + 0] * tb[0] + d[a * 3 + 1] * tb[3] + d[a * 3 + 2] * tb[6];
+      v[a * 3 + 1] = d[a * 3 + 0] * tb[1] + d[a * 3 + 1] * tb[4] + d[a * 3 + 2] * tb[7];
+      v[a * 3 + 2] = d[a * 3 + 0] * tb[2] + d[a * 3 + 1] * tb[5] + d[a * 3 + 2] * tb[8];
+    }
+  }
+  if (has) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dst[q] = v[q];
+  }
+}
+
+// lanes per row for k_ilu_decompose_lanes: the smallest power of two above
+// the longest row's block count (diagonal included), 0 when that exceeds 64
+static unsigned decompose_lanes(const std::vector<unsigned> &colind, unsigned nblk) {
+  unsigned mx = 0;
+  for (unsigned i = 0; i < nblk; ++i) mx = std::max(mx, colind[i + 1] - colind[i] + 1);
+  unsigned g = 8;
+  while (g <= mx) g *= 2;
+  return g <= 64 ? g : 0u;
+}
+
 // A slot -> ILU slot by column lookup in the ILU row (the row2crs map of
 // CopyValue), -1 where the ILU pattern lacks the column.
 static std::vector<int> build_a2l(const dfm2h_ilu_s &P, const Pattern &A) {
@@ -183,6 +314,7 @@ static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
   P->colind_d.upload(P->colind);
   P->rowptr_d.upload(P->rowptr);
   P->diaind_d.upload(P->diaind);
+  P->dec_lanes = decompose_lanes(P->colind, P->nblk);
   P->vcrs.alloc(P->ncrs * 9);
   P->vdia.alloc((size_t)P->nblk * 9);
   std::vector<unsigned> rows;
@@ -372,13 +504,30 @@ int dfm2h_ilu_decompose(dfm2h_ilu P, int *ok, int *n_singular) {
   return guard(__func__, [&] {
     need_ilu(P);
     P->nsing_d.zero();
+    const unsigned G = P->dec_lanes;
     for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l) {
       const unsigned nrows = P->levptr_f[l + 1] - P->levptr_f[l];
-      hipLaunchKernelGGL(k_ilu_decompose_level, dim3(ceil_div(nrows, 64)), dim3(64), 0, stream(),
-                         P->levrows_f.p + P->levptr_f[l], nrows, P->colind_d.p, P->rowptr_d.p,
-                         P->diaind_d.p, P->vcrs.p, P->vdia.p, P->nsing_d.p);
+      const unsigned *rows = P->levrows_f.p + P->levptr_f[l];
+      if (G == 0) {
+        hipLaunchKernelGGL(k_ilu_decompose_level, dim3(ceil_div(nrows, 64)), dim3(64), 0,
+                           stream(), rows, nrows, P->colind_d.p, P->rowptr_d.p, P->diaind_d.p,
+                           P->vcrs.p, P->vdia.p, P->nsing_d.p);
+        continue;
+      }
+      const dim3 grid(ceil_div((size_t)nrows * G, 64));
+#define DFM2H_DEC_LANES(g)                                                                    \
+  hipLaunchKernelGGL(k_ilu_decompose_lanes<g>, grid, dim3(64), 0, stream(), rows, nrows,     \
+                     P->colind_d.p, P->rowptr_d.p, P->diaind_d.p, P->vcrs.p, P->vdia.p,      \
+                     P->nsing_d.p)
+      switch (G) {
+        case 8: DFM2H_DEC_LANES(8); break;
+        case 16: DFM2H_DEC_LANES(16); break;
+        case 32: DFM2H_DEC_LANES(32); break;
+        default: DFM2H_DEC_LANES(64); break;
+      }
+#undef DFM2H_DEC_LANES
     }
-    check_launch("k_ilu_decompose_level");
+    check_launch("k_ilu_decompose");
     int ns = 0;
     DFM2H_HIP(hipMemcpyAsync(&ns, P->nsing_d.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
     DFM2H_HIP(hipStreamSynchronize(stream()));

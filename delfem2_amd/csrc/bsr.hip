@@ -4,6 +4,8 @@
 #include "handles.hpp"
 #include "tet.hpp"
 
+#include <cstdlib>
+
 namespace dfm2h {
 
 struct AsmArgs {
@@ -74,12 +76,189 @@ __device__ inline void mass_block(double m[9], double dm, bool same) {
   for (int k = 0; k < 9; ++k) m[k] = (k % 4 == 0) ? d : 0.0;
 }
 
+// One element's contribution to the destination block (li, lj) of row node
+// i, per kind (the reference's element matrix entry, same expressions).
+template <int KIND>
+__device__ inline void element_block(const AsmArgs &A, unsigned i, double vol, const double di[3],
+                                     const double dj[3], bool same, double myu, double lam,
+                                     double blk[9]) {
+  const double w = vol;
+  if (KIND == kAsmStatic) {
+    emat_block(blk, di, dj, w, myu, lam);
+  } else if (KIND == kAsmNewmark) {  // (&emat)[k] = eM[k] + (beta dt dt) eK[k]
+    double kb[9], mb[9];
+    emat_block<true>(kb, di, dj, w, myu, lam);
+    mass_block(mb, A.rho * vol * 0.05, same);
+    const double s = A.beta_nb * A.dt * A.dt;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) blk[k] = mb[k] + s * kb[k];
+  } else if (KIND == kAsmStiffwarp) {  // R_i K_e R_i^T, lumped mass on the diagonal block
+    double kb[9], Ri[9];
+    emat_block<true>(kb, di, dj, w, myu, lam);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ri[k] = A.rot[(size_t)i * 9 + k];
+    rotate_block(blk, kb, Ri);
+    if (same) {
+      blk[0] += A.rho * vol * 0.25 / (A.dt * A.dt);
+      blk[4] += A.rho * vol * 0.25 / (A.dt * A.dt);
+      blk[8] += A.rho * vol * 0.25 / (A.dt * A.dt);
+    }
+  } else {  // backward Euler: K_e, lumped mass on the diagonal block
+    emat_block<true>(blk, di, dj, w, myu, lam);
+    if (same) {
+      blk[0] += A.rho * vol * 0.25 / (A.dt * A.dt);
+      blk[4] += A.rho * vol * 0.25 / (A.dt * A.dt);
+      blk[8] += A.rho * vol * 0.25 / (A.dt * A.dt);
+    }
+  }
+}
+
+// One element's contribution to the right-hand side of row node i (local
+// index li, gradient row di): the addend the reference's per-element eres
+// puts into vec_b at node i, per kind, written to out (the caller adds it).
+// P: the element's node coordinates (read by the stiffness-warping kind only).
+template <int KIND>
+__device__ inline void element_rhs_term(const AsmArgs &A, unsigned i, int li, uint4 n,
+                                        const TetGeo &G, const double (&P)[4][3],
+                                        const double di[3], double myu, double lam,
+                                        double out[3]) {
+  const double w = G.vol;
+  const double vol = G.vol;
+  const unsigned id[4] = {n.x, n.y, n.z, n.w};
+  if (KIND == kAsmNewmark) {
+    double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
+                    vol * A.rho * A.g2 * 0.25};
+    double kb[9], mb[9], x[3];
+#pragma unroll
+    for (int jn = 0; jn < 4; ++jn) {  // - K_e u
+      emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+      load_node3(x, A.disp, id[jn]);
+      add_block_vec(er, kb, x, -1.);
+    }
+#pragma unroll
+    for (int jn = 0; jn < 4; ++jn) {  // - M_e a
+      mass_block(mb, A.rho * vol * 0.05, jn == li);
+      load_node3(x, A.acc, id[jn]);
+      add_block_vec(er, mb, x, -1.);
+    }
+#pragma unroll
+    for (int jn = 0; jn < 4; ++jn) {  // - dt K_e v
+      emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+      load_node3(x, A.velo, id[jn]);
+      add_block_vec(er, kb, x, -A.dt);
+    }
+#pragma unroll
+    for (int jn = 0; jn < 4; ++jn) {  // - (0.5 dt^2) K_e a
+      emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+      load_node3(x, A.acc, id[jn]);
+      add_block_vec(er, kb, x, -0.5 * A.dt * A.dt);
+    }
+    out[0] = er[0];
+    out[1] = er[1];
+    out[2] = er[2];
+  } else if (KIND == kAsmBEuler) {
+    double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
+                    vol * A.rho * A.g2 * 0.25};
+    double kb[9], u[3], v[3];
+#pragma unroll
+    for (int jn = 0; jn < 4; ++jn) {  // - K_e (u + dt v), three subtractions per pair
+      emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+      load_node3(u, A.disp, id[jn]);
+      load_node3(v, A.velo, id[jn]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        er[a] -= kb[3 * a + 0] * (u[0] + A.dt * v[0]);
+        er[a] -= kb[3 * a + 1] * (u[1] + A.dt * v[1]);
+        er[a] -= kb[3 * a + 2] * (u[2] + A.dt * v[2]);
+      }
+    }
+    out[0] = er[0] / A.dt;
+    out[1] = er[1] / A.dt;
+    out[2] = er[2] / A.dt;
+  } else if (KIND == kAsmStiffwarp) {
+    double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
+                    vol * A.rho * A.g2 * 0.25};
+    double Ri[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ri[k] = A.rot[(size_t)i * 9 + k];
+#pragma unroll
+    for (int jn = 0; jn < 4; ++jn) {
+      double kb[9], E[9], u[3], v[3];
+      emat_block<true>(kb, di, G.d[jn], w, myu, lam);
+      rotate_block(E, kb, Ri);
+      load_node3(u, A.disp, id[jn]);
+      load_node3(v, A.velo, id[jn]);
+      const double *Pj = P[jn];
+      const double Pj1[3] = {Ri[0] * Pj[0] + Ri[1] * Pj[1] + Ri[2] * Pj[2],
+                             Ri[3] * Pj[0] + Ri[4] * Pj[1] + Ri[5] * Pj[2],
+                             Ri[6] * Pj[0] + Ri[7] * Pj[1] + Ri[8] * Pj[2]};
+      const double uj1[3] = {Pj[0] + u[0] + A.dt * v[0] - Pj1[0],
+                             Pj[1] + u[1] + A.dt * v[1] - Pj1[1],
+                             Pj[2] + u[2] + A.dt * v[2] - Pj1[2]};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        er[a] -= E[3 * a + 0] * uj1[0];
+        er[a] -= E[3 * a + 1] * uj1[1];
+        er[a] -= E[3 * a + 2] * uj1[2];
+      }
+    }
+    out[0] = er[0] / A.dt;
+    out[1] = er[1] / A.dt;
+    out[2] = er[2] / A.dt;
+  } else {
+    // eres = vol*rho*g*0.25 - K_e u_e (fem_solidlinear.h:214-221 and
+    // AddEmatEvecScale3 femutil.h:252-268, scale -1), then vec_b += eres
+    double er0 = vol * A.rho * A.g0 * 0.25;
+    double er1 = vol * A.rho * A.g1 * 0.25;
+    double er2 = vol * A.rho * A.g2 * 0.25;
+#pragma unroll
+    for (int jn = 0; jn < 4; ++jn) {
+      double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+      if (A.disp) {
+        u0 = A.disp[(size_t)id[jn] * 3 + 0];
+        u1 = A.disp[(size_t)id[jn] * 3 + 1];
+        u2 = A.disp[(size_t)id[jn] * 3 + 2];
+      }
+      double kb[9];
+      emat_block(kb, di, G.d[jn], w, myu, lam);
+      er0 += -1. * (kb[0] * u0 + kb[1] * u1 + kb[2] * u2);
+      er1 += -1. * (kb[3] * u0 + kb[4] * u1 + kb[5] * u2);
+      er2 += -1. * (kb[6] * u0 + kb[7] * u1 + kb[8] * u2);
+    }
+    out[0] = er0;
+    out[1] = er1;
+    out[2] = er2;
+  }
+}
+
+// every neighbour of row node i in an element must own a slot in row i
+__device__ inline void check_row_slots(const AsmArgs &A, unsigned i, uint4 n) {
+  const unsigned id[4] = {n.x, n.y, n.z, n.w};
+  const unsigned r0 = A.colind[i], r1 = A.colind[i + 1];
+#pragma unroll
+  for (int jn = 0; jn < 4; ++jn) {
+    const unsigned col = id[jn];
+    if (col == i) continue;
+    unsigned lo = r0, hi = r1;
+    while (lo < hi) {
+      const unsigned mid = (lo + hi) >> 1;
+      if (A.rowptr[mid] < col) lo = mid + 1; else hi = mid;
+    }
+    bool found = lo < r1 && A.rowptr[lo] == col;
+    if (!found)  // pattern rows need not be sorted: fall back to a scan
+      for (unsigned s = r0; s < r1 && !found; ++s) found = A.rowptr[s] == col;
+    if (!found) atomicOr(A.missing, 1);
+  }
+}
+
 // Gather assembly, one thread per destination block: off-diagonal slot s
 // (t < ncrs) or the diagonal block and right-hand side of row t-ncrs.  Each
 // thread walks the elements around its row node in ascending element order
 // and adds their contribution to its block; the additions therefore happen
 // in exactly the order of the reference's sequential element loop with
 // Merge<4,4,3,3>, and no two threads write the same address (no atomics).
+// This form computes each element's geometry where it is used; the listed
+// form below is the one run, this one serves meshes of 2^28 tets or more.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,174 +289,20 @@ __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
     tet_geometry(G, P);
     const double myu = A.myu_e ? A.myu_e[e] : A.myu;
     const double lam = A.lambda_e ? A.lambda_e[e] : A.lambda;
-    const double w = G.vol;
     double di[3], dj[3];
     pick_row(di, G, li);
     pick_row(dj, G, lj);
     double blk[9];
-    if (KIND == kAsmStatic) {
-      emat_block(blk, di, dj, w, myu, lam);
-    } else if (KIND == kAsmNewmark) {  // (&emat)[k] = eM[k] + (beta dt dt) eK[k]
-      double kb[9], mb[9];
-      emat_block<true>(kb, di, dj, w, myu, lam);
-      mass_block(mb, A.rho * G.vol * 0.05, li == lj);
-      const double s = A.beta_nb * A.dt * A.dt;
-#pragma unroll
-      for (int k = 0; k < 9; ++k) blk[k] = mb[k] + s * kb[k];
-    } else if (KIND == kAsmStiffwarp) {  // R_i K_e R_i^T, lumped mass on the diagonal block
-      double kb[9], Ri[9];
-      emat_block<true>(kb, di, dj, w, myu, lam);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Ri[k] = A.rot[(size_t)i * 9 + k];
-      rotate_block(blk, kb, Ri);
-      if (li == lj) {
-        blk[0] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
-        blk[4] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
-        blk[8] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
-      }
-    } else {  // backward Euler: K_e, lumped mass on the diagonal block
-      emat_block<true>(blk, di, dj, w, myu, lam);
-      if (li == lj) {
-        blk[0] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
-        blk[4] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
-        blk[8] += A.rho * G.vol * 0.25 / (A.dt * A.dt);
-      }
-    }
+    element_block<KIND>(A, i, G.vol, di, dj, li == lj, myu, lam, blk);
 #pragma unroll
     for (int k = 0; k < 9; ++k) acc[k] += blk[k];
-    if (diag && KIND == kAsmNewmark) {
-      const double vol = G.vol;
-      double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
-                      vol * A.rho * A.g2 * 0.25};
-      const unsigned id[4] = {n.x, n.y, n.z, n.w};
-      double kb[9], mb[9], x[3];
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {  // - K_e u
-        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
-        load_node3(x, A.disp, id[jn]);
-        add_block_vec(er, kb, x, -1.);
-      }
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {  // - M_e a
-        mass_block(mb, A.rho * vol * 0.05, jn == li);
-        load_node3(x, A.acc, id[jn]);
-        add_block_vec(er, mb, x, -1.);
-      }
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {  // - dt K_e v
-        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
-        load_node3(x, A.velo, id[jn]);
-        add_block_vec(er, kb, x, -A.dt);
-      }
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {  // - (0.5 dt^2) K_e a
-        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
-        load_node3(x, A.acc, id[jn]);
-        add_block_vec(er, kb, x, -0.5 * A.dt * A.dt);
-      }
-      bacc[0] += er[0];
-      bacc[1] += er[1];
-      bacc[2] += er[2];
-    }
-    if (diag && KIND == kAsmBEuler) {
-      const double vol = G.vol;
-      double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
-                      vol * A.rho * A.g2 * 0.25};
-      const unsigned id[4] = {n.x, n.y, n.z, n.w};
-      double kb[9], u[3], v[3];
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {  // - K_e (u + dt v), three subtractions per pair
-        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
-        load_node3(u, A.disp, id[jn]);
-        load_node3(v, A.velo, id[jn]);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          er[a] -= kb[3 * a + 0] * (u[0] + A.dt * v[0]);
-          er[a] -= kb[3 * a + 1] * (u[1] + A.dt * v[1]);
-          er[a] -= kb[3 * a + 2] * (u[2] + A.dt * v[2]);
-        }
-      }
-      bacc[0] += er[0] / A.dt;
-      bacc[1] += er[1] / A.dt;
-      bacc[2] += er[2] / A.dt;
-    }
-    if (diag && KIND == kAsmStiffwarp) {
-      const double vol = G.vol;
-      double er[3] = {vol * A.rho * A.g0 * 0.25, vol * A.rho * A.g1 * 0.25,
-                      vol * A.rho * A.g2 * 0.25};
-      const unsigned id[4] = {n.x, n.y, n.z, n.w};
-      double Ri[9];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Ri[k] = A.rot[(size_t)i * 9 + k];
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {
-        double kb[9], E[9], u[3], v[3];
-        emat_block<true>(kb, di, G.d[jn], w, myu, lam);
-        rotate_block(E, kb, Ri);
-        load_node3(u, A.disp, id[jn]);
-        load_node3(v, A.velo, id[jn]);
-        const double *Pj = P[jn];
-        const double Pj1[3] = {Ri[0] * Pj[0] + Ri[1] * Pj[1] + Ri[2] * Pj[2],
-                               Ri[3] * Pj[0] + Ri[4] * Pj[1] + Ri[5] * Pj[2],
-                               Ri[6] * Pj[0] + Ri[7] * Pj[1] + Ri[8] * Pj[2]};
-        const double uj1[3] = {Pj[0] + u[0] + A.dt * v[0] - Pj1[0],
-                               Pj[1] + u[1] + A.dt * v[1] - Pj1[1],
-                               Pj[2] + u[2] + A.dt * v[2] - Pj1[2]};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          er[a] -= E[3 * a + 0] * uj1[0];
-          er[a] -= E[3 * a + 1] * uj1[1];
-          er[a] -= E[3 * a + 2] * uj1[2];
-        }
-      }
-      bacc[0] += er[0] / A.dt;
-      bacc[1] += er[1] / A.dt;
-      bacc[2] += er[2] / A.dt;
-    }
-    if (diag && KIND == kAsmStatic) {
-      // eres = vol*rho*g*0.25 - K_e u_e (fem_solidlinear.h:214-221 and
-      // AddEmatEvecScale3 femutil.h:252-268, scale -1), then vec_b += eres
-      const double vol = G.vol;
-      double er0 = vol * A.rho * A.g0 * 0.25;
-      double er1 = vol * A.rho * A.g1 * 0.25;
-      double er2 = vol * A.rho * A.g2 * 0.25;
-      const unsigned id[4] = {n.x, n.y, n.z, n.w};
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {
-        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
-        if (A.disp) {
-          u0 = A.disp[(size_t)id[jn] * 3 + 0];
-          u1 = A.disp[(size_t)id[jn] * 3 + 1];
-          u2 = A.disp[(size_t)id[jn] * 3 + 2];
-        }
-        double kb[9];
-        emat_block(kb, di, G.d[jn], w, myu, lam);
-        er0 += -1. * (kb[0] * u0 + kb[1] * u1 + kb[2] * u2);
-        er1 += -1. * (kb[3] * u0 + kb[4] * u1 + kb[5] * u2);
-        er2 += -1. * (kb[6] * u0 + kb[7] * u1 + kb[8] * u2);
-      }
-      bacc[0] += er0;
-      bacc[1] += er1;
-      bacc[2] += er2;
-    }
     if (diag) {
-      const unsigned id[4] = {n.x, n.y, n.z, n.w};
-      // every neighbour of i in this element must own a slot in row i
-      const unsigned r0 = A.colind[i], r1 = A.colind[i + 1];
-#pragma unroll
-      for (int jn = 0; jn < 4; ++jn) {
-        const unsigned col = id[jn];
-        if (col == i) continue;
-        unsigned lo = r0, hi = r1;
-        while (lo < hi) {
-          const unsigned mid = (lo + hi) >> 1;
-          if (A.rowptr[mid] < col) lo = mid + 1; else hi = mid;
-        }
-        bool found = lo < r1 && A.rowptr[lo] == col;
-        if (!found)  // pattern rows need not be sorted: fall back to a scan
-          for (unsigned s = r0; s < r1 && !found; ++s) found = A.rowptr[s] == col;
-        if (!found) atomicOr(A.missing, 1);
-      }
+      double tr[3];
+      element_rhs_term<KIND>(A, i, li, n, G, P, di, myu, lam, tr);
+      bacc[0] += tr[0];
+      bacc[1] += tr[1];
+      bacc[2] += tr[2];
+      check_row_slots(A, i, n);
     }
   }
 #pragma unroll
@@ -287,6 +312,180 @@ __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
     A.vec_b[(size_t)i * 3 + 1] = bacc[1];
     A.vec_b[(size_t)i * 3 + 2] = bacc[2];
   }
+}
+
+// Per-tet record of the listed assembly: volume, the four gradient rows and
+// the element's Lame parameters (tet_geometry's values).
+constexpr int kTetRec = 16;  // doubles: vol, d[4][3], myu, lambda, unused
+
+__global__ void __launch_bounds__(256) k_tet_records(AsmArgs A, size_t ntet, double *rec) {
+  const size_t e = (size_t)wg_id(true) * blockDim.x + threadIdx.x;
+  if (e >= ntet) return;
+  double P[4][3];
+  load_tet(P, A.xyz, A.tet[e]);
+  TetGeo G;
+  tet_geometry(G, P);
+  double *r = rec + e * kTetRec;
+  r[0] = G.vol;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r[1 + 3 * k + a] = G.d[k][a];
+  r[13] = A.myu_e ? A.myu_e[e] : A.myu;
+  r[14] = A.lambda_e ? A.lambda_e[e] : A.lambda;
+  r[15] = 0.0;
+}
+
+// Element entry of destination block t: the elements k_assemble_tet_gather
+// would add there, in its order, packed as e | li << 28 | lj << 30.  ptr ==
+// nullptr: count them into cnt[t] (and flag missing slots); else write them
+// from ptr[t].
+__global__ void __launch_bounds__(256)
+k_asm_lists(AsmArgs A, const unsigned *__restrict__ ptr, unsigned *out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool diag = t >= A.ncrs;
+  if (t >= A.ncrs + A.nblk) return;
+  const unsigned i = diag ? (unsigned)(t - A.ncrs) : A.rowof[t];
+  const unsigned j = diag ? i : A.rowptr[t];
+  unsigned c = 0;
+  const size_t w = ptr ? ptr[t] : 0;
+  const unsigned q1 = A.elsup_ind[i + 1];
+  for (unsigned q = A.elsup_ind[i]; q < q1; ++q) {
+    const unsigned e = A.elsup[q];
+    const uint4 n = A.tet[e];
+    const int li = local_index(n, i);
+    const int lj = diag ? li : local_index(n, j);
+    if (lj < 0) continue;
+    if (ptr)
+      out[w + c] = e | ((unsigned)li & 3u) << 28 | ((unsigned)lj & 3u) << 30;
+    else if (diag)
+      check_row_slots(A, i, n);
+    ++c;
+  }
+  if (!ptr) out[t] = c;
+}
+
+// The gather assembly over the element lists, off-diagonal blocks (slots
+// t < ncrs): the same additions in the same order as k_assemble_tet_gather,
+// each element's geometry read from its record instead of recomputed, and
+// no walk over the elements that do not touch the block.  Workgroups are
+// numbered per XCD (wg_id): each XCD's L2 then serves one contiguous run of
+// rows and the records their elements share.
+template <int KIND>
+__global__ void __launch_bounds__(256)
+k_assemble_offdiag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
+                         const unsigned *__restrict__ ent, const double *__restrict__ rec) {
+  const size_t t = (size_t)wg_id(true) * blockDim.x + threadIdx.x;
+  if (t >= A.ncrs) return;
+  const unsigned i = A.rowof[t];
+  double *dst = A.vcrs + t * 9;
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = dst[q];
+  // elements in rounds of U: the entries, then their records, are requested
+  // together, then added in order
+  constexpr int U = 4;
+  const unsigned q0 = ptr[t], q1 = ptr[t + 1];
+  for (unsigned qb = q0; qb < q1; qb += U) {
+    unsigned en[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) en[u] = qb + u < q1 ? ent[qb + u] : 0u;
+    double vol[U], my[U], la[U], di[U][3], dj[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (qb + u < q1) {
+        const unsigned li = (en[u] >> 28) & 3u, lj = en[u] >> 30;
+        const double *r = rec + (size_t)(en[u] & 0x0FFFFFFFu) * kTetRec;
+        vol[u] = r[0];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          di[u][a] = r[1 + 3 * li + a];
+          dj[u][a] = r[1 + 3 * lj + a];
+        }
+        my[u] = r[13];
+        la[u] = r[14];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (qb + u < q1) {
+        double blk[9];
+        element_block<KIND>(A, i, vol[u], di[u], dj[u], ((en[u] >> 28) & 3u) == (en[u] >> 30),
+                            my[u], la[u], blk);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] += blk[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) dst[q] = acc[q];
+}
+
+// Diagonal blocks and right-hand side over the element lists: 32 lanes per
+// row node, each computing one element's diagonal block and right-hand-side
+// addend (12 values, into LDS); then lane q < 9 adds value q of every element
+// to the diagonal entry q, and lanes 9..11 the addends to vec_b, in element
+// order -- the additions and order of k_assemble_tet_gather's diagonal
+// thread, with the elements' work spread over the lanes.  Rows with more
+// than 32 elements take several rounds.
+constexpr int kDiagLanes = 32;
+
+template <int KIND>
+__global__ void __launch_bounds__(64, 2)
+k_assemble_diag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
+                      const unsigned *__restrict__ ent, const double *__restrict__ rec) {
+  __shared__ double term[64][13];  // 12 values per lane, padded against bank conflicts
+  const unsigned lane = threadIdx.x, g0 = lane & ~(kDiagLanes - 1u), l = lane & (kDiagLanes - 1u);
+  const unsigned i = wg_id(true) * (64 / kDiagLanes) + lane / kDiagLanes;
+  const bool live = i < A.nblk;
+  const size_t t = A.ncrs + (live ? i : 0u);
+  const unsigned q0 = live ? ptr[t] : 0u, q1 = live ? ptr[t + 1] : 0u;
+  // the running sum of lane l < 12: diagonal entry l, or vec_b component l - 9
+  double sum = 0.0;
+  if (live && l < 9) sum = A.vdia[(size_t)i * 9 + l];
+  if (live && l >= 9 && l < 12) sum = A.vec_b[(size_t)i * 3 + (l - 9)];
+  unsigned rounds = q1 - q0;  // the wavefront's longest list, in rounds of 32
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    rounds = max(rounds, (unsigned)__shfl_xor((int)rounds, off, 64));
+  rounds = (rounds + kDiagLanes - 1) / kDiagLanes;
+  for (unsigned c = 0; c < rounds; ++c) {
+    const unsigned qb = q0 + c * kDiagLanes, q = qb + l;
+    if (q < q1) {
+      const unsigned en = ent[q];
+      const unsigned e = en & 0x0FFFFFFFu;
+      const int li = (int)((en >> 28) & 3u);
+      const double *r = rec + (size_t)e * kTetRec;
+      TetGeo G;
+      G.vol = r[0];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) G.d[k][a] = r[1 + 3 * k + a];
+      const double myu = r[13], lam = r[14];
+      double di[3];
+      pick_row(di, G, li);
+      double blk[9], tr[3];
+      element_block<KIND>(A, i, G.vol, di, di, true, myu, lam, blk);
+      const uint4 n = A.tet[e];
+      double P[4][3];
+      if (KIND == kAsmStiffwarp) load_tet(P, A.xyz, n);
+      element_rhs_term<KIND>(A, i, li, n, G, P, di, myu, lam, tr);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) term[lane][k] = blk[k];
+      term[lane][9] = tr[0];
+      term[lane][10] = tr[1];
+      term[lane][11] = tr[2];
+    }
+    __syncthreads();
+    if (l < 12 && qb < q1) {
+      const unsigned m = min(q1 - qb, (unsigned)kDiagLanes);
+      for (unsigned k = 0; k < m; ++k) sum += term[g0 + k][l];
+    }
+    __syncthreads();
+  }
+  if (live && l < 9) A.vdia[(size_t)i * 9 + l] = sum;
+  if (live && l >= 9 && l < 12) A.vec_b[(size_t)i * 3 + (l - 9)] = sum;
 }
 
 // SetFixedBC (ls_block_sparse.cpp:397-472): entries in a fixed row or a
@@ -339,6 +538,15 @@ static void need_pattern(dfm2h_bsr A) {
   DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_STATE, "SetPattern has not been called");
 }
 
+// The walking form runs for meshes whose element ids do not fit the packed
+// entries (2^28 tets or more), or when the environment variable
+// DFM2H_ASSEMBLY_WALK is 1 (read at every call; lets the two forms be
+// compared in one process).
+static bool walk_assembly(dfm2h_tetmesh M) {
+  const char *v = std::getenv("DFM2H_ASSEMBLY_WALK");
+  return (v && v[0] == '1') || M->ntet >= (1ull << 28);
+}
+
 // Shared host side of the gather assembly: checks, argument block, launch,
 // missing-slot report.  `a` carries the kind's extra fields.
 static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu_e,
@@ -375,18 +583,75 @@ static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu
   a.elsup = M->elsup.p;
   a.missing = missing.p;
   const dim3 grid(ceil_div(n, 256)), block(256);
-  if (kind == kAsmNewmark)
-    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmNewmark>, grid, block, 0, stream(), a);
-  else if (kind == kAsmStiffwarp)
-    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmStiffwarp>, grid, block, 0, stream(), a);
-  else if (kind == kAsmBEuler)
-    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmBEuler>, grid, block, 0, stream(), a);
-  else
-    hipLaunchKernelGGL(k_assemble_tet_gather<kAsmStatic>, grid, block, 0, stream(), a);
-  check_launch("k_assemble_tet_gather");
+  if (walk_assembly(M)) {
+    if (kind == kAsmNewmark)
+      hipLaunchKernelGGL(k_assemble_tet_gather<kAsmNewmark>, grid, block, 0, stream(), a);
+    else if (kind == kAsmStiffwarp)
+      hipLaunchKernelGGL(k_assemble_tet_gather<kAsmStiffwarp>, grid, block, 0, stream(), a);
+    else if (kind == kAsmBEuler)
+      hipLaunchKernelGGL(k_assemble_tet_gather<kAsmBEuler>, grid, block, 0, stream(), a);
+    else
+      hipLaunchKernelGGL(k_assemble_tet_gather<kAsmStatic>, grid, block, 0, stream(), a);
+    check_launch("k_assemble_tet_gather");
+  } else {
+    if (A->asm_mesh_serial != M->serial) {  // element lists for this mesh
+      A->asm_mesh_serial = 0;
+      DBuf<unsigned> cnt;
+      cnt.alloc(n);
+      hipLaunchKernelGGL(k_asm_lists, grid, block, 0, stream(), a, (const unsigned *)nullptr,
+                         cnt.p);
+      check_launch("k_asm_lists");
+      std::vector<unsigned> ptr(n + 1, 0);
+      DFM2H_HIP(hipMemcpyAsync(ptr.data() + 1, cnt.p, n * sizeof(unsigned),
+                               hipMemcpyDeviceToHost, stream()));
+      int h_miss = 0;
+      DFM2H_HIP(hipMemcpyAsync(&h_miss, missing.p, sizeof(int), hipMemcpyDeviceToHost,
+                               stream()));
+      DFM2H_HIP(hipStreamSynchronize(stream()));
+      size_t tot = 0;
+      for (size_t q = 1; q <= n; ++q) {
+        tot += ptr[q];
+        DFM2H_REQUIRE(tot < (1ull << 32), DFM2H_ERR_INVALID, "too many element entries");
+        ptr[q] = (unsigned)tot;
+      }
+      A->asm_ptr.upload(ptr);
+      A->asm_ent.alloc(std::max<size_t>(tot, 1));
+      hipLaunchKernelGGL(k_asm_lists, grid, block, 0, stream(), a, A->asm_ptr.p, A->asm_ent.p);
+      check_launch("k_asm_lists");
+      A->asm_missing = h_miss;
+      A->asm_mesh_serial = M->serial;
+    }
+    A->asm_rec.alloc(std::max<size_t>(M->ntet * kTetRec, 1));
+    if (M->ntet) {
+      hipLaunchKernelGGL(k_tet_records, dim3(ceil_div(M->ntet, 256)), block, 0, stream(), a,
+                         M->ntet, A->asm_rec.p);
+      check_launch("k_tet_records");
+    }
+    const unsigned *pp = A->asm_ptr.p, *pe = A->asm_ent.p;
+    const double *pr = A->asm_rec.p;
+    const dim3 go(ceil_div(std::max<size_t>(A->pat.ncrs, 1), 256));
+    const dim3 gd(ceil_div(A->nblk, 64 / kDiagLanes)), bd(64);
+#define DFM2H_ASM_LISTS(K)                                                                   \
+  do {                                                                                       \
+    if (A->pat.ncrs)                                                                         \
+      hipLaunchKernelGGL(k_assemble_offdiag_lists<K>, go, block, 0, stream(), a, pp, pe, pr); \
+    hipLaunchKernelGGL(k_assemble_diag_lists<K>, gd, bd, 0, stream(), a, pp, pe, pr);        \
+  } while (0)
+    if (kind == kAsmNewmark)
+      DFM2H_ASM_LISTS(kAsmNewmark);
+    else if (kind == kAsmStiffwarp)
+      DFM2H_ASM_LISTS(kAsmStiffwarp);
+    else if (kind == kAsmBEuler)
+      DFM2H_ASM_LISTS(kAsmBEuler);
+    else
+      DFM2H_ASM_LISTS(kAsmStatic);
+#undef DFM2H_ASM_LISTS
+    check_launch("k_assemble_lists");
+  }
   int h_missing = 0;
   DFM2H_HIP(hipMemcpyAsync(&h_missing, missing.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
   DFM2H_HIP(hipStreamSynchronize(stream()));
+  if (!walk_assembly(M) && A->asm_missing) h_missing = 1;  // found when listing
   A->version++;
   DFM2H_REQUIRE(!h_missing, DFM2H_ERR_PATTERN,
                 "an element edge has no block in the matrix pattern");

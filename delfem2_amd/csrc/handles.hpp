@@ -90,9 +90,17 @@ struct dfm2h_bsr_s {
   dfm2h::PackedPlan plan;  // all rows, CRS block order
   dfm2h::PackedVals<double> pk64;
   dfm2h::PackedVals<float> pk32;
+  // gather assembly: per destination block (off-diagonal slots, then the
+  // diagonal blocks) the entries (element | li << 28 | lj << 30) of the
+  // elements it receives, ascending; built on first use with a mesh
+  uint64_t asm_mesh_serial = 0;  // serial of that mesh (0: none yet)
+  int asm_missing = 0;           // an element edge lacks a slot in the pattern
+  dfm2h::DBuf<unsigned> asm_ptr, asm_ent;
+  dfm2h::DBuf<double> asm_rec;  // per-tet records (geometry, Lame), rewritten every call
 };
 
 struct dfm2h_tetmesh_s {
+  uint64_t serial = 0;  // unique per created mesh
   size_t ntet = 0, np = 0;
   unsigned max_elsup = 0;           // most elements around one node
   dfm2h::DBuf<unsigned> tet;        // [4*ntet]
@@ -111,6 +119,7 @@ struct dfm2h_ilu_s {
   // A slot -> ILU slot map for CopyValue (built against `mapped_for`)
   const dfm2h_bsr_s *mapped_for = nullptr;
   dfm2h::DBuf<int> a2l_d;
+  bool a2l_identity = false;  // a2l[s] == s for every slot (same pattern)
   // level schedules: rows grouped by dependency level, ascending inside a level
   std::vector<unsigned> levptr_f, levptr_b;
   dfm2h::DBuf<unsigned> levrows_f;

@@ -262,6 +262,17 @@ static std::vector<int> build_a2l(const dfm2h_ilu_s &P, const Pattern &A) {
 
 using namespace dfm2h;
 
+// Builds the A -> ILU slot map for CopyValue against A; notes when it is the
+// identity (ILU(0) on A's own pattern), where CopyValue is one device copy.
+static void set_a2l(dfm2h_ilu P, dfm2h_bsr A) {
+  const std::vector<int> a2l = build_a2l(*P, A->pat);
+  bool id = a2l.size() == P->ncrs;
+  for (size_t s = 0; id && s < a2l.size(); ++s) id = a2l[s] == (int)s;
+  P->a2l_d.upload(a2l);
+  P->a2l_identity = id;
+  P->mapped_for = A;
+}
+
 static void need_ilu(dfm2h_ilu P) {
   DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null preconditioner handle");
   DFM2H_REQUIRE(P->has_pattern, DFM2H_ERR_STATE, "SetPattern0 has not been called");
@@ -355,9 +366,7 @@ static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
     }
     P->bplan.build(groups, s0, s1, P->rowptr, true);
   }
-  const std::vector<int> a2l = build_a2l(*P, pa);
-  P->a2l_d.upload(a2l);
-  P->mapped_for = A;
+  set_a2l(P, A);
   P->nsing_d.alloc(1);
   P->has_pattern = true;
   P->pattern_serial++;
@@ -477,20 +486,22 @@ int dfm2h_ilu_copy_value(dfm2h_ilu P, dfm2h_bsr A) {
     need_ilu(P);
     DFM2H_REQUIRE(A && A->has_pattern, DFM2H_ERR_STATE, "matrix has no pattern");
     DFM2H_REQUIRE(A->nblk == P->nblk, DFM2H_ERR_INVALID, "size mismatch");
-    if (P->mapped_for != A) {
-      const std::vector<int> a2l = build_a2l(*P, A->pat);
-      P->a2l_d.upload(a2l);
-      P->mapped_for = A;
-    }
-    if (P->ncrs) {
-      hipLaunchKernelGGL(k_zero, dim3(ceil_div(P->ncrs * 9, 256)), dim3(256), 0, stream(),
-                         P->vcrs.p, P->ncrs * 9);
-      check_launch("k_zero");
-    }
-    if (A->pat.ncrs) {
-      hipLaunchKernelGGL(k_copy_value, dim3(ceil_div(A->pat.ncrs * 9, 256)), dim3(256), 0,
-                         stream(), A->pat.ncrs, P->a2l_d.p, A->vcrs.p, P->vcrs.p);
-      check_launch("k_copy_value");
+    if (P->mapped_for != A) set_a2l(P, A);
+    if (P->a2l_identity) {  // same pattern, same slots: a plain copy
+      if (P->ncrs)
+        DFM2H_HIP(hipMemcpyAsync(P->vcrs.p, A->vcrs.p, sizeof(double) * 9 * P->ncrs,
+                                 hipMemcpyDeviceToDevice, stream()));
+    } else {
+      if (P->ncrs) {
+        hipLaunchKernelGGL(k_zero, dim3(ceil_div(P->ncrs * 9, 256)), dim3(256), 0, stream(),
+                           P->vcrs.p, P->ncrs * 9);
+        check_launch("k_zero");
+      }
+      if (A->pat.ncrs) {
+        hipLaunchKernelGGL(k_copy_value, dim3(ceil_div(A->pat.ncrs * 9, 256)), dim3(256), 0,
+                           stream(), A->pat.ncrs, P->a2l_d.p, A->vcrs.p, P->vcrs.p);
+        check_launch("k_copy_value");
+      }
     }
     if (P->nblk)
       DFM2H_HIP(hipMemcpyAsync(P->vdia.p, A->vdia.p, sizeof(double) * 9 * P->nblk,

@@ -3,6 +3,7 @@
 #include "handles.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -221,8 +222,10 @@ int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet, s
                   "mesh too large for 32-bit indices");
     for (size_t i = 0; i < ntet * 4; ++i)
       DFM2H_REQUIRE(tet[i] < np, DFM2H_ERR_INVALID, "tet node index out of range");
+    static std::atomic<uint64_t> mesh_serial{0};
     auto *M = new dfm2h_tetmesh_s;
     try {
+      M->serial = ++mesh_serial;
       M->ntet = ntet;
       M->np = np;
       std::vector<unsigned> ind(np + 1, 0), es(ntet * 4);

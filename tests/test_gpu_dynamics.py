@@ -164,6 +164,50 @@ def test_stiffwarp_assembly_bit_exact(gpu, order):
     assert np.array_equal(bits(vd), bits(C.vdia))
 
 
+@pytest.mark.parametrize("order", ["natural", "color4"])
+def test_listed_assembly_equals_walking_form(gpu, order, monkeypatch):
+    """Every kind of the gather assembly: the listed form (element lists
+    built once per matrix and mesh, per-tet geometry records) gives the bits
+    of the walking form (DFM2H_ASSEMBLY_WALK=1) on the first call (lists
+    built), a repeated call (lists reused) and a second mesh handle of the
+    same tets (lists rebuilt)."""
+    p = W.cube(4, sigma=0.5, seed=35, ordering=order)
+    u, v, a = ((W.uniform(k, p.ndof) - 0.5) * s for k, s in ((46, 1e-2), (47, 1e-1), (48, 1.0)))
+    R = W.uniform(49, p.np_ * 9) - 0.5
+    g = [0.1, -0.4, -2.0]
+    rho, dt, beta, gamma = 1.2, 0.025, 0.3, 0.55
+    G = DynamicSystem(p, "newmark")
+    meshes = [G.mesh, S.TetMesh(p.tet, p.np_)]
+    du, dv, da, dR = dev(u), dev(v), dev(a), dev(R)
+
+    def run(kind, mesh):
+        G.A.setZero()
+        b = torch.zeros(p.ndof, dtype=torch.float64, device="cuda")
+        if kind == "static":
+            ok = S.MergeLinSys_SolidLinear_Static_MeshTet3D(G.A, b, G.myu, G.lam, rho, g, G.xyz,
+                                                            mesh, du)
+        elif kind == "newmark":
+            ok = S.MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(
+                G.A, b, G.myu, G.lam, rho, g, dt, gamma, beta, G.xyz, mesh, du, dv, da)
+        elif kind == "beuler":
+            ok = S.MergeLinSys_SolidLinear_BEuler_MeshTet3D(G.A, b, G.myu, G.lam, rho, g, dt,
+                                                            G.xyz, mesh, du, dv)
+        else:
+            ok = S.MergeLinSys_SolidStiffwarp_BEuler_MeshTet3D(G.A, b, G.myu, G.lam, rho, g, dt,
+                                                               G.xyz, mesh, du, dv, dR)
+        assert ok
+        return (b.cpu().numpy(),) + tuple(G.A.download())
+
+    for kind in ("static", "newmark", "beuler", "stiffwarp"):
+        print(f"[asm forms] {kind} {order}", flush=True)
+        monkeypatch.setenv("DFM2H_ASSEMBLY_WALK", "1")
+        ref = run(kind, meshes[0])
+        monkeypatch.delenv("DFM2H_ASSEMBLY_WALK")
+        for mesh in (meshes[0], meshes[0], meshes[1]):
+            out = run(kind, mesh)
+            assert all(np.array_equal(bits(x), bits(y)) for x, y in zip(out, ref)), kind
+
+
 def test_bicgstab_vs_oracle(gpu):
     """Device Solve_PBiCGStab on the warped (non-symmetric) system vs the
     oracle's: iteration counts within 1, x rel-L2 <= 1e-7, true residual."""

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <type_traits>
 
 namespace dfm2h {
@@ -634,22 +635,36 @@ __global__ void k_mixed_out(double *__restrict__ x, const double *__restrict__ x
 // ---------------------------------------------------------------------------
 // host side
 
+// A captured run of `chunk` fused iterations.
+struct ChunkGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> sig;
+  unsigned chunk = 0;
+  void drop() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    sig.clear();
+  }
+};
+
+// iterations per graph launch: large chunks save launches in long solves,
+// small ones bound the iterations that still run (as immediate returns)
+// after the device has finished, which dominates short solves
+constexpr unsigned kChunkLarge = 16, kChunkSmall = 4;
+
 template <typename T>
 struct PcgBufs {
   using elem = T;
   DBuf<T> r, x, Pr, p;
   DBuf<double> partials, partials2, partials3, partials4, hist;
   DBuf<PcgScal> S;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<uintptr_t> sig;
-  unsigned chunk = 0;
+  ChunkGraph g[2];  // iterations per launch: [0] kChunkLarge, [1] kChunkSmall
   void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-    sig.clear();
+    g[0].drop();
+    g[1].drop();
   }
   ~PcgBufs() { drop_graph(); }
 };
@@ -961,37 +976,76 @@ static void launch_iteration(const PcgArgs<T, M> &a, dfm2h_ilu P, int shape,
   hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a, 1);
 }
 
-// Captures `chunk` iterations into B's graph unless its signature matches.
-template <typename T, class F>
-static void ensure_graph(PcgBufs<T> &B, hipStream_t s, const std::vector<uintptr_t> &sig,
+// Captures `chunk` iterations into G unless its signature matches.
+template <class F>
+static void ensure_graph(ChunkGraph &G, hipStream_t s, const std::vector<uintptr_t> &sig,
                          unsigned chunk, F &&one_iteration) {
-  if (B.exec && B.sig == sig) return;
-  B.drop_graph();
+  if (G.exec && G.sig == sig) return;
+  G.drop();
   DFM2H_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (unsigned k = 0; k < chunk; ++k) one_iteration();
-  hipError_t e = hipStreamEndCapture(s, &B.graph);
+  hipError_t e = hipStreamEndCapture(s, &G.graph);
   DFM2H_REQUIRE(e == hipSuccess, DFM2H_ERR_HIP,
                 std::string("graph capture: ") + hipGetErrorString(e));
-  DFM2H_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
-  B.sig = sig;
-  B.chunk = chunk;
+  DFM2H_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+  G.sig = sig;
+  G.chunk = chunk;
 }
 
-// Launches the captured chunks until the device says done (polled one chunk
-// behind, so the device never idles) or nchunks ran; returns the status.
+// Captures B's large- and small-chunk graphs of the iteration.
+template <typename T, class F>
+static void ensure_graphs(PcgBufs<T> &B, hipStream_t s, unsigned max_iters,
+                          const std::function<std::vector<uintptr_t>(unsigned)> &sig,
+                          F &&one_iteration) {
+  const unsigned cl = std::max(1u, std::min(kChunkLarge, max_iters));
+  const unsigned cs = std::max(1u, std::min(kChunkSmall, max_iters));
+  ensure_graph(B.g[0], s, sig(cl), cl, one_iteration);
+  ensure_graph(B.g[1], s, sig(cs), cs, one_iteration);
+}
+
+// Launches captured chunks until the device says done (polled one chunk
+// behind, so the device never idles) or max_iters iterations were launched;
+// returns the status.  Small chunks run first, and again once the rate at
+// which |r|^2 has been falling predicts the end within two large chunks of
+// the iterations already launched; large chunks in between.  Which graph
+// runs changes no result: every kernel reads the device-side state.
 template <typename T>
-static void run_chunks(PcgWork &W, PcgBufs<T> &B, unsigned nchunks, PcgScal &status) {
+static void run_chunks(PcgWork &W, PcgBufs<T> &B, unsigned max_iters, bool mixed,
+                       PcgScal &status) {
   hipStream_t s = W.s;
-  bool finished = false;
-  for (unsigned c = 0; c < nchunks && !finished; ++c) {
-    DFM2H_HIP(hipGraphLaunch(B.exec, s));
+  bool finished = false, small = true;
+  unsigned launched = 0;
+  double prev_it = 0.0, prev_l = 0.0;
+  bool have_prev = false;
+  for (unsigned c = 0; launched < max_iters && !finished; ++c) {
+    const ChunkGraph &G = B.g[small ? 1 : 0];
+    DFM2H_HIP(hipGraphLaunch(G.exec, s));
+    launched += G.chunk;
     DFM2H_HIP(hipMemcpyAsync(&W.host_status[c & 1], B.S.p, sizeof(PcgScal),
                              hipMemcpyDeviceToHost, s));
     DFM2H_HIP(hipEventRecord(W.ev_poll[c & 1], s));
-    if (c > 0) {
-      DFM2H_HIP(hipEventSynchronize(W.ev_poll[(c - 1) & 1]));
-      finished = W.host_status[(c - 1) & 1].done != 0;
+    if (c == 0) continue;
+    DFM2H_HIP(hipEventSynchronize(W.ev_poll[(c - 1) & 1]));
+    const PcgScal &st = W.host_status[(c - 1) & 1];
+    finished = st.done != 0;
+    // log |r|^2 of the start (|r0|^2 or |b|^2) and of this state, the target
+    const double l_start = mixed ? std::log(st.sq_b) : -std::log(st.inv_sq0);
+    const double l_target = l_start + 2.0 * std::log(st.tol);
+    const double l = std::log(st.sq);
+    double rate = 0.0;  // decrease of log |r|^2 per iteration, recent if possible
+    if (have_prev && st.it > prev_it)
+      rate = (prev_l - l) / (st.it - prev_it);
+    else if (st.it > 0)
+      rate = (l_start - l) / st.it;
+    if (st.it > prev_it || !have_prev) {
+      prev_it = st.it;
+      prev_l = l;
+      have_prev = true;
     }
+    const double in_flight = (double)launched - st.it;
+    const double left = rate > 0.0 && std::isfinite(l) ? (l - l_target) / rate - in_flight
+                                                       : 1.0e30;
+    small = left < 2.0 * B.g[0].chunk;
   }
   DFM2H_HIP(hipMemcpyAsync(&W.host_status[0], B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, s));
   DFM2H_HIP(hipStreamSynchronize(s));
@@ -1101,10 +1155,9 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
-  const unsigned chunk = std::max(1u, std::min(16u, max_nitr));
-  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk,
-               [&] { launch_iteration(a, P, shape, Lo, s); });
-  run_chunks(W, B, (max_nitr + chunk - 1) / chunk, status);
+  ensure_graphs(B, s, max_nitr, [&](unsigned c) { return graph_sig(a, A, P, c); },
+                [&] { launch_iteration(a, P, shape, Lo, s); });
+  run_chunks(W, B, max_nitr, false, status);
 }
 
 // One mixed iteration: an FP32 PCG step whose beta step also decides
@@ -1152,10 +1205,10 @@ static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
   check_launch("mixed prologue");
-  const unsigned chunk = std::max(1u, std::min(16u, max_inner));
-  ensure_graph(B, s, graph_sig(a, A, P, chunk), chunk,
-               [&] { launch_mixed_iteration(a, P, shape, Lo, s); });
-  run_chunks(W, B, (max_inner + chunk - 1) / chunk + 1, status);
+  ensure_graphs(B, s, max_inner, [&](unsigned c) { return graph_sig(a, A, P, c); },
+                [&] { launch_mixed_iteration(a, P, shape, Lo, s); });
+  // one large chunk beyond the cap: the reliable update that follows it
+  run_chunks(W, B, max_inner + B.g[0].chunk, true, status);
 }
 
 static void fence_in(PcgWork &W) {

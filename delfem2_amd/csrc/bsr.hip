@@ -488,6 +488,94 @@ k_assemble_diag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
   if (live && l >= 9 && l < 12) A.vec_b[(size_t)i * 3 + (l - 9)] = sum;
 }
 
+// The Newmark-beta diagonal blocks and right-hand side with four lanes per
+// element, one per element node jn (eight elements per 32-lane row group and
+// round): lane (element, jn) forms K_e(li, jn) and M_e(li, jn) and the four
+// products the right-hand side needs at node jn -- K u, M a, K v, K a, each
+// row in add_block_vec's association -- and the lane with jn = li also the
+// element's diagonal block.  Twelve lanes then apply them in the order of
+// element_rhs_term: per element eres = body force, then -1 (K u) over jn,
+// -1 (M a) over jn, -dt (K v) over jn, -(0.5 dt^2) (K a) over jn; and the
+// diagonal block and vec_b receive the elements in ascending order.  Same
+// values as k_assemble_diag_lists<kAsmNewmark>, with a quarter of its
+// per-lane work and registers.
+__global__ void __launch_bounds__(64)
+k_assemble_diag_newmark(AsmArgs A, const unsigned *__restrict__ ptr,
+                        const unsigned *__restrict__ ent, const double *__restrict__ rec) {
+  constexpr unsigned EPR = kDiagLanes / 4;  // elements per round
+  __shared__ double prod[64][13];           // 4 products x 3 rows per lane, padded
+  __shared__ double dblk[2 * EPR][10];      // per element of the round: its block, its volume
+  const unsigned lane = threadIdx.x, grp = lane / kDiagLanes, g0 = grp * kDiagLanes;
+  const unsigned l = lane & (kDiagLanes - 1u), u = l / 4, jn = l % 4;
+  const unsigned i = wg_id(true) * (64 / kDiagLanes) + grp;
+  const bool live = i < A.nblk;
+  const size_t t = A.ncrs + (live ? i : 0u);
+  const unsigned q0 = live ? ptr[t] : 0u, q1 = live ? ptr[t + 1] : 0u;
+  double sum = 0.0;  // lane l < 9: diagonal entry l; 9..11: vec_b component l - 9
+  if (live && l < 9) sum = A.vdia[(size_t)i * 9 + l];
+  if (live && l >= 9 && l < 12) sum = A.vec_b[(size_t)i * 3 + (l - 9)];
+  unsigned rounds = q1 - q0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    rounds = max(rounds, (unsigned)__shfl_xor((int)rounds, off, 64));
+  rounds = (rounds + EPR - 1) / EPR;
+  for (unsigned c = 0; c < rounds; ++c) {
+    const unsigned qb = q0 + c * EPR, q = qb + u;
+    if (q < q1) {
+      const unsigned en = ent[q];
+      const unsigned e = en & 0x0FFFFFFFu;
+      const unsigned li = (en >> 28) & 3u;
+      const double *r = rec + (size_t)e * kTetRec;
+      const double vol = r[0], myu = r[13], lam = r[14];
+      const double di[3] = {r[1 + 3 * li], r[2 + 3 * li], r[3 + 3 * li]};
+      const double dj[3] = {r[1 + 3 * jn], r[2 + 3 * jn], r[3 + 3 * jn]};
+      const uint4 n = A.tet[e];
+      const unsigned node = jn == 0 ? n.x : jn == 1 ? n.y : jn == 2 ? n.z : n.w;
+      double kb[9], mb[9], xu[3], xv[3], xa[3];
+      emat_block<true>(kb, di, dj, vol, myu, lam);
+      mass_block(mb, A.rho * vol * 0.05, jn == li);
+      load_node3(xu, A.disp, node);
+      load_node3(xv, A.velo, node);
+      load_node3(xa, A.acc, node);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        prod[lane][0 + a] = kb[3 * a + 0] * xu[0] + kb[3 * a + 1] * xu[1] + kb[3 * a + 2] * xu[2];
+        prod[lane][3 + a] = mb[3 * a + 0] * xa[0] + mb[3 * a + 1] * xa[1] + mb[3 * a + 2] * xa[2];
+        prod[lane][6 + a] = kb[3 * a + 0] * xv[0] + kb[3 * a + 1] * xv[1] + kb[3 * a + 2] * xv[2];
+        prod[lane][9 + a] = kb[3 * a + 0] * xa[0] + kb[3 * a + 1] * xa[1] + kb[3 * a + 2] * xa[2];
+      }
+      if (jn == li) {  // (&emat)[k] = eM[k] + (beta dt dt) eK[k] of the block (li, li)
+        const double sc = A.beta_nb * A.dt * A.dt;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dblk[grp * EPR + u][k] = mb[k] + sc * kb[k];
+        dblk[grp * EPR + u][9] = vol;
+      }
+    }
+    __syncthreads();
+    if (l < 12 && qb < q1) {
+      const unsigned m = min(q1 - qb, EPR);
+      const double g = l == 9 ? A.g0 : l == 10 ? A.g1 : A.g2;
+      const unsigned a = l - 9;
+      for (unsigned k = 0; k < m; ++k) {
+        if (l < 9) {
+          sum += dblk[grp * EPR + k][l];
+        } else {
+          const unsigned b = g0 + 4 * k;
+          double er = dblk[grp * EPR + k][9] * A.rho * g * 0.25;
+          for (int j = 0; j < 4; ++j) er += -1. * prod[b + j][0 + a];
+          for (int j = 0; j < 4; ++j) er += -1. * prod[b + j][3 + a];
+          for (int j = 0; j < 4; ++j) er += -A.dt * prod[b + j][6 + a];
+          for (int j = 0; j < 4; ++j) er += -0.5 * A.dt * A.dt * prod[b + j][9 + a];
+          sum += er;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (live && l < 9) A.vdia[(size_t)i * 9 + l] = sum;
+  if (live && l >= 9 && l < 12) A.vec_b[(size_t)i * 3 + (l - 9)] = sum;
+}
+
 // SetFixedBC (ls_block_sparse.cpp:397-472): entries in a fixed row or a
 // fixed column become 0; the diagonal entry of a fixed DOF becomes 1.
 __global__ void k_set_fixed_bc(unsigned nblk, size_t ncrs, const unsigned *__restrict__ rowof,
@@ -637,8 +725,12 @@ static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu
       hipLaunchKernelGGL(k_assemble_offdiag_lists<K>, go, block, 0, stream(), a, pp, pe, pr); \
     hipLaunchKernelGGL(k_assemble_diag_lists<K>, gd, bd, 0, stream(), a, pp, pe, pr);        \
   } while (0)
-    if (kind == kAsmNewmark)
-      DFM2H_ASM_LISTS(kAsmNewmark);
+    if (kind == kAsmNewmark) {
+      if (A->pat.ncrs)
+        hipLaunchKernelGGL(k_assemble_offdiag_lists<kAsmNewmark>, go, block, 0, stream(), a, pp,
+                           pe, pr);
+      hipLaunchKernelGGL(k_assemble_diag_newmark, gd, bd, 0, stream(), a, pp, pe, pr);
+    }
     else if (kind == kAsmStiffwarp)
       DFM2H_ASM_LISTS(kAsmStiffwarp);
     else if (kind == kAsmBEuler)

@@ -337,7 +337,10 @@ __global__ void __launch_bounds__(256) k_tet_records(AsmArgs A, size_t ntet, dou
 }
 
 // Element entry of destination block t: the elements k_assemble_tet_gather
-// would add there, in its order, packed as e | li << 28 | lj << 30.  ptr ==
+// would add there, in its order, packed as p | li << 28 | lj << 30 where p is
+// the element id for a diagonal block (t >= ncrs: the row node's whole
+// element list) and, for an off-diagonal block, the element's position in
+// its row node's list (so a row's records can be staged once).  ptr ==
 // nullptr: count them into cnt[t] (and flag missing slots); else write them
 // from ptr[t].
 __global__ void __launch_bounds__(256)
@@ -357,7 +360,8 @@ k_asm_lists(AsmArgs A, const unsigned *__restrict__ ptr, unsigned *out) {
     const int lj = diag ? li : local_index(n, j);
     if (lj < 0) continue;
     if (ptr)
-      out[w + c] = e | ((unsigned)li & 3u) << 28 | ((unsigned)lj & 3u) << 30;
+      out[w + c] = (diag ? e : q - A.elsup_ind[i]) | ((unsigned)li & 3u) << 28 |
+                   ((unsigned)lj & 3u) << 30;
     else if (diag)
       check_row_slots(A, i, n);
     ++c;
@@ -382,6 +386,7 @@ k_assemble_offdiag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
   double acc[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) acc[q] = dst[q];
+  const unsigned *row_ent = ent + ptr[A.ncrs + i];  // the row node's elements
   // elements in rounds of U: the entries, then their records, are requested
   // together, then added in order
   constexpr int U = 4;
@@ -395,7 +400,8 @@ k_assemble_offdiag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
     for (int u = 0; u < U; ++u) {
       if (qb + u < q1) {
         const unsigned li = (en[u] >> 28) & 3u, lj = en[u] >> 30;
-        const double *r = rec + (size_t)(en[u] & 0x0FFFFFFFu) * kTetRec;
+        const unsigned e = row_ent[en[u] & 0x0FFFFFFFu] & 0x0FFFFFFFu;
+        const double *r = rec + (size_t)e * kTetRec;
         vol[u] = r[0];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -419,6 +425,54 @@ k_assemble_offdiag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
   }
 #pragma unroll
   for (int q = 0; q < 9; ++q) dst[q] = acc[q];
+}
+
+// Off-diagonal blocks, 32 lanes per row node: the records of the node's
+// elements (its diagonal list, at most kRowElems) are staged in LDS with
+// 16-byte loads, then lane l adds the elements of the row's slots l, l + 32,
+// ... from there -- the additions of k_assemble_offdiag_lists, with a record
+// read from HBM once per row instead of nine scattered loads per block.
+constexpr unsigned kRowElems = 32;
+
+template <int KIND>
+__global__ void __launch_bounds__(64)
+k_assemble_offdiag_rows(AsmArgs A, const unsigned *__restrict__ ptr,
+                        const unsigned *__restrict__ ent, const double *__restrict__ rec) {
+  __shared__ double recs[2][kRowElems][kTetRec];
+  const unsigned lane = threadIdx.x, grp = lane / 32, l = lane & 31u;
+  const unsigned i = wg_id(true) * 2 + grp;
+  const bool live = i < A.nblk;
+  const unsigned d0 = live ? ptr[A.ncrs + i] : 0u, d1 = live ? ptr[A.ncrs + i + 1] : 0u;
+  const unsigned m = min(d1 - d0, kRowElems);  // the host guarantees d1 - d0 <= kRowElems
+  for (unsigned k = l; k < m * (kTetRec / 2); k += 32) {
+    const unsigned rr = k / (kTetRec / 2), part = k % (kTetRec / 2);
+    const unsigned e = ent[d0 + rr] & 0x0FFFFFFFu;
+    const double2 v = *reinterpret_cast<const double2 *>(rec + (size_t)e * kTetRec + 2 * part);
+    recs[grp][rr][2 * part] = v.x;
+    recs[grp][rr][2 * part + 1] = v.y;
+  }
+  __syncthreads();
+  const unsigned s0 = live ? A.colind[i] : 0u, s1 = live ? A.colind[i + 1] : 0u;
+  for (unsigned sl = s0 + l; sl < s1; sl += 32) {
+    double *dst = A.vcrs + (size_t)sl * 9;
+    double acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = dst[q];
+    const unsigned q1 = ptr[sl + 1];
+    for (unsigned q = ptr[sl]; q < q1; ++q) {
+      const unsigned en = ent[q];
+      const unsigned li = (en >> 28) & 3u, lj = en >> 30;
+      const double *r = recs[grp][en & 0x0FFFFFFFu];
+      const double di[3] = {r[1 + 3 * li], r[2 + 3 * li], r[3 + 3 * li]};
+      const double dj[3] = {r[1 + 3 * lj], r[2 + 3 * lj], r[3 + 3 * lj]};
+      double blk[9];
+      element_block<KIND>(A, i, r[0], di, dj, li == lj, r[13], r[14], blk);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc[k] += blk[k];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dst[q] = acc[q];
+  }
 }
 
 // Diagonal blocks and right-hand side over the element lists: 32 lanes per
@@ -505,6 +559,7 @@ k_assemble_diag_newmark(AsmArgs A, const unsigned *__restrict__ ptr,
   constexpr unsigned EPR = kDiagLanes / 4;  // elements per round
   __shared__ double prod[64][13];           // 4 products x 3 rows per lane, padded
   __shared__ double dblk[2 * EPR][10];      // per element of the round: its block, its volume
+  __shared__ double eres[2 * EPR][3];       // per element of the round: its vec_b addend
   const unsigned lane = threadIdx.x, grp = lane / kDiagLanes, g0 = grp * kDiagLanes;
   const unsigned l = lane & (kDiagLanes - 1u), u = l / 4, jn = l % 4;
   const unsigned i = wg_id(true) * (64 / kDiagLanes) + grp;
@@ -552,23 +607,21 @@ k_assemble_diag_newmark(AsmArgs A, const unsigned *__restrict__ ptr,
       }
     }
     __syncthreads();
-    if (l < 12 && qb < q1) {
+    if (l < 3 * EPR && qb + l / 3 < q1) {  // element k = l / 3, row a = l % 3: its eres_a
+      const unsigned k = l / 3, a = l % 3, b = g0 + 4 * k;
+      const double g = a == 0 ? A.g0 : a == 1 ? A.g1 : A.g2;
+      double er = dblk[grp * EPR + k][9] * A.rho * g * 0.25;
+      for (int j = 0; j < 4; ++j) er += -1. * prod[b + j][0 + a];
+      for (int j = 0; j < 4; ++j) er += -1. * prod[b + j][3 + a];
+      for (int j = 0; j < 4; ++j) er += -A.dt * prod[b + j][6 + a];
+      for (int j = 0; j < 4; ++j) er += -0.5 * A.dt * A.dt * prod[b + j][9 + a];
+      eres[grp * EPR + k][a] = er;
+    }
+    __syncthreads();
+    if (l < 12 && qb < q1) {  // the elements in ascending order
       const unsigned m = min(q1 - qb, EPR);
-      const double g = l == 9 ? A.g0 : l == 10 ? A.g1 : A.g2;
-      const unsigned a = l - 9;
-      for (unsigned k = 0; k < m; ++k) {
-        if (l < 9) {
-          sum += dblk[grp * EPR + k][l];
-        } else {
-          const unsigned b = g0 + 4 * k;
-          double er = dblk[grp * EPR + k][9] * A.rho * g * 0.25;
-          for (int j = 0; j < 4; ++j) er += -1. * prod[b + j][0 + a];
-          for (int j = 0; j < 4; ++j) er += -1. * prod[b + j][3 + a];
-          for (int j = 0; j < 4; ++j) er += -A.dt * prod[b + j][6 + a];
-          for (int j = 0; j < 4; ++j) er += -0.5 * A.dt * A.dt * prod[b + j][9 + a];
-          sum += er;
-        }
-      }
+      for (unsigned k = 0; k < m; ++k)
+        sum += l < 9 ? dblk[grp * EPR + k][l] : eres[grp * EPR + k][l - 9];
     }
     __syncthreads();
   }
@@ -719,16 +772,21 @@ static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu
     const double *pr = A->asm_rec.p;
     const dim3 go(ceil_div(std::max<size_t>(A->pat.ncrs, 1), 256));
     const dim3 gd(ceil_div(A->nblk, 64 / kDiagLanes)), bd(64);
+    const bool rows = M->max_elsup <= kRowElems;
+#define DFM2H_ASM_OFFDIAG(K)                                                                  \
+  do {                                                                                       \
+    if (A->pat.ncrs && rows)                                                                 \
+      hipLaunchKernelGGL(k_assemble_offdiag_rows<K>, gd, bd, 0, stream(), a, pp, pe, pr);    \
+    else if (A->pat.ncrs)                                                                    \
+      hipLaunchKernelGGL(k_assemble_offdiag_lists<K>, go, block, 0, stream(), a, pp, pe, pr); \
+  } while (0)
 #define DFM2H_ASM_LISTS(K)                                                                   \
   do {                                                                                       \
-    if (A->pat.ncrs)                                                                         \
-      hipLaunchKernelGGL(k_assemble_offdiag_lists<K>, go, block, 0, stream(), a, pp, pe, pr); \
+    DFM2H_ASM_OFFDIAG(K);                                                                    \
     hipLaunchKernelGGL(k_assemble_diag_lists<K>, gd, bd, 0, stream(), a, pp, pe, pr);        \
   } while (0)
     if (kind == kAsmNewmark) {
-      if (A->pat.ncrs)
-        hipLaunchKernelGGL(k_assemble_offdiag_lists<kAsmNewmark>, go, block, 0, stream(), a, pp,
-                           pe, pr);
+      DFM2H_ASM_OFFDIAG(kAsmNewmark);
       hipLaunchKernelGGL(k_assemble_diag_newmark, gd, bd, 0, stream(), a, pp, pe, pr);
     }
     else if (kind == kAsmStiffwarp)
@@ -738,6 +796,7 @@ static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu
     else
       DFM2H_ASM_LISTS(kAsmStatic);
 #undef DFM2H_ASM_LISTS
+#undef DFM2H_ASM_OFFDIAG
     check_launch("k_assemble_lists");
   }
   int h_missing = 0;

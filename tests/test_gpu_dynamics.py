@@ -208,6 +208,52 @@ def test_listed_assembly_equals_walking_form(gpu, order, monkeypatch):
             assert all(np.array_equal(bits(x), bits(y)) for x, y in zip(out, ref)), kind
 
 
+def test_listed_assembly_many_elements_per_node(gpu, monkeypatch):
+    """A fan of 40 tets around one node (more than a row's 32 staged
+    records): the listed assembly's per-block path gives the walking form's
+    bits, static and Newmark-beta."""
+    nring = 20
+    ang = 2 * np.pi * np.arange(nring) / nring
+    xyz = np.concatenate([[0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, -1.0],
+                          np.stack([np.cos(ang), np.sin(ang), 0.1 * np.sin(3 * ang)], 1).ravel()])
+    tets = []
+    for k in range(nring):
+        a_, b_ = 3 + k, 3 + (k + 1) % nring
+        tets += [[0, 1, a_, b_], [0, 2, b_, a_]]
+    tet = np.array(tets, np.uint32)
+    npt = xyz.size // 3
+    ind, ps = oracle.psup(tet, npt)
+    A = S.CMatrixSparse()
+    A.Initialize(npt, 3, True)
+    A.SetPattern(ind, ps)
+    mesh = S.TetMesh(tet, npt)
+    ne = len(tet)
+    myu = dev(1.0 + W.uniform(51, ne))
+    lam = dev(0.5 + W.uniform(52, ne))
+    du, dv, da = (dev((W.uniform(k, 3 * npt) - 0.5) * 0.1) for k in (53, 54, 55))
+    dx = dev(xyz)
+    g = [0.1, -0.4, -2.0]
+
+    def run(kind):
+        A.setZero()
+        b = torch.zeros(3 * npt, dtype=torch.float64, device="cuda")
+        if kind == "static":
+            ok = S.MergeLinSys_SolidLinear_Static_MeshTet3D(A, b, myu, lam, 1.1, g, dx, mesh, du)
+        else:
+            ok = S.MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(A, b, myu, lam, 1.1, g, 0.02, 0.6,
+                                                                 0.3, dx, mesh, du, dv, da)
+        assert ok
+        return (b.cpu().numpy(),) + tuple(A.download())
+
+    for kind in ("static", "newmark"):
+        monkeypatch.setenv("DFM2H_ASSEMBLY_WALK", "1")
+        ref = run(kind)
+        monkeypatch.delenv("DFM2H_ASSEMBLY_WALK")
+        out = run(kind)
+        assert all(np.array_equal(bits(x), bits(y)) for x, y in zip(out, ref)), kind
+        assert np.abs(ref[2]).max() > 0
+
+
 def test_bicgstab_vs_oracle(gpu):
     """Device Solve_PBiCGStab on the warped (non-symmetric) system vs the
     oracle's: iteration counts within 1, x rel-L2 <= 1e-7, true residual."""

@@ -826,8 +826,7 @@ void ensure_batch_vals(dfm2h_batch B, PackedVals<T> &av, PackedVals<T> &fv, Pack
     const unsigned nsl = B->aplan.nslices;
     dv.dia.alloc(std::max<size_t>((size_t)nsl * kSliceVals, 1));
     if (nsl) {
-      const size_t n = (size_t)nsl * kSliceVals;
-      hipLaunchKernelGGL(k_pack_dia<T>, dim3(ceil_div(n, 256)), dim3(256), 0, B->s, nsl,
+      hipLaunchKernelGGL(k_pack_dia<T>, dim3(pack_grid(nsl)), dim3(256), 0, B->s, nsl,
                          B->row_l0.p, P->vdia.p, dv.dia.p);
       check_launch("k_pack_dia");
     }

@@ -30,15 +30,13 @@ void ensure_packed(const PackedPlan &P, PackedVals<T> &V, const double *crs, con
   if (V.version == version && (V.val.p || P.nq == 0)) return;
   V.val.alloc(P.nq * kSliceVals);
   if (P.nq) {
-    const size_t n = P.nq * kSliceVals;
-    hipLaunchKernelGGL(k_pack_blocks<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, P.nq, P.src.p,
+    hipLaunchKernelGGL(k_pack_blocks<T>, dim3(pack_grid(P.nq)), dim3(256), 0, s, P.nq, P.src.p,
                        crs, V.val.p);
   }
   if (dia) {
     V.dia.alloc((size_t)P.nslices * kSliceVals);
     if (P.nslices) {
-      const size_t n = (size_t)P.nslices * kSliceVals;
-      hipLaunchKernelGGL(k_pack_dia<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, P.nslices,
+      hipLaunchKernelGGL(k_pack_dia<T>, dim3(pack_grid(P.nslices)), dim3(256), 0, s, P.nslices,
                          P.row.p, dia, V.dia.p);
     }
   }

@@ -148,32 +148,50 @@ struct PackedVals {
 // ---------------------------------------------------------------------------
 // packing kernels (gather from CRS storage, optional rounding to T)
 
+// One thread per (block-slot or slice, lane < 63): the lane's three values
+// (one row of a 3x3 block: contiguous in CRS) are read together and written
+// to their places in the layout.  Launch with pack_grid(count) x 256.
+inline unsigned pack_grid(size_t count) {
+  return (unsigned)((count * 63 + 255) / 256);
+}
+
+template <typename T>
+__device__ inline void put_lane(T *__restrict__ out, size_t q, unsigned lane, const double *v) {
+  if (planar_vals<T>()) {
+    T *o = out + q * kSliceVals + lane;
+    o[0] = v ? (T)v[0] : (T)0;
+    o[63] = v ? (T)v[1] : (T)0;
+    o[126] = v ? (T)v[2] : (T)0;
+  } else {
+    T *o = out + (q * 63 + lane) * 3;
+    o[0] = v ? (T)v[0] : (T)0;
+    o[1] = v ? (T)v[1] : (T)0;
+    o[2] = v ? (T)v[2] : (T)0;
+  }
+}
+
 template <typename T>
 __global__ void k_pack_blocks(size_t nq, const int *__restrict__ src, const double *__restrict__ crs,
                               T *__restrict__ val) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // over nq*21*9
-  if (t >= nq * kSliceVals) return;
-  const size_t q = t / kSliceVals;
-  const unsigned rem = (unsigned)(t % kSliceVals);
-  const unsigned lane = planar_vals<T>() ? rem % 63 : rem / 3;
-  const unsigned tt = planar_vals<T>() ? rem / 63 : rem % 3;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // over nq*63
+  if (t >= nq * 63) return;
+  const size_t q = t / 63;
+  const unsigned lane = (unsigned)(t - q * 63);
   const unsigned c = lane / kSliceRows, r = lane % kSliceRows;
   const int s = src[q * kSliceRows + r];
-  val[t] = s >= 0 ? (T)crs[(size_t)s * 9 + c * 3 + tt] : (T)0;
+  put_lane(val, q, lane, s >= 0 ? crs + (size_t)s * 9 + c * 3 : nullptr);
 }
 
 template <typename T>
 __global__ void k_pack_dia(unsigned nslices, const unsigned *__restrict__ row,
                            const double *__restrict__ dia_crs, T *__restrict__ dia) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nslices * kSliceVals) return;
-  const size_t s = t / kSliceVals;
-  const unsigned rem = (unsigned)(t % kSliceVals);
-  const unsigned lane = planar_vals<T>() ? rem % 63 : rem / 3;
-  const unsigned tt = planar_vals<T>() ? rem / 63 : rem % 3;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // over nslices*63
+  if (t >= (size_t)nslices * 63) return;
+  const size_t s = t / 63;
+  const unsigned lane = (unsigned)(t - s * 63);
   const unsigned c = lane / kSliceRows, r = lane % kSliceRows;
   const unsigned i = row[s * kSliceRows + r];
-  dia[t] = i != ~0u ? (T)dia_crs[(size_t)i * 9 + c * 3 + tt] : (T)0;
+  put_lane(dia, s, lane, i != ~0u ? dia_crs + (size_t)i * 9 + c * 3 : nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -808,8 +808,7 @@ static void ensure_dinv(dfm2h_bsr A, dfm2h_ilu P, PackedVals<T> &V, hipStream_t 
   const unsigned nsl = A->plan.nslices;
   V.dia.alloc((size_t)nsl * kSliceVals);
   if (nsl) {
-    const size_t n = (size_t)nsl * kSliceVals;
-    hipLaunchKernelGGL(k_pack_dia<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, nsl,
+    hipLaunchKernelGGL(k_pack_dia<T>, dim3(pack_grid(nsl)), dim3(256), 0, s, nsl,
                        P->work->fu.row_l0.p, P->vdia.p, V.dia.p);
     check_launch("k_pack_dia");
   }

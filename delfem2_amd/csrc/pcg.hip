@@ -593,17 +593,42 @@ k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *
   if (threadIdx.x == 0) dst[wg_id(a.xcd)] = part;
 }
 
-// p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue)
+// p = beta p + Pr (mode 1), or p = Pr (mode 0, prologue).  Four entries per
+// thread with 16-byte loads (FP32; two for FP64), one pass over the grid
+// pupdate_grid(n); the remainder of n / 4 by the first threads.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
+inline unsigned pupdate_grid(size_t n) { return std::max(1u, ceil_div(n / 4, kVecBlock)); }
+
 template <typename T, typename M>
 __global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T, M> a, int mode) {
   if (a.S->done) return;
   const double beta = a.S->beta;
-  for (size_t i = (size_t)blockIdx.x * kVecBlock + threadIdx.x; i < a.n;
-       i += (size_t)gridDim.x * kVecBlock) {
-    if (mode == 0)
-      a.p[i] = a.Pr[i];
-    else
-      a.p[i] = (T)(beta * (double)a.p[i] + (double)a.Pr[i]);
+  using V = typename Vec4<T>::type;
+  const size_t k = (size_t)blockIdx.x * kVecBlock + threadIdx.x, n4 = a.n / 4;
+  if (k < n4) {
+    const V z = reinterpret_cast<const V *>(a.Pr)[k];
+    V &pk = reinterpret_cast<V *>(a.p)[k];
+    if (mode == 0) {
+      pk = z;
+    } else {
+      const V p = pk;
+      pk = V{(T)(beta * (double)p.x + (double)z.x), (T)(beta * (double)p.y + (double)z.y),
+             (T)(beta * (double)p.z + (double)z.z), (T)(beta * (double)p.w + (double)z.w)};
+    }
+  }
+  if (k < a.n - 4 * n4) {
+    const size_t i = 4 * n4 + k;
+    a.p[i] = mode == 0 ? a.Pr[i] : (T)(beta * (double)a.p[i] + (double)a.Pr[i]);
   }
 }
 
@@ -972,7 +997,7 @@ static void launch_iteration(const PcgArgs<T, M> &a, dfm2h_ilu P, int shape,
   hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, shape, Lo, s);
   launch_scalar<kScalConvBeta>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a, 1);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
 }
 
 // Captures `chunk` iterations into G unless its signature matches.
@@ -1152,7 +1177,7 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   a.xcd = shape_xcd(shape);
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
   ensure_graphs(B, s, max_nitr, [&](unsigned c) { return graph_sig(a, A, P, c); },
                 [&] { launch_iteration(a, P, shape, Lo, s); });
@@ -1177,7 +1202,7 @@ static void launch_mixed_iteration(const PcgArgs<T, float> &a, dfm2h_ilu P, int 
   hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, shape, Lo, s);
   launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(gv), dim3(kVecBlock), 0, s, a, 1);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
 }
 
 template <typename T>
@@ -1202,7 +1227,7 @@ static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   a.xcd = shape_xcd(shape);
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(gv), dim3(kVecBlock), 0, s, a, 0);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("mixed prologue");
   ensure_graphs(B, s, max_inner, [&](unsigned c) { return graph_sig(a, A, P, c); },
                 [&] { launch_mixed_iteration(a, P, shape, Lo, s); });
@@ -1487,7 +1512,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
           launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
         else
           launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
-        hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(gv), dim3(kVecBlock), 0, W.s, a, 1);
+        hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, W.s, a, 1);
         DFM2H_HIP(hipEventRecord(ev[8], W.s));
         DFM2H_HIP(hipEventSynchronize(ev[8]));
         if (rep == 0) continue;  // warm-up iteration

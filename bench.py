@@ -65,6 +65,40 @@ def kernel_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0, sv: int 
     }
 
 
+def host_cpu() -> dict:
+    """CPU model and logical core count of the machine the CPU baseline ran on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+def stream_copy_gbs(mib: int = 1024, reps: int = 10) -> float:
+    """Measured device-to-device copy rate (read + write bytes per second, GB/s)
+    of a plain copy of `mib` MiB, the achievable counterpart of the 8 TB/s
+    specification peak."""
+    import torch
+
+    src = torch.empty(mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * (mib << 20) / (ms * 1e-3) / 1e9
+
+
 def level0_rows(psup_ind, psup, nb: int):
     """(rows with no lower neighbour, rows with no upper neighbour) of the
     one-ring pattern: the sizes of forward / backward level 0."""
@@ -118,7 +152,7 @@ def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
                    f"Decompose {t2 - t1:.3f} s, {n_it} PCG iterations {t3 - t2:.2f} s "
                    f"({t_it * 1e3:.1f} ms/iter) extrapolated to {iters_fp64} iterations"),
         "dof_iter_per_s": round(prob.ndof * n_it / (t3 - t2), 1),
-        "host": platform.processor() or platform.machine(),
+        **host_cpu(),
     }
 
 
@@ -155,7 +189,7 @@ def cpu_baseline_batch(bat, it64, budget_s: float = 15.0):
             "cores": 1, "kind": kind,
             "sample": f"first {done} of {bat.nmesh} meshes solved in {t:.2f} s; scaled by "
                       f"sum(ndof x iterations) of all meshes / of the sample",
-            "host": platform.processor() or platform.machine()}
+            **host_cpu()}
 
 
 def bench_batch(args):
@@ -515,6 +549,13 @@ def main():
         step_mixed()
     torch.cuda.synchronize()
 
+    # CopyValue + Decompose alone (part of every step below; T_pcg = step - this)
+    t0 = time.perf_counter()
+    for _ in range(max(args.steps, 3)):
+        G.factor()
+    torch.cuda.synchronize()
+    t_factor = (time.perf_counter() - t0) / max(args.steps, 3)
+
     # ---- timed: FP32 + FP64 refinement (the headline) ----
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -566,6 +607,7 @@ def main():
         v, name, tms = G.ilu.sweep_shape(fp32)
         shapes[prec] = {"shape": name, "tune_ms_per_sweep_pair": [round(t, 5) for t in tms]}
 
+    copy_gbs = stream_copy_gbs()
     b_iter64 = pcg_iter_bytes(nb, nnzb, 8, n0, nu0)
     t_it64 = t_fp64 / max(it64, 1)
 
@@ -604,7 +646,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_launch": int(launch_bytes), "avg_launch_ms": round(launch_ms, 5)},
+                     "bytes_per_launch": int(launch_bytes), "avg_launch_ms": round(launch_ms, 5),
+                     "stream_copy_gbs": round(copy_gbs, 1),
+                     "frac_of_stream_copy": round(dom_gbs / copy_gbs, 4)},
+        "factor_ms": round(t_factor * 1e3, 3),
+        "pcg_s": {"mixed": round(t_mixed - t_factor, 5), "fp64": round(t_fp64 - t_factor, 5)},
         "kernels": kern,
         "sweep_shape": shapes,
         "cpu_baseline": cpu,

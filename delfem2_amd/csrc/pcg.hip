@@ -436,32 +436,61 @@ __device__ inline void fwd_level0(const PcgArgs<T, M> &a, const SliceLane &L, un
 }
 
 // r += -alpha Ap, x += alpha p, partial (r,r); then forward level 0.
-// One wavefront per matrix slice (rows in order, components in lanes).
-// Mixed form with FP32 vectors, right after a reliable update: x32 is first
-// folded into x64.
+// Each wavefront takes kUpdSlices consecutive matrix slices (rows in order,
+// components in lanes): all their entries are loaded before any is stored,
+// so the slices' loads are in flight together.  Mixed form with FP32
+// vectors, right after a reliable update: x32 is first folded into x64.
+constexpr unsigned kUpdSlices = 2;
+
+inline unsigned update_grid(unsigned nsl) {
+  return std::max(1u, ceil_div(nsl, kSlicesPerBlock * kUpdSlices));
+}
+
 template <typename T, typename M>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   if (a.S->done) return;
   const double alpha = a.S->alpha;
-  const SliceLane L = slice_lane(0, a.xcd);
-  const bool sl = L.s < a.a_nsl;
-  const unsigned i = row_of(a.a_plan, a.a_gv, L, 0, sl);
   const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && a.S->upd;  // an update just ran
-  double s2 = 0.0, srp = 0.0;
-  T rn = 0;
-  if (i != ~0u) {
-    const size_t e = kVS * (size_t)i + L.c;
-    rn = (T)((double)a.r[e] + (-alpha) * (double)a.Pr[e]);
-    a.r[e] = rn;
-    if (fold) {  // x64 += x32, x32 = alpha p
-      a.x64[e] += (double)a.x[e];
-      a.x[e] = (T)(alpha * (double)a.p[e]);
-    } else {
-      a.x[e] = (T)((double)a.x[e] + alpha * (double)a.p[e]);
+  const unsigned wv = wg_id(a.xcd) * kSlicesPerBlock + threadIdx.x / kWave;
+  SliceLane L[kUpdSlices];
+  bool sl[kUpdSlices];
+  unsigned i[kUpdSlices];
+  T r0[kUpdSlices], q0[kUpdSlices], x0[kUpdSlices], p0[kUpdSlices];
+#pragma unroll
+  for (unsigned h = 0; h < kUpdSlices; ++h) {
+    L[h].lane = threadIdx.x & (kWave - 1);
+    L[h].s = wv * kUpdSlices + h;
+    L[h].c = L[h].lane / kSliceRows;
+    L[h].r = L[h].lane - L[h].c * kSliceRows;
+    sl[h] = L[h].s < a.a_nsl;
+    i[h] = row_of(a.a_plan, a.a_gv, L[h], 0, sl[h]);
+    r0[h] = q0[h] = x0[h] = p0[h] = 0;
+    if (i[h] != ~0u) {
+      const size_t e = kVS * (size_t)i[h] + L[h].c;
+      r0[h] = a.r[e];
+      q0[h] = a.Pr[e];
+      x0[h] = a.x[e];
+      p0[h] = a.p[e];
     }
-    s2 = (double)rn * (double)rn;
   }
-  if (sl) fwd_level0(a, L, i, rn, srp);
+  double s2 = 0.0, srp = 0.0;
+#pragma unroll
+  for (unsigned h = 0; h < kUpdSlices; ++h) {
+    T rn = 0;
+    if (i[h] != ~0u) {
+      const size_t e = kVS * (size_t)i[h] + L[h].c;
+      rn = (T)((double)r0[h] + (-alpha) * (double)q0[h]);
+      a.r[e] = rn;
+      if (fold) {  // x64 += x32, x32 = alpha p
+        a.x64[e] += (double)x0[h];
+        a.x[e] = (T)(alpha * (double)p0[h]);
+      } else {
+        a.x[e] = (T)((double)x0[h] + alpha * (double)p0[h]);
+      }
+      s2 += (double)rn * (double)rn;
+    }
+    if (sl[h]) fwd_level0(a, L[h], i[h], rn, srp);
+  }
   s2 = block_reduce_sum<kPackBlock>(s2);
   const bool iso = a.upd_part_off != ~0u;  // rows with neither lower nor upper blocks exist
   if (iso) srp = block_reduce_sum<kPackBlock>(srp);
@@ -789,7 +818,7 @@ static SweepLayout sweep_layout(dfm2h_bsr A, dfm2h_ilu P, int split) {
     }
   if (P->work->fu.any_iso) {  // the update kernel sums those rows' (r, Pr)
     Lo.upd_off = off;
-    off += slice_grid(A->plan.nslices);
+    off += update_grid(A->plan.nslices);
   } else {
     Lo.upd_off = ~0u;
   }
@@ -994,9 +1023,10 @@ static void launch_iteration(const PcgArgs<T, M> &a, dfm2h_ilu P, int shape,
   const unsigned gv = vec_grid(a.n);
   hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
-  hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, shape, Lo, s);
-  launch_scalar<kScalConvBeta>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
+  launch_scalar<kScalConvBeta>(s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3, Lo.n3,
+                               a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
 }
 
@@ -1199,9 +1229,10 @@ static void launch_mixed_iteration(const PcgArgs<T, float> &a, dfm2h_ilu P, int 
   hipLaunchKernelGGL((k_pcg_spmv<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64,
                                  a.partials4, gs);
-  hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, s, a);
   launch_sweeps_fused(a, P, shape, Lo, s);
-  launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, gs, a.partials3, Lo.n3, a.hist);
+  launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3, Lo.n3,
+                                 a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
 }
 
@@ -1500,7 +1531,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         else
           launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[2], W.s));
-        hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(gs), dim3(kPackBlock), 0, W.s, a);
+        hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[3], W.s));
         DFM2H_HIP(hipEventRecord(ev[4], W.s));
         DFM2H_HIP(hipEventRecord(ev[5], W.s));
@@ -1509,9 +1540,11 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         launch_sweeps_fused(a, P, shape, Lo, W.s, 2);
         DFM2H_HIP(hipEventRecord(ev[7], W.s));
         if constexpr (is_mixed<T, M>())
-          launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
+          launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3,
+                                         n3, a.hist);
         else
-          launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, gs, a.partials3, n3, a.hist);
+          launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3,
+                                        n3, a.hist);
         hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, W.s, a, 1);
         DFM2H_HIP(hipEventRecord(ev[8], W.s));
         DFM2H_HIP(hipEventSynchronize(ev[8]));

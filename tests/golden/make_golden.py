@@ -158,6 +158,11 @@ def main():
         p16 = W.cube(16, sigma=0.5, seed=20261203, ordering=order)
         o16 = ref_case(p16)
         meta["cases"][f"c16_sigma05_{order}"] = summary(o16)
+    # --- C2 at full size (64^3, E lognormal(0, 0.5) seed 20261203, colour-4 order: the
+    # headline configuration): hashes, iterations, history, solution norms
+    p64 = W.cube(64, sigma=0.5, seed=20261203, ordering="color4")
+    o64 = ref_case(p64, maxit=20000)
+    meta["cases"]["c64_sigma05_color4"] = summary(o64)
     # --- SIMP cantilever 8x4x4 (point-load patch, x=0 clamped)
     pc = W.cantilever(8, 4, 4, seed=7)
     oc = ref_case(pc)

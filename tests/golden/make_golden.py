@@ -32,9 +32,9 @@ def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000, fill=0):
-    """Full reference pipeline on one problem (ILU(fill): Initialize_ILUk);
-    returns a dict of outputs."""
+def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000, fill=0, pcg=True):
+    """Full reference pipeline on one problem (ILU(fill): Initialize_ILUk;
+    pcg=False stops after SolvePrecond); returns a dict of outputs."""
     R = oracle.ref()
     ind = np.zeros(prob.np_ + 1, np.uint32)
     n = R.ref_psup(prob.tet, prob.ne, prob.np_, ind, None)
@@ -60,6 +60,11 @@ def ref_case(prob: W.Problem, disp=None, tol=1e-8, maxit=5000, fill=0):
     rp, di, lc, ld = sysr.ilu()
     pre = bb.copy()
     sysr.solve_precond(pre)
+    if not pcg:
+        return dict(psup_ind=ind, psup=psup, merge_status=st, vcrs_asm=vcrs_asm,
+                    vdia_asm=vdia_asm, b_asm=b_asm, vcrs=vcrs, vdia=vdia, b=bb, matvec_x=xv,
+                    matvec_y=yv, ilu_ok=ok, ilu_colind=ci, ilu_rowptr=rp, ilu_diaind=di,
+                    ilu_vcrs=lc, ilu_vdia=ld, precond_b=pre)
     r = bb.copy()
     x, hist = sysr.solve_pcg(r, tol, maxit)
     return dict(psup_ind=ind, psup=psup, merge_status=st, vcrs_asm=vcrs_asm, vdia_asm=vdia_asm,
@@ -75,6 +80,8 @@ def summary(out: dict) -> dict:
             d[k] = {"sha256": sha(v), "size": int(v.size), "dtype": str(v.dtype)}
         else:
             d[k] = v if not isinstance(v, (np.bool_, bool)) else bool(v)
+    if "hist" not in out:
+        return d
     d["iterations"] = int(out["hist"].size - 1)
     d["hist"] = [float(h) for h in out["hist"]]
     d["x_max_abs"] = float(np.abs(out["x"]).max())
@@ -163,6 +170,13 @@ def main():
     p64 = W.cube(64, sigma=0.5, seed=20261203, ordering="color4")
     o64 = ref_case(p64, maxit=20000)
     meta["cases"]["c64_sigma05_color4"] = summary(o64)
+    del o64
+    # --- C3 at full size (128^3, E lognormal(0, 1), nu ~ U[0.25, 0.35], colour-4): up to
+    # SolvePrecond (the reference's PCG would take minutes here)
+    p128 = W.cube(128, sigma=1.0, nu_range=(0.25, 0.35), ordering="color4")
+    o128 = ref_case(p128, pcg=False)
+    meta["cases"]["c128_sigma1_color4"] = summary(o128)
+    del o128
     # --- SIMP cantilever 8x4x4 (point-load patch, x=0 clamped)
     pc = W.cantilever(8, 4, 4, seed=7)
     oc = ref_case(pc)

@@ -187,6 +187,51 @@ def test_c2_full_size_vs_reference(gpu):
     assert (torch.linalg.norm(xm - x) / torch.linalg.norm(x)).item() <= MIXED_X_TOL
 
 
+def test_c3_full_size_vs_reference(gpu):
+    """C3 itself (128^3, 6,440,067 DOF, E lognormal(0, 1), nu ~ U[0.25, 0.35],
+    colour-4 order): the device's assembled and constrained A and b, MatVec,
+    ILU(0) factors and SolvePrecond(b) hash to the compiled reference's
+    SHA-256 (the reference's PCG is not run at this size; the solvers are
+    checked at C3 by test_c3_mixed_128cube_to_1e10)."""
+    import hashlib
+    import json
+    import os
+    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+    m = meta["cases"]["c128_sigma1_color4"]
+
+    def sha(a):
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+    prob = W.cube(128, sigma=1.0, nu_range=(0.25, 0.35), ordering="color4")
+    print(f"[c3] ndof={prob.ndof}", flush=True)
+    G = DeviceSystem(prob)
+    G.A.setZero()
+    G.b.zero_()
+    assert S.MergeLinSys_SolidLinear_Static_MeshTet3D(G.A, G.b, G.myu, G.lam, prob.rho, prob.g,
+                                                      G.xyz, G.mesh)
+    vc, vd = G.A.download()
+    assert sha(vc) == m["vcrs_asm"]["sha256"] and sha(vd) == m["vdia_asm"]["sha256"]
+    assert sha(G.b.cpu().numpy()) == m["b_asm"]["sha256"]
+    G.A.SetFixedBC(G.flag)
+    S.setRHS_Zero(G.b, G.flag, 0)
+    vc, vd = G.A.download()
+    b = G.b.cpu().numpy()
+    assert sha(vc) == m["vcrs"]["sha256"] and sha(vd) == m["vdia"]["sha256"]
+    assert sha(b) == m["b"]["sha256"]
+    del vc, vd
+    xv = W.uniform(777, prob.ndof) - 0.5
+    yg = dev(W.uniform(778, prob.ndof) - 0.5)
+    G.A.MatVec(yg, 1.5, dev(xv), -0.25)
+    assert sha(yg.cpu().numpy()) == m["matvec_y"]["sha256"]
+    assert G.factor() == m["ilu_ok"]
+    _, _, lc, ld = G.ilu.download()
+    assert sha(lc) == m["ilu_vcrs"]["sha256"] and sha(ld) == m["ilu_vdia"]["sha256"]
+    del lc, ld
+    pg = dev(b)
+    G.ilu.SolvePrecond(pg)
+    assert sha(pg.cpu().numpy()) == m["precond_b"]["sha256"]
+
+
 def test_pcg_c1_golden(gpu):
     import os
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))

@@ -177,6 +177,9 @@ def main():
     o128 = ref_case(p128, pcg=False)
     meta["cases"]["c128_sigma1_color4"] = summary(o128)
     del o128
+    # --- C4 at full size (96x48x48 SIMP cantilever, colour-4): up to SolvePrecond
+    pc4 = W.cantilever(96, 48, 48, seed=20261204, ordering="color4")
+    meta["cases"]["c4_cantilever_color4"] = summary(ref_case(pc4, pcg=False))
     # --- SIMP cantilever 8x4x4 (point-load patch, x=0 clamped)
     pc = W.cantilever(8, 4, 4, seed=7)
     oc = ref_case(pc)

@@ -136,79 +136,28 @@ def test_pcg_fp64_vs_oracle(gpu, case):
     assert torch.linalg.norm(rr - r).item() <= 1e-6 * torch.linalg.norm(G.b).item()
 
 
-def test_c2_full_size_vs_reference(gpu):
-    """C2 itself (64^3, 823,875 DOF, colour-4 order) against the compiled
-    reference's fixtures: assembled and constrained A and b, MatVec, ILU(0)
-    factors and SolvePrecond(b) bit for bit (SHA-256); the fused FP64 PCG
-    within one iteration of the reference's count (556), with its initial
-    residual and solution norm (rel 1e-7 -- the solution itself is too large
-    to commit); the mixed solver's solution within 1e-6 of the FP64 one."""
-    import hashlib
+def reference_case(name):
     import json
     import os
     meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
-    m = meta["cases"]["c64_sigma05_color4"]
+    return meta["cases"][name]
 
-    def sha(a):
-        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
-    prob = W.cube(64, sigma=0.5, seed=20261203, ordering="color4")
-    print(f"[c2] ndof={prob.ndof}", flush=True)
+def sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def device_matches_reference(prob, m):
+    """Assembly, BCs, MatVec, ILU(0) and SolvePrecond(b) on the device, each
+    compared with the compiled reference's SHA-256 in m; returns the system."""
     G = DeviceSystem(prob)
     G.A.setZero()
     G.b.zero_()
     assert S.MergeLinSys_SolidLinear_Static_MeshTet3D(G.A, G.b, G.myu, G.lam, prob.rho, prob.g,
                                                       G.xyz, G.mesh)
-    vc, vd = G.A.download()
-    assert sha(vc) == m["vcrs_asm"]["sha256"] and sha(vd) == m["vdia_asm"]["sha256"]
-    assert sha(G.b.cpu().numpy()) == m["b_asm"]["sha256"]
-    G.A.SetFixedBC(G.flag)
-    S.setRHS_Zero(G.b, G.flag, 0)
-    vc, vd = G.A.download()
-    b = G.b.cpu().numpy()
-    assert sha(vc) == m["vcrs"]["sha256"] and sha(vd) == m["vdia"]["sha256"]
-    assert sha(b) == m["b"]["sha256"]
-    xv = W.uniform(777, prob.ndof) - 0.5
-    yg = dev(W.uniform(778, prob.ndof) - 0.5)
-    G.A.MatVec(yg, 1.5, dev(xv), -0.25)
-    assert sha(yg.cpu().numpy()) == m["matvec_y"]["sha256"]
-    assert G.factor() == m["ilu_ok"]
-    rp, di, lc, ld = G.ilu.download()
-    assert sha(lc) == m["ilu_vcrs"]["sha256"] and sha(ld) == m["ilu_vdia"]["sha256"]
-    pg = dev(b)
-    G.ilu.SolvePrecond(pg)
-    assert sha(pg.cpu().numpy()) == m["precond_b"]["sha256"]
-    x, hist, _ = G.solve(1e-8, 20000)
-    assert abs((len(hist) - 1) - m["iterations"]) <= 1
-    assert hist[0] == pytest.approx(m["hist"][0], rel=1e-13)
-    xn = torch.linalg.norm(x).item()
-    assert abs(xn - m["x_norm"]) <= FP64_X_TOL * m["x_norm"]
-    xm, _, _ = G.solve_mixed(1e-8)
-    assert (torch.linalg.norm(xm - x) / torch.linalg.norm(x)).item() <= MIXED_X_TOL
-
-
-def test_c3_full_size_vs_reference(gpu):
-    """C3 itself (128^3, 6,440,067 DOF, E lognormal(0, 1), nu ~ U[0.25, 0.35],
-    colour-4 order): the device's assembled and constrained A and b, MatVec,
-    ILU(0) factors and SolvePrecond(b) hash to the compiled reference's
-    SHA-256 (the reference's PCG is not run at this size; the solvers are
-    checked at C3 by test_c3_mixed_128cube_to_1e10)."""
-    import hashlib
-    import json
-    import os
-    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
-    m = meta["cases"]["c128_sigma1_color4"]
-
-    def sha(a):
-        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
-
-    prob = W.cube(128, sigma=1.0, nu_range=(0.25, 0.35), ordering="color4")
-    print(f"[c3] ndof={prob.ndof}", flush=True)
-    G = DeviceSystem(prob)
-    G.A.setZero()
-    G.b.zero_()
-    assert S.MergeLinSys_SolidLinear_Static_MeshTet3D(G.A, G.b, G.myu, G.lam, prob.rho, prob.g,
-                                                      G.xyz, G.mesh)
+    if G.f_ext is not None:
+        G.b.add_(G.f_ext)
     vc, vd = G.A.download()
     assert sha(vc) == m["vcrs_asm"]["sha256"] and sha(vd) == m["vdia_asm"]["sha256"]
     assert sha(G.b.cpu().numpy()) == m["b_asm"]["sha256"]
@@ -230,6 +179,45 @@ def test_c3_full_size_vs_reference(gpu):
     pg = dev(b)
     G.ilu.SolvePrecond(pg)
     assert sha(pg.cpu().numpy()) == m["precond_b"]["sha256"]
+    return G
+
+
+def test_c2_full_size_vs_reference(gpu):
+    """C2 itself (64^3, 823,875 DOF, colour-4 order) against the compiled
+    reference's fixtures: assembled and constrained A and b, MatVec, ILU(0)
+    factors and SolvePrecond(b) bit for bit (SHA-256); the fused FP64 PCG
+    within one iteration of the reference's count (556), with its initial
+    residual and solution norm (rel 1e-7 -- the solution itself is too large
+    to commit); the mixed solver's solution within 1e-6 of the FP64 one."""
+    m = reference_case("c64_sigma05_color4")
+    prob = W.cube(64, sigma=0.5, seed=20261203, ordering="color4")
+    print(f"[c2] ndof={prob.ndof}", flush=True)
+    G = device_matches_reference(prob, m)
+    x, hist, _ = G.solve(1e-8, 20000)
+    assert abs((len(hist) - 1) - m["iterations"]) <= 1
+    assert hist[0] == pytest.approx(m["hist"][0], rel=1e-13)
+    xn = torch.linalg.norm(x).item()
+    assert abs(xn - m["x_norm"]) <= FP64_X_TOL * m["x_norm"]
+    xm, _, _ = G.solve_mixed(1e-8)
+    assert (torch.linalg.norm(xm - x) / torch.linalg.norm(x)).item() <= MIXED_X_TOL
+
+
+@pytest.mark.parametrize("case", ["c3", "c4"])
+def test_c3_c4_full_size_vs_reference(gpu, case):
+    """C3 (128^3, 6,440,067 DOF, E lognormal(0, 1), nu ~ U[0.25, 0.35]) and C4
+    (96x48x48 SIMP cantilever, 698,691 DOF), colour-4 order: the device's
+    assembled and constrained A and b, MatVec, ILU(0) factors and
+    SolvePrecond(b) hash to the compiled reference's SHA-256 (the
+    reference's PCG is not run at these sizes; the solvers are checked there
+    by test_c3_mixed_128cube_to_1e10 and the C4 adjoint test)."""
+    if case == "c3":
+        m = reference_case("c128_sigma1_color4")
+        prob = W.cube(128, sigma=1.0, nu_range=(0.25, 0.35), ordering="color4")
+    else:
+        m = reference_case("c4_cantilever_color4")
+        prob = W.cantilever(96, 48, 48, seed=20261204, ordering="color4")
+    print(f"[{case}] ndof={prob.ndof}", flush=True)
+    device_matches_reference(prob, m)
 
 
 def test_pcg_c1_golden(gpu):

@@ -258,6 +258,118 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args):
+    """SIMP compliance gradient of the 96x48x48 cantilever (1,327,104 tets,
+    698,691 DOF, rho_e ~ U(0.2, 1), E_e = rho_e^3, nu = 0.3, x=0 clamped, unit
+    load on the central 3x3 nodes of the x=max face, colour-4 order): one
+    step = E from rho, assembly, fixed BCs, ILU(0) CopyValue + Decompose,
+    FP64 PCG to 1e-10 for u, the adjoint (K lam = f: the forward system, so
+    lam = u) and the per-element contraction dC/drho_e = -3 rho_e^2 lam_e^T
+    K_e^1 u_e.  Value: seconds per gradient (all 1.33M elements)."""
+    import numpy as np
+    import torch
+
+    from delfem2_amd import solver as S
+    from delfem2_amd import workload as W
+    from delfem2_amd.adjoint import ElasticSystem, compliance_and_grad, unit_lame
+
+    tol, p = 1e-10, 3.0
+    print("[bench] building the 96x48x48 cantilever", file=sys.stderr, flush=True)
+    prob = W.cantilever(96, 48, 48, seed=20261204, ordering="color4")
+    rho_np = prob.extra["density"]
+    rho = torch.from_numpy(rho_np).cuda()
+    G = ElasticSystem(prob)
+    for _ in range(args.warmup):
+        compliance_and_grad(G, rho, p, tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        Cv, grad, u = compliance_and_grad(G, rho, p, tol)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    # one more gradient, phase by phase (synchronised: for the breakdown only)
+    ph = {}
+
+    def timed(name, fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ph[name] = round((time.perf_counter() - t) * 1e3, 3)
+        return r
+
+    timed("assembly_bc_ms", lambda: (G.set_modulus(rho ** p), G.assemble()))
+    timed("ilu_copy_decompose_ms", G.factor)
+    uu, hist, _ = timed("forward_pcg_ms", lambda: G.solve(tol, 20000))
+    ph["forward_iterations"] = len(hist) - 1
+    # dC/du = f: the adjoint system is the forward one, so the backward pass
+    # takes lam = u (what a second, deterministic solve would return)
+    r = G.b.clone()
+    S.setRHS_Zero(r, G.flag, 0)
+    assert torch.equal(r, G.b)
+    lam = uu
+    ph["adjoint_pcg_ms"] = 0.0
+    ph["adjoint_iterations"] = 0
+    timed("contraction_ms", lambda: G.adjoint_grad(uu, lam))
+    nb, nnzb = prob.np_, G.A.ncrs
+    # the contraction reads coordinates (4 nodes), tets, u and lam per element
+    # and writes one value: algorithmic bytes per element
+    con_bytes = prob.ne * (16 + 4 * 24 + 2 * 4 * 24 + 8)
+    ach = con_bytes / (ph["contraction_ms"] * 1e-3) / 1e9
+    cpu = None
+    if args.cpu_iters > 0:
+        import oracle
+        print("[bench] CPU baseline sample (oracle restatement)", file=sys.stderr, flush=True)
+        m1, l1 = unit_lame(prob.extra["nu"])
+        E = rho_np ** p
+        t_a = time.perf_counter()
+        ind, ps = oracle.psup(prob.tet, prob.np_)
+        Cs = oracle.System(ind, ps)
+        b = np.zeros(prob.ndof)
+        Cs.assemble(b, E * m1, E * l1, prob.rho, prob.g, prob.xyz, prob.tet)
+        b += prob.f_ext
+        Cs.set_fixed_bc(prob.flag)
+        oracle.set_rhs_zero(b, prob.flag)
+        t_b = time.perf_counter()
+        Cs.factor()
+        t_c = time.perf_counter()
+        n_it = max(5, min(args.cpu_iters, 100))
+        uc, hc = Cs.solve_pcg(b.copy(), 1e-30, n_it)
+        t_d = time.perf_counter()
+        oracle.adjoint_grad(m1, l1, prob.xyz, prob.tet, uc, uc)
+        t_e = time.perf_counter()
+        t_it = (t_d - t_c) / max(len(hc) - 1, 1)
+        n_tot = ph["forward_iterations"] + ph["adjoint_iterations"]
+        value = (t_b - t_a) + (t_c - t_b) + n_tot * t_it + (t_e - t_d)
+        cpu = {"value": round(value, 3), "unit": "s per gradient (extrapolated)", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle restatement on the same problem: pattern+assembly+BC "
+                         f"{t_b - t_a:.2f} s, Decompose {t_c - t_b:.3f} s, {len(hc) - 1} PCG "
+                         f"iterations {t_d - t_c:.2f} s ({t_it * 1e3:.1f} ms/iter) extrapolated "
+                         f"to {n_tot} (the forward solve; the adjoint reuses it), "
+                         f"contraction {t_e - t_d:.3f} s",
+               **host_cpu()}
+    out = {
+        "metric": "SIMP compliance gradient dC/drho of the 96x48x48 tet cantilever (s)",
+        "value": round(t_step, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SIMP densities)",
+        "config": {"workload": f"adjoint compliance gradient, cantilever 96x48x48 hex x6 tets, "
+                               f"{prob.ndof} DOF, {prob.ne} elements, rho~U(0.2,1), E=rho^3, "
+                               f"nu=0.3, ILU(0)-PCG to {tol} (adjoint = forward for the "
+                               f"compliance), colour-4",
+                   "ndof": prob.ndof, "elements": prob.ne},
+        "compliance": float(Cv),
+        "phases": ph,
+        "roofline": {"bound": "hbm", "kernel": "k_adjoint_grad_tet", "achieved": round(ach, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_launch": int(con_bytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_newmark(args):
     """Implicit Newmark-beta time stepping of the 64^3 cube (823,875 DOF, the
     headline cube's mesh, material and z=0 clamp; gravity (0, 0, -3)): one
@@ -467,11 +579,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cube",
-                    choices=["cube", "cube128", "batch512", "newmark64"],
+                    choices=["cube", "cube128", "batch512", "newmark64", "adjoint96"],
                     help="cube: the 823,875-DOF single solve to 1e-8 (headline); cube128: "
                          "the 6,440,067-DOF mixed-precision solve to 1e-10; batch512: 512 "
                          "independent meshes in one batched solve; newmark64: implicit "
-                         "Newmark-beta time steps of the 823,875-DOF cube")
+                         "Newmark-beta time steps of the 823,875-DOF cube; adjoint96: the "
+                         "compliance gradient of the 96x48x48 cantilever")
     ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
     ap.add_argument("--sigma", type=float, default=0.5)
     ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
@@ -515,6 +628,9 @@ def main():
         return
     if args.workload == "newmark64":
         bench_newmark(args)
+        return
+    if args.workload == "adjoint96":
+        bench_adjoint(args)
         return
 
     import torch

@@ -80,8 +80,13 @@ class ElasticTetSolve(torch.autograd.Function):
         sys: ElasticSystem = ctx.sys
         r = grad_u.detach().to(torch.float64).contiguous().clone()
         S.setRHS_Zero(r, sys.flag, 0)
-        lam = torch.zeros_like(r)
-        S.Solve_PCG(r, lam, ctx.tol, ctx.max_nitr, sys.A, sys.ilu)
+        if torch.equal(r, sys.b):
+            # the adjoint system is the forward one (compliance: dJ/du = f), and
+            # the solver is deterministic: its solution would be u, bit for bit
+            lam = u
+        else:
+            lam = torch.zeros_like(r)
+            S.Solve_PCG(r, lam, ctx.tol, ctx.max_nitr, sys.A, sys.ilu)
         return sys.adjoint_grad(u, lam), None, None, None
 
 

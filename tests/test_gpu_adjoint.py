@@ -38,6 +38,23 @@ def test_contraction_kernel_bit_exact(gpu):
     assert np.array_equal(bits(g), bits(gc))
 
 
+def test_compliance_adjoint_reuses_forward_solution(gpu):
+    """For the compliance the adjoint system is the forward one; the backward
+    pass then takes lam = u.  A second, explicit solve of that system gives
+    the same bits (the solver is deterministic), so the gradient is the one
+    the explicit adjoint solve would give (up to how torch forms 3 rho^2)."""
+    prob = W.cantilever(8, 4, 4, seed=7, ordering="color4")
+    sys = ElasticSystem(prob)
+    rho = torch.from_numpy(prob.extra["density"]).cuda()
+    C, grad, u = compliance_and_grad(sys, rho, tol=1e-10)
+    r = sys.b.clone()
+    lam = torch.zeros_like(r)
+    S.Solve_PCG(r, lam, 1e-10, 20000, sys.A, sys.ilu)
+    assert torch.equal(lam, u)
+    g = (3.0 * rho ** 2 * sys.adjoint_grad(u, lam)).cpu().numpy()
+    assert np.allclose(g, grad.cpu().numpy(), rtol=1e-14, atol=0.0)
+
+
 def cpu_compliance_grad(prob, rho, p=3.0):
     """CPU adjoint: oracle assembly / ILU / PCG (tol 1e-12), lam = u."""
     E = rho ** p

@@ -775,6 +775,9 @@ struct PcgWork {
   // mixed solver options: Krylov vector precision, forced-update period
   int mixed_vectors64 = 0;
   unsigned mixed_period = 50;
+  // iterations per graph launch (the small chunk at the start and near the
+  // predicted end, the large one in between)
+  unsigned chunk_large = kChunkLarge, chunk_small = kChunkSmall;
   PcgWork() {
     DFM2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     DFM2H_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
@@ -1048,11 +1051,11 @@ static void ensure_graph(ChunkGraph &G, hipStream_t s, const std::vector<uintptr
 
 // Captures B's large- and small-chunk graphs of the iteration.
 template <typename T, class F>
-static void ensure_graphs(PcgBufs<T> &B, hipStream_t s, unsigned max_iters,
+static void ensure_graphs(const PcgWork &W, PcgBufs<T> &B, hipStream_t s, unsigned max_iters,
                           const std::function<std::vector<uintptr_t>(unsigned)> &sig,
                           F &&one_iteration) {
-  const unsigned cl = std::max(1u, std::min(kChunkLarge, max_iters));
-  const unsigned cs = std::max(1u, std::min(kChunkSmall, max_iters));
+  const unsigned cl = std::max(1u, std::min(W.chunk_large, max_iters));
+  const unsigned cs = std::max(1u, std::min(W.chunk_small, max_iters));
   ensure_graph(B.g[0], s, sig(cl), cl, one_iteration);
   ensure_graph(B.g[1], s, sig(cs), cs, one_iteration);
 }
@@ -1209,7 +1212,7 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
-  ensure_graphs(B, s, max_nitr, [&](unsigned c) { return graph_sig(a, A, P, c); },
+  ensure_graphs(W, B, s, max_nitr, [&](unsigned c) { return graph_sig(a, A, P, c); },
                 [&] { launch_iteration(a, P, shape, Lo, s); });
   run_chunks(W, B, max_nitr, false, status);
 }
@@ -1260,7 +1263,7 @@ static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("mixed prologue");
-  ensure_graphs(B, s, max_inner, [&](unsigned c) { return graph_sig(a, A, P, c); },
+  ensure_graphs(W, B, s, max_inner, [&](unsigned c) { return graph_sig(a, A, P, c); },
                 [&] { launch_mixed_iteration(a, P, shape, Lo, s); });
   // one large chunk beyond the cap: the reliable update that follows it
   run_chunks(W, B, max_inner + B.g[0].chunk, true, status);
@@ -1455,6 +1458,17 @@ int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_p
     PcgWork &W = work_of(P);
     W.mixed_vectors64 = fp64_vectors ? 1 : 0;
     W.mixed_period = update_period;
+  });
+}
+
+int dfm2h_pcg_set_graph_chunks(dfm2h_ilu P, unsigned large, unsigned small) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null handle");
+    DFM2H_REQUIRE(small >= 1 && small <= large && large <= 1024, DFM2H_ERR_INVALID,
+                  "need 1 <= small <= large <= 1024");
+    PcgWork &W = work_of(P);
+    W.chunk_large = large;
+    W.chunk_small = small;
   });
 }
 

@@ -312,6 +312,12 @@ class CPreconditionerILU:
         N.call("dfm2h_pcg_set_mixed_options", self.h, int(bool(fp64_vectors)),
                int(update_period))
 
+    def set_graph_chunks(self, large: int = 16, small: int = 4) -> None:
+        """Iterations per HIP-graph launch of the fused solvers: small chunks
+        at the start and near the predicted end, large ones in between (no
+        reference counterpart; changes no result)."""
+        N.call("dfm2h_pcg_set_graph_chunks", self.h, int(large), int(small))
+
     def set_sweep_shape(self, fp32: bool, shape: int) -> None:
         """Sweep launch shape of the fused solvers (-1 = autotune)."""
         N.call("dfm2h_pcg_set_shape", self.h, int(bool(fp32)), int(shape))

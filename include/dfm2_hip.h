@@ -289,6 +289,14 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
  * vectors, period 50 (measured on MI355X: with delta = 0.1 it keeps the
  * inner iteration count of the 64^3 cube at the FP64 solver's). */
 int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_period);
+/* HIP-graph chunking of the fused solvers (no reference counterpart): the
+ * iterations are captured in graphs of `large` and of `small` iterations;
+ * small ones run first and once the residual's recent rate of decrease
+ * predicts the end within two large ones (fewer iterations run, as
+ * immediate returns, after convergence), large ones in between.  Defaults
+ * 16 and 4; small == large gives a fixed chunk.  The chunking changes no
+ * result.  Requires 1 <= small <= large <= 1024. */
+int dfm2h_pcg_set_graph_chunks(dfm2h_ilu P, unsigned large, unsigned small);
 /* Launch shape of the fused solvers (no reference counterpart): shapes
  * 0..5 set the triangular sweeps' {wavefronts per 21-row slice, blocks per
  * load round} from { {1,8}, {1,4}, {2,8}, {2,4}, {1,16}, {2,16} }; with two

@@ -349,6 +349,32 @@ def test_assembly_deterministic_and_c2_properties(gpu):
     assert torch.linalg.norm(xm - x).item() / torch.linalg.norm(x).item() <= MIXED_X_TOL
 
 
+def test_graph_chunking_changes_no_result(gpu):
+    """The iterations are launched in HIP graphs of a large and a small
+    number of iterations (adaptive); any chunking gives the same iterates:
+    FP64 and mixed solutions bit-identical, same iteration counts, for the
+    default (16, 4), fixed chunks of 1 and 7, and (9, 2).  Bad arguments
+    are rejected."""
+    prob = W.cube(12, sigma=0.5, seed=11, ordering="color4")
+    G = DeviceSystem(prob)
+    assert G.assemble() and G.factor()
+    ref = None
+    for large, small in ((16, 4), (1, 1), (7, 7), (9, 2)):
+        G.ilu.set_graph_chunks(large, small)
+        x, h, _ = G.solve(1e-8, 5000)
+        xm, hm, nm = G.solve_mixed(1e-8)
+        out = (x.cpu().numpy(), len(h), xm.cpu().numpy(), nm)
+        if ref is None:
+            ref = out
+        else:
+            assert np.array_equal(bits(out[0]), bits(ref[0])) and out[1] == ref[1]
+            assert np.array_equal(bits(out[2]), bits(ref[2])) and out[3] == ref[3]
+    G.ilu.set_graph_chunks()
+    for bad in ((0, 0), (4, 8), (2048, 4)):
+        with pytest.raises(Exception):
+            G.ilu.set_graph_chunks(*bad)
+
+
 def test_sweep_shapes_and_autotuner(gpu):
     """Every launch shape of the fused solvers gives the oracle's solution
     (FP64 sweep shapes 0/1/4 keep the summation order: same iterations as the

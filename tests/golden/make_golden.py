@@ -180,6 +180,9 @@ def main():
     # --- C4 at full size (96x48x48 SIMP cantilever, colour-4): up to SolvePrecond
     pc4 = W.cantilever(96, 48, 48, seed=20261204, ordering="color4")
     meta["cases"]["c4_cantilever_color4"] = summary(ref_case(pc4, pcg=False))
+    # --- C5 at full size: the union system of the 512 meshes (block diagonal, so the
+    # reference run on it is its run on every mesh), up to SolvePrecond
+    meta["cases"]["c5_union_color4"] = summary(ref_case(W.batch().union, pcg=False))
     # --- SIMP cantilever 8x4x4 (point-load patch, x=0 clamped)
     pc = W.cantilever(8, 4, 4, seed=7)
     oc = ref_case(pc)

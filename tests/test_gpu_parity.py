@@ -202,20 +202,25 @@ def test_c2_full_size_vs_reference(gpu):
     assert (torch.linalg.norm(xm - x) / torch.linalg.norm(x)).item() <= MIXED_X_TOL
 
 
-@pytest.mark.parametrize("case", ["c3", "c4"])
-def test_c3_c4_full_size_vs_reference(gpu, case):
-    """C3 (128^3, 6,440,067 DOF, E lognormal(0, 1), nu ~ U[0.25, 0.35]) and C4
-    (96x48x48 SIMP cantilever, 698,691 DOF), colour-4 order: the device's
-    assembled and constrained A and b, MatVec, ILU(0) factors and
-    SolvePrecond(b) hash to the compiled reference's SHA-256 (the
-    reference's PCG is not run at these sizes; the solvers are checked there
-    by test_c3_mixed_128cube_to_1e10 and the C4 adjoint test)."""
+@pytest.mark.parametrize("case", ["c3", "c4", "c5"])
+def test_c3_c4_c5_full_size_vs_reference(gpu, case):
+    """C3 (128^3, 6,440,067 DOF, E lognormal(0, 1), nu ~ U[0.25, 0.35]), C4
+    (96x48x48 SIMP cantilever, 698,691 DOF) and C5 (the union system of the
+    512 meshes, 5,125,209 DOF), colour-4 order: the device's assembled and
+    constrained A and b, MatVec, ILU(0) factors and SolvePrecond(b) hash to
+    the compiled reference's SHA-256 (the reference's PCG is not run at
+    these sizes; the solvers are checked there by
+    test_c3_mixed_128cube_to_1e10, the C4 adjoint test and the C5 batch
+    test)."""
     if case == "c3":
         m = reference_case("c128_sigma1_color4")
         prob = W.cube(128, sigma=1.0, nu_range=(0.25, 0.35), ordering="color4")
-    else:
+    elif case == "c4":
         m = reference_case("c4_cantilever_color4")
         prob = W.cantilever(96, 48, 48, seed=20261204, ordering="color4")
+    else:
+        m = reference_case("c5_union_color4")
+        prob = W.batch().union
     print(f"[{case}] ndof={prob.ndof}", flush=True)
     device_matches_reference(prob, m)
 

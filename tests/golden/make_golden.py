@@ -234,6 +234,22 @@ def main():
         if scheme == "stiffwarp":
             case["assembly"]["rot"] = sha(rot)
         meta["cases"][f"dyn_{scheme}_c4"] = case
+    # --- the Newmark-beta assembly at the bench's size (64^3, colour-4, uniform Lame pair:
+    # the reference driver's signature), seeded u / v / a: hashes of A and b
+    pn = W.cube(64, seed=5, ordering="color4")
+    pn.g = np.array([0.0, -0.5, -3.0])
+    run = oracle.DynamicRun(pn, "newmark", dt=0.03, gamma_newmark=0.6, beta_newmark=0.36,
+                            use_ref=True)
+    u, v, a = ((W.uniform(k, pn.ndof) - 0.5) * s for k, s in ((61, 1e-2), (62, 1e-1), (63, 1.0)))
+    b = np.zeros(pn.ndof)
+    run.sys.set_matrix(*[np.zeros_like(m) for m in run.sys.matrix()])
+    run.sys.assemble_newmark_uniform(b, pn.myu[0], pn.lam[0], pn.rho, pn.g, 0.03, 0.6, 0.36,
+                                     pn.xyz, pn.tet, u, v, a)
+    vc, vd = run.sys.matrix()
+    meta["cases"]["dyn_newmark_c64"] = {"assembly": {"vcrs": sha(vc), "vdia": sha(vd),
+                                                     "b": sha(b)},
+                                        "seeds": [61, 62, 63], "dt": 0.03, "gamma": 0.6,
+                                        "beta": 0.36, "g": pn.g.tolist()}
 
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **arrays)
     with open(os.path.join(HERE, "golden.json"), "w") as f:

@@ -254,6 +254,33 @@ def test_listed_assembly_many_elements_per_node(gpu, monkeypatch):
         assert np.abs(ref[2]).max() > 0
 
 
+def test_newmark_assembly_64cube_vs_reference(gpu):
+    """The Newmark-beta assembly at the bench's size (64^3, colour-4 order,
+    uniform Lame pair, seeded u / v / a): A and b hash to the compiled
+    reference driver's SHA-256."""
+    import hashlib
+    import json
+    import os
+    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+    m = meta["cases"]["dyn_newmark_c64"]
+
+    def sha(a):
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+    p = W.cube(64, seed=5, ordering="color4")
+    print(f"[newmark 64] ndof={p.ndof}", flush=True)
+    u, v, a = ((W.uniform(k, p.ndof) - 0.5) * s for k, s in ((61, 1e-2), (62, 1e-1), (63, 1.0)))
+    G = DynamicSystem(p, "newmark")
+    G.A.setZero()
+    bd = torch.zeros(p.ndof, dtype=torch.float64, device="cuda")
+    assert S.MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D(
+        G.A, bd, float(p.myu[0]), float(p.lam[0]), p.rho, m["g"], m["dt"], m["gamma"], m["beta"],
+        G.xyz, G.mesh, dev(u), dev(v), dev(a))
+    vc, vd = G.A.download()
+    assert sha(vc) == m["assembly"]["vcrs"] and sha(vd) == m["assembly"]["vdia"]
+    assert sha(bd.cpu().numpy()) == m["assembly"]["b"]
+
+
 def test_bicgstab_vs_oracle(gpu):
     """Device Solve_PBiCGStab on the warped (non-symmetric) system vs the
     oracle's: iteration counts within 1, x rel-L2 <= 1e-7, true residual."""

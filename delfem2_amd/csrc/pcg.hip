@@ -1232,11 +1232,13 @@ static void launch_mixed_iteration(const PcgArgs<T, float> &a, dfm2h_ilu P, int 
   hipLaunchKernelGGL((k_pcg_spmv<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
   launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64,
                                  a.partials4, gs);
-  hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, s, a);
+  hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, s,
+                     a);
   launch_sweeps_fused(a, P, shape, Lo, s);
   launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3, Lo.n3,
                                  a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a,
+                     1);
 }
 
 template <typename T>
@@ -1261,7 +1263,8 @@ static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   a.xcd = shape_xcd(shape);
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a,
+                     0);
   check_launch("mixed prologue");
   ensure_graphs(W, B, s, max_inner, [&](unsigned c) { return graph_sig(a, A, P, c); },
                 [&] { launch_mixed_iteration(a, P, shape, Lo, s); });
@@ -1545,7 +1548,8 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         else
           launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
         DFM2H_HIP(hipEventRecord(ev[2], W.s));
-        hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, W.s, a);
+        hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0,
+                           W.s, a);
         DFM2H_HIP(hipEventRecord(ev[3], W.s));
         DFM2H_HIP(hipEventRecord(ev[4], W.s));
         DFM2H_HIP(hipEventRecord(ev[5], W.s));
@@ -1559,7 +1563,8 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         else
           launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3,
                                         n3, a.hist);
-        hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, W.s, a, 1);
+        hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, W.s,
+                           a, 1);
         DFM2H_HIP(hipEventRecord(ev[8], W.s));
         DFM2H_HIP(hipEventSynchronize(ev[8]));
         if (rep == 0) continue;  // warm-up iteration

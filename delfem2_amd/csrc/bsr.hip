@@ -318,22 +318,33 @@ __global__ void __launch_bounds__(256) k_assemble_tet_gather(AsmArgs A) {
 // the element's Lame parameters (tet_geometry's values).
 constexpr int kTetRec = 16;  // doubles: vol, d[4][3], myu, lambda, unused
 
+// (256 threads; the workgroup's records go through LDS so that the global
+// stores are contiguous 16-byte runs)
 __global__ void __launch_bounds__(256) k_tet_records(AsmArgs A, size_t ntet, double *rec) {
-  const size_t e = (size_t)wg_id(true) * blockDim.x + threadIdx.x;
-  if (e >= ntet) return;
-  double P[4][3];
-  load_tet(P, A.xyz, A.tet[e]);
-  TetGeo G;
-  tet_geometry(G, P);
-  double *r = rec + e * kTetRec;
-  r[0] = G.vol;
+  __shared__ double lr[256][kTetRec + 1];  // padded against bank conflicts
+  const size_t e0 = (size_t)wg_id(true) * 256, e = e0 + threadIdx.x;
+  if (e < ntet) {
+    double P[4][3];
+    load_tet(P, A.xyz, A.tet[e]);
+    TetGeo G;
+    tet_geometry(G, P);
+    double *r = lr[threadIdx.x];
+    r[0] = G.vol;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 4; ++k)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) r[1 + 3 * k + a] = G.d[k][a];
-  r[13] = A.myu_e ? A.myu_e[e] : A.myu;
-  r[14] = A.lambda_e ? A.lambda_e[e] : A.lambda;
-  r[15] = 0.0;
+      for (int a = 0; a < 3; ++a) r[1 + 3 * k + a] = G.d[k][a];
+    r[13] = A.myu_e ? A.myu_e[e] : A.myu;
+    r[14] = A.lambda_e ? A.lambda_e[e] : A.lambda;
+    r[15] = 0.0;
+  }
+  __syncthreads();
+  const size_t n = min((size_t)256, ntet - e0) * (kTetRec / 2);  // 16-byte pieces
+  double2 *out = reinterpret_cast<double2 *>(rec + e0 * kTetRec);
+  for (size_t j = threadIdx.x; j < n; j += 256) {
+    const size_t rr = j / (kTetRec / 2), f = 2 * (j % (kTetRec / 2));
+    out[j] = double2{lr[rr][f], lr[rr][f + 1]};
+  }
 }
 
 // Element entry of destination block t: the elements k_assemble_tet_gather
@@ -444,6 +455,18 @@ k_assemble_offdiag_rows(AsmArgs A, const unsigned *__restrict__ ptr,
   const bool live = i < A.nblk;
   const unsigned d0 = live ? ptr[A.ncrs + i] : 0u, d1 = live ? ptr[A.ncrs + i + 1] : 0u;
   const unsigned m = min(d1 - d0, kRowElems);  // the host guarantees d1 - d0 <= kRowElems
+  // this lane's first block: its value and element range are requested
+  // together with the records
+  const unsigned s0 = live ? A.colind[i] : 0u, s1 = live ? A.colind[i + 1] : 0u;
+  double acc0[9];
+  unsigned qa0 = 0, qb0 = 0;
+  if (s0 + l < s1) {
+    const double *d = A.vcrs + (size_t)(s0 + l) * 9;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc0[q] = d[q];
+    qa0 = ptr[s0 + l];
+    qb0 = ptr[s0 + l + 1];
+  }
   for (unsigned k = l; k < m * (kTetRec / 2); k += 32) {
     const unsigned rr = k / (kTetRec / 2), part = k % (kTetRec / 2);
     const unsigned e = ent[d0 + rr] & 0x0FFFFFFFu;
@@ -452,14 +475,14 @@ k_assemble_offdiag_rows(AsmArgs A, const unsigned *__restrict__ ptr,
     recs[grp][rr][2 * part + 1] = v.y;
   }
   __syncthreads();
-  const unsigned s0 = live ? A.colind[i] : 0u, s1 = live ? A.colind[i + 1] : 0u;
   for (unsigned sl = s0 + l; sl < s1; sl += 32) {
     double *dst = A.vcrs + (size_t)sl * 9;
+    const bool first = sl == s0 + l;
     double acc[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] = dst[q];
-    const unsigned q1 = ptr[sl + 1];
-    for (unsigned q = ptr[sl]; q < q1; ++q) {
+    for (int q = 0; q < 9; ++q) acc[q] = first ? acc0[q] : dst[q];
+    const unsigned q1 = first ? qb0 : ptr[sl + 1];
+    for (unsigned q = first ? qa0 : ptr[sl]; q < q1; ++q) {
       const unsigned en = ent[q];
       const unsigned li = (en >> 28) & 3u, lj = en >> 30;
       const double *r = recs[grp][en & 0x0FFFFFFFu];

@@ -707,7 +707,7 @@ struct ChunkGraph {
 // iterations per graph launch: large chunks save launches in long solves,
 // small ones bound the iterations that still run (as immediate returns)
 // after the device has finished, which dominates short solves
-constexpr unsigned kChunkLarge = 16, kChunkSmall = 4;
+constexpr unsigned kChunkLarge = 16, kChunkSmall = 2;
 
 template <typename T>
 struct PcgBufs {

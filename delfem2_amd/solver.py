@@ -312,7 +312,7 @@ class CPreconditionerILU:
         N.call("dfm2h_pcg_set_mixed_options", self.h, int(bool(fp64_vectors)),
                int(update_period))
 
-    def set_graph_chunks(self, large: int = 16, small: int = 4) -> None:
+    def set_graph_chunks(self, large: int = 16, small: int = 2) -> None:
         """Iterations per HIP-graph launch of the fused solvers: small chunks
         at the start and near the predicted end, large ones in between (no
         reference counterpart; changes no result)."""

@@ -294,7 +294,7 @@ int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_p
  * small ones run first and once the residual's recent rate of decrease
  * predicts the end within two large ones (fewer iterations run, as
  * immediate returns, after convergence), large ones in between.  Defaults
- * 16 and 4; small == large gives a fixed chunk.  The chunking changes no
+ * 16 and 2; small == large gives a fixed chunk.  The chunking changes no
  * result.  Requires 1 <= small <= large <= 1024. */
 int dfm2h_pcg_set_graph_chunks(dfm2h_ilu P, unsigned large, unsigned small);
 /* Launch shape of the fused solvers (no reference counterpart): shapes

@@ -358,13 +358,13 @@ def test_graph_chunking_changes_no_result(gpu):
     """The iterations are launched in HIP graphs of a large and a small
     number of iterations (adaptive); any chunking gives the same iterates:
     FP64 and mixed solutions bit-identical, same iteration counts, for the
-    default (16, 4), fixed chunks of 1 and 7, and (9, 2).  Bad arguments
-    are rejected."""
+    default (16, 2), (16, 4), fixed chunks of 1 and 7, and (9, 3).  Bad
+    arguments are rejected."""
     prob = W.cube(12, sigma=0.5, seed=11, ordering="color4")
     G = DeviceSystem(prob)
     assert G.assemble() and G.factor()
     ref = None
-    for large, small in ((16, 4), (1, 1), (7, 7), (9, 2)):
+    for large, small in ((16, 2), (16, 4), (1, 1), (7, 7), (9, 3)):
         G.ilu.set_graph_chunks(large, small)
         x, h, _ = G.solve(1e-8, 5000)
         xm, hm, nm = G.solve_mixed(1e-8)

@@ -81,22 +81,11 @@ def host_cpu() -> dict:
 
 def stream_copy_gbs(mib: int = 1024, reps: int = 10) -> float:
     """Measured device-to-device copy rate (read + write bytes per second, GB/s)
-    of a plain copy of `mib` MiB, the achievable counterpart of the 8 TB/s
-    specification peak."""
-    import torch
+    of the library's plain 16-byte-vector copy of `mib` MiB, the achievable
+    counterpart of the 8 TB/s specification peak."""
+    from delfem2_amd import solver as S
 
-    src = torch.empty(mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del src, dst
-    return 2.0 * (mib << 20) / (ms * 1e-3) / 1e9
+    return S.measure_copy_gbs(mib << 20, reps)
 
 
 def level0_rows(psup_ind, psup, nb: int):

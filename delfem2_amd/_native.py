@@ -100,6 +100,7 @@ SIGNATURES = {
     "dfm2h_solve_pcg_mixed": [_vp, _vp, _vp, _vp, _d, _d, _u, _u, _vp, _sz, _psz, _psz],
     "dfm2h_adjoint_grad_tet": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "dfm2h_pcg_kernel_times": [_vp, _vp, _i, _u, _vp],
+    "dfm2h_measure_copy": [_sz, _u, _vp],
     "dfm2h_pcg_set_shape": [_vp, _i, _i],
     "dfm2h_pcg_set_mixed_options": [_vp, _i, _u],
     "dfm2h_pcg_set_graph_chunks": [_vp, _u, _u],

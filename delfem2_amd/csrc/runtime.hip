@@ -116,6 +116,14 @@ k_psup_rows(unsigned np, const unsigned *__restrict__ tet, const unsigned *__res
 
 using namespace dfm2h;
 
+// plain 16-byte copy, one vector per thread: the measured copy rate the
+// kernels' bandwidth is compared with (a probe of grid sizes and of several
+// vectors per thread found this form the fastest on MI355X)
+__global__ void k_copy16(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 extern "C" {
 
 int dfm2h_init(int device) {
@@ -134,6 +142,33 @@ int dfm2h_synchronize(void) {
 }
 
 const char *dfm2h_last_error(void) { return g_last_error.c_str(); }
+
+int dfm2h_measure_copy(size_t bytes, unsigned reps, double *gbs) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(gbs && bytes >= 16 && reps >= 1, DFM2H_ERR_INVALID, "bad argument");
+    const size_t n = bytes / 16;
+    DBuf<float4> a, b;
+    a.alloc(n);
+    b.alloc(n);
+    DFM2H_HIP(hipMemsetAsync(a.p, 0, n * 16, stream()));
+    hipEvent_t e0, e1;
+    DFM2H_HIP(hipEventCreate(&e0));
+    DFM2H_HIP(hipEventCreate(&e1));
+    const dim3 grid(ceil_div(n, 256));
+    hipLaunchKernelGGL(k_copy16, grid, dim3(256), 0, stream(), a.p, b.p, n);  // warm-up
+    DFM2H_HIP(hipEventRecord(e0, stream()));
+    for (unsigned r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_copy16, grid, dim3(256), 0, stream(), a.p, b.p, n);
+    DFM2H_HIP(hipEventRecord(e1, stream()));
+    check_launch("k_copy16");
+    DFM2H_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DFM2H_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+  });
+}
 
 int dfm2h_malloc(void **p, size_t bytes) {
   return guard(__func__, [&] {

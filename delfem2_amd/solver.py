@@ -453,6 +453,15 @@ def pcg_kernel_times(mat: CMatrixSparse, ilu: CPreconditionerILU, fp32: bool,
                      "refine_step", "conv_step"], ms.tolist()))
 
 
+def measure_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Read + write rate (GB/s) of a plain 16-byte-vector device copy of
+    `nbytes` bytes on the solver stream: the achievable bandwidth the kernels'
+    rates are compared with."""
+    out = np.zeros(1)
+    N.call("dfm2h_measure_copy", int(nbytes), int(reps), _hptr(out))
+    return float(out[0])
+
+
 def Solve_PCG_Mixed(b_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: float,
                     mat: CMatrixSparse, ilu: CPreconditionerILU, delta: float = 1e-1,
                     max_outer: int = 200, max_inner: int = 20000):

@@ -329,6 +329,11 @@ int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name,
  * scratch vectors; nothing the caller owns is touched. */
 int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep,
                            double *ms);
+/* Measurement (no reference counterpart): read + write bytes per second
+ * (GB/s) of a plain device copy of `bytes` bytes (16-byte vectors, one per
+ * thread), averaged over `reps` launches on the library stream -- the
+ * achievable counterpart of the HBM peak for the kernels' rooflines. */
+int dfm2h_measure_copy(size_t bytes, unsigned reps, double *gbs);
 
 /* ---- adjoint (no reference counterpart) --------------------------------- */
 /* Material sensitivity of an objective J(u) with K(E) u = f and

@@ -78,6 +78,33 @@ void tune_store(const char *kind, int bucket, int fp32, int variant, double ms);
 int dof_bucket(size_t ndof);
 
 // ---------------------------------------------------------------------------
+// timing events (RAII over hipEventCreate; destroyed on every exit path)
+
+template <int N>
+struct TimingEvents {
+  hipEvent_t e[N] = {};
+  TimingEvents() {
+    for (auto &x : e) {
+      const hipError_t err = hipEventCreate(&x);
+      if (err != hipSuccess) {
+        x = nullptr;
+        destroy();
+        DFM2H_HIP(err);
+      }
+    }
+  }
+  TimingEvents(const TimingEvents &) = delete;
+  TimingEvents &operator=(const TimingEvents &) = delete;
+  ~TimingEvents() { destroy(); }
+  void destroy() {
+    for (auto &x : e)
+      if (x) (void)hipEventDestroy(x);
+    for (auto &x : e) x = nullptr;
+  }
+  hipEvent_t operator[](int i) const { return e[i]; }
+};
+
+// ---------------------------------------------------------------------------
 // device buffer (RAII over hipMalloc)
 
 template <class T>

@@ -1535,8 +1535,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       const unsigned n3 = Lo.n3;
       // segments: spmv | alpha | update | (two empty) | fwd | bwd | beta+pupdate
       constexpr int kSeg = 8;
-      hipEvent_t ev[kSeg + 1];
-      for (auto &e : ev) DFM2H_HIP(hipEventCreate(&e));
+      TimingEvents<kSeg + 1> ev;
       double acc[kSeg] = {0};
       for (unsigned rep = 0; rep <= nrep; ++rep) {
         DFM2H_HIP(hipEventRecord(ev[0], W.s));
@@ -1575,7 +1574,6 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         }
       }
       check_launch("kernel timing");
-      for (auto &e : ev) (void)hipEventDestroy(e);
       ms[0] = acc[0] / nrep;  // SpMV
       ms[1] = acc[2] / nrep;  // update + forward level 0
       ms[2] = acc[5] / nrep;  // forward levels >= 1

@@ -151,21 +151,17 @@ int dfm2h_measure_copy(size_t bytes, unsigned reps, double *gbs) {
     a.alloc(n);
     b.alloc(n);
     DFM2H_HIP(hipMemsetAsync(a.p, 0, n * 16, stream()));
-    hipEvent_t e0, e1;
-    DFM2H_HIP(hipEventCreate(&e0));
-    DFM2H_HIP(hipEventCreate(&e1));
+    TimingEvents<2> ev;
     const dim3 grid(ceil_div(n, 256));
     hipLaunchKernelGGL(k_copy16, grid, dim3(256), 0, stream(), a.p, b.p, n);  // warm-up
-    DFM2H_HIP(hipEventRecord(e0, stream()));
+    DFM2H_HIP(hipEventRecord(ev[0], stream()));
     for (unsigned r = 0; r < reps; ++r)
       hipLaunchKernelGGL(k_copy16, grid, dim3(256), 0, stream(), a.p, b.p, n);
-    DFM2H_HIP(hipEventRecord(e1, stream()));
+    DFM2H_HIP(hipEventRecord(ev[1], stream()));
     check_launch("k_copy16");
-    DFM2H_HIP(hipEventSynchronize(e1));
+    DFM2H_HIP(hipEventSynchronize(ev[1]));
     float ms = 0.f;
-    DFM2H_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    DFM2H_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
   });
 }

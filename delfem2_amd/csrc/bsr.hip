@@ -597,10 +597,14 @@ k_assemble_diag_newmark(AsmArgs A, const unsigned *__restrict__ ptr,
   for (int off = 32; off > 0; off >>= 1)
     rounds = max(rounds, (unsigned)__shfl_xor((int)rounds, off, 64));
   rounds = (rounds + EPR - 1) / EPR;
+  // the row's first 32 list entries, one per lane of the row group: the
+  // rounds take them by shuffle instead of a dependent load each
+  const unsigned ent_l = q0 + l < q1 ? ent[q0 + l] : 0u;
   for (unsigned c = 0; c < rounds; ++c) {
-    const unsigned qb = q0 + c * EPR, q = qb + u;
+    const unsigned qb = q0 + c * EPR, q = qb + u, src = c * EPR + u;
+    const unsigned en_s = (unsigned)__shfl((int)ent_l, (int)(g0 + (src & (kDiagLanes - 1u))), 64);
     if (q < q1) {
-      const unsigned en = ent[q];
+      const unsigned en = src < kDiagLanes ? en_s : ent[q];
       const unsigned e = en & 0x0FFFFFFFu;
       const unsigned li = (en >> 28) & 3u;
       const double *r = rec + (size_t)e * kTetRec;

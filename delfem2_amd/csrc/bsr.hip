@@ -444,6 +444,7 @@ k_assemble_offdiag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
 // ... from there -- the additions of k_assemble_offdiag_lists, with a record
 // read from HBM once per row instead of nine scattered loads per block.
 constexpr unsigned kRowElems = 32;
+constexpr unsigned kOffdiagBatch = 4;  // list entries of a block loaded together
 
 template <int KIND>
 __global__ void __launch_bounds__(64)
@@ -482,16 +483,25 @@ k_assemble_offdiag_rows(AsmArgs A, const unsigned *__restrict__ ptr,
 #pragma unroll
     for (int q = 0; q < 9; ++q) acc[q] = first ? acc0[q] : dst[q];
     const unsigned q1 = first ? qb0 : ptr[sl + 1];
-    for (unsigned q = first ? qa0 : ptr[sl]; q < q1; ++q) {
-      const unsigned en = ent[q];
-      const unsigned li = (en >> 28) & 3u, lj = en >> 30;
-      const double *r = recs[grp][en & 0x0FFFFFFFu];
-      const double di[3] = {r[1 + 3 * li], r[2 + 3 * li], r[3 + 3 * li]};
-      const double dj[3] = {r[1 + 3 * lj], r[2 + 3 * lj], r[3 + 3 * lj]};
-      double blk[9];
-      element_block<KIND>(A, i, r[0], di, dj, li == lj, r[13], r[14], blk);
+    // the block's list entries four at a time, their loads in flight
+    // together; the elements are still added in list order
+    for (unsigned q = first ? qa0 : ptr[sl]; q < q1; q += kOffdiagBatch) {
+      unsigned ens[kOffdiagBatch];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) acc[k] += blk[k];
+      for (unsigned u = 0; u < kOffdiagBatch; ++u) ens[u] = q + u < q1 ? ent[q + u] : 0u;
+#pragma unroll
+      for (unsigned u = 0; u < kOffdiagBatch; ++u) {
+        if (q + u >= q1) break;
+        const unsigned en = ens[u];
+        const unsigned li = (en >> 28) & 3u, lj = en >> 30;
+        const double *r = recs[grp][en & 0x0FFFFFFFu];
+        const double di[3] = {r[1 + 3 * li], r[2 + 3 * li], r[3 + 3 * li]};
+        const double dj[3] = {r[1 + 3 * lj], r[2 + 3 * lj], r[3 + 3 * lj]};
+        double blk[9];
+        element_block<KIND>(A, i, r[0], di, dj, li == lj, r[13], r[14], blk);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] += blk[k];
+      }
     }
 #pragma unroll
     for (int q = 0; q < 9; ++q) dst[q] = acc[q];

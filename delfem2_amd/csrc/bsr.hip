@@ -586,7 +586,7 @@ k_assemble_diag_lists(AsmArgs A, const unsigned *__restrict__ ptr,
 // diagonal block and vec_b receive the elements in ascending order.  Same
 // values as k_assemble_diag_lists<kAsmNewmark>, with a quarter of its
 // per-lane work and registers.
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 5)
 k_assemble_diag_newmark(AsmArgs A, const unsigned *__restrict__ ptr,
                         const unsigned *__restrict__ ent, const double *__restrict__ rec) {
   constexpr unsigned EPR = kDiagLanes / 4;  // elements per round
@@ -610,19 +610,33 @@ k_assemble_diag_newmark(AsmArgs A, const unsigned *__restrict__ ptr,
   // the row's first 32 list entries, one per lane of the row group: the
   // rounds take them by shuffle instead of a dependent load each
   const unsigned ent_l = q0 + l < q1 ? ent[q0 + l] : 0u;
-  for (unsigned c = 0; c < rounds; ++c) {
-    const unsigned qb = q0 + c * EPR, q = qb + u, src = c * EPR + u;
+  // this lane's list entry of round c (~0u past the list; a wavefront-uniform
+  // call) and its element's node jn, requested a round ahead
+  auto entry = [&](unsigned c) -> unsigned {
+    const unsigned src = c * EPR + u, q = q0 + src;
     const unsigned en_s = (unsigned)__shfl((int)ent_l, (int)(g0 + (src & (kDiagLanes - 1u))), 64);
-    if (q < q1) {
-      const unsigned en = src < kDiagLanes ? en_s : ent[q];
+    return q < q1 ? (src < kDiagLanes ? en_s : ent[q]) : ~0u;
+  };
+  const unsigned *const tet_nodes = reinterpret_cast<const unsigned *>(A.tet);
+  auto node_of = [&](unsigned en) -> unsigned {
+    return en == ~0u ? 0u : tet_nodes[(size_t)(en & 0x0FFFFFFFu) * 4 + jn];
+  };
+  unsigned en_next = rounds > 0 ? entry(0) : ~0u;
+  unsigned node_next = node_of(en_next);
+  for (unsigned c = 0; c < rounds; ++c) {
+    const unsigned qb = q0 + c * EPR;
+    const unsigned en = en_next, node = node_next;
+    if (c + 1 < rounds) {
+      en_next = entry(c + 1);
+      node_next = node_of(en_next);
+    }
+    if (en != ~0u) {
       const unsigned e = en & 0x0FFFFFFFu;
       const unsigned li = (en >> 28) & 3u;
       const double *r = rec + (size_t)e * kTetRec;
       const double vol = r[0], myu = r[13], lam = r[14];
       const double di[3] = {r[1 + 3 * li], r[2 + 3 * li], r[3 + 3 * li]};
       const double dj[3] = {r[1 + 3 * jn], r[2 + 3 * jn], r[3 + 3 * jn]};
-      const uint4 n = A.tet[e];
-      const unsigned node = jn == 0 ? n.x : jn == 1 ? n.y : jn == 2 ? n.z : n.w;
       double kb[9], mb[9], xu[3], xv[3], xa[3];
       emat_block<true>(kb, di, dj, vol, myu, lam);
       mass_block(mb, A.rho * vol * 0.05, jn == li);

@@ -468,12 +468,30 @@ k_assemble_offdiag_rows(AsmArgs A, const unsigned *__restrict__ ptr,
     qa0 = ptr[s0 + l];
     qb0 = ptr[s0 + l + 1];
   }
-  for (unsigned k = l; k < m * (kTetRec / 2); k += 32) {
-    const unsigned rr = k / (kTetRec / 2), part = k % (kTetRec / 2);
-    const unsigned e = ent[d0 + rr] & 0x0FFFFFFFu;
-    const double2 v = *reinterpret_cast<const double2 *>(rec + (size_t)e * kTetRec + 2 * part);
-    recs[grp][rr][2 * part] = v.x;
-    recs[grp][rr][2 * part + 1] = v.y;
+  // the node's records into LDS, 16 bytes per lane and step: every step's
+  // list entry is requested first, then every step's record part, so the
+  // steps' loads are in flight together
+  constexpr unsigned kParts = kTetRec / 2, kSteps = kRowElems * kParts / 32;
+  unsigned es[kSteps];
+#pragma unroll
+  for (unsigned st = 0; st < kSteps; ++st) {
+    const unsigned k = l + 32 * st;
+    es[st] = k < m * kParts ? ent[d0 + k / kParts] & 0x0FFFFFFFu : 0u;
+  }
+  double2 vs[kSteps];
+#pragma unroll
+  for (unsigned st = 0; st < kSteps; ++st) {
+    const unsigned k = l + 32 * st;
+    if (k < m * kParts)
+      vs[st] = *reinterpret_cast<const double2 *>(rec + (size_t)es[st] * kTetRec + 2 * (k % kParts));
+  }
+#pragma unroll
+  for (unsigned st = 0; st < kSteps; ++st) {
+    const unsigned k = l + 32 * st;
+    if (k < m * kParts) {
+      recs[grp][k / kParts][2 * (k % kParts)] = vs[st].x;
+      recs[grp][k / kParts][2 * (k % kParts) + 1] = vs[st].y;
+    }
   }
   __syncthreads();
   for (unsigned sl = s0 + l; sl < s1; sl += 32) {

@@ -162,6 +162,7 @@ int dfm2h_measure_copy(size_t bytes, unsigned reps, double *gbs) {
     DFM2H_HIP(hipEventSynchronize(ev[1]));
     float ms = 0.f;
     DFM2H_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    DFM2H_REQUIRE(ms > 0.f, DFM2H_ERR_INVALID, "copy too short to time");
     *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
   });
 }

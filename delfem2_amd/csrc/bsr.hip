@@ -708,13 +708,20 @@ __global__ void k_set_fixed_bc(unsigned nblk, size_t ncrs, const unsigned *__res
   const bool diag = t >= ncrs;
   const unsigned i = diag ? (unsigned)(t - ncrs) : rowof[t];
   const unsigned j = diag ? i : rowptr[t];
+  bool fri[3], fcj[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    fri[a] = flag[(size_t)i * 3 + a] != 0;
+    fcj[a] = flag[(size_t)j * 3 + a] != 0;
+  }
+  if (!(fri[0] || fri[1] || fri[2] || fcj[0] || fcj[1] || fcj[2])) return;  // block untouched
   double *v = diag ? vdia + (size_t)i * 9 : vcrs + t * 9;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const bool fr = flag[(size_t)i * 3 + a] != 0;
-      const bool fc = flag[(size_t)j * 3 + b] != 0;
+      const bool fr = fri[a];
+      const bool fc = fcj[b];
       if (diag) {
         if (fr || fc) v[a * 3 + b] = (a == b && fr) ? 1.0 : 0.0;
       } else {

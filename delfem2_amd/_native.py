@@ -60,6 +60,8 @@ SIGNATURES = {
     "dfm2h_bsr_upload": [_vp, _vp, _vp],
     "dfm2h_bsr_download": [_vp, _vp, _vp],
     "dfm2h_bsr_info": [_vp, _psz, _psz, _pvp, _pvp],
+    "dfm2h_bsr_values_changed": [_vp],
+    "dfm2h_bsr_merge": [_vp, _u, _vp, _u, _vp, _vp],
     "dfm2h_bsr_matvec": [_vp, _vp, _d, _vp, _d],
     "dfm2h_bsr_set_fixed_bc": [_vp, _vp],
     "dfm2h_set_rhs_zero": [_vp, _vp, _sz, _i],

@@ -30,7 +30,11 @@ def unit_lame(nu: np.ndarray):
 
 
 class ElasticSystem(DeviceSystem):
-    """A DeviceSystem whose element moduli are a tensor input."""
+    """A DeviceSystem whose element moduli are a tensor input.
+
+    ``state`` changes whenever the device matrix, right-hand side and factors
+    are rebuilt, so a backward pass can tell whether they still hold the
+    forward solve's system."""
 
     def __init__(self, prob, device: int = 0, nu=None):
         super().__init__(prob, device)
@@ -38,12 +42,28 @@ class ElasticSystem(DeviceSystem):
         m1, l1 = unit_lame(nu if np.ndim(nu) else np.full(prob.ne, float(nu)))
         self.mu1 = torch.from_numpy(np.ascontiguousarray(m1)).to(self.dev)
         self.lam1 = torch.from_numpy(np.ascontiguousarray(l1)).to(self.dev)
-        self.E_version = None
+        self.state = 0
 
     def set_modulus(self, E: torch.Tensor) -> None:
         E = E.detach().to(self.dev, torch.float64).contiguous()
         self.myu = (E * self.mu1).contiguous()
         self.lam = (E * self.lam1).contiguous()
+
+    def assemble(self, disp=None) -> bool:
+        self.state += 1
+        return super().assemble(disp)
+
+    def factor(self) -> bool:
+        self.state += 1
+        return super().factor()
+
+    def build(self, E: torch.Tensor) -> None:
+        """K(E), f and the ILU(0) factors of K(E) on the device."""
+        self.set_modulus(E)
+        if not self.assemble():
+            raise RuntimeError("element edge missing from the matrix pattern")
+        if not self.factor():
+            raise RuntimeError("ILU(0) met more than 10 singular pivots")
 
     def adjoint_grad(self, u: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:
         """g_e = -lam_e^T K_e^1 u_e for every element (device kernel)."""
@@ -54,39 +74,52 @@ class ElasticSystem(DeviceSystem):
         return g
 
 
+def _converged(hist, tol: float, what: str) -> None:
+    """Raises when a Solve_PCG history stops short of the tolerance."""
+    if len(hist) > 1 and not hist[-1] < tol * hist[0]:
+        raise RuntimeError(f"{what} did not converge: |r|/|r0| = {hist[-1] / hist[0]:.3e} "
+                           f"after {len(hist) - 1} iterations (tol {tol:.1e})")
+
+
 class ElasticTetSolve(torch.autograd.Function):
     """u = K(E)^-1 f on the GPU; backward by one adjoint PCG solve with the
-    forward factors and the per-element contraction kernel."""
+    forward factors and the per-element contraction kernel.  Each solve must
+    reach the tolerance (else RuntimeError).  When the system was rebuilt
+    for other moduli between forward and backward (another load case, a line
+    search), backward rebuilds K(E) and its factors for the forward E first."""
 
     @staticmethod
     def forward(ctx, E: torch.Tensor, sys: ElasticSystem, tol: float = 1e-10,
                 max_nitr: int = 20000):
-        sys.set_modulus(E)
-        if not sys.assemble():
-            raise RuntimeError("element edge missing from the matrix pattern")
-        if not sys.factor():
-            raise RuntimeError("ILU(0) met more than 10 singular pivots")
+        sys.build(E)
         u, hist, _ = sys.solve(tol, max_nitr)
+        _converged(hist, tol, "forward solve")
         ctx.sys = sys
         ctx.tol = tol
         ctx.max_nitr = max_nitr
         ctx.hist = hist
-        ctx.save_for_backward(u)
+        ctx.state = sys.state
+        ctx.f = sys.b.clone()
+        ctx.save_for_backward(u, E.detach().clone())
         return u
 
     @staticmethod
     def backward(ctx, grad_u: torch.Tensor):
-        (u,) = ctx.saved_tensors
+        u, E = ctx.saved_tensors
         sys: ElasticSystem = ctx.sys
+        if sys.state != ctx.state:  # K, f or the factors belong to other moduli now
+            sys.build(E)
+            ctx.state = sys.state
         r = grad_u.detach().to(torch.float64).contiguous().clone()
         S.setRHS_Zero(r, sys.flag, 0)
-        if torch.equal(r, sys.b):
+        if torch.equal(r, ctx.f):
             # the adjoint system is the forward one (compliance: dJ/du = f), and
             # the solver is deterministic: its solution would be u, bit for bit
             lam = u
         else:
             lam = torch.zeros_like(r)
-            S.Solve_PCG(r, lam, ctx.tol, ctx.max_nitr, sys.A, sys.ilu)
+            hist = S.Solve_PCG(r, lam, ctx.tol, ctx.max_nitr, sys.A, sys.ilu)
+            _converged(hist, ctx.tol, "adjoint solve")
         return sys.adjoint_grad(u, lam), None, None, None
 
 

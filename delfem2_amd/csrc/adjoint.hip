@@ -60,6 +60,8 @@ extern "C" int dfm2h_adjoint_grad_tet(dfm2h_tetmesh M, const double *xyz, const 
     DFM2H_REQUIRE(M, DFM2H_ERR_INVALID, "null mesh handle");
     DFM2H_REQUIRE(xyz && myu1_e && lambda1_e && u && lam && grad, DFM2H_ERR_INVALID,
                   "null argument");
+    for (const double *p : {xyz, myu1_e, lambda1_e, u, lam, (const double *)grad})
+      require_device(p, "a device array argument");
     if (!M->ntet) return;
     hipLaunchKernelGGL(k_adjoint_grad_tet, dim3(ceil_div(M->ntet, 256)), dim3(256), 0, stream(),
                        M->ntet, reinterpret_cast<const uint4 *>(M->tet.p), xyz, myu1_e,

@@ -947,6 +947,7 @@ void need_batch(dfm2h_batch B) {
   DFM2H_REQUIRE(B->A->pattern_serial == B->a_serial && B->P->pattern_serial == B->p_serial,
                 DFM2H_ERR_STATE, "matrix / preconditioner pattern changed since batch_create");
   DFM2H_REQUIRE(B->P->decomposed, DFM2H_ERR_STATE, "preconditioner not decomposed");
+  flush_merge(B->A);
 }
 
 }  // namespace
@@ -1145,6 +1146,8 @@ int dfm2h_batch_solve(dfm2h_batch B, const double *b, double *x, double tol, int
   return guard(__func__, [&] {
     need_batch(B);
     DFM2H_REQUIRE(b && x, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(b);
+    DFM2H_DEVICE_ARG(x);
     DFM2H_REQUIRE(max_nitr >= 1, DFM2H_ERR_INVALID, "max_nitr must be >= 1");
     DFM2H_REQUIRE(!fp32 || (delta > 0.0 && delta < 1.0), DFM2H_ERR_INVALID,
                   "delta must be in (0, 1)");

@@ -746,6 +746,59 @@ __global__ void k_fill_rowof(unsigned nblk, const unsigned *colind, unsigned *ro
   for (unsigned s = colind[i]; s < colind[i + 1]; ++s) rowof[s] = i;
 }
 
+// Staged Merge additions: one thread per (destination, component); run r =
+// {dst, first, count} adds blocks first .. first+count-1 (call order) to
+// its destination one after another -- the reference's `+=` sequence.
+__global__ void k_merge_apply(size_t nrun, const unsigned *__restrict__ run, size_t ncrs,
+                              const double *__restrict__ val, double *vcrs, double *vdia) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrun * 9) return;
+  const size_t r = t / 9;
+  const unsigned k = (unsigned)(t - r * 9);
+  const size_t dst = run[3 * r], first = run[3 * r + 1], cnt = run[3 * r + 2];
+  double *out = dst < ncrs ? vcrs + 9 * dst + k : vdia + 9 * (dst - ncrs) + k;
+  double acc = *out;
+  for (size_t c = 0; c < cnt; ++c) acc += val[9 * (first + c) + k];
+  *out = acc;
+}
+
+// Blocks staged beyond this count are applied before more are staged.
+constexpr size_t kMergeChunk = size_t(1) << 21;
+
+void flush_merge(dfm2h_bsr_s *A) {
+  const size_t ne = A->mg_dst.size();
+  if (!ne) return;
+  // stable counting sort of the staged blocks by destination
+  const size_t nd = A->pat.ncrs + A->nblk;
+  std::vector<unsigned> start(nd + 1, 0);
+  for (size_t e = 0; e < ne; ++e) start[A->mg_dst[e] + 1]++;
+  for (size_t d = 0; d < nd; ++d) start[d + 1] += start[d];
+  std::vector<unsigned> run;  // {dst, first, count} per touched destination
+  for (size_t d = 0; d < nd; ++d)
+    if (start[d + 1] > start[d])
+      run.insert(run.end(), {(unsigned)d, start[d], start[d + 1] - start[d]});
+  std::vector<double> val(9 * ne);
+  for (size_t e = 0; e < ne; ++e) {
+    const unsigned pos = start[A->mg_dst[e]]++;
+    std::memcpy(&val[9 * (size_t)pos], &A->mg_val[9 * e], 9 * sizeof(double));
+  }
+  if (A->mg_val_d.n < val.size()) A->mg_val_d.alloc(val.size());
+  if (A->mg_run_d.n < run.size()) A->mg_run_d.alloc(run.size());
+  DFM2H_HIP(hipMemcpyAsync(A->mg_val_d.p, val.data(), val.size() * sizeof(double),
+                           hipMemcpyHostToDevice, stream()));
+  DFM2H_HIP(hipMemcpyAsync(A->mg_run_d.p, run.data(), run.size() * sizeof(unsigned),
+                           hipMemcpyHostToDevice, stream()));
+  const size_t nrun = run.size() / 3;
+  hipLaunchKernelGGL(k_merge_apply, dim3(ceil_div(nrun * 9, 256)), dim3(256), 0, stream(), nrun,
+                     A->mg_run_d.p, A->pat.ncrs, A->mg_val_d.p, A->vcrs.p, A->vdia.p);
+  check_launch("k_merge_apply");
+  // the host arrays above are released on return: wait for the copies
+  DFM2H_HIP(hipStreamSynchronize(stream()));
+  A->mg_dst.clear();
+  A->mg_val.clear();
+  A->version++;
+}
+
 }  // namespace dfm2h
 
 using namespace dfm2h;
@@ -753,6 +806,7 @@ using namespace dfm2h;
 static void need_pattern(dfm2h_bsr A) {
   DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
   DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_STATE, "SetPattern has not been called");
+  flush_merge(A);
 }
 
 // The walking form runs for meshes whose element ids do not fit the packed
@@ -772,6 +826,9 @@ static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu
   need_pattern(A);
   DFM2H_REQUIRE(M, DFM2H_ERR_INVALID, "null mesh handle");
   DFM2H_REQUIRE(vec_b && g && xyz, DFM2H_ERR_INVALID, "null argument");
+  for (const double *p : {(const double *)vec_b, myu_e, lambda_e, xyz, a.disp, a.velo, a.acc,
+                          a.rot})
+    require_device(p, "a vector argument of the assembly");
   DFM2H_REQUIRE(M->np == A->nblk, DFM2H_ERR_INVALID, "mesh node count != matrix block rows");
   const size_t n = A->pat.ncrs + A->nblk;
   if (!n) return;
@@ -963,7 +1020,8 @@ int dfm2h_bsr_set_pattern(dfm2h_bsr A, const unsigned *colind, size_t ncolind,
       A->plan.build({all}, s0, s1, P.rowptr, true);
     }
     A->has_pattern = true;
-    A->pattern_serial++;
+    A->pattern_serial = next_serial();
+    A->mg_mark.assign(A->nblk, ~0u);
     A->version++;
     DFM2H_HIP(hipStreamSynchronize(stream()));
   });
@@ -972,6 +1030,8 @@ int dfm2h_bsr_set_pattern(dfm2h_bsr A, const unsigned *colind, size_t ncolind,
 int dfm2h_bsr_set_zero(dfm2h_bsr A) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
+    A->mg_dst.clear();  // staged additions are overwritten anyway
+    A->mg_val.clear();
     A->vcrs.zero();
     A->vdia.zero();
     A->version++;
@@ -981,6 +1041,7 @@ int dfm2h_bsr_set_zero(dfm2h_bsr A) {
 int dfm2h_bsr_upload(dfm2h_bsr A, const double *vcrs, const double *vdia) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
+    flush_merge(A);
     if (vcrs) {
       need_pattern(A);
       A->vcrs.upload(vcrs, A->pat.ncrs * 9);
@@ -994,6 +1055,7 @@ int dfm2h_bsr_upload(dfm2h_bsr A, const double *vcrs, const double *vdia) {
 int dfm2h_bsr_download(dfm2h_bsr A, double *vcrs, double *vdia) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
+    flush_merge(A);
     if (vcrs) {
       need_pattern(A);
       A->vcrs.download(vcrs, A->pat.ncrs * 9);
@@ -1006,10 +1068,66 @@ int dfm2h_bsr_download(dfm2h_bsr A, double *vcrs, double *vdia) {
 int dfm2h_bsr_info(dfm2h_bsr A, size_t *nblk, size_t *ncrs, double **vcrs, double **vdia) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
+    if (vcrs || vdia) flush_merge(A);  // the device arrays hold every staged addition
     if (nblk) *nblk = A->nblk;
     if (ncrs) *ncrs = A->pat.ncrs;
     if (vcrs) *vcrs = A->vcrs.p;
     if (vdia) *vdia = A->vdia.p;
+  });
+}
+
+int dfm2h_bsr_values_changed(dfm2h_bsr A) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
+    A->version++;
+  });
+}
+
+// ls_block_sparse.h:260-310: row by row, the row's slots are marked by
+// column (the reference's merge_buffer), then each column's block is added
+// -- emat[irow][irow] when the column is the row itself, into the diagonal
+// block -- and the marks are cleared.  A missing slot ends the call with
+// the blocks before it staged, as the reference returns false there.
+int dfm2h_bsr_merge(dfm2h_bsr A, unsigned nrow, const unsigned *ip_row, unsigned ncol,
+                    const unsigned *ip_col, const double *emat) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(A, DFM2H_ERR_INVALID, "null matrix handle");
+    DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_STATE, "SetPattern has not been called");
+    DFM2H_REQUIRE((ip_row && ip_col && emat) || nrow == 0 || ncol == 0, DFM2H_ERR_INVALID,
+                  "null argument");
+    for (unsigned r = 0; r < nrow; ++r)
+      DFM2H_REQUIRE(ip_row[r] < A->nblk, DFM2H_ERR_INVALID, "row block index out of range");
+    for (unsigned c = 0; c < ncol; ++c)
+      DFM2H_REQUIRE(ip_col[c] < A->nblk, DFM2H_ERR_INVALID, "column block index out of range");
+    if (A->mg_dst.size() + (size_t)nrow * ncol > kMergeChunk) flush_merge(A);
+    const Pattern &pt = A->pat;
+    std::vector<unsigned> &mark = A->mg_mark;
+    for (unsigned r = 0; r < nrow; ++r) {
+      const unsigned i = ip_row[r];
+      for (unsigned q = pt.colind[i]; q < pt.colind[i + 1]; ++q) mark[pt.rowptr[q]] = q;
+      bool missing = false;
+      for (unsigned c = 0; c < ncol && !missing; ++c) {
+        const unsigned j = ip_col[c];
+        unsigned dst;
+        const double *in;
+        if (i == j) {
+          DFM2H_REQUIRE(r < ncol, DFM2H_ERR_INVALID,
+                        "a diagonal block takes emat[irow][irow], which needs irow < ncol");
+          dst = (unsigned)pt.ncrs + i;
+          in = emat + 9 * ((size_t)r * ncol + r);
+        } else if (mark[j] != ~0u) {
+          dst = mark[j];
+          in = emat + 9 * ((size_t)r * ncol + c);
+        } else {
+          missing = true;
+          break;
+        }
+        A->mg_dst.push_back(dst);
+        A->mg_val.insert(A->mg_val.end(), in, in + 9);
+      }
+      for (unsigned q = pt.colind[i]; q < pt.colind[i + 1]; ++q) mark[pt.rowptr[q]] = ~0u;
+      DFM2H_REQUIRE(!missing, DFM2H_ERR_PATTERN, "a block of the element is not in the pattern");
+    }
   });
 }
 
@@ -1018,6 +1136,8 @@ int dfm2h_bsr_matvec(dfm2h_bsr A, double *y, double alpha, const double *x, doub
     need_pattern(A);
     DFM2H_REQUIRE(y && x, DFM2H_ERR_INVALID, "null vector");
     DFM2H_REQUIRE(y != x, DFM2H_ERR_INVALID, "y and x must not alias");
+    DFM2H_DEVICE_ARG(y);
+    DFM2H_DEVICE_ARG(x);
     if (!A->nblk) return;
     ensure_packed(A->plan, A->pk64, A->vcrs.p, A->vdia.p, A->version, stream());
     hipLaunchKernelGGL((k_sliced_matvec<double, 3>), dim3(slice_grid(A->plan.nslices)), dim3(kPackBlock),
@@ -1031,6 +1151,7 @@ int dfm2h_bsr_set_fixed_bc(dfm2h_bsr A, const int *flag) {
   return guard(__func__, [&] {
     need_pattern(A);
     DFM2H_REQUIRE(flag, DFM2H_ERR_INVALID, "null flag");
+    DFM2H_DEVICE_ARG(flag);
     const size_t n = A->pat.ncrs + A->nblk;
     if (!n) return;
     hipLaunchKernelGGL(k_set_fixed_bc, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), A->nblk,
@@ -1045,6 +1166,8 @@ int dfm2h_set_rhs_zero(double *b, const int *flag, size_t n, int nz) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE((b && flag) || n == 0, DFM2H_ERR_INVALID, "null vector");
     if (!n) return;
+    DFM2H_DEVICE_ARG(b);
+    DFM2H_DEVICE_ARG(flag);
     hipLaunchKernelGGL(k_set_rhs_zero, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), b, flag, n,
                        nz);
     check_launch("k_set_rhs_zero");
@@ -1055,6 +1178,9 @@ int dfm2h_xplusay(double *X, size_t n, const int *flag, double alpha, const doub
   return guard(__func__, [&] {
     DFM2H_REQUIRE((X && flag && Y) || n == 0, DFM2H_ERR_INVALID, "null vector");
     if (!n) return;
+    DFM2H_DEVICE_ARG(X);
+    DFM2H_DEVICE_ARG(flag);
+    DFM2H_DEVICE_ARG(Y);
     hipLaunchKernelGGL(k_xplusay, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), X, n, flag,
                        alpha, Y);
     check_launch("k_xplusay");
@@ -1130,6 +1256,8 @@ int dfm2h_xplusaybz(double *X, size_t n, const int *flag, double alpha, const do
   return guard(__func__, [&] {
     DFM2H_REQUIRE((X && flag && Y && Z) || n == 0, DFM2H_ERR_INVALID, "null vector");
     if (!n) return;
+    for (const void *p : {(const void *)X, (const void *)flag, (const void *)Y, (const void *)Z})
+      require_device(p, "a vector argument");
     hipLaunchKernelGGL(k_xplusaybzcw<false>, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), X,
                        n, flag, alpha, Y, beta, Z, 0.0, (const double *)nullptr);
     check_launch("k_xplusaybz");
@@ -1141,6 +1269,9 @@ int dfm2h_xplusaybzcw(double *X, size_t n, const int *flag, double alpha, const 
   return guard(__func__, [&] {
     DFM2H_REQUIRE((X && flag && Y && Z && W) || n == 0, DFM2H_ERR_INVALID, "null vector");
     if (!n) return;
+    for (const void *p : {(const void *)X, (const void *)flag, (const void *)Y, (const void *)Z,
+                          (const void *)W})
+      require_device(p, "a vector argument");
     hipLaunchKernelGGL(k_xplusaybzcw<true>, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), X,
                        n, flag, alpha, Y, beta, Z, gamma, W);
     check_launch("k_xplusaybzcw");

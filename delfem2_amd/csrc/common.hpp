@@ -64,7 +64,17 @@ int guard(const char *fn, F &&f) {
   }
 }
 
+// the calling thread's stream (dfm2h_set_stream)
 hipStream_t stream();
+// process-wide unique serial numbers (meshes, patterns): a handle created
+// at the address of a destroyed one never carries the same serial
+uint64_t next_serial();
+// Throws DFM2H_ERR_INVALID unless p (when non-null) is memory a kernel can
+// read: device, managed, or pinned host memory.  Pageable host memory --
+// e.g. a std::vector's data() passed where the reference took a host
+// array -- is rejected before anything is launched.
+void require_device(const void *p, const char *what);
+#define DFM2H_DEVICE_ARG(p) ::dfm2h::require_device((p), #p)
 // checks the last kernel launch
 void check_launch(const char *what);
 

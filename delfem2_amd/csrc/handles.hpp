@@ -22,6 +22,10 @@ struct Pattern {
 struct PcgWork;  // pcg.hip
 void destroy_pcg_work(PcgWork *);
 
+// Applies the matrix's staged per-element Merge additions (bsr.hip); every
+// entry point that reads or writes the matrix's values calls it first.
+void flush_merge(dfm2h_bsr_s *A);
+
 // Packs CRS values (and optional per-row diagonal blocks) into V when the
 // source version differs from the packed one.
 template <typename T>
@@ -84,7 +88,16 @@ struct dfm2h_bsr_s {
   dfm2h::Pattern pat;
   dfm2h::DBuf<double> vcrs, vdia;
   uint64_t version = 1;  // bumped whenever values change
-  uint64_t pattern_serial = 0;  // bumped whenever the pattern is set
+  uint64_t pattern_serial = 0;  // process-unique, set with the pattern (next_serial)
+  // Per-element Merge (dfm2h_bsr_merge): additions staged on the host in
+  // call order -- destination (slot < ncrs: off-diagonal slot; ncrs + i:
+  // diagonal block i) and the 3x3 block -- and applied on the device, in
+  // that order per destination, before the values are next used
+  std::vector<unsigned> mg_dst;
+  std::vector<double> mg_val;   // 9 per staged block
+  std::vector<unsigned> mg_mark;  // per column block: its slot in the current row, or ~0u
+  dfm2h::DBuf<double> mg_val_d;
+  dfm2h::DBuf<unsigned> mg_run_d;  // per touched destination: {dst, first, count}
   dfm2h::PackedPlan plan;  // all rows, CRS block order
   dfm2h::PackedVals<double> pk64;
   dfm2h::PackedVals<float> pk32;
@@ -114,8 +127,10 @@ struct dfm2h_ilu_s {
   std::vector<unsigned> colind, rowptr, diaind;  // host (sorted pattern)
   dfm2h::DBuf<unsigned> colind_d, rowptr_d, diaind_d;
   dfm2h::DBuf<double> vcrs, vdia;
-  // A slot -> ILU slot map for CopyValue (built against `mapped_for`)
+  // A slot -> ILU slot map for CopyValue, built against the matrix
+  // `mapped_for` with pattern serial `mapped_serial`
   const dfm2h_bsr_s *mapped_for = nullptr;
+  uint64_t mapped_serial = 0;
   dfm2h::DBuf<int> a2l_d;
   bool a2l_identity = false;  // a2l[s] == s for every slot (same pattern)
   // level schedules: rows grouped by dependency level, ascending inside a level
@@ -132,7 +147,7 @@ struct dfm2h_ilu_s {
   dfm2h::PackedVals<double> f64, b64;
   dfm2h::PackedVals<float> f32, b32;
   uint64_t version = 1;
-  uint64_t pattern_serial = 0;  // bumped whenever the pattern is set
+  uint64_t pattern_serial = 0;  // process-unique, set with the pattern (next_serial)
   dfm2h::PcgWork *work = nullptr;
   ~dfm2h_ilu_s() { dfm2h::destroy_pcg_work(work); }
 };

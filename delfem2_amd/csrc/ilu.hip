@@ -271,6 +271,7 @@ static void set_a2l(dfm2h_ilu P, dfm2h_bsr A) {
   P->a2l_d.upload(a2l);
   P->a2l_identity = id;
   P->mapped_for = A;
+  P->mapped_serial = A->pattern_serial;
 }
 
 static void need_ilu(dfm2h_ilu P) {
@@ -369,7 +370,7 @@ static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
   set_a2l(P, A);
   P->nsing_d.alloc(1);
   P->has_pattern = true;
-  P->pattern_serial++;
+  P->pattern_serial = next_serial();
   P->decomposed = false;
   P->version++;
   DFM2H_HIP(hipStreamSynchronize(stream()));
@@ -486,7 +487,10 @@ int dfm2h_ilu_copy_value(dfm2h_ilu P, dfm2h_bsr A) {
     need_ilu(P);
     DFM2H_REQUIRE(A && A->has_pattern, DFM2H_ERR_STATE, "matrix has no pattern");
     DFM2H_REQUIRE(A->nblk == P->nblk, DFM2H_ERR_INVALID, "size mismatch");
-    if (P->mapped_for != A) set_a2l(P, A);
+    flush_merge(A);
+    // the map is rebuilt for another matrix, or for a matrix created where
+    // the mapped one was (serials are process-unique)
+    if (P->mapped_for != A || P->mapped_serial != A->pattern_serial) set_a2l(P, A);
     if (P->a2l_identity) {  // same pattern, same slots: a plain copy
       if (P->ncrs)
         DFM2H_HIP(hipMemcpyAsync(P->vcrs.p, A->vcrs.p, sizeof(double) * 9 * P->ncrs,
@@ -553,6 +557,7 @@ int dfm2h_ilu_forward(dfm2h_ilu P, double *vec) {
   return guard(__func__, [&] {
     need_ilu(P);
     DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(vec);
     ilu_forward_levels_double(P, vec, vec);
   });
 }
@@ -561,6 +566,7 @@ int dfm2h_ilu_backward(dfm2h_ilu P, double *vec) {
   return guard(__func__, [&] {
     need_ilu(P);
     DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(vec);
     ilu_backward_levels_double(P, vec);
   });
 }
@@ -569,6 +575,7 @@ int dfm2h_ilu_solve_precond(dfm2h_ilu P, double *vec) {
   return guard(__func__, [&] {
     need_ilu(P);
     DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(vec);
     ilu_forward_levels_double(P, vec, vec);
     ilu_backward_levels_double(P, vec);
   });

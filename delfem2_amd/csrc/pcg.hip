@@ -1300,6 +1300,7 @@ static void need_solver(dfm2h_bsr A, dfm2h_ilu P) {
   DFM2H_REQUIRE(A && P, DFM2H_ERR_INVALID, "null handle");
   DFM2H_REQUIRE(A->has_pattern && P->has_pattern, DFM2H_ERR_STATE, "pattern not set");
   DFM2H_REQUIRE(A->nblk == P->nblk, DFM2H_ERR_INVALID, "matrix / preconditioner size mismatch");
+  flush_merge(A);
 }
 
 // mixed solve with Krylov vectors of precision T; returns the status
@@ -1345,6 +1346,8 @@ int dfm2h_vec_dot(const double *x, const double *y, size_t n, double *out) {
       return;
     }
     DFM2H_REQUIRE(x && y, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(x);
+    DFM2H_DEVICE_ARG(y);
     static thread_local DBuf<double> partials, res;
     const unsigned g = vec_grid(n);
     if (partials.n < g) partials.alloc(kVecGridMax);
@@ -1362,6 +1365,8 @@ int dfm2h_vec_add_scaled(double *y, double alpha, const double *x, size_t n) {
   return guard(__func__, [&] {
     if (!n) return;
     DFM2H_REQUIRE(x && y, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(x);
+    DFM2H_DEVICE_ARG(y);
     hipLaunchKernelGGL(k_add_scaled, dim3(vec_grid(n)), dim3(256), 0, stream(), y, alpha, x, n);
     check_launch("k_add_scaled");
   });
@@ -1371,6 +1376,8 @@ int dfm2h_vec_scale_and_add(double *y, double beta, const double *x, size_t n) {
   return guard(__func__, [&] {
     if (!n) return;
     DFM2H_REQUIRE(x && y, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(x);
+    DFM2H_DEVICE_ARG(y);
     hipLaunchKernelGGL(k_scale_and_add, dim3(vec_grid(n)), dim3(256), 0, stream(), y, beta, x, n);
     check_launch("k_scale_and_add");
   });
@@ -1380,6 +1387,8 @@ int dfm2h_vec_copy(double *dst, const double *src, size_t n) {
   return guard(__func__, [&] {
     if (!n || dst == src) return;
     DFM2H_REQUIRE(dst && src, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(dst);
+    DFM2H_DEVICE_ARG(src);
     DFM2H_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
   });
 }
@@ -1388,6 +1397,7 @@ int dfm2h_vec_set_zero(double *x, size_t n) {
   return guard(__func__, [&] {
     if (!n) return;
     DFM2H_REQUIRE(x, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(x);
     hipLaunchKernelGGL(k_fill<double>, dim3(vec_grid(n)), dim3(256), 0, stream(), x, 0.0, n);
     check_launch("k_fill");
   });
@@ -1398,6 +1408,8 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
   return guard(__func__, [&] {
     need_solver(A, P);
     DFM2H_REQUIRE(r && x, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(r);
+    DFM2H_DEVICE_ARG(x);
     PcgWork &W = work_of(P);
     PcgBufs<double> &B = W.f64;
     ensure_fusion(A, P, W);
@@ -1435,6 +1447,8 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
   return guard(__func__, [&] {
     need_solver(A, P);
     DFM2H_REQUIRE(b && x, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(b);
+    DFM2H_DEVICE_ARG(x);
     DFM2H_REQUIRE(delta > 0.0 && delta < 1.0, DFM2H_ERR_INVALID, "delta must be in (0, 1)");
     DFM2H_REQUIRE(max_outer >= 1 && max_inner >= 1, DFM2H_ERR_INVALID, "caps must be >= 1");
     PcgWork &W = work_of(P);

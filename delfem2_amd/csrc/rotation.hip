@@ -198,6 +198,9 @@ int dfm2h_rotation_at_points(dfm2h_bsr A, const double *xyz, const double *disp,
   return guard(__func__, [&] {
     DFM2H_REQUIRE(A && xyz && R, DFM2H_ERR_INVALID, "null argument");
     DFM2H_REQUIRE(A->has_pattern, DFM2H_ERR_INVALID, "matrix has no pattern");
+    DFM2H_DEVICE_ARG(xyz);
+    DFM2H_DEVICE_ARG(disp);
+    DFM2H_DEVICE_ARG(R);
     if (!A->nblk) return;
     hipLaunchKernelGGL(k_rotation_at_points, dim3(ceil_div(A->nblk, 128)), dim3(128), 0,
                        stream(), (unsigned)A->nblk, A->pat.colind_d.p, A->pat.rowptr_d.p, xyz,

@@ -15,10 +15,37 @@
 namespace dfm2h {
 
 static thread_local std::string g_last_error;
-static hipStream_t g_stream = nullptr;
+// The stream of the calling thread: a thread that sets one does not
+// redirect another thread's calls, so separate handles can be driven from
+// separate threads on separate streams.
+static thread_local hipStream_t g_stream = nullptr;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 hipStream_t stream() { return g_stream; }
+
+uint64_t next_serial() {
+  static std::atomic<uint64_t> serial{0};
+  return ++serial;
+}
+
+void require_device(const void *p, const char *what) {
+  if (!p) return;
+  hipPointerAttribute_t at{};
+  const hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // clear the failure so later launch checks do not see it
+    throw Error(DFM2H_ERR_INVALID, std::string(what) +
+                                       " is not device memory (a host pointer where a "
+                                       "device pointer is required)");
+  }
+  const bool ok = at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged ||
+                  at.type == hipMemoryTypeUnified ||
+                  (at.type == hipMemoryTypeHost && at.devicePointer != nullptr);
+  if (!ok)
+    throw Error(DFM2H_ERR_INVALID, std::string(what) +
+                                       " is not device memory (a host pointer where a "
+                                       "device pointer is required)");
+}
 
 void check_launch(const char *what) {
   hipError_t e = hipGetLastError();
@@ -180,6 +207,7 @@ int dfm2h_free(void *p) {
 
 int dfm2h_memcpy_htod(void *dst, const void *src, size_t bytes) {
   return guard(__func__, [&] {
+    DFM2H_DEVICE_ARG(dst);
     DFM2H_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
     DFM2H_HIP(hipStreamSynchronize(stream()));
   });
@@ -187,6 +215,7 @@ int dfm2h_memcpy_htod(void *dst, const void *src, size_t bytes) {
 
 int dfm2h_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   return guard(__func__, [&] {
+    DFM2H_DEVICE_ARG(src);
     DFM2H_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
     DFM2H_HIP(hipStreamSynchronize(stream()));
   });
@@ -194,6 +223,8 @@ int dfm2h_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
 
 int dfm2h_memcpy_dtod(void *dst, const void *src, size_t bytes) {
   return guard(__func__, [&] {
+    DFM2H_DEVICE_ARG(dst);
+    DFM2H_DEVICE_ARG(src);
     DFM2H_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
   });
 }
@@ -254,10 +285,9 @@ int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet, s
                   "mesh too large for 32-bit indices");
     for (size_t i = 0; i < ntet * 4; ++i)
       DFM2H_REQUIRE(tet[i] < np, DFM2H_ERR_INVALID, "tet node index out of range");
-    static std::atomic<uint64_t> mesh_serial{0};
     auto *M = new dfm2h_tetmesh_s;
     try {
-      M->serial = ++mesh_serial;
+      M->serial = next_serial();
       M->ntet = ntet;
       M->np = np;
       std::vector<unsigned> ind(np + 1, 0), es(ntet * 4);

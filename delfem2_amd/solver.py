@@ -118,6 +118,27 @@ class CMatrixSparse:
         N.call("dfm2h_bsr_set_zero", self.h)
         return True
 
+    def Merge(self, aIpRow, aIpCol, emat) -> bool:
+        """ls_block_sparse.h:260-310 Merge: adds emat[nrow][ncol][3][3] (host)
+        at the blocks (aIpRow[i], aIpCol[j]), the diagonal block taking
+        emat[i][i]; staged in call order and applied on the device before the
+        values are next used.  False when a block is missing from the pattern
+        (the blocks before it are added, as in the reference)."""
+        ir = np.ascontiguousarray(aIpRow, np.uint32).ravel()
+        ic = np.ascontiguousarray(aIpCol, np.uint32).ravel()
+        em = np.ascontiguousarray(emat, np.float64)
+        if em.size != ir.size * ic.size * 9:
+            raise ValueError("emat must hold nrow x ncol 3x3 blocks")
+        st = N.load().dfm2h_bsr_merge(self.h, ir.size, _hptr(ir), ic.size, _hptr(ic), _hptr(em))
+        if st == N.ERR_PATTERN:
+            return False
+        N.check(st)
+        return True
+
+    def values_changed(self) -> None:
+        """Declares that the device value arrays were written directly."""
+        N.call("dfm2h_bsr_values_changed", self.h)
+
     @property
     def ncrs(self) -> int:
         n = C.c_size_t(0)

@@ -8,6 +8,9 @@
  *   ViewAsVectorX<REAL>  view_vectorx.h:26-55 (a non-owning view of DEVICE memory)
  *   AddScaledVec / ScaleAndAddVec / AddMatVec / SolvePrecond / Dot
  *                        view_vectorx.h:61-109 (found by ADL)
+ *   Merge<nrow,ncol,3,3,double>  ls_block_sparse.h:260-310 (found by ADL, so
+ *                        the reference's MergeLinSys_* templates instantiate
+ *                        on CMatrixSparse below)
  *   MergeLinSys_SolidLinear_Static_MeshTet3D  fem_solidlinear.h:192-237
  *   MergeLinSys_SolidLinear_NewmarkBeta_MeshTet3D fem_solidlinear.h:331-379
  *   MergeLinSys_SolidLinear_BEuler_MeshTet3D  fem_solidlinear.h:381-443
@@ -35,6 +38,7 @@
 #include <cstddef>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dfm2_hip.h"
@@ -159,6 +163,24 @@ inline void SolvePrecond(ViewAsVectorXd &Pr_vec, const CPreconditionerILU &ilu) 
   ilu.SolvePrecond(Pr_vec.p);
 }
 inline double Dot(const ViewAsVectorXd &x, const ViewAsVectorXd &y) { return x.dot(y); }
+
+// ls_block_sparse.h:260-310 with the reference's signature.  An
+// unqualified Merge<4, 4, 3, 3, double>(mat_A, aIP, aIP, emat, buffer) in
+// the reference's element loops (fem_solidlinear.h:192-530) finds this one
+// by ADL when mat_A is a delfem2::hip::CMatrixSparse: the blocks are staged
+// and applied on the device in call order (bit-identical sums); the
+// buffer is not needed.  False when a block is not in the pattern.
+template <int nrow, int ncol, int ndimrow, int ndimcol, typename T>
+bool Merge(CMatrixSparse &A, const unsigned int aIpRow[nrow], const unsigned int aIpCol[ncol],
+           const T emat[nrow][ncol][ndimrow][ndimcol],
+           std::vector<unsigned int> & /*merge_buffer*/) {
+  static_assert(ndimrow == 3 && ndimcol == 3, "3x3 blocks only");
+  static_assert(std::is_same<T, double>::value, "the device matrix holds doubles");
+  const int st = dfm2h_bsr_merge(A.h, nrow, aIpRow, ncol, aIpCol, &emat[0][0][0][0]);
+  if (st == DFM2H_ERR_PATTERN) return false;
+  check(st);
+  return true;
+}
 
 // Fused device Solve_PCG (lsitrsol.h:80-148): same arguments and return
 // value; Pr and p are scratch in the reference and are not touched here.
@@ -461,6 +483,13 @@ class LinearSystemSolver_BlockSparseILU {
 };
 
 }  // namespace hip
+
+// Makes the name Merge a template visible in namespace delfem2, so that the
+// reference's element loops (which call Merge<...>(mat_A, ...) unqualified)
+// parse under C++17 even where ls_block_sparse.h is not included before
+// them; the call itself resolves by overload (delfem2::hip::CMatrixSparse).
+using hip::Merge;
+
 }  // namespace delfem2
 
 #endif  // DELFEM2_HIP_HPP

@@ -6,9 +6,13 @@
  *   -> Solve_PCG
  *
  * Plain pointers and sizes only.  Unless stated otherwise, array arguments
- * named *_dev are device pointers and the others are host pointers.  Every
- * call is ordered on the library stream (dfm2h_set_stream); calls that
- * return host values synchronise that stream.  Every function returns a
+ * named *_dev are device pointers and the others are host pointers; a
+ * device argument that is pageable host memory (where the reference took a
+ * host array) is rejected with DFM2H_ERR_INVALID before anything runs.
+ * Every call is ordered on the calling thread's stream (dfm2h_set_stream);
+ * calls that return host values synchronise that stream.  A handle is not
+ * thread-safe; separate handles may be used from separate threads at the
+ * same time, each thread on its own stream.  Every function returns a
  * status code (DFM2H_OK == 0); dfm2h_last_error() describes the last
  * failure of the calling thread.  No C++ exception crosses this boundary.
  *
@@ -50,8 +54,9 @@ typedef struct dfm2h_batch_s *dfm2h_batch;     /* many meshes, one solver     */
 /* Selects the HIP device for this thread.  No reference counterpart (the
  * reference is host-only). */
 int dfm2h_init(int device);
-/* The stream every later call is ordered on (a hipStream_t; NULL = default
- * stream).  No reference counterpart. */
+/* The stream every later call of the calling thread is ordered on (a
+ * hipStream_t; NULL = default stream); other threads keep their own.  No
+ * reference counterpart. */
 int dfm2h_set_stream(void *hip_stream);
 void *dfm2h_get_stream(void);
 int dfm2h_synchronize(void);
@@ -87,9 +92,29 @@ int dfm2h_bsr_set_zero(dfm2h_bsr A);
  * as public host vectors; here they live in HBM).  Either pointer may be NULL. */
 int dfm2h_bsr_upload(dfm2h_bsr A, const double *val_crs, const double *val_dia);
 int dfm2h_bsr_download(dfm2h_bsr A, double *val_crs, double *val_dia);
-/* Sizes and the device arrays (nullable outputs). */
+/* Sizes and the device arrays (nullable outputs).  The arrays are for
+ * reading; a caller that writes through them must call
+ * dfm2h_bsr_values_changed afterwards, or later calls keep using copies
+ * packed from the old values. */
 int dfm2h_bsr_info(dfm2h_bsr A, size_t *nblk, size_t *ncrs,
                    double **val_crs_dev, double **val_dia_dev);
+/* Declares that the values were written outside the library (through the
+ * arrays of dfm2h_bsr_info).  No reference counterpart (the reference's
+ * members are plain host vectors). */
+int dfm2h_bsr_values_changed(dfm2h_bsr A);
+/* ls_block_sparse.h:260-310 Merge<nrow, ncol, 3, 3, double>(A, ip_row,
+ * ip_col, emat, merge_buffer): adds the element blocks emat[nrow][ncol][3][3]
+ * (host, row-major; ip_row / ip_col host) into A, row by row: block (irow,
+ * jcol) goes to stored slot (ip_row[irow], ip_col[jcol]), or -- when the two
+ * nodes are the same -- emat[irow][irow] goes to the diagonal block, as
+ * there.  The additions are staged on the host and applied on the device,
+ * in call order per block, before A's values are next used: bit-identical
+ * to the reference's sequential `+=`.  Returns DFM2H_ERR_PATTERN when a slot
+ * is missing; the blocks before it (in the reference's order) are still
+ * added and the rest of the element is not, as the reference returns false
+ * there. */
+int dfm2h_bsr_merge(dfm2h_bsr A, unsigned nrow, const unsigned *ip_row,
+                    unsigned ncol, const unsigned *ip_col, const double *emat);
 /* ls_block_sparse.cpp:215-273 MatVec: y = alpha*A*x + beta*y (y is scaled
  * by beta first, unconditionally, then off-diagonal blocks in stored order,
  * then the diagonal block: bit-identical to the reference). */

@@ -9,7 +9,9 @@ outputs under tests/golden/:
   iteration counts, residual histories and scalar summaries.
 
 Usage (needs oracle/_ref built, i.e. the reference tree at build time):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py meshes     # only the mesh hashes, into
+                                                  # the existing golden.json
 """
 from __future__ import annotations
 
@@ -87,6 +89,24 @@ def summary(out: dict) -> dict:
     d["x_max_abs"] = float(np.abs(out["x"]).max())
     d["x_norm"] = float(np.linalg.norm(out["x"]))
     return d
+
+
+MESH_HASH_DIMS = [(10, 10, 10), (64, 64, 64), (96, 48, 48)]
+
+
+def mesh_hashes() -> dict:
+    """SHA-256 of the reference's MeshTet3D_VoxelGrid arrays (integer node
+    coordinates as float64, tets as uint32) at the C1, C2 and C4 grid sizes."""
+    R = oracle.ref()
+    out = {}
+    for (nx, ny, nz) in MESH_HASH_DIMS:
+        npn = (nx + 1) * (ny + 1) * (nz + 1)
+        xyz = np.zeros(npn * 3)
+        tet = np.zeros(nx * ny * nz * 24, np.uint32)
+        R.ref_mesh_tet_voxel(nx, ny, nz, xyz, tet)
+        out[f"{nx}x{ny}x{nz}"] = {"xyz": sha(xyz), "tet": sha(tet), "np": npn,
+                                  "ntet": int(tet.size // 4)}
+    return out
 
 
 def main():
@@ -251,11 +271,29 @@ def main():
                                         "seeds": [61, 62, 63], "dt": 0.03, "gamma": 0.6,
                                         "beta": 0.36, "g": pn.g.tolist()}
 
+    meta["mesh_hashes"] = mesh_hashes()
+
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **arrays)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1)
     print("wrote", os.path.join(HERE, "golden.npz"), os.path.join(HERE, "golden.json"))
 
 
+def update_meshes():
+    """Adds the mesh hashes to the existing golden.json, nothing else."""
+    if not oracle.ref_available():
+        raise SystemExit("oracle/_ref/libdfm2ref.so not built (needs the reference tree)")
+    path = os.path.join(HERE, "golden.json")
+    with open(path) as f:
+        meta = json.load(f)
+    meta["mesh_hashes"] = mesh_hashes()
+    with open(path, "w") as f:
+        json.dump(meta, f, indent=1)
+    print("updated", path)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "meshes":
+        update_meshes()
+        raise SystemExit(0)
     main()

@@ -64,3 +64,24 @@ def test_time_stepping_against_reference(gpu):
     for k in ("newmark_u_rel", "newmark_v_rel", "newmark_a_rel", "stiffwarp_u_rel",
               "stiffwarp_v_rel"):
         assert out[k] < 1e-7, k
+
+
+MERGE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "dropin_merge")
+
+
+def test_reference_element_loops_on_device_matrix(gpu):
+    """The reference's MergeLinSys_* templates (static, Newmark-beta,
+    backward Euler, stiffness-warped), unchanged, on
+    delfem2::hip::CMatrixSparse (Merge found by ADL) and on the reference's
+    host CMatrixSparse<double>: A and b bit-identical, and the device bulk
+    assembly of each kind bit-identical to both; a host flag array passed to
+    SetFixedBC is refused with DFM2H_ERR_INVALID."""
+    if not os.path.exists(MERGE):
+        pytest.skip("merge binary not built (needs the reference headers at build time)")
+    r = subprocess.run([MERGE], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    for kind in ("static", "newmark", "beuler", "stiffwarp"):
+        assert out[f"{kind}_template_same"] and out[f"{kind}_bulk_same"], kind
+    assert out["host_pointer_status"] == 1

@@ -62,6 +62,18 @@ def test_mesh_and_pattern_full(dims):
     assert np.array_equal(ps, G[key + "_psup"])
 
 
+@pytest.mark.parametrize("dims", [(10, 10, 10), (64, 64, 64), (96, 48, 48)])
+def test_mesh_hashes_at_config_sizes(dims):
+    """The workload's mesh (every GPU test and the bench consume it) and the
+    oracle's, against SHA-256 of the reference's MeshTet3D_VoxelGrid arrays
+    at the C1, C2 and C4 grid sizes (integer coordinates, natural order)."""
+    ref = META["mesh_hashes"]["%dx%dx%d" % dims]
+    for xyz, tet in (W.mesh_tet_voxel(*dims), oracle.mesh_tet_voxel(*dims)):
+        assert xyz.size == 3 * ref["np"] and tet.size == 4 * ref["ntet"]
+        assert sha(np.ascontiguousarray(xyz, np.float64)) == ref["xyz"]
+        assert sha(np.ascontiguousarray(tet, np.uint32)) == ref["tet"]
+
+
 def test_element_matrices_bit_exact():
     P, D, ml = G["elem_P"], G["elem_disp"], G["elem_myu_lambda"]
     for e in range(P.shape[0]):

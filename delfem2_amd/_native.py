@@ -94,6 +94,7 @@ SIGNATURES = {
     "dfm2h_ilu_download": [_vp, _vp, _vp, _vp, _vp],
     "dfm2h_ilu_levels": [_vp, _pu, _pu],
     "dfm2h_vec_dot": [_vp, _vp, _sz, _pd],
+    "dfm2h_vec_dot_noalloc": [_vp, _vp, _sz, _pd],
     "dfm2h_vec_add_scaled": [_vp, _d, _vp, _sz],
     "dfm2h_vec_scale_and_add": [_vp, _d, _vp, _sz],
     "dfm2h_vec_copy": [_vp, _vp, _sz],

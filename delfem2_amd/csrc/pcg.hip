@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <mutex>
 #include <type_traits>
 
 namespace dfm2h {
@@ -1336,6 +1337,50 @@ static PcgScal mixed_solve(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B, 
   return st;
 }
 
+namespace dfm2h {
+
+// Dot on the calling thread's stream: one partial per workgroup into
+// partials[0 .. vec_grid(n)), their fixed-order sum into *res, read back.
+// Every partial the sum reads is written first, so nothing needs clearing.
+static void dot_into(const double *x, const double *y, size_t n, double *partials, double *res,
+                     double *out) {
+  const unsigned g = vec_grid(n);
+  hipLaunchKernelGGL(k_dot, dim3(g), dim3(kVecBlock), 0, stream(), x, y, n, partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kVecBlock), 0, stream(), partials, g, res);
+  check_launch("k_dot");
+  DFM2H_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  DFM2H_HIP(hipStreamSynchronize(stream()));
+}
+
+// Scratch of the allocation-free dot: kVecGridMax workgroup partials, then
+// the result.  Each GPU holds its own copy until the process ends; calls on
+// one GPU share it.
+constexpr size_t kDotScratch = kVecGridMax + 1;
+static __device__ __align__(256) double scratch[kDotScratch];
+
+// The device address of `scratch` on the current device, looked up once
+// per device (the host-side name is not a device address).
+static double *scratch_on_current_device() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double *addr[kMaxDevices] = {};
+  int dev = 0;
+  DFM2H_HIP(hipGetDevice(&dev));
+  DFM2H_REQUIRE(dev >= 0 && dev < kMaxDevices, DFM2H_ERR_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    DFM2H_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DFM2H_REQUIRE(p != nullptr, DFM2H_ERR_HIP, "no device address for the dot scratch");
+    addr[dev] = static_cast<double *>(p);
+  }
+  return addr[dev];
+}
+
+}  // namespace dfm2h
+
+using namespace dfm2h;
+
 extern "C" {
 
 int dfm2h_vec_dot(const double *x, const double *y, size_t n, double *out) {
@@ -1349,15 +1394,28 @@ int dfm2h_vec_dot(const double *x, const double *y, size_t n, double *out) {
     DFM2H_DEVICE_ARG(x);
     DFM2H_DEVICE_ARG(y);
     static thread_local DBuf<double> partials, res;
-    const unsigned g = vec_grid(n);
-    if (partials.n < g) partials.alloc(kVecGridMax);
+    if (partials.n < kVecGridMax) partials.alloc(kVecGridMax);
     if (!res.p) res.alloc(1);
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kVecBlock), 0, stream(), x, y, n, partials.p);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kVecBlock), 0, stream(), partials.p, g,
-                       res.p);
-    check_launch("k_dot");
-    DFM2H_HIP(hipMemcpyAsync(out, res.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
-    DFM2H_HIP(hipStreamSynchronize(stream()));
+    dot_into(x, y, n, partials.p, res.p, out);
+  });
+}
+
+int dfm2h_vec_dot_noalloc(const double *x, const double *y, size_t n, double *out) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(out, DFM2H_ERR_INVALID, "null output");
+    if (n == 0) {
+      *out = 0.0;
+      return;
+    }
+    DFM2H_REQUIRE(x && y, DFM2H_ERR_INVALID, "null vector");
+    DFM2H_DEVICE_ARG(x);
+    DFM2H_DEVICE_ARG(y);
+    // the partials of vec_grid(n) workgroups and the result must fit the
+    // scratch (vec_grid never exceeds kVecGridMax: any n fits)
+    DFM2H_REQUIRE(vec_grid(n) + 1 <= kDotScratch, DFM2H_ERR_INVALID,
+                  "too many partial sums for the dot scratch");
+    double *sc = scratch_on_current_device();
+    dot_into(x, y, n, sc, sc + kVecGridMax, out);
   });
 }
 

@@ -250,6 +250,17 @@ int dfm2h_ilu_levels(dfm2h_ilu P, unsigned *n_forward, unsigned *n_backward);
 /* view_vectorx.h:104-109 Dot -> host double (deterministic tree order). */
 int dfm2h_vec_dot(const double *x_dev, const double *y_dev, size_t n,
                   double *out);
+/* No counterpart in the reference: dfm2h_vec_dot for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and summation order, so
+ * the result is bit-identical to dfm2h_vec_dot; it calls no hipMalloc /
+ * hipFree.  Its scratch (2048 workgroup partials and the result, 16 KiB) is
+ * a static device array that each GPU holds until the process ends; every
+ * call on one GPU shares it, so calls that overlap on one GPU (other
+ * threads or streams) race.  Any n is accepted (the partials never exceed
+ * the scratch). */
+int dfm2h_vec_dot_noalloc(const double *x_dev, const double *y_dev, size_t n,
+                          double *out);
 /* view_vectorx.h:61-71 AddScaledVec: y += alpha*x. */
 int dfm2h_vec_add_scaled(double *y_dev, double alpha, const double *x_dev,
                          size_t n);

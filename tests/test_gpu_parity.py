@@ -198,14 +198,15 @@ def test_c2_full_size_vs_reference(gpu):
     assert hist[0] == pytest.approx(m["hist"][0], rel=1e-13)
     # the whole residual history against the reference's (557 entries): the
     # device's dot products sum in another order, so entries drift apart by
-    # rounding only -- tight early on, within 2% to the end
+    # rounding only (measured on MI355X: 2.5e-13 over the first 100 entries,
+    # 6.9e-3 at most over all 557)
     h_dev = np.asarray(hist)
     h_ref = np.asarray(m["hist"])
     k = min(h_dev.size, h_ref.size)
     dev_rel = np.abs(h_dev[:k] / h_ref[:k] - 1.0)
     print(f"[c2] history rel. deviation: first 100 max {dev_rel[:100].max():.2e}, "
           f"all {k} max {dev_rel.max():.2e}", flush=True)
-    assert dev_rel[:100].max() <= 1e-6
+    assert dev_rel[:100].max() <= 1e-10
     assert dev_rel.max() <= 2e-2
     xn = torch.linalg.norm(x).item()
     assert abs(xn - m["x_norm"]) <= FP64_X_TOL * m["x_norm"]

@@ -4,12 +4,13 @@ MI355X, FP32-inner + FP64-refinement vs plain FP64 ILU(0)-PCG.
 
 Workload: 64^3 hexes x 6 tets (274,625 nodes, 1,572,864 tets), E_e =
 exp(0.5 z_e) from the seeded portable RNG, nu = 0.3, z=0 clamped, body
-force (0,0,-1e-3), 8-colour node order.  One step = CopyValue + Decompose
+force (0,0,-1e-3), 4-colour node order.  One step = CopyValue + Decompose
 + the full solve to relative residual 1e-8 with the matrix, factors' pattern
 and right-hand side resident in HBM.  The JSON line reports the
 FP32+refinement time (value) beside the FP64 time, DOF*iterations/s, the
-HBM roofline of the dominant kernel and the CPU reference timed on a
-bounded sample.
+HBM roofline of the dominant kernel and the CPU reference (the reference's
+own code, compiled) solving the same cube in full, in its natural node order
+and in the GPU's.
 """
 from __future__ import annotations
 
@@ -107,11 +108,12 @@ def pcg_iter_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0, sv: in
     return sum(kernel_bytes(nb, nnzb, s, n0, nu0, sv).values())
 
 
-def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
-    """The CPU reference (oracle/_ref, the reference's own code compiled) or,
-    without it, the oracle restatement, on a bounded sample of the same
-    workload: full set-up + factorisation, then `max_cpu_iters` PCG
-    iterations; time-to-1e-8 extrapolated with the FP64 iteration count."""
+def cpu_reference_run(prob, tol: float = 1e-8, max_it: int = 20000, keep_x: bool = False) -> dict:
+    """The CPU reference (oracle/_ref: the reference's own code compiled) or,
+    without it, the oracle restatement, on the whole workload, single
+    thread: pattern + assembly + BC, Decompose, and Solve_PCG to `tol`, each
+    timed.  Returns the phase times, the iteration count and the final
+    relative residual."""
     import numpy as np
 
     import oracle
@@ -121,26 +123,99 @@ def cpu_baseline(prob, iters_fp64: int, max_cpu_iters: int):
     Sys = oracle.RefSystem(ind, ps) if kind == "reference" else oracle.System(ind, ps)
     b = np.zeros(prob.ndof)
     Sys.assemble(b, prob.myu, prob.lam, prob.rho, prob.g, prob.xyz, prob.tet)
+    if prob.f_ext is not None:
+        b += prob.f_ext
     Sys.set_fixed_bc(prob.flag)
     oracle.set_rhs_zero(b, prob.flag)
     t1 = time.perf_counter()
     Sys.factor()
     t2 = time.perf_counter()
-    _, hist = Sys.solve_pcg(b.copy(), 1e-8, max_cpu_iters)
+    x, hist = Sys.solve_pcg(b.copy(), tol, max_it)
     t3 = time.perf_counter()
-    n_it = len(hist) - 2 if len(hist) - 1 > max_cpu_iters else len(hist) - 1
-    n_it = max(n_it, 1)
-    t_it = (t3 - t2) / n_it
-    t_solve = (t2 - t1) + t_it * iters_fp64
-    return {
-        "value": round(t_solve, 3),
-        "unit": "s (time-to-1e-8 = Decompose + FP64 PCG, iterations x measured s/iteration)",
+    # a run stopped by the cap ends with a repeated entry (lsitrsol.h:146)
+    out = {"kind": kind, "setup_s": t1 - t0, "decompose_s": t2 - t1, "pcg_s": t3 - t2,
+           "iterations": int(min(len(hist) - 1, max_it)),
+           "rel_residual": float(hist[-1] / hist[0]) if hist[0] > 0 else 0.0}
+    if keep_x:
+        out["x"] = x
+    return out
+
+
+def cpu_worker(spec: str) -> None:
+    """Child-process entry (--cpu-worker n:sigma:ordering): the CPU reference
+    run of one cube workload, as one JSON line on stdout."""
+    from delfem2_amd import workload as W
+    n, sigma, ordering = spec.split(":")
+    prob = W.cube(int(n), sigma=float(sigma), seed=20261203, ordering=ordering)
+    out = cpu_reference_run(prob)
+    out["ndof"] = prob.ndof
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(n: int, sigma: float, ordering: str) -> dict:
+    """The CPU reference solved in full to 1e-8, single thread, in the
+    reference's own (natural) node order -- how its callers run it -- and in
+    the GPU workload's order, as two child processes side by side (one core
+    each).  value = the natural-order time-to-1e-8 (Decompose + PCG)."""
+    import subprocess
+
+    orders = ["natural"] + ([ordering] if ordering != "natural" else [])
+    procs = {o: subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                                  f"{n}:{sigma}:{o}"], stdout=subprocess.PIPE, text=True)
+             for o in orders}
+    runs = {}
+    for o, p in procs.items():
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"CPU reference run ({o} order) failed")
+        runs[o] = json.loads(out.strip().splitlines()[-1])
+    nat = runs["natural"]
+
+    def solve_s(r):
+        return r["decompose_s"] + r["pcg_s"]
+
+    res = {
+        "value": round(solve_s(nat), 3),
+        "unit": "s (time-to-1e-8 = Decompose + FP64 PCG, measured in full)",
         "cores": 1,
-        "kind": kind,
-        "sample": (f"{prob.name} ({prob.ndof} DOF): pattern+assembly+BC {t1 - t0:.2f} s, "
-                   f"Decompose {t2 - t1:.3f} s, {n_it} PCG iterations {t3 - t2:.2f} s "
-                   f"({t_it * 1e3:.1f} ms/iter) extrapolated to {iters_fp64} iterations"),
-        "dof_iter_per_s": round(prob.ndof * n_it / (t3 - t2), 1),
+        "kind": nat["kind"],
+        "measured_iterations": nat["iterations"],
+        "sample": (f"the whole workload ({nat['ndof']} DOF) in the reference's natural node "
+                   f"order: pattern+assembly+BC {nat['setup_s']:.2f} s, Decompose "
+                   f"{nat['decompose_s']:.3f} s, {nat['iterations']} PCG iterations "
+                   f"{nat['pcg_s']:.2f} s ({nat['pcg_s'] / max(nat['iterations'], 1) * 1e3:.1f} "
+                   f"ms/iter), final relative residual {nat['rel_residual']:.2e}; the "
+                   f"{ordering} order beside it, in a second process on another core"),
+        "dof_iter_per_s": round(nat["ndof"] * nat["iterations"] / nat["pcg_s"], 1),
+        **host_cpu(),
+    }
+    for o in orders[1:]:
+        r = runs[o]
+        res[o] = {"value": round(solve_s(r), 3), "measured_iterations": r["iterations"],
+                  "setup_s": round(r["setup_s"], 3), "decompose_s": round(r["decompose_s"], 3),
+                  "pcg_s": round(r["pcg_s"], 3), "rel_residual": r["rel_residual"],
+                  "dof_iter_per_s": round(r["ndof"] * r["iterations"] / r["pcg_s"], 1)}
+    return res
+
+
+def cpu_baseline_sample(prob, iters_fp64: int, max_cpu_iters: int) -> dict:
+    """For workloads whose full CPU solve takes many minutes (128^3): set-up,
+    Decompose and `max_cpu_iters` PCG iterations of the CPU reference, the
+    time-to-tolerance extrapolated with the GPU's FP64 iteration count."""
+    r = cpu_reference_run(prob, 1e-30, max_cpu_iters)
+    n_it = max(r["iterations"], 1)
+    t_it = r["pcg_s"] / n_it
+    return {
+        "value": round(r["decompose_s"] + t_it * iters_fp64, 3),
+        "unit": "s (Decompose + FP64 PCG, iterations x measured s/iteration)",
+        "cores": 1,
+        "kind": r["kind"],
+        "measured_iterations": n_it,
+        "sample": (f"{prob.name} ({prob.ndof} DOF): pattern+assembly+BC {r['setup_s']:.2f} s, "
+                   f"Decompose {r['decompose_s']:.3f} s, {n_it} PCG iterations "
+                   f"{r['pcg_s']:.2f} s ({t_it * 1e3:.1f} ms/iter) extrapolated to "
+                   f"{iters_fp64} iterations"),
+        "dof_iter_per_s": round(prob.ndof * n_it / r["pcg_s"], 1),
         **host_cpu(),
     }
 
@@ -308,35 +383,27 @@ def bench_adjoint(args):
     cpu = None
     if args.cpu_iters > 0:
         import oracle
-        print("[bench] CPU baseline sample (oracle restatement)", file=sys.stderr, flush=True)
+        print("[bench] CPU baseline: the reference's forward solve in full",
+              file=sys.stderr, flush=True)
         m1, l1 = unit_lame(prob.extra["nu"])
         E = rho_np ** p
-        t_a = time.perf_counter()
-        ind, ps = oracle.psup(prob.tet, prob.np_)
-        Cs = oracle.System(ind, ps)
-        b = np.zeros(prob.ndof)
-        Cs.assemble(b, E * m1, E * l1, prob.rho, prob.g, prob.xyz, prob.tet)
-        b += prob.f_ext
-        Cs.set_fixed_bc(prob.flag)
-        oracle.set_rhs_zero(b, prob.flag)
-        t_b = time.perf_counter()
-        Cs.factor()
-        t_c = time.perf_counter()
-        n_it = max(5, min(args.cpu_iters, 100))
-        uc, hc = Cs.solve_pcg(b.copy(), 1e-30, n_it)
+        cprob = W.cantilever(96, 48, 48, seed=20261204, ordering="color4")
+        cprob.myu, cprob.lam = E * m1, E * l1
+        r = cpu_reference_run(cprob, tol, 20000, keep_x=True)
+        uc = r.pop("x")
+        # the contraction has no reference counterpart: the oracle's, timed
         t_d = time.perf_counter()
         oracle.adjoint_grad(m1, l1, prob.xyz, prob.tet, uc, uc)
-        t_e = time.perf_counter()
-        t_it = (t_d - t_c) / max(len(hc) - 1, 1)
-        n_tot = ph["forward_iterations"] + ph["adjoint_iterations"]
-        value = (t_b - t_a) + (t_c - t_b) + n_tot * t_it + (t_e - t_d)
-        cpu = {"value": round(value, 3), "unit": "s per gradient (extrapolated)", "cores": 1,
-               "kind": "port",
-               "sample": f"oracle restatement on the same problem: pattern+assembly+BC "
-                         f"{t_b - t_a:.2f} s, Decompose {t_c - t_b:.3f} s, {len(hc) - 1} PCG "
-                         f"iterations {t_d - t_c:.2f} s ({t_it * 1e3:.1f} ms/iter) extrapolated "
-                         f"to {n_tot} (the forward solve; the adjoint reuses it), "
-                         f"contraction {t_e - t_d:.3f} s",
+        t_con = time.perf_counter() - t_d
+        value = r["setup_s"] + r["decompose_s"] + r["pcg_s"] + t_con
+        cpu = {"value": round(value, 3), "unit": "s per gradient (measured)", "cores": 1,
+               "kind": r["kind"], "measured_iterations": r["iterations"],
+               "sample": f"the whole gradient, single thread: the reference's pattern + "
+                         f"assembly + BC {r['setup_s']:.2f} s, Decompose "
+                         f"{r['decompose_s']:.3f} s, forward PCG to {tol} in "
+                         f"{r['iterations']} iterations {r['pcg_s']:.2f} s (the adjoint reuses "
+                         f"it: lam = u), and the oracle's element contraction {t_con:.3f} s "
+                         f"(no reference counterpart)",
                **host_cpu()}
     out = {
         "metric": "SIMP compliance gradient dC/drho of the 96x48x48 tet cantilever (s)",
@@ -532,7 +599,7 @@ def bench_cube128(args):
     cpu = None
     if args.cpu_iters > 0:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(prob, it64, min(args.cpu_iters, 20))
+        cpu = cpu_baseline_sample(prob, it64, min(args.cpu_iters, 20))
         cpu["unit"] = "s (time-to-1e-10 = Decompose + FP64 PCG, iterations x measured s/iteration)"
     n0, nu0 = level0_rows(G.psup_ind, G.psup, nb)
     b32 = pcg_iter_bytes(nb, nnzb, 4, n0, nu0, 4 if args.mixed_vectors == 32 else 8)
@@ -586,13 +653,19 @@ def main():
                     help="Krylov vector precision of the mixed solver (matrix and factors "
                          "are FP32 either way)")
     ap.add_argument("--cpu-iters", type=int, default=200,
-                    help="PCG iterations of the bounded CPU sample (0 = skip)")
+                    help="CPU baseline: 0 = skip; cube: any other value runs the reference's "
+                         "full solves; other workloads: PCG iterations of the bounded sample")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--spmv-reps", type=int, default=50,
                     help="launches per kernel class when timing the fused kernels")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MB (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker)
+        return
 
     if args.copy_mb:
         import torch
@@ -723,10 +796,29 @@ def main():
     b_iter64 = pcg_iter_bytes(nb, nnzb, 8, n0, nu0)
     t_it64 = t_fp64 / max(it64, 1)
 
+    # one-off packing of A's sliced FP32 / FP64 copies (redone only when A's
+    # values change, so outside the timed steps): a step right after A is
+    # marked changed, minus a plain step
+    t_pack = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step_mixed()
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t
+        G.A.values_changed()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step_mixed()
+        torch.cuda.synchronize()
+        t_pack.append(time.perf_counter() - t - t_plain)
+    pack_ms = sorted(t_pack)[1] * 1e3
+
     cpu = None
     if args.cpu_iters > 0:
-        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(prob, it64, args.cpu_iters)
+        print("[bench] CPU reference (natural and workload order, full solves)",
+              file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.n, args.sigma, args.ordering)
 
     out = {
         "metric": "time-to-1e-8 PCG at 823k DOF: FP32+refine vs FP64 (s, DOF·iter/s, %HBM)",
@@ -763,6 +855,7 @@ def main():
                      "frac_of_stream_copy": round(dom_gbs / copy_gbs, 4)},
         "factor_ms": round(t_factor * 1e3, 3),
         "pcg_s": {"mixed": round(t_mixed - t_factor, 5), "fp64": round(t_fp64 - t_factor, 5)},
+        "pack_ms": round(pack_ms, 3),  # A's sliced copies, packed once, outside the steps
         "kernels": kern,
         "sweep_shape": shapes,
         "cpu_baseline": cpu,

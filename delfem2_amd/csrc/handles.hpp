@@ -114,6 +114,10 @@ struct dfm2h_tetmesh_s {
   uint64_t serial = 0;  // unique per created mesh
   size_t ntet = 0, np = 0;
   unsigned max_elsup = 0;           // most elements around one node
+  // nodes with more elements than the LDS form of the device pattern build
+  // takes, and where each one's candidate run starts (4 per element)
+  size_t nbig = 0, big_cand = 0;
+  dfm2h::DBuf<unsigned> big, big_off;
   dfm2h::DBuf<unsigned> tet;        // [4*ntet]
   dfm2h::DBuf<unsigned> elsup_ind;  // [np+1]
   dfm2h::DBuf<unsigned> elsup;      // element ids, ascending per node
@@ -136,6 +140,7 @@ struct dfm2h_ilu_s {
   // level schedules: rows grouped by dependency level, ascending inside a level
   std::vector<unsigned> levptr_f, levptr_b;
   dfm2h::DBuf<unsigned> levrows_f;
+  bool flev_ranges = false;  // every forward level is a consecutive, ascending row range
   dfm2h::DBuf<int> nsing_d;
   unsigned dec_lanes = 0;  // lanes per row of the factorisation kernel (0: one thread per row)
   // sliced sweeps: forward plan (lower blocks + D^-1), backward plan (upper blocks)

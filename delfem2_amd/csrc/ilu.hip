@@ -334,6 +334,8 @@ static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
     std::vector<unsigned> s0(P->nblk), s1(P->nblk);
     build_levels(P->nblk, P->colind, P->rowptr, P->diaind, true, P->levptr_f, rows);
     P->levrows_f.upload(rows);
+    P->flev_ranges = true;  // level l is rows [levptr_f[l], levptr_f[l+1]) in order
+    for (unsigned k = 0; P->flev_ranges && k < P->nblk; ++k) P->flev_ranges = rows[k] == k;
     std::vector<std::vector<unsigned>> groups;
     for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l)
       groups.emplace_back(rows.begin() + P->levptr_f[l], rows.begin() + P->levptr_f[l + 1]);

@@ -221,6 +221,26 @@ __device__ inline unsigned wg_id(bool xcd) {
   return x * q + min(x, rem) + b / kXcds;
 }
 
+// Workgroup number by segments: the logical workgroups form nseg segments
+// [seg[k], seg[k+1]) whose sizes are multiples of 8 except the last; XCD x
+// takes run x of each segment's eight equal runs (the last segment's runs
+// follow the grid's own remainder, so the map is a bijection).  For a
+// matrix whose rows are grouped by colour, one segment per colour gives
+// every XCD the same spatial slab of every colour: the neighbours a row
+// gathers then lie in its own XCD's L2.  nseg = 1 is wg_id(true).
+constexpr unsigned kMaxSeg = 8;
+__device__ inline unsigned seg_wg_id(const unsigned *seg, unsigned nseg) {
+  const unsigned b = blockIdx.x, x = b % kXcds;
+  unsigned k = b / kXcds;  // the k-th workgroup that XCD x runs
+  for (unsigned s = 0; s + 1 < nseg; ++s) {
+    const unsigned run = (seg[s + 1] - seg[s]) / kXcds;
+    if (k < run) return seg[s] + x * run + k;
+    k -= run;
+  }
+  const unsigned n = seg[nseg] - seg[nseg - 1], q = n / kXcds, rem = n % kXcds;
+  return seg[nseg - 1] + x * q + min(x, rem) + k;
+}
+
 __device__ inline SliceLane slice_lane(unsigned slice0, bool xcd = false) {
   SliceLane L;
   L.lane = threadIdx.x & (kWave - 1);

@@ -179,6 +179,9 @@ struct PcgArgs {
   double *partials4;  // mixed: |r_true - r_recursive|^2 partials of a reliable update
   unsigned upd_part_off;
   unsigned xcd;  // workgroups of the slice kernels numbered per XCD (wg_id)
+  // the SpMV's numbering with xcd set: segments of its grid (seg_wg_id)
+  unsigned nseg;
+  unsigned seg[kMaxSeg + 1];
   PcgScal *S;
   // mixed precision: FP64 solution part (FP32 vectors), right-hand side,
   // matrix and outer history
@@ -373,6 +376,8 @@ __device__ inline double residual_row(const PcgArgs<T, M> &a, const SliceLane &L
 }
 
 // blocks per load round of the SpMV
+// (FP64: 8 or 16 per round measured no faster at 64^3: 71.1 / 78.7 us
+// against 71.6 us with 4)
 template <typename T, typename M>
 constexpr int spmv_unroll() {
   return sizeof(M) == 4 ? 8 : 4;
@@ -385,7 +390,12 @@ template <typename T, typename M>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
   if (a.S->done) return;
   const bool refine = is_mixed<T, M>() && a.S->upd;  // grid-uniform
-  const SliceLane L = slice_lane(0, a.xcd);
+  const unsigned w = a.xcd ? seg_wg_id(a.seg, a.nseg) : blockIdx.x;
+  SliceLane L;
+  L.lane = threadIdx.x & (kWave - 1);
+  L.s = w * kSlicesPerBlock + threadIdx.x / kWave;
+  L.c = L.lane / kSliceRows;
+  L.r = L.lane - L.c * kSliceRows;
   const RowSlots R = row_slots(a.a_plan, a.a_gv, L, 0, L.s < a.a_nsl);
   const unsigned i = R.i;
   double part = 0.0, part2 = 0.0, part4 = 0.0;
@@ -408,7 +418,6 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
     part4 = block_reduce_sum<kPackBlock>(part4);
   }
   if (threadIdx.x == 0) {
-    const unsigned w = wg_id(a.xcd);
     a.partials[w] = part;
     if (refine) {
       a.partials2[w] = part2;
@@ -452,7 +461,9 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   if (a.S->done) return;
   const double alpha = a.S->alpha;
   const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && a.S->upd;  // an update just ran
-  const unsigned wv = wg_id(a.xcd) * kSlicesPerBlock + threadIdx.x / kWave;
+  // no gathers here: the dispatch order serves (an XCD-ordered numbering
+  // measured slower on MI355X: FP64 64^3 14.2 -> 16.9 us)
+  const unsigned wv = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
   SliceLane L[kUpdSlices];
   bool sl[kUpdSlices];
   unsigned i[kUpdSlices];
@@ -496,7 +507,7 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   const bool iso = a.upd_part_off != ~0u;  // rows with neither lower nor upper blocks exist
   if (iso) srp = block_reduce_sum<kPackBlock>(srp);
   if (threadIdx.x == 0) {
-    const unsigned w = wg_id(a.xcd);
+    const unsigned w = blockIdx.x;
     a.partials2[w] = s2;
     if (iso) a.partials3[a.upd_part_off + w] = srp;
   }
@@ -730,6 +741,7 @@ struct Fusion {
   const void *A = nullptr;
   uint64_t a_serial = ~0ull, p_serial = ~0ull, serial = 0;
   DBuf<unsigned> l0mask, isomask, row_l0;
+  std::vector<unsigned> spmv_seg;  // segments of the SpMV grid (seg_wg_id)
   bool any_iso = false;  // rows with neither lower nor upper blocks
   PackedVals<double> d64;
   PackedVals<float> d32;
@@ -851,6 +863,23 @@ static void ensure_fusion(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W) {
   F.isomask.upload(iso);
   F.any_iso = std::any_of(iso.begin(), iso.end(), [](unsigned m) { return m != 0; });
   F.row_l0.upload(row);
+  // SpMV segments: one per forward level when the levels are consecutive row
+  // ranges (a colour ordering), boundaries rounded to multiples of 8
+  // workgroups; else one segment
+  {
+    const unsigned gs = slice_grid(nsl);
+    const size_t nlev = P->levptr_f.size() - 1;
+    const bool ranges = P->flev_ranges && nlev >= 2 && nlev <= kMaxSeg;
+    F.spmv_seg.assign(1, 0u);
+    if (ranges)
+      for (size_t l = 1; l < nlev; ++l) {
+        const unsigned wg = (unsigned)(P->levptr_f[l] / (kSliceRows * kSlicesPerBlock));
+        const unsigned b = std::min(gs, (wg + kXcds / 2) / kXcds * kXcds);
+        if (b > F.spmv_seg.back()) F.spmv_seg.push_back(b);
+      }
+    if (F.spmv_seg.back() < gs) F.spmv_seg.push_back(gs);
+    if (F.spmv_seg.size() < 2) F.spmv_seg.push_back(gs);
+  }
   F.d64.version = ~0ull;
   F.d32.version = ~0ull;
   F.A = A;
@@ -905,6 +934,8 @@ static PcgArgs<T, M> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Pa
   a.partials4 = B.partials4.p;
   a.upd_part_off = 0;  // set per sweep shape
   a.xcd = 0;           // set per launch shape
+  a.nseg = (unsigned)F.spmv_seg.size() - 1;
+  for (unsigned k = 0; k <= a.nseg; ++k) a.seg[k] = F.spmv_seg[k];
   a.hist = B.hist.p;
   a.S = B.S.p;
   return a;
@@ -1139,8 +1170,13 @@ static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T, M>
   DFM2H_HIP(hipEventCreate(&e1));
   ms.assign(kNumSweepShapes, 0.0);
   const unsigned gs = slice_grid(a0.a_nsl);
-  int best = 0;
-  for (int v = 0; v < kNumSweepShapes; ++v) {
+  // default: split1_u8 with the XCD-ordered numbering, which moves fewer
+  // bytes (measured FP64 64^3: SpMV fetch 395 -> 343 MB per launch, sweeps
+  // -12%) at equal or lower time
+  constexpr int kDefault = kNumSweepBase;
+  int best = kDefault;
+  for (int i = 0; i < kNumSweepShapes; ++i) {
+    const int v = i == 0 ? kDefault : (i <= kDefault ? i - 1 : i);  // the default first
     // FP64 keeps the reference's summation order (one wavefront per slice)
     if (sizeof(M) == 8 && kSweepSplit[v] != 1) continue;
     const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[v]);
@@ -1160,8 +1196,8 @@ static int tune_sweeps(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, const PcgArgs<T, M>
     float t = 0.f;
     DFM2H_HIP(hipEventElapsedTime(&t, e0, e1));
     ms[v] = (double)t / kRep;
-    // another shape must beat the default (shape 0) by 3% to replace it
-    if (ms[v] < (best == 0 ? 0.97 : 1.0) * ms[best]) best = v;
+    // another shape must beat the default by 3% to replace it
+    if (v != kDefault && ms[v] < (best == kDefault ? 0.97 : 1.0) * ms[best]) best = v;
   }
   check_launch("launch-shape autotuner");
   (void)hipEventDestroy(e0);

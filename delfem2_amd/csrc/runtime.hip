@@ -98,24 +98,23 @@ int dof_bucket(size_t ndof) { return ndof < 1 ? 0 : (int)std::floor(std::log2((d
 
 // ---------------------------------------------------------------------------
 // one-ring pattern on the device: one thread per node gathers the nodes of
-// its surrounding elements into LDS, then emits the distinct ones other
-// than itself in ascending order (a selection over at most 3 x 32
-// candidates) -- the same rows as JArray_PSuP_MeshElem + JArray_Sort.
-// Pass 1 (psup_ind == nullptr) writes each row's count into out; pass 2
-// writes the row at out[psup_ind[i]].
+// its surrounding elements (other than itself; a degenerate element can
+// repeat a node) into a candidate buffer, then emits the distinct ones in
+// ascending order by repeated selection -- the same rows as
+// JArray_PSuP_MeshElem + JArray_Sort.  Nodes with at most kPsupMaxElem
+// elements keep their candidates in LDS; the others (any count) in a global
+// buffer, one run of 4 x their element count per such node.  Pass 1
+// (psup_ind == nullptr) writes each row's count into out; pass 2 writes the
+// row at out[psup_ind[i]].
 
 constexpr int kPsupMaxElem = 32;  // Kuhn-split voxel meshes: 24
 constexpr int kPsupCand = 3 * kPsupMaxElem;
 constexpr int kPsupBlock = 64;
 
-__global__ void __launch_bounds__(kPsupBlock)
-k_psup_rows(unsigned np, const unsigned *__restrict__ tet, const unsigned *__restrict__ esi,
-            const unsigned *__restrict__ es, const unsigned *__restrict__ psup_ind,
-            unsigned *__restrict__ out) {
-  __shared__ unsigned cand[kPsupBlock][kPsupCand + 1];
-  const unsigned i = blockIdx.x * kPsupBlock + threadIdx.x;
-  if (i >= np) return;
-  unsigned *c = cand[threadIdx.x];
+__device__ inline void psup_row(unsigned i, unsigned *c, const unsigned *__restrict__ tet,
+                                const unsigned *__restrict__ esi,
+                                const unsigned *__restrict__ es,
+                                const unsigned *__restrict__ psup_ind, unsigned *__restrict__ out) {
   unsigned nc = 0;
   for (unsigned k = esi[i]; k < esi[i + 1]; ++k) {
     const unsigned e = es[k];
@@ -137,6 +136,29 @@ k_psup_rows(unsigned np, const unsigned *__restrict__ tet, const unsigned *__res
     last = best;
   }
   if (!psup_ind) out[i] = cnt;
+}
+
+__global__ void __launch_bounds__(kPsupBlock)
+k_psup_rows(unsigned np, const unsigned *__restrict__ tet, const unsigned *__restrict__ esi,
+            const unsigned *__restrict__ es, const unsigned *__restrict__ psup_ind,
+            unsigned *__restrict__ out) {
+  __shared__ unsigned cand[kPsupBlock][kPsupCand + 1];
+  const unsigned i = blockIdx.x * kPsupBlock + threadIdx.x;
+  if (i >= np || esi[i + 1] - esi[i] > (unsigned)kPsupMaxElem) return;
+  psup_row(i, cand[threadIdx.x], tet, esi, es, psup_ind, out);
+}
+
+// the nodes big[0 .. nbig) with more elements; node big[b]'s candidates at
+// cand + coff[b] (4 per element around it: enough for the 3 others)
+__global__ void __launch_bounds__(kPsupBlock)
+k_psup_rows_big(unsigned nbig, const unsigned *__restrict__ big,
+                const unsigned *__restrict__ coff, unsigned *__restrict__ cand,
+                const unsigned *__restrict__ tet, const unsigned *__restrict__ esi,
+                const unsigned *__restrict__ es, const unsigned *__restrict__ psup_ind,
+                unsigned *__restrict__ out) {
+  const unsigned b = blockIdx.x * kPsupBlock + threadIdx.x;
+  if (b >= nbig) return;
+  psup_row(big[b], cand + coff[b], tet, esi, es, psup_ind, out);
 }
 
 }  // namespace dfm2h
@@ -297,6 +319,26 @@ int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet, s
       for (size_t e = 0; e < ntet; ++e)
         for (int k = 0; k < 4; ++k) es[cur[tet[e * 4 + k]]++] = (unsigned)e;
       for (size_t q = 0; q < np; ++q) M->max_elsup = std::max(M->max_elsup, ind[q + 1] - ind[q]);
+      // nodes beyond the LDS form of the device pattern build, and where
+      // their candidate runs start (4 per element around the node)
+      {
+        std::vector<unsigned> big, coff;
+        size_t tot = 0;
+        for (size_t q = 0; q < np; ++q) {
+          const unsigned d = ind[q + 1] - ind[q];
+          if (d <= (unsigned)kPsupMaxElem) continue;
+          big.push_back((unsigned)q);
+          coff.push_back((unsigned)tot);
+          tot += 4 * (size_t)d;
+        }
+        DFM2H_REQUIRE(tot < (1ull << 32), DFM2H_ERR_INVALID, "mesh too large for 32-bit indices");
+        M->nbig = big.size();
+        M->big_cand = tot;
+        if (!big.empty()) {
+          M->big.upload(big);
+          M->big_off.upload(coff);
+        }
+      }
       M->tet.upload(tet, ntet * 4);
       M->elsup_ind.upload(ind);
       M->elsup.upload(es);
@@ -313,20 +355,26 @@ int dfm2h_psup_tet_device(dfm2h_tetmesh M, unsigned *psup_ind, unsigned *psup,
                           size_t psup_cap, size_t *npsup) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(M && psup_ind && npsup, DFM2H_ERR_INVALID, "null argument");
-    DFM2H_REQUIRE(M->max_elsup <= (unsigned)kPsupMaxElem, DFM2H_ERR_INVALID,
-                  "a node has more than 32 elements around it; use dfm2h_psup_tet");
     const size_t np = M->np;
     psup_ind[0] = 0;
     if (np == 0) {
       *npsup = 0;
       return;
     }
-    DBuf<unsigned> cnt;
+    DBuf<unsigned> cnt, cand;
     cnt.alloc(np + 1);
-    const dim3 g(ceil_div(np, kPsupBlock));
-    hipLaunchKernelGGL(k_psup_rows, g, dim3(kPsupBlock), 0, stream(), (unsigned)np, M->tet.p,
-                       M->elsup_ind.p, M->elsup.p, (const unsigned *)nullptr, cnt.p);
-    check_launch("k_psup_rows");
+    if (M->nbig) cand.alloc(M->big_cand);
+    const dim3 g(ceil_div(np, kPsupBlock)), gb(ceil_div(std::max<size_t>(M->nbig, 1), kPsupBlock));
+    auto rows = [&](const unsigned *ind, unsigned *out) {
+      hipLaunchKernelGGL(k_psup_rows, g, dim3(kPsupBlock), 0, stream(), (unsigned)np, M->tet.p,
+                         M->elsup_ind.p, M->elsup.p, ind, out);
+      if (M->nbig)
+        hipLaunchKernelGGL(k_psup_rows_big, gb, dim3(kPsupBlock), 0, stream(),
+                           (unsigned)M->nbig, M->big.p, M->big_off.p, cand.p, M->tet.p,
+                           M->elsup_ind.p, M->elsup.p, ind, out);
+      check_launch("k_psup_rows");
+    };
+    rows(nullptr, cnt.p);
     DFM2H_HIP(hipMemcpyAsync(psup_ind + 1, cnt.p, np * sizeof(unsigned), hipMemcpyDeviceToHost,
                              stream()));
     DFM2H_HIP(hipStreamSynchronize(stream()));
@@ -341,9 +389,7 @@ int dfm2h_psup_tet_device(dfm2h_tetmesh M, unsigned *psup_ind, unsigned *psup,
     DBuf<unsigned> ind, out;
     ind.upload(psup_ind, np + 1);
     out.alloc(std::max<size_t>(tot, 1));
-    hipLaunchKernelGGL(k_psup_rows, g, dim3(kPsupBlock), 0, stream(), (unsigned)np, M->tet.p,
-                       M->elsup_ind.p, M->elsup.p, ind.p, out.p);
-    check_launch("k_psup_rows");
+    rows(ind.p, out.p);
     if (tot)
       DFM2H_HIP(hipMemcpyAsync(psup, out.p, tot * sizeof(unsigned), hipMemcpyDeviceToHost,
                                stream()));

@@ -79,8 +79,8 @@ def JArray_PSuP_MeshElem(tet: np.ndarray, np_: int):
 
 
 def JArray_PSuP_MeshElem_Device(mesh: "TetMesh"):
-    """The same pattern built on the device from a TetMesh handle (at most 32
-    elements around a node): (psup_ind, psup) on the host."""
+    """The same pattern built on the device from a TetMesh handle (any number
+    of elements around a node): (psup_ind, psup) on the host."""
     ind = np.zeros(mesh.np_ + 1, np.uint32)
     n = C.c_size_t(0)
     N.call("dfm2h_psup_tet_device", mesh.h, _hptr(ind), None, 0, C.byref(n))

@@ -139,9 +139,9 @@ int dfm2h_tetmesh_create(dfm2h_tetmesh *out, const unsigned *tet, size_t ntet,
 /* The same pattern built on the device from the mesh handle (one thread per
  * node over its surrounding elements; msh_topology_uniform.cpp:301-316 +
  * jagarray.cpp:72-92, identical output).  Host outputs as dfm2h_psup_tet:
- * psup_ind[np+1] always, psup when psup_cap >= *npsup.  Limit: at most 32
- * elements around a node (DFM2H_ERR_INVALID otherwise; the host build has
- * no limit). */
+ * psup_ind[np+1] always, psup when psup_cap >= *npsup.  Any number of
+ * elements around a node (up to 32 the candidates stay in LDS, beyond that
+ * in a device buffer). */
 int dfm2h_psup_tet_device(dfm2h_tetmesh M, unsigned *psup_ind, unsigned *psup,
                           size_t psup_cap, size_t *npsup);
 int dfm2h_tetmesh_destroy(dfm2h_tetmesh M);

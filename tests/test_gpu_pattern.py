@@ -1,12 +1,14 @@
 """Device one-ring pattern (dfm2h_psup_tet_device) against the host build
 (dfm2h_psup_tet) and the oracle (msh_topology_uniform.cpp:301-316 +
-jagarray.cpp:72-92): identical psup_ind and psup arrays."""
+jagarray.cpp:72-92): identical psup_ind and psup arrays, for voxel meshes
+and for meshes with nodes of any element count."""
 import numpy as np
 import pytest
 
 import oracle
 from delfem2_amd import solver as S
 from delfem2_amd import workload as W
+from delfem2_amd.problem import DeviceSystem
 
 pytestmark = pytest.mark.gpu
 
@@ -48,9 +50,58 @@ def test_device_pattern_c3_size(gpu):
     assert np.array_equal(ind_d, ind_h) and np.array_equal(ps_d, ps_h)
 
 
-def test_device_pattern_degree_limit(gpu):
-    # node 0 in 40 tets: beyond the device build's 32-element limit
-    tet = np.array([[0, 1 + 3 * k, 2 + 3 * k, 3 + 3 * k] for k in range(40)], np.uint32).ravel()
-    mesh = S.TetMesh(tet, 121)
-    with pytest.raises(S.N.DFM2HError):
-        S.JArray_PSuP_MeshElem_Device(mesh)
+def ball_mesh(nsurf=150, ninner=40, seed=5):
+    """A non-voxel mesh: the Delaunay tetrahedralisation (scipy) of 150
+    points on the unit sphere, the centre and seeded interior points, tets
+    oriented positive.  Nodes of the sphere meet up to 74 tets."""
+    from scipy.spatial import Delaunay
+    k = np.arange(nsurf) + 0.5
+    th = np.arccos(1 - 2 * k / nsurf)
+    ph = np.pi * (1 + 5 ** 0.5) * k
+    surf = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    inner = (W.uniform(seed, 3 * ninner).reshape(-1, 3) - 0.5) * 0.9
+    inner = inner[np.linalg.norm(inner, axis=1) < 0.45]
+    pts = np.vstack([[0.0, 0.0, 0.0], inner, surf])
+    tet = Delaunay(pts).simplices.astype(np.int64)
+    P = pts[tet]
+    vol = np.einsum("ij,ij->i", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0])
+    tet[vol < 0] = tet[vol < 0][:, [0, 2, 1, 3]]
+    return np.ascontiguousarray(pts.ravel()), np.ascontiguousarray(tet.ravel(), np.uint32)
+
+
+def test_device_pattern_high_degree_nodes(gpu):
+    """Nodes with more than 32 elements around them (the device build keeps
+    their candidates in global memory): a 40-tet fan around one node and an
+    unstructured ball mesh (up to 74 tets at a node) give the oracle's
+    pattern; the ball is then assembled, factorised and solved through the
+    device pattern, against the oracle (x rel-L2 <= 1e-7)."""
+    fan = np.array([[0, 1 + 3 * k, 2 + 3 * k, 3 + 3 * k] for k in range(40)], np.uint32).ravel()
+    xyz, ball = ball_mesh()
+    for tet, np_ in ((fan, 121), (ball, xyz.size // 3)):
+        deg = np.bincount(tet.astype(np.int64), minlength=np_)
+        assert deg.max() > 32
+        mesh = S.TetMesh(tet, np_)
+        ind_d, ps_d = S.JArray_PSuP_MeshElem_Device(mesh)
+        ind_o, ps_o = oracle.psup(tet, np_)
+        assert np.array_equal(ind_d.astype(np.int64), np.asarray(ind_o, np.int64))
+        assert np.array_equal(ps_d.astype(np.int64), np.asarray(ps_o, np.int64))
+    ne = ball.size // 4
+    myu, lam = W.lame(np.ones(ne), np.full(ne, 0.3))
+    flag = np.zeros(xyz.size, np.int32)
+    flag.reshape(-1, 3)[xyz.reshape(-1, 3)[:, 2] < -0.6] = 1
+    prob = W.Problem("ball", xyz, ball, myu, lam, flag)
+    G = DeviceSystem(prob)
+    assert G.assemble() and G.factor()
+    x, hist, _ = G.solve(1e-10, 2000)
+    ind, ps = oracle.psup(prob.tet, prob.np_)
+    C = oracle.System(ind, ps)
+    b = np.zeros(prob.ndof)
+    C.assemble(b, prob.myu, prob.lam, prob.rho, prob.g, prob.xyz, prob.tet)
+    C.set_fixed_bc(prob.flag)
+    oracle.set_rhs_zero(b, prob.flag)
+    vc, vd = G.A.download()
+    assert np.array_equal(vc.view(np.uint64), C.vcrs.view(np.uint64))
+    C.factor()
+    xc, hc = C.solve_pcg(b.copy(), 1e-10, 2000)
+    assert hist[-1] < 1e-10 * hist[0] and abs(len(hist) - len(hc)) <= 1
+    assert np.linalg.norm(x.cpu().numpy() - xc) / np.linalg.norm(xc) <= 1e-7

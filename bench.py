@@ -67,17 +67,23 @@ def kernel_bytes(nb: int, nnzb: int, s: int, n0: int = 0, nu0: int = 0, sv: int 
 
 
 def host_cpu() -> dict:
-    """CPU model and logical core count of the machine the CPU baseline ran on."""
-    model = platform.processor() or platform.machine()
+    """CPU model (the kernel's model name, with family / model / stepping,
+    clock and cache size as /proc/cpuinfo reports them) and logical core
+    count of the machine the CPU baseline ran on."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if not line.strip():
+                    break  # the first processor's block
+                k, _, v = line.partition(":")
+                info[k.strip()] = v.strip()
     except OSError:
         pass
-    return {"cpu_model": model, "nproc": os.cpu_count()}
+    model = info.get("model name") or platform.processor() or platform.machine()
+    detail = ", ".join(f"{k} {info[k]}" for k in ("cpu family", "model", "stepping", "cpu MHz",
+                                                  "cache size") if k in info)
+    return {"cpu_model": f"{model} ({detail})" if detail else model, "nproc": os.cpu_count()}
 
 
 def stream_copy_gbs(mib: int = 1024, reps: int = 10) -> float:
@@ -523,6 +529,96 @@ def bench_newmark(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_stiffwarp(args):
+    """Stiffness-warped backward-Euler time stepping of the 64^3 cube
+    (823,875 DOF, the headline cube's mesh, material and z=0 clamp; gravity
+    (0, 0, -3)): one step = node rotations (polar decomposition over each
+    one-ring), assembly of R K R^T + lumped M / dt^2 and its residual, fixed
+    BCs, ILU(0) CopyValue + Decompose, device BiCGStab to 1e-4 (1000
+    iterations at most) and the u / v updates -- the step of
+    603_Fem3DStiffwarp/main.cpp:180-227 with its dt and tolerance; E = 1,
+    nu = 0.3 everywhere (the reference's driver takes one Lame pair).  Value:
+    seconds per step (steps from rest; the state evolves)."""
+    import numpy as np
+    import torch
+
+    import delfem2_amd.solver as S
+    from delfem2_amd import workload as W
+    from delfem2_amd.problem import DynamicSystem
+
+    dt, tol = 0.03, 1e-4
+    print(f"[bench] building the {args.n}^3 stiffness-warped problem", file=sys.stderr,
+          flush=True)
+    # one Lame pair for the whole mesh: the reference's driver takes scalars
+    prob = W.cube(args.n, sigma=0.0, seed=20261203, ordering=args.ordering)
+    prob.g = np.array([0.0, 0.0, -3.0])
+    G = DynamicSystem(prob, "stiffwarp", dt=dt)
+    for _ in range(args.warmup):
+        G.step(tol, 1000)
+    torch.cuda.synchronize()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        its.append(G.step(tol, 1000))
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    ph = {}
+
+    def timed(name, fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ph[name] = round((time.perf_counter() - t) * 1e3, 3)
+        return r
+
+    timed("rotations_assembly_bc_ms", G.assemble)
+    timed("ilu_copy_decompose_ms", G.factor)
+    x = torch.zeros_like(G.b)
+    r = G.b.clone()
+    hist = timed("bicgstab_ms", lambda: S.Solve_PBiCGStab(r, x, tol, 1000, G.A, G.ilu))
+    ph["bicgstab_iterations"] = len(hist)
+    nb, nnzb = prob.np_, G.A.ncrs
+    kb = kernel_bytes(nb, nnzb, 8)
+    # per iteration: two MatVecs, two full ILU solves (level 0 included), and
+    # the vector updates (s: 3 passes, x / r: 7, p: 4 of n doubles)
+    it_bytes = (2 * spmv_bytes(nb, nnzb, 8, True) + 2 * (kb["fwd_sweep"] + kb["bwd_sweep"])
+                + 14 * 8 * 3 * nb)
+    ach = it_bytes * max(len(hist), 1) / (ph["bicgstab_ms"] * 1e-3) / 1e9
+    cpu = None
+    if args.cpu_iters > 0:
+        import oracle
+        print("[bench] CPU baseline (one stiffness-warped step)", file=sys.stderr, flush=True)
+        kind = "reference" if oracle.ref_available() else "port"
+        run = oracle.DynamicRun(prob, "stiffwarp", dt=dt, use_ref=kind == "reference")
+        tc = time.perf_counter()
+        it_c = run.step(tol, 1000)
+        tc = time.perf_counter() - tc
+        cpu = {"value": round(tc, 3), "unit": "s/step", "cores": 1, "kind": kind,
+               "measured_iterations": it_c,
+               "sample": f"one stiffness-warped step from rest of the same {args.n}^3 problem "
+                         f"({it_c} BiCGStab iterations to {tol})", **host_cpu()}
+    out = {
+        "metric": "stiffness-warped backward-Euler time step of the 823k-DOF tet cube (s/step)",
+        "value": round(t_step, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded voxel tet cube, lognormal E)",
+        "config": {"workload": f"stiffness-warped step, tet cube {args.n}^3 hex x6 tets, "
+                               f"{prob.ndof} DOF, E=1, nu=0.3, rho=1, "
+                               f"g=(0,0,-3), dt={dt}, ILU(0)-BiCGStab to {tol}, "
+                               f"{args.ordering} order",
+                   "ndof": prob.ndof},
+        "bicgstab_iterations_per_step": its,
+        "phases": ph,
+        "roofline": {"bound": "hbm", "phase": "bicgstab_ms", "achieved": round(ach, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_iteration": int(it_bytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def kernel_table(G, S, nb, nnzb, n0, nu0, step_mixed, step_fp64, args) -> dict:
     """Per-kernel times of the fused iterations (HIP events on the solver
     stream, right after a solve of the same precision: same buffers, same
@@ -635,12 +731,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cube",
-                    choices=["cube", "cube128", "batch512", "newmark64", "adjoint96"],
+                    choices=["cube", "cube128", "batch512", "newmark64", "adjoint96",
+                             "stiffwarp64"],
                     help="cube: the 823,875-DOF single solve to 1e-8 (headline); cube128: "
                          "the 6,440,067-DOF mixed-precision solve to 1e-10; batch512: 512 "
                          "independent meshes in one batched solve; newmark64: implicit "
                          "Newmark-beta time steps of the 823,875-DOF cube; adjoint96: the "
-                         "compliance gradient of the 96x48x48 cantilever")
+                         "compliance gradient of the 96x48x48 cantilever; stiffwarp64: "
+                         "stiffness-warped time steps (BiCGStab) of the 823,875-DOF cube")
     ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
     ap.add_argument("--sigma", type=float, default=0.5)
     ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
@@ -693,6 +791,9 @@ def main():
         return
     if args.workload == "adjoint96":
         bench_adjoint(args)
+        return
+    if args.workload == "stiffwarp64":
+        bench_stiffwarp(args)
         return
 
     import torch

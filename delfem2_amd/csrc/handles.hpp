@@ -26,6 +26,17 @@ void destroy_pcg_work(PcgWork *);
 // entry point that reads or writes the matrix's values calls it first.
 void flush_merge(dfm2h_bsr_s *A);
 
+struct BicgWork;  // krylov.hip
+void destroy_bicg_work(BicgWork *);
+
+// ILU(0) sweeps on FP64 vectors, level by level on stream s (ilu.hip):
+// forward vec = L^-1 src (src may be vec), backward vec = U^-1 vec; with
+// done set the kernels return at once when *done is nonzero.
+void ilu_forward_levels_double(dfm2h_ilu_s *P, const double *src, double *vec, hipStream_t s,
+                               const int *done = nullptr);
+void ilu_backward_levels_double(dfm2h_ilu_s *P, double *vec, hipStream_t s,
+                                const int *done = nullptr);
+
 // Packs CRS values (and optional per-row diagonal blocks) into V when the
 // source version differs from the packed one.
 template <typename T>
@@ -154,5 +165,9 @@ struct dfm2h_ilu_s {
   uint64_t version = 1;
   uint64_t pattern_serial = 0;  // process-unique, set with the pattern (next_serial)
   dfm2h::PcgWork *work = nullptr;
-  ~dfm2h_ilu_s() { dfm2h::destroy_pcg_work(work); }
+  dfm2h::BicgWork *bicg = nullptr;  // the BiCGStab solver's buffers and graphs
+  ~dfm2h_ilu_s() {
+    dfm2h::destroy_pcg_work(work);
+    dfm2h::destroy_bicg_work(bicg);
+  }
 };

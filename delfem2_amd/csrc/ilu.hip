@@ -281,23 +281,24 @@ static void need_ilu(dfm2h_ilu P) {
 
 namespace dfm2h {
 
-void ilu_forward_levels_double(dfm2h_ilu P, const double *src, double *vec) {
-  ensure_packed(P->fplan, P->f64, P->vcrs.p, P->vdia.p, P->version, stream());
+void ilu_forward_levels_double(dfm2h_ilu P, const double *src, double *vec, hipStream_t s,
+                               const int *done) {
+  ensure_packed(P->fplan, P->f64, P->vcrs.p, P->vdia.p, P->version, s);
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
-    hipLaunchKernelGGL((k_sliced_fwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
+    hipLaunchKernelGGL((k_sliced_fwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
                        P->fplan.group_ptr[l], nsl, view(P->fplan), P->f64.val.p, P->f64.dia.p,
-                       src, vec);
+                       src, vec, done);
   }
   check_launch("k_sliced_fwd");
 }
 
-void ilu_backward_levels_double(dfm2h_ilu P, double *vec) {
-  ensure_packed(P->bplan, P->b64, P->vcrs.p, nullptr, P->version, stream());
+void ilu_backward_levels_double(dfm2h_ilu P, double *vec, hipStream_t s, const int *done) {
+  ensure_packed(P->bplan, P->b64, P->vcrs.p, nullptr, P->version, s);
   for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
-    hipLaunchKernelGGL((k_sliced_bwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, stream(),
-                       P->bplan.group_ptr[l], nsl, view(P->bplan), P->b64.val.p, vec);
+    hipLaunchKernelGGL((k_sliced_bwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
+                       P->bplan.group_ptr[l], nsl, view(P->bplan), P->b64.val.p, vec, done);
   }
   check_launch("k_sliced_bwd");
 }
@@ -560,7 +561,7 @@ int dfm2h_ilu_forward(dfm2h_ilu P, double *vec) {
     need_ilu(P);
     DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
     DFM2H_DEVICE_ARG(vec);
-    ilu_forward_levels_double(P, vec, vec);
+    ilu_forward_levels_double(P, vec, vec, stream());
   });
 }
 
@@ -569,7 +570,7 @@ int dfm2h_ilu_backward(dfm2h_ilu P, double *vec) {
     need_ilu(P);
     DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
     DFM2H_DEVICE_ARG(vec);
-    ilu_backward_levels_double(P, vec);
+    ilu_backward_levels_double(P, vec, stream());
   });
 }
 
@@ -578,8 +579,8 @@ int dfm2h_ilu_solve_precond(dfm2h_ilu P, double *vec) {
     need_ilu(P);
     DFM2H_REQUIRE(vec, DFM2H_ERR_INVALID, "null vector");
     DFM2H_DEVICE_ARG(vec);
-    ilu_forward_levels_double(P, vec, vec);
-    ilu_backward_levels_double(P, vec);
+    ilu_forward_levels_double(P, vec, vec, stream());
+    ilu_backward_levels_double(P, vec, stream());
   });
 }
 

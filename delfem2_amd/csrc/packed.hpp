@@ -521,10 +521,13 @@ __device__ inline void sliced_fwd_row(const SliceLane &L, bool valid, unsigned i
   }
 }
 
+// done (nullable): the kernel returns at once when *done is set (a fused
+// solver's convergence flag)
 template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
 k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val,
-             const T *__restrict__ dia, const T *src, T *vec) {
+             const T *__restrict__ dia, const T *src, T *vec, const int *done) {
+  if (done && *done) return;
   const SliceLane L = slice_lane(slice0);
   if (L.s >= slice0 + nsl) return;  // whole wavefront
   const unsigned i = L.c < 3 ? P.row[L.s * kSliceRows + L.r] : ~0u;
@@ -536,7 +539,9 @@ k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ va
 // backward sweep of one level: vec_i -= sum U_ij vec_j (U scaled by D^-1)
 template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
-k_sliced_bwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val, T *vec) {
+k_sliced_bwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val, T *vec,
+             const int *done) {
+  if (done && *done) return;
   const SliceLane L = slice_lane(slice0);
   if (L.s >= slice0 + nsl || L.c >= 3) return;
   const unsigned i = P.row[L.s * kSliceRows + L.r];

@@ -316,6 +316,15 @@ def test_bicgstab_vs_oracle(gpu):
     res = bd.clone()
     G.A.MatVec(res, -1.0, x, 1.0)
     assert torch.linalg.norm(res).item() < 1e-9 * torch.linalg.norm(bd).item()
+    # the history entry by entry (dot products in another order: rounding)
+    k = min(len(hd), len(hc))
+    assert np.allclose(hd[:k], hc[:k], rtol=1e-6, atol=0.0)
+    # the iteration cap: max_niter - 1 iterations, one entry each, as the oracle
+    r = bd.clone()
+    h4 = S.Solve_PBiCGStab(r, x, 1e-30, 4, G.A, G.ilu)
+    _, hc4 = C.solve_pbicgstab(b.copy(), 1e-30, 4)
+    assert len(h4) == len(hc4) == 3
+    assert np.allclose(h4, hc4, rtol=1e-9, atol=0.0)
     # zero right-hand side: x = 0 and the history is |b| alone
     z = torch.zeros_like(bd)
     hz = S.Solve_PBiCGStab(z, x, 1e-10, 1000, G.A, G.ilu)

@@ -5,6 +5,8 @@
 #include "tet.hpp"
 
 #include <cstdlib>
+#include <mutex>
+#include <set>
 
 namespace dfm2h {
 
@@ -762,6 +764,16 @@ __global__ void k_merge_apply(size_t nrun, const unsigned *__restrict__ run, siz
   *out = acc;
 }
 
+namespace {
+std::mutex g_live_mu;
+std::set<const dfm2h_bsr_s *> g_live;  // matrix handles not yet destroyed
+}  // namespace
+
+bool bsr_alive(const dfm2h_bsr_s *A) {
+  std::lock_guard<std::mutex> lk(g_live_mu);
+  return g_live.count(A) != 0;
+}
+
 // Blocks staged beyond this count are applied before more are staged.
 constexpr size_t kMergeChunk = size_t(1) << 21;
 
@@ -973,12 +985,22 @@ int dfm2h_bsr_create(dfm2h_bsr *out, size_t nblk, unsigned len, int is_dia) {
       delete A;
       throw;
     }
+    {
+      std::lock_guard<std::mutex> lk(g_live_mu);
+      g_live.insert(A);
+    }
     *out = A;
   });
 }
 
 int dfm2h_bsr_destroy(dfm2h_bsr A) {
-  return guard(__func__, [&] { delete A; });
+  return guard(__func__, [&] {
+    {
+      std::lock_guard<std::mutex> lk(g_live_mu);
+      g_live.erase(A);
+    }
+    delete A;
+  });
 }
 
 int dfm2h_bsr_set_pattern(dfm2h_bsr A, const unsigned *colind, size_t ncolind,

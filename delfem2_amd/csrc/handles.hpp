@@ -25,6 +25,8 @@ void destroy_pcg_work(PcgWork *);
 // Applies the matrix's staged per-element Merge additions (bsr.hip); every
 // entry point that reads or writes the matrix's values calls it first.
 void flush_merge(dfm2h_bsr_s *A);
+// whether A is a matrix handle that has been created and not destroyed
+bool bsr_alive(const dfm2h_bsr_s *A);
 
 struct BicgWork;  // krylov.hip
 void destroy_bicg_work(BicgWork *);
@@ -153,6 +155,10 @@ struct dfm2h_ilu_s {
   dfm2h::DBuf<unsigned> levrows_f;
   bool flev_ranges = false;  // every forward level is a consecutive, ascending row range
   dfm2h::DBuf<int> nsing_d;
+  dfm2h::DBuf<unsigned char> sing_d;  // per row: the pivot was singular
+  // the matrix the last CopyValue read, its pattern serial and value version
+  const dfm2h_bsr_s *cv_from = nullptr;
+  uint64_t cv_serial = 0, cv_version = 0;
   unsigned dec_lanes = 0;  // lanes per row of the factorisation kernel (0: one thread per row)
   // sliced sweeps: forward plan (lower blocks + D^-1), backward plan (upper blocks)
   dfm2h::PackedPlan fplan, bplan;

@@ -28,8 +28,10 @@ __global__ void k_zero(double *p, size_t n) {
 // one thread per row, the reference's IKJ loop verbatim in arithmetic:
 //   for k in L(i) ascending, for j in U(k) ascending, (i,j) in pattern:
 //     A_ij -= A_ik * U_kj            (or D_i when j == i)
-//   D_i <- inverse(D_i) unless |det| <= 1e-30 (counted, left as is)
-//   U_ij <- D_i * U_ij
+//   D_i <- inverse(D_i) unless |det| <= 1e-30 (counted and marked in
+//          sing[i], left as is)
+//   U_ij <- D_i * U_ij   (skipped with scale_u = 0: the state in which the
+//          reference returns at its 11th singular pivot)
 // Row i's columns are sorted, so the slot of j in row i is found by a
 // pointer that only moves forward (j ascends for a fixed k).
 
@@ -42,7 +44,7 @@ __global__ void __launch_bounds__(64)
 k_ilu_decompose_level(const unsigned *__restrict__ rows, unsigned nrows,
                       const unsigned *__restrict__ colind, const unsigned *__restrict__ rowptr,
                       const unsigned *__restrict__ diaind, double *vcrs, double *vdia,
-                      int *nsing) {
+                      int *nsing, unsigned char *sing, int scale_u) {
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows) return;
   const unsigned i = rows[t];
@@ -99,8 +101,9 @@ k_ilu_decompose_level(const unsigned *__restrict__ rows, unsigned nrows,
     }
   } else {
     atomicAdd(nsing, 1);
+    sing[i] = 1;
   }
-  for (unsigned s = idia; s < i1; ++s) {
+  for (unsigned s = idia; scale_u && s < i1; ++s) {
     double *v = vcrs + (size_t)s * 9;
     double tb[9];
 #pragma unroll
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(64)
 k_ilu_decompose_lanes(const unsigned *__restrict__ rows, unsigned nrows,
                       const unsigned *__restrict__ colind, const unsigned *__restrict__ rowptr,
                       const unsigned *__restrict__ diaind, double *vcrs, double *vdia,
-                      int *nsing) {
+                      int *nsing, unsigned char *sing, int scale_u) {
   static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "G: power of two <= 64");
   const unsigned lane = threadIdx.x;  // blockDim.x == 64: one wavefront
   const unsigned gb = lane & ~(G - 1u), t = lane & (G - 1u);
@@ -213,12 +216,13 @@ k_ilu_decompose_lanes(const unsigned *__restrict__ rows, unsigned nrows,
       for (int q = 0; q < 9; ++q) v[q] = a[q];
     } else {
       atomicAdd(nsing, 1);
+      sing[i] = 1;
     }
   }
   double d[9];  // D_i^-1 (or D_i itself when singular), from the diagonal lane
 #pragma unroll
   for (int q = 0; q < 9; ++q) d[q] = __shfl(v[q], (int)(gb + (nL & (G - 1u))), 64);
-  if (has && t > nL) {
+  if (scale_u && has && t > nL) {
     double tb[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) tb[q] = v[q];
@@ -277,6 +281,69 @@ static void set_a2l(dfm2h_ilu P, dfm2h_bsr A) {
 static void need_ilu(dfm2h_ilu P) {
   DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null preconditioner handle");
   DFM2H_REQUIRE(P->has_pattern, DFM2H_ERR_STATE, "SetPattern0 has not been called");
+}
+
+// Factorises the rows rows[0 .. nrows) (one dependency level, or one row),
+// with or without the U scaling.
+static void decompose_rows(dfm2h_ilu P, const unsigned *rows, unsigned nrows, int scale_u) {
+  if (!nrows) return;
+  const unsigned G = P->dec_lanes;
+  if (G == 0) {
+    hipLaunchKernelGGL(k_ilu_decompose_level, dim3(ceil_div(nrows, 64)), dim3(64), 0, stream(),
+                       rows, nrows, P->colind_d.p, P->rowptr_d.p, P->diaind_d.p, P->vcrs.p,
+                       P->vdia.p, P->nsing_d.p, P->sing_d.p, scale_u);
+    return;
+  }
+  const dim3 grid(ceil_div((size_t)nrows * G, 64));
+#define DFM2H_DEC_LANES(g)                                                                    \
+  hipLaunchKernelGGL(k_ilu_decompose_lanes<g>, grid, dim3(64), 0, stream(), rows, nrows,     \
+                     P->colind_d.p, P->rowptr_d.p, P->diaind_d.p, P->vcrs.p, P->vdia.p,      \
+                     P->nsing_d.p, P->sing_d.p, scale_u)
+  switch (G) {
+    case 8: DFM2H_DEC_LANES(8); break;
+    case 16: DFM2H_DEC_LANES(16); break;
+    case 32: DFM2H_DEC_LANES(32); break;
+    default: DFM2H_DEC_LANES(64); break;
+  }
+#undef DFM2H_DEC_LANES
+}
+
+// The reference factorises rows in ascending order and returns at its 11th
+// singular pivot (ls_ilu_block_sparse.cpp:331-333): rows before that row i*
+// factorised, row i* eliminated but its U part not scaled, later rows as
+// CopyValue left them.  The level-scheduled factorisation ran every row;
+// rows before i* hold the reference's values already (they depend on
+// earlier rows only), so rows from i* on are copied again from the matrix
+// CopyValue read -- when that matrix still exists unchanged -- and row i* is
+// redone without the U scaling.  Without that matrix the rows after i* stay
+// factorised.
+static void stop_at_eleventh_singular(dfm2h_ilu P) {
+  std::vector<unsigned char> sing(P->nblk);
+  DFM2H_HIP(hipMemcpyAsync(sing.data(), P->sing_d.p, P->nblk, hipMemcpyDeviceToHost, stream()));
+  DFM2H_HIP(hipStreamSynchronize(stream()));
+  unsigned stop = P->nblk, cnt = 0;
+  for (unsigned i = 0; i < P->nblk && stop == P->nblk; ++i)
+    if (sing[i] && ++cnt == 11) stop = i;
+  const dfm2h_bsr_s *A = P->cv_from;
+  if (stop == P->nblk || !A || !bsr_alive(A) || A->pattern_serial != P->cv_serial ||
+      A->version != P->cv_version || P->mapped_for != A || P->mapped_serial != A->pattern_serial)
+    return;
+  const size_t l0 = P->colind[stop], a0 = A->pat.colind[stop];
+  if (P->ncrs > l0)
+    hipLaunchKernelGGL(k_zero, dim3(ceil_div((P->ncrs - l0) * 9, 256)), dim3(256), 0, stream(),
+                       P->vcrs.p + l0 * 9, (P->ncrs - l0) * 9);
+  if (A->pat.ncrs > a0)
+    hipLaunchKernelGGL(k_copy_value, dim3(ceil_div((A->pat.ncrs - a0) * 9, 256)), dim3(256), 0,
+                       stream(), A->pat.ncrs - a0, P->a2l_d.p + a0, A->vcrs.p + a0 * 9,
+                       P->vcrs.p);
+  DFM2H_HIP(hipMemcpyAsync(P->vdia.p + (size_t)stop * 9, A->vdia.p + (size_t)stop * 9,
+                           sizeof(double) * 9 * (P->nblk - stop), hipMemcpyDeviceToDevice,
+                           stream()));
+  DBuf<unsigned> row;
+  row.upload(&stop, 1);
+  decompose_rows(P, row.p, 1, 0);
+  check_launch("ILU stop row");
+  DFM2H_HIP(hipStreamSynchronize(stream()));
 }
 
 namespace dfm2h {
@@ -372,6 +439,7 @@ static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
   }
   set_a2l(P, A);
   P->nsing_d.alloc(1);
+  P->sing_d.alloc(std::max<size_t>(P->nblk, 1));
   P->has_pattern = true;
   P->pattern_serial = next_serial();
   P->decomposed = false;
@@ -513,6 +581,9 @@ int dfm2h_ilu_copy_value(dfm2h_ilu P, dfm2h_bsr A) {
     if (P->nblk)
       DFM2H_HIP(hipMemcpyAsync(P->vdia.p, A->vdia.p, sizeof(double) * 9 * P->nblk,
                                hipMemcpyDeviceToDevice, stream()));
+    P->cv_from = A;
+    P->cv_serial = A->pattern_serial;
+    P->cv_version = A->version;
     P->decomposed = false;
     P->version++;
   });
@@ -522,35 +593,17 @@ int dfm2h_ilu_decompose(dfm2h_ilu P, int *ok, int *n_singular) {
   return guard(__func__, [&] {
     need_ilu(P);
     P->nsing_d.zero();
-    const unsigned G = P->dec_lanes;
-    for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l) {
-      const unsigned nrows = P->levptr_f[l + 1] - P->levptr_f[l];
-      const unsigned *rows = P->levrows_f.p + P->levptr_f[l];
-      if (G == 0) {
-        hipLaunchKernelGGL(k_ilu_decompose_level, dim3(ceil_div(nrows, 64)), dim3(64), 0,
-                           stream(), rows, nrows, P->colind_d.p, P->rowptr_d.p, P->diaind_d.p,
-                           P->vcrs.p, P->vdia.p, P->nsing_d.p);
-        continue;
-      }
-      const dim3 grid(ceil_div((size_t)nrows * G, 64));
-#define DFM2H_DEC_LANES(g)                                                                    \
-  hipLaunchKernelGGL(k_ilu_decompose_lanes<g>, grid, dim3(64), 0, stream(), rows, nrows,     \
-                     P->colind_d.p, P->rowptr_d.p, P->diaind_d.p, P->vcrs.p, P->vdia.p,      \
-                     P->nsing_d.p)
-      switch (G) {
-        case 8: DFM2H_DEC_LANES(8); break;
-        case 16: DFM2H_DEC_LANES(16); break;
-        case 32: DFM2H_DEC_LANES(32); break;
-        default: DFM2H_DEC_LANES(64); break;
-      }
-#undef DFM2H_DEC_LANES
-    }
+    P->sing_d.zero();
+    for (size_t l = 0; l + 1 < P->levptr_f.size(); ++l)
+      decompose_rows(P, P->levrows_f.p + P->levptr_f[l], P->levptr_f[l + 1] - P->levptr_f[l], 1);
     check_launch("k_ilu_decompose");
     int ns = 0;
     DFM2H_HIP(hipMemcpyAsync(&ns, P->nsing_d.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
     DFM2H_HIP(hipStreamSynchronize(stream()));
+    if (ns > 10) stop_at_eleventh_singular(P);
     if (ok) *ok = ns > 10 ? 0 : 1;
-    if (n_singular) *n_singular = ns;
+    // the reference stops counting where it stops factorising
+    if (n_singular) *n_singular = ns > 10 ? 11 : ns;
     P->decomposed = true;
     P->version++;
   });

@@ -229,10 +229,14 @@ int dfm2h_ilu_info(dfm2h_ilu P, size_t *nblk, size_t *ncrs, unsigned *colind);
 int dfm2h_ilu_copy_value(dfm2h_ilu P, dfm2h_bsr A);
 /* ls_ilu_block_sparse.cpp:152-420 Decompose() (3x3 branch 286-356).  Rows
  * are factorised level by level, each row in the reference's IKJ order, so
- * the factors are bit-identical.  *ok = 1 (true) or 0 when more than 10
- * pivots were singular (|det| <= 1e-30; the reference then stops, here the
- * remaining rows are still processed and the factors are unspecified).
- * Either output may be NULL.  Returns DFM2H_OK in both cases. */
+ * the factors are bit-identical.  *ok = 1 (true), or 0 when more than 10
+ * pivots were singular (|det| <= 1e-30): the reference then returns at its
+ * 11th singular pivot in row order, and the factors are left as it leaves
+ * them -- rows before that row factorised, that row eliminated but its U
+ * part not scaled, later rows as CopyValue wrote them (restored from the
+ * matrix CopyValue read; if that matrix was destroyed or changed since,
+ * the later rows stay factorised) -- and *n_singular = 11.  Either output
+ * may be NULL.  Returns DFM2H_OK in both cases. */
 int dfm2h_ilu_decompose(dfm2h_ilu P, int *ok, int *n_singular);
 /* ls_ilu_block_sparse.h:38-41 SolvePrecond(vec) = ForwardSubstitution
  * (cpp:544-573) then BackwardSubstitution (cpp:721-749), in place. */

@@ -312,7 +312,32 @@ def test_edge_cases(gpu):
     G.A.setZero()
     G.ilu.CopyValue(G.A)
     assert not G.ilu.Decompose()
-    assert G.ilu.n_singular == prob.np_
+    assert G.ilu.n_singular == 11  # the reference's count where it stops
+
+
+@pytest.mark.parametrize("ordering", ["natural", "color4"])
+def test_decompose_stops_at_eleventh_singular_pivot(gpu, ordering):
+    """14 rows with exactly zero pivots; the 11th in row order is row 80.
+    The device factorises level by level, then puts the rows from row 80 on
+    back into the state the reference's row-order loop leaves when it
+    returns false there (ls_ilu_block_sparse.cpp:331-333): factors equal to
+    the oracle's bit for bit, ok = False, 11 singular pivots counted."""
+    from test_oracle_invariants import singular_rows_case
+    p, Cs, rows = singular_rows_case(ordering)
+    A = S.CMatrixSparse()
+    A.Initialize(p.np_, 3, True)
+    A.SetPattern(Cs.colind, Cs.rowptr)
+    A.upload(Cs.vcrs, Cs.vdia)
+    P = S.CPreconditionerILU()
+    P.SetPattern0(A)
+    P.CopyValue(A)
+    assert not P.Decompose() and P.n_singular == 11
+    assert not Cs.factor() and Cs.n_singular == 11
+    _, _, lc, ld = P.download()
+    assert np.array_equal(bits(lc), bits(Cs.l_vcrs))
+    assert np.array_equal(bits(ld), bits(Cs.l_vdia))
+    # the rows after the stop row are CopyValue's (unfactorised) values
+    assert np.array_equal(bits(ld[9 * 81:]), bits(Cs.vdia[9 * 81:]))
 
 
 def test_missing_pattern_entry_reported(gpu):

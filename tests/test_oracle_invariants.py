@@ -245,6 +245,37 @@ def test_decompose_singular_count():
     assert S.n_singular == 11  # stops at the 11th, as the reference
 
 
+def singular_rows_case(ordering):
+    """A 4^3 cube whose rows 10, 17, ..., 101 (14 rows) have their diagonal
+    and lower blocks zeroed: each of those pivots is exactly zero, the 11th
+    is row 80, and later rows still factorise."""
+    p = W.cube(4, sigma=0.5, seed=3, ordering=ordering)
+    S, b = assemble(p)
+    rows = list(range(10, 102, 7))
+    for i in rows:
+        S.vdia[9 * i:9 * i + 9] = 0.0
+        for s in range(S.colind[i], S.colind[i + 1]):
+            if S.rowptr[s] < i:
+                S.vcrs[9 * s:9 * s + 9] = 0.0
+    return p, S, rows
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference not built")
+@pytest.mark.parametrize("ordering", ["natural", "color4"])
+def test_decompose_stop_state_matches_reference(ordering):
+    """Decompose returns false at the 11th singular pivot and leaves the rows
+    as they are then (ls_ilu_block_sparse.cpp:331-333): the oracle's factor
+    arrays equal the compiled reference's bit for bit."""
+    p, S, rows = singular_rows_case(ordering)
+    R = oracle.RefSystem(S.colind, S.rowptr)
+    R.set_matrix(S.vcrs, S.vdia)
+    assert not S.factor() and S.n_singular == 11
+    assert not R.factor()
+    _, _, rc, rd = R.ilu()
+    assert np.array_equal(S.l_vcrs.view(np.uint64), rc.view(np.uint64))
+    assert np.array_equal(S.l_vdia.view(np.uint64), rd.view(np.uint64))
+
+
 @pytest.mark.parametrize("fill", [1, 2])
 def test_iluk_defining_property(fill):
     # ILU(k): the pattern contains A's; (L + D)(I + U) == A on every position of

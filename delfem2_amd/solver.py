@@ -471,7 +471,7 @@ def pcg_kernel_times(mat: CMatrixSparse, ilu: CPreconditionerILU, fp32: bool,
     ms = np.zeros(8)
     N.call("dfm2h_pcg_kernel_times", mat.h, ilu.h, int(bool(fp32)), int(nrep), _hptr(ms))
     return dict(zip(["spmv", "update", "fwd_sweep", "bwd_sweep", "pupdate", "alpha_step",
-                     "refine_step", "conv_step"], ms.tolist()))
+                     "spmv_back_to_back", "unused"], ms.tolist()))
 
 
 def measure_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:

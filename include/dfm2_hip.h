@@ -364,7 +364,8 @@ int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name,
  * ms[8] = { SpMV, r/x update with forward level 0, forward levels >= 1,
  * backward levels >= 1 (backward level 0 has no work: its (r, Pr) share is
  * summed in the forward sweep), stop test + beta step + p update, alpha
- * step, 0, 0 } (the FP32 reliable-update residual runs inside the SpMV when
+ * step, the SpMV again timed over nrep launches back to back (no events
+ * between kernels: its duration plus the dispatch gap), 0 } (the FP32 reliable-update residual runs inside the SpMV when
  * due, its decision inside the beta step).  The kernels run on the solver's
  * scratch vectors; nothing the caller owns is touched. */
 int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep,

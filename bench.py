@@ -506,8 +506,9 @@ def bench_newmark(args):
         it_c = run.step(tol, 1000)
         tc = time.perf_counter() - tc
         cpu = {"value": round(tc, 3), "unit": "s/step", "cores": 1, "kind": kind,
+               "measured_iterations": it_c,
                "sample": f"one Newmark step from rest of the same {args.n}^3 problem "
-                         f"({it_c} PCG iterations to {tol})"}
+                         f"({it_c} PCG iterations to {tol})", **host_cpu()}
     out = {
         "metric": "Newmark-beta implicit time step of the 823k-DOF tet cube (s/step)",
         "value": round(t_step, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,

@@ -12,7 +12,9 @@
 * streams are per thread: two threads, each with its own handles and its
   own stream, give results bit-identical to single-thread runs;
 * CopyValue after its matrix handle was replaced by one with another
-  pattern rebuilds the slot map (ls_ilu_block_sparse.cpp:1009-1047).
+  pattern rebuilds the slot map (ls_ilu_block_sparse.cpp:1009-1047);
+* SetZero drops staged blocks; a direct write to the device values declared
+  with dfm2h_bsr_values_changed reaches the solver's packed copies.
 """
 import ctypes as C
 import threading
@@ -286,3 +288,43 @@ def test_copy_value_after_matrix_replaced(gpu):
     assert np.array_equal(bits(lc), bits(exp.ravel()))
     assert np.array_equal(bits(ld), bits(vd2))
     print("second handle at the first one's address:", A2.h.value == addr1)
+
+
+def test_set_zero_discards_staged_blocks(gpu):
+    """Blocks staged by Merge and not yet applied are dropped by SetZero, as
+    the reference's setZero clears what its Merge calls had added."""
+    prob = W.cube(6, sigma=0.5, seed=7)
+    ind, ps = host_pattern(prob)
+    A = matrix(ind, ps, prob.np_)
+    A.upload(W.uniform(66, ps.size * 9), W.uniform(67, prob.np_ * 9))
+    merge_all(A, prob.tet, W.uniform(68, prob.ne * 144) - 0.5)
+    A.setZero()
+    vc, vd = A.download()
+    assert not vc.any() and not vd.any()
+
+
+def test_values_changed_reaches_the_solver_copies(gpu):
+    """A write through the device arrays of dfm2h_bsr_info, declared with
+    dfm2h_bsr_values_changed, is what the next factorisation and solve use:
+    doubling A on the device halves the solution bit for bit (a power-of-two
+    scaling is exact through the block inverses, ILU(0) and PCG, and leaves
+    the residual history unchanged).  Without the declaration the solver
+    would keep its packed copies of the old values."""
+    prob = W.cube(6, sigma=0.5, seed=8, ordering="color4")
+    G = DeviceSystem(prob)
+    assert G.assemble() and G.factor()
+    x1, h1, _ = G.solve(1e-10, 5000)
+    lib = N.load()
+    nblk, ncrs = C.c_size_t(), C.c_size_t()
+    pc, pd = C.c_void_p(), C.c_void_p()
+    N.check(lib.dfm2h_bsr_info(G.A.h, C.byref(nblk), C.byref(ncrs), C.byref(pc), C.byref(pd)))
+    for ptr, n in ((pc, ncrs.value * 9), (pd, nblk.value * 9)):
+        tmp = torch.empty(n, dtype=torch.float64, device="cuda")
+        N.check(lib.dfm2h_vec_copy(C.c_void_p(tmp.data_ptr()), ptr, n))
+        N.check(lib.dfm2h_vec_add_scaled(ptr, C.c_double(1.0), C.c_void_p(tmp.data_ptr()), n))
+        torch.cuda.synchronize()
+    G.A.values_changed()
+    assert G.factor()
+    x2, h2, _ = G.solve(1e-10, 5000)
+    assert np.array_equal(bits(np.array(h1)), bits(np.array(h2)))
+    assert np.array_equal(bits((x1 * 0.5).cpu().numpy()), bits(x2.cpu().numpy()))

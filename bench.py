@@ -643,6 +643,7 @@ def kernel_table(G, S, nb, nnzb, n0, nu0, step_mixed, step_fp64, args) -> dict:
                              "gbs": round(b / (ms[k] * 1e-3) / 1e9, 1)}
         for k in ("alpha_step", "spmv_back_to_back"):
             kern[prec][k + "_ms"] = round(ms[k], 5)
+        kern[prec]["beta_step_folded"] = ms["folded"]
     return kern
 
 

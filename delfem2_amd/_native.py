@@ -107,6 +107,7 @@ SIGNATURES = {
     "dfm2h_pcg_set_shape": [_vp, _i, _i],
     "dfm2h_pcg_set_mixed_options": [_vp, _i, _u],
     "dfm2h_pcg_set_graph_chunks": [_vp, _u, _u],
+    "dfm2h_pcg_set_scalar_fold": [_vp, _i],
     "dfm2h_pcg_get_shape": [_vp, _i, _pi, C.POINTER(C.c_char_p), _vp, _sz, _psz],
     "dfm2h_batch_create": [_pvp, _vp, _vp, _vp, _sz],
     "dfm2h_batch_destroy": [_vp],

@@ -198,13 +198,12 @@ __device__ inline double reduce_partials(const double *partials, unsigned n) {
 // for an unused set), each formed in exactly reduce_partials' order, with
 // the loads of U strides of every set issued before their adds: the
 // partials were just written by other XCDs, so every load round is a full
-// memory latency, and one round replaces up to U x NS of them.
-template <int BS, int NS>
-__device__ inline void reduce_partial_sets(const double *const (&p)[NS], const unsigned (&n)[NS],
-                                           double (&t)[NS]) {
-  constexpr int U = 8;
-  constexpr int NW = BS / kWave;
-  double s[NS];
+// memory latency, and one round replaces up to U x NS of them.  Neither U
+// nor the other sets change a set's sum.  partial_sums_thread forms each
+// thread's share, partial_sums_block the block's; reduce_partial_sets both.
+template <int BS, int NS, int U = 8>
+__device__ inline void partial_sums_thread(const double *const (&p)[NS], const unsigned (&n)[NS],
+                                           double (&s)[NS]) {
   unsigned nmax = 0;
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
@@ -226,6 +225,13 @@ __device__ inline void reduce_partial_sets(const double *const (&p)[NS], const u
       for (int u = 0; u < U; ++u)
         if (base + u * BS + threadIdx.x < n[k]) s[k] += v[k][u];
   }
+}
+
+// The block's sums of the per-thread sums s (fixed wave, then fixed wave
+// order), in every thread.
+template <int BS, int NS>
+__device__ inline void partial_sums_block(const double (&s)[NS], double (&t)[NS]) {
+  constexpr int NW = BS / kWave;
   __shared__ double sm[NS][NW];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
@@ -243,6 +249,14 @@ __device__ inline void reduce_partial_sets(const double *const (&p)[NS], const u
     t[k] = a;
   }
   __syncthreads();
+}
+
+template <int BS, int NS, int U = 8>
+__device__ inline void reduce_partial_sets(const double *const (&p)[NS], const unsigned (&n)[NS],
+                                           double (&t)[NS]) {
+  double s[NS];
+  partial_sums_thread<BS, NS, U>(p, n, s);
+  partial_sums_block<BS, NS>(s, t);
 }
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

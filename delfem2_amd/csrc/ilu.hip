@@ -389,7 +389,6 @@ int dfm2h_ilu_destroy(dfm2h_ilu P) {
 // known: device copies, level schedules, sliced sweep plans, the A -> ILU
 // slot map.
 static void finish_pattern(dfm2h_ilu P, dfm2h_bsr A) {
-  const Pattern &pa = A->pat;
   P->ncrs = P->colind[P->nblk];
   P->colind_d.upload(P->colind);
   P->rowptr_d.upload(P->rowptr);

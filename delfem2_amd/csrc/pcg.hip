@@ -53,6 +53,70 @@ struct PcgScal {
   int pad3;
 };
 
+// Field by field copy of the state (a whole-struct copy keeps a per-thread
+// stack copy alive, which the compiler then places in LDS for every thread).
+__device__ __attribute__((always_inline)) inline void copy_state(PcgScal &d, const PcgScal &s) {
+  d.rPr = s.rPr;
+  d.alpha = s.alpha;
+  d.beta = s.beta;
+  d.inv_sq0 = s.inv_sq0;
+  d.tol = s.tol;
+  d.sq = s.sq;
+  d.it = s.it;
+  d.done = s.done;
+  d.converged = s.converged;
+  d.max_nitr = s.max_nitr;
+  d.nan = s.nan;
+  d.pad = s.pad;
+  d.sq_b = s.sq_b;
+  d.rmax2 = s.rmax2;
+  d.delta2 = s.delta2;
+  d.upd = s.upd;
+  d.nupd = s.nupd;
+  d.max_upd = s.max_upd;
+  d.final_ = s.final_;
+  d.it_upd = s.it_upd;
+  d.period = s.period;
+  d.restart = s.restart;
+  d.pad3 = s.pad3;
+}
+
+// State fields written by earlier kernels are read with vector loads at
+// agent scope, never through the scalar data cache: a uniform load of them
+// can otherwise return a scalar-cache line filled before the write
+// (measured: the mixed solver's one-workgroup steps then saw stale
+// restart / rmax2 fields and took other iterates).
+template <typename V>
+__device__ __attribute__((always_inline)) inline V ld_state(const V *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __attribute__((always_inline)) inline void load_state(PcgScal &d, const PcgScal *s) {
+  d.rPr = ld_state(&s->rPr);
+  d.alpha = ld_state(&s->alpha);
+  d.beta = ld_state(&s->beta);
+  d.inv_sq0 = ld_state(&s->inv_sq0);
+  d.tol = ld_state(&s->tol);
+  d.sq = ld_state(&s->sq);
+  d.it = ld_state(&s->it);
+  d.done = ld_state(&s->done);
+  d.converged = ld_state(&s->converged);
+  d.max_nitr = ld_state(&s->max_nitr);
+  d.nan = ld_state(&s->nan);
+  d.pad = ld_state(&s->pad);
+  d.sq_b = ld_state(&s->sq_b);
+  d.rmax2 = ld_state(&s->rmax2);
+  d.delta2 = ld_state(&s->delta2);
+  d.upd = ld_state(&s->upd);
+  d.nupd = ld_state(&s->nupd);
+  d.max_upd = ld_state(&s->max_upd);
+  d.final_ = ld_state(&s->final_);
+  d.it_upd = ld_state(&s->it_upd);
+  d.period = ld_state(&s->period);
+  d.restart = ld_state(&s->restart);
+  d.pad3 = ld_state(&s->pad3);
+}
+
 constexpr int kVecBlock = 256;
 constexpr unsigned kVecGridMax = 2048;
 constexpr int kScalBlock = 1024;  // the one-workgroup scalar steps
@@ -177,12 +241,19 @@ struct PcgArgs {
   double *partials2;  // (r,r) partials of the update kernel
   double *partials3;  // (r,Pr) partials of the fused iteration
   double *partials4;  // mixed: |r_true - r_recursive|^2 partials of a reliable update
+  double *partials5;  // mixed: FP64 |b - A x|^2 partials of a reliable update
   unsigned upd_part_off;
   unsigned xcd;  // workgroups of the slice kernels numbered per XCD (wg_id)
   // the SpMV's numbering with xcd set: segments of its grid (seg_wg_id)
   unsigned nseg;
   unsigned seg[kMaxSeg + 1];
-  PcgScal *S;
+  // iteration state, two copies: S is read by the SpMV, the r/x update and
+  // the separate beta step, S1 by the sweeps and the folded p update.  The
+  // separate scalar-step kernels write both; the p update with the beta step
+  // folded in writes S.
+  PcgScal *S, *S1;
+  // partial counts of the beta step's two sets
+  unsigned n_beta2, n_beta3;
   // mixed precision: FP64 solution part (FP32 vectors), right-hand side,
   // matrix and outer history
   double *x64;
@@ -219,127 +290,201 @@ enum ScalMode {
                         // iteration; then beta
 };
 
-// pa/na: first partial set, pb/nb: second (ConvBeta, AlphaMixed, BetaMixed),
-// pc/nc: third (AlphaMixed)
+// One scalar step on a private copy S of the state, from the partial sums t,
+// t2, t3; hist entries are written when `writer`.  Every caller with the
+// same state and sums computes the same new state.
 template <int MODE>
-__global__ void __launch_bounds__(kScalBlock)
-k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
-             const double *__restrict__ pb, unsigned nb, const double *__restrict__ pc,
-             unsigned nc, double *hist) {
-  if (MODE != kScalInit && MODE != kScalInitMixed && S->done) return;
-  const bool upd = S->upd != 0;  // block-uniform
-  const bool two = MODE == kScalConvBeta || MODE == kScalBetaMixed || (MODE == kScalAlphaMixed && upd);
-  const bool three = MODE == kScalAlphaMixed && upd;
-  const double *const ps[3] = {pa, pb, pc};
-  const unsigned ns[3] = {na, two ? nb : 0u, three ? nc : 0u};
-  double ts[3];
-  reduce_partial_sets<kScalBlock, 3>(ps, ns, ts);
-  const double t = ts[0], t2 = ts[1], t3 = ts[2];
-  if (threadIdx.x != 0) return;
+__device__ __attribute__((always_inline)) inline void scal_step(PcgScal &S, double t, double t2,
+                                                                double t3, double *hist,
+                                                                bool writer) {
+  const bool upd = S.upd != 0;  // before this step
   if (MODE == kScalInit) {  // |r0|^2: history[0]; early return when < 1e-30
-    hist[0] = sqrt(t);
-    S->sq = t;
-    S->it = 0;
-    S->converged = 0;
-    S->nan = 0;
-    S->done = 0;
+    if (writer) hist[0] = sqrt(t);
+    S.sq = t;
+    S.it = 0;
+    S.converged = 0;
+    S.nan = 0;
+    S.done = 0;
     if (t < 1.0e-30) {
-      S->done = 1;
-      S->converged = 1;
+      S.done = 1;
+      S.converged = 1;
     } else if (!(t == t) || t > 1.0e308) {
-      S->done = 1;
-      S->nan = 1;
+      S.done = 1;
+      S.nan = 1;
     }
-    S->inv_sq0 = 1.0 / t;
+    S.inv_sq0 = 1.0 / t;
   } else if (MODE == kScalAlpha) {  // alpha = rPr / pAp
-    S->alpha = S->rPr / t;
+    S.alpha = S.rPr / t;
   } else if (MODE == kScalRPr0) {
-    S->rPr = t;
+    S.rPr = t;
   } else if (MODE == kScalConvBeta) {
     // push |r|, stop on sqrt(|r|^2/|r0|^2) < tol; else beta = rPr1/rPr
-    const int it = S->it + 1;
-    S->it = it;
-    S->sq = t;
-    hist[it] = sqrt(t);
-    if (sqrt(t * S->inv_sq0) < S->tol) {
-      S->done = 1;
-      S->converged = 1;
+    const int it = S.it + 1;
+    S.it = it;
+    S.sq = t;
+    if (writer) hist[it] = sqrt(t);
+    if (sqrt(t * S.inv_sq0) < S.tol) {
+      S.done = 1;
+      S.converged = 1;
     } else if (!(t == t) || t > 1.0e308) {
-      S->done = 1;
-      S->nan = 1;
-    } else if (it >= S->max_nitr) {
-      S->done = 1;
+      S.done = 1;
+      S.nan = 1;
+    } else if (it >= S.max_nitr) {
+      S.done = 1;
     } else {
-      S->beta = t2 / S->rPr;
-      S->rPr = t2;
+      S.beta = t2 / S.rPr;
+      S.rPr = t2;
     }
   } else if (MODE == kScalInitMixed) {
-    hist[0] = sqrt(t);
-    S->sq_b = t;
-    S->rmax2 = t;
-    S->sq = t;
-    S->it = 0;
-    S->nupd = 0;
-    S->upd = 0;
-    S->final_ = 0;
-    S->it_upd = 0;
-    S->restart = 0;
-    S->converged = 0;
-    S->nan = 0;
-    S->done = 0;
+    if (writer) hist[0] = sqrt(t);
+    S.sq_b = t;
+    S.rmax2 = t;
+    S.sq = t;
+    S.it = 0;
+    S.nupd = 0;
+    S.upd = 0;
+    S.final_ = 0;
+    S.it_upd = 0;
+    S.restart = 0;
+    S.converged = 0;
+    S.nan = 0;
+    S.done = 0;
     if (t < 1.0e-30) {
-      S->done = 1;
-      S->converged = 1;
+      S.done = 1;
+      S.converged = 1;
     } else if (!(t == t) || t > 1.0e308) {
-      S->done = 1;
-      S->nan = 1;
+      S.done = 1;
+      S.nan = 1;
     }
   } else if (MODE == kScalAlphaMixed) {
     // t: (p, Ap); t2: FP64 |b - A x|^2 of the reliable update, if one ran
     if (upd) {
-      const int k = S->nupd + 1;
-      S->nupd = k;
-      hist[k] = sqrt(t2);
-      S->rmax2 = t2;
-      if (sqrt(t2 / S->sq_b) < S->tol) {
-        S->done = 1;
-        S->converged = 1;
+      const int k = S.nupd + 1;
+      S.nupd = k;
+      if (writer) hist[k] = sqrt(t2);
+      S.rmax2 = t2;
+      if (sqrt(t2 / S.sq_b) < S.tol) {
+        S.done = 1;
+        S.converged = 1;
       } else if (!(t2 == t2) || t2 > 1.0e308) {
-        S->done = 1;
-        S->nan = 1;
-      } else if (S->final_ || k >= S->max_upd) {
-        S->done = 1;
+        S.done = 1;
+        S.nan = 1;
+      } else if (S.final_ || k >= S.max_upd) {
+        S.done = 1;
       }
     }
     if (upd && (!(t > 0.0) || t3 > 1.0e-2 * t2)) {
       // the recursive residual the current direction was built from is more
       // than 10% off the true one (or reached zero, p = 0): no step now, and
       // the next direction restarts from the true residual
-      S->alpha = 0.0;
-      S->restart = 1;
+      S.alpha = 0.0;
+      S.restart = 1;
     } else {
-      S->alpha = S->rPr / t;
+      S.alpha = S.rPr / t;
     }
   } else {  // kScalBetaMixed: t: FP32 |r|^2 after the update, t2: (r, Pr)
-    const int it = S->it + 1;
-    S->it = it;
-    S->sq = t;
-    S->upd = 0;
+    const int it = S.it + 1;
+    S.it = it;
+    S.sq = t;
+    S.upd = 0;
     if (!(t == t) || t > 1.0e308) {
-      S->done = 1;
-      S->nan = 1;
+      S.done = 1;
+      S.nan = 1;
       return;
     }
-    if (t < S->delta2 * S->rmax2 || sqrt(t / S->sq_b) < S->tol || it >= S->max_nitr ||
-        (S->period > 0 && it - S->it_upd >= S->period)) {
-      S->upd = 1;
-      S->final_ = it >= S->max_nitr ? 1 : 0;
-      S->it_upd = it;
+    if (t < S.delta2 * S.rmax2 || sqrt(t / S.sq_b) < S.tol || it >= S.max_nitr ||
+        (S.period > 0 && it - S.it_upd >= S.period)) {
+      S.upd = 1;
+      S.final_ = it >= S.max_nitr ? 1 : 0;
+      S.it_upd = it;
     }
-    S->beta = S->restart ? 0.0 : t2 / S->rPr;
-    S->restart = 0;
-    S->rPr = t2;
+    S.beta = S.restart ? 0.0 : t2 / S.rPr;
+    S.restart = 0;
+    S.rPr = t2;
   }
+}
+
+// The fields scal_step<MODE> may change, from s into d (the folded step
+// copies the rest unchanged, so the whole state need not stay in registers).
+template <int MODE>
+__device__ __attribute__((always_inline)) inline void store_step_fields(PcgScal &d,
+                                                                        const PcgScal &s) {
+  if constexpr (MODE == kScalAlpha) {
+    d.alpha = s.alpha;
+  } else if constexpr (MODE == kScalAlphaMixed) {
+    d.nupd = s.nupd;
+    d.rmax2 = s.rmax2;
+    d.done = s.done;
+    d.converged = s.converged;
+    d.nan = s.nan;
+    d.alpha = s.alpha;
+    d.restart = s.restart;
+  } else if constexpr (MODE == kScalConvBeta) {
+    d.it = s.it;
+    d.sq = s.sq;
+    d.done = s.done;
+    d.converged = s.converged;
+    d.nan = s.nan;
+    d.beta = s.beta;
+    d.rPr = s.rPr;
+  } else if constexpr (MODE == kScalBetaMixed) {
+    d.it = s.it;
+    d.sq = s.sq;
+    d.upd = s.upd;
+    d.done = s.done;
+    d.nan = s.nan;
+    d.final_ = s.final_;
+    d.it_upd = s.it_upd;
+    d.beta = s.beta;
+    d.restart = s.restart;
+    d.rPr = s.rPr;
+  } else {
+    copy_state(d, s);
+  }
+}
+
+// The folded beta step keeps the state in LDS: the first kStateWords threads
+// each load one 32-bit word of it early and store it into the block's copy
+// (and, in workgroup 0, into the output state: the words the step leaves
+// unchanged) before the block's reduction barrier.
+constexpr unsigned kStateWords = sizeof(PcgScal) / 4;
+static_assert(sizeof(PcgScal) % 4 == 0 && kStateWords <= (unsigned)kWave,
+              "the state's words fit one wavefront");
+
+__device__ inline unsigned state_word(const PcgScal *S) {
+  return threadIdx.x < kStateWords ? ld_state(reinterpret_cast<const unsigned *>(S) + threadIdx.x)
+                                   : 0u;
+}
+
+__device__ inline void put_state_word(PcgScal *block_copy, PcgScal *out, unsigned w) {
+  if (threadIdx.x < kStateWords) {
+    reinterpret_cast<unsigned *>(block_copy)[threadIdx.x] = w;
+    if (blockIdx.x == 0) reinterpret_cast<unsigned *>(out)[threadIdx.x] = w;
+  }
+}
+
+// pa/na: first partial set, pb/nb: second (ConvBeta, AlphaMixed, BetaMixed),
+// pc/nc: third (AlphaMixed).  Writes the new state to S[0] and S[1] (the
+// copies the iteration reads at its two points; equal here).
+template <int MODE>
+__global__ void __launch_bounds__(kScalBlock)
+k_pcg_scalar(PcgScal *S, const double *__restrict__ pa, unsigned na,
+             const double *__restrict__ pb, unsigned nb, const double *__restrict__ pc,
+             unsigned nc, double *hist) {
+  if (MODE != kScalInit && MODE != kScalInitMixed && ld_state(&S->done)) return;
+  const bool upd = ld_state(&S->upd) != 0;  // block-uniform
+  const bool two = MODE == kScalConvBeta || MODE == kScalBetaMixed || (MODE == kScalAlphaMixed && upd);
+  const bool three = MODE == kScalAlphaMixed && upd;
+  const double *const ps[3] = {pa, pb, pc};
+  const unsigned ns[3] = {na, two ? nb : 0u, three ? nc : 0u};
+  double ts[3];
+  reduce_partial_sets<kScalBlock, 3>(ps, ns, ts);
+  if (threadIdx.x != 0) return;
+  PcgScal st;
+  load_state(st, S);
+  scal_step<MODE>(st, ts[0], ts[1], ts[2], hist, true);
+  copy_state(S[0], st);
+  copy_state(S[1], st);
 }
 
 // the mixed solver's FP64 solution component j (x64 + x32 for FP32 vectors)
@@ -384,12 +529,12 @@ constexpr int spmv_unroll() {
 }
 
 // Ap -> Pr, partial (p, Ap).  Mixed form with a reliable update due: also
-// r = b - A x in FP64 (partials2: |r|^2), for the update kernel that
-// follows.
+// r = b - A x in FP64 (partials5: |r|^2; partials4: its change), for the
+// alpha step that follows.
 template <typename T, typename M>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
-  if (a.S->done) return;
-  const bool refine = is_mixed<T, M>() && a.S->upd;  // grid-uniform
+  if (ld_state(&a.S->done)) return;
+  const bool refine = is_mixed<T, M>() && ld_state(&a.S->upd);  // grid-uniform
   const unsigned w = a.xcd ? seg_wg_id(a.seg, a.nseg) : blockIdx.x;
   SliceLane L;
   L.lane = threadIdx.x & (kWave - 1);
@@ -420,7 +565,7 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
   if (threadIdx.x == 0) {
     a.partials[w] = part;
     if (refine) {
-      a.partials2[w] = part2;
+      a.partials5[w] = part2;
       a.partials4[w] = part4;
     }
   }
@@ -458,9 +603,10 @@ inline unsigned update_grid(unsigned nsl) {
 
 template <typename T, typename M>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
-  if (a.S->done) return;
-  const double alpha = a.S->alpha;
-  const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && a.S->upd;  // an update just ran
+  if (ld_state(&a.S->done)) return;
+  const double alpha = ld_state(&a.S->alpha);
+  // an update just ran
+  const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && ld_state(&a.S->upd);
   // no gathers here: the dispatch order serves (an XCD-ordered numbering
   // measured slower on MI355X: FP64 64^3 14.2 -> 16.9 us)
   const unsigned wv = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
@@ -574,7 +720,7 @@ k_pcg_fwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *
     }
     load_slot_round<M, U, SPLIT, NT>(R0, L, half, R.kl, R.off, a.f_plan.col, a.f_val);
   }
-  if (a.S->done) return;
+  if (ld_state(&a.S1->done)) return;
   T t = rv;
   if (i != ~0u)
     t = sliced_row_sum_pre<T, M, true, U, SPLIT, kVS, NT>(t, (T)1, R0, L, half, R.len, R.kl,
@@ -619,7 +765,7 @@ k_pcg_bwd(PcgArgs<T, M> a, unsigned slice0, unsigned nsl, GroupView gv, double *
     }
     load_slot_round<M, U, SPLIT, NT>(R0, L, half, R.kl, R.off, a.b_plan.col, a.b_val);
   }
-  if (a.S->done) return;
+  if (ld_state(&a.S1->done)) return;
   if (i != ~0u)
     t = sliced_row_sum_pre<T, M, true, U, SPLIT, kVS, NT>(t, (T)1, R0, L, half, R.len, R.kl,
                                                           R.kmax, R.off, a.b_plan.col, a.b_val,
@@ -652,8 +798,8 @@ inline unsigned pupdate_grid(size_t n) { return std::max(1u, ceil_div(n / 4, kVe
 
 template <typename T, typename M>
 __global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T, M> a, int mode) {
-  if (a.S->done) return;
-  const double beta = a.S->beta;
+  if (ld_state(&a.S->done)) return;
+  const double beta = ld_state(&a.S->beta);
   using V = typename Vec4<T>::type;
   const size_t k = (size_t)blockIdx.x * kVecBlock + threadIdx.x, n4 = a.n / 4;
   if (k < n4) {
@@ -671,6 +817,54 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_pupdate(PcgArgs<T, M> a, int 
     const size_t i = 4 * n4 + k;
     a.p[i] = mode == 0 ? a.Pr[i] : (T)(beta * (double)a.p[i] + (double)a.Pr[i]);
   }
+}
+
+// Workgroup size of the p update that runs the beta step itself: the
+// scalar kernel's, so that its partial sums come out in the same order.
+constexpr int kFoldBlock = kScalBlock;
+
+inline unsigned pupdate_grid_folded(size_t n) { return std::max(1u, ceil_div(n / 4, kFoldBlock)); }
+
+// p = beta p + Pr with the beta step run here: every workgroup sums the
+// update's (r,r) partials and the sweeps' (r, Pr) partials itself, with the
+// scalar kernel's reduction, and workgroup 0 stores the new state in S (the
+// copy the next SpMV reads).  Same products as k_pcg_pupdate's mode 1.
+template <typename T, typename M>
+__global__ void __launch_bounds__(kFoldBlock) k_pcg_pupdate_folded(PcgArgs<T, M> a) {
+  __shared__ PcgScal sst;  // the block's copy of the state
+  const unsigned sw = state_word(a.S1);
+  using V = typename Vec4<T>::type;
+  const size_t k = (size_t)blockIdx.x * kFoldBlock + threadIdx.x, n4 = a.n / 4;
+  const bool body = k < n4, tail = k < a.n - 4 * n4;
+  V z{}, p{};
+  T zt = 0, pt = 0;
+  if (body) {
+    z = reinterpret_cast<const V *>(a.Pr)[k];
+    p = reinterpret_cast<const V *>(a.p)[k];
+  }
+  if (tail) {
+    zt = a.Pr[4 * n4 + k];
+    pt = a.p[4 * n4 + k];
+  }
+  const double *const ps[2] = {a.partials2, a.partials3};
+  const unsigned ns[2] = {a.n_beta2, a.n_beta3};
+  double s2[2], ts[2];
+  partial_sums_thread<kFoldBlock, 2>(ps, ns, s2);
+  put_state_word(&sst, a.S, sw);
+  partial_sums_block<kFoldBlock, 2>(s2, ts);
+  constexpr int kMode = is_mixed<T, M>() ? kScalBetaMixed : kScalConvBeta;
+  if (threadIdx.x == 0 && !sst.done) {
+    scal_step<kMode>(sst, ts[0], ts[1], 0.0, a.hist, blockIdx.x == 0);
+    if (blockIdx.x == 0) store_step_fields<kMode>(*a.S, sst);
+  }
+  __syncthreads();
+  if (sst.done) return;  // block-uniform
+  const double beta = sst.beta;
+  if (body)
+    reinterpret_cast<V *>(a.p)[k] =
+        V{(T)(beta * (double)p.x + (double)z.x), (T)(beta * (double)p.y + (double)z.y),
+          (T)(beta * (double)p.z + (double)z.z), (T)(beta * (double)p.w + (double)z.w)};
+  if (tail) a.p[4 * n4 + k] = (T)(beta * (double)pt + (double)zt);
 }
 
 // x = 0 (and x64 = 0 for FP32 vectors), r = b, partial |b|^2 (FP64)
@@ -725,7 +919,7 @@ template <typename T>
 struct PcgBufs {
   using elem = T;
   DBuf<T> r, x, Pr, p;
-  DBuf<double> partials, partials2, partials3, partials4, hist;
+  DBuf<double> partials, partials2, partials3, partials4, partials5, hist;
   DBuf<PcgScal> S;
   ChunkGraph g[2];  // iterations per launch: [0] kChunkLarge, [1] kChunkSmall
   void drop_graph() {
@@ -765,7 +959,6 @@ static unsigned split_grid(unsigned nsl, int split) {
 constexpr int kNumSweepBase = 6;
 constexpr int kNumSweepShapes = 2 * kNumSweepBase;
 constexpr int kSweepSplit[kNumSweepShapes] = {1, 1, 2, 2, 1, 2, 1, 1, 2, 2, 1, 2};
-constexpr int kSweepU[kNumSweepShapes] = {8, 4, 8, 4, 16, 16, 8, 4, 8, 4, 16, 16};
 const char *const kSweepNames[kNumSweepShapes] = {
     "split1_u8",     "split1_u4",     "split2_u8",     "split2_u4",
     "split1_u16",    "split2_u16",    "split1_u8_xcd", "split1_u4_xcd",
@@ -791,6 +984,8 @@ struct PcgWork {
   // iterations per graph launch (the small chunk at the start and near the
   // predicted end, the large one in between)
   unsigned chunk_large = kChunkLarge, chunk_small = kChunkSmall;
+  // beta step folded into the p update: -1 by size, 0 never, 1 always
+  int fold_mode = -1;
   PcgWork() {
     DFM2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     DFM2H_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
@@ -936,8 +1131,10 @@ static PcgArgs<T, M> make_args(dfm2h_bsr A, dfm2h_ilu P, PcgBufs<T> &B, const Pa
   a.xcd = 0;           // set per launch shape
   a.nseg = (unsigned)F.spmv_seg.size() - 1;
   for (unsigned k = 0; k <= a.nseg; ++k) a.seg[k] = F.spmv_seg[k];
+  a.partials5 = B.partials5.p;
   a.hist = B.hist.p;
   a.S = B.S.p;
+  a.S1 = B.S.p + 1;
   return a;
 }
 
@@ -967,12 +1164,13 @@ static void ensure_bufs(PcgBufs<T> &B, dfm2h_bsr A, dfm2h_ilu P, unsigned max_ni
   if (B.partials.n < np) B.partials.alloc(np);
   if (B.partials2.n < gs) B.partials2.alloc(gs);
   if (B.partials4.n < gs) B.partials4.alloc(gs);
+  if (B.partials5.n < gs) B.partials5.alloc(gs);
   const size_t n3 = std::max<size_t>(sweep_layout(A, P, 2).n3, 1);  // the larger layout
   if (B.partials3.n < n3) B.partials3.alloc(n3);
   if (B.hist.n < (size_t)max_nitr + 2) B.hist.alloc((size_t)max_nitr + 2);
   if (!B.S.p) {
-    B.S.alloc(1);
-    DFM2H_HIP(hipMemsetAsync(B.S.p, 0, sizeof(PcgScal), stream()));
+    B.S.alloc(2);
+    DFM2H_HIP(hipMemsetAsync(B.S.p, 0, 2 * sizeof(PcgScal), stream()));
   }
 }
 
@@ -1048,21 +1246,50 @@ static void launch_scalar(hipStream_t s, PcgScal *S, const double *pa, unsigned 
                      nc, hist);
 }
 
-// One FP64 iteration (lsitrsol.h:107-146): spmv, alpha, update (+ forward
-// level 0), forward levels >= 1, backward levels >= 1, stop test + beta,
-// p update.
+// The alpha step (a one-workgroup scalar kernel) and the r/x update; the
+// beta step (stop test included) and the p update, the former as a separate
+// scalar kernel or folded into the latter.  Both forms give the same bits.
+template <typename T, typename M>
+static void launch_alpha_update(const PcgArgs<T, M> &a, hipStream_t s) {
+  const unsigned gs = slice_grid(a.a_nsl);
+  if constexpr (is_mixed<T, M>())
+    launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials5, gs, a.hist64,
+                                   a.partials4, gs);
+  else
+    launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
+  hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, s, a);
+}
+
+template <typename T, typename M>
+static void launch_beta_pupdate(const PcgArgs<T, M> &a, bool fold, hipStream_t s) {
+  if (fold) {
+    hipLaunchKernelGGL((k_pcg_pupdate_folded<T, M>), dim3(pupdate_grid_folded(a.n)),
+                       dim3(kFoldBlock), 0, s, a);
+    return;
+  }
+  if constexpr (is_mixed<T, M>())
+    launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, a.n_beta2, a.partials3, a.n_beta3,
+                                  a.hist);
+  else
+    launch_scalar<kScalConvBeta>(s, a.S, a.partials2, a.n_beta2, a.partials3, a.n_beta3, a.hist);
+  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
+}
+
+// One iteration (lsitrsol.h:107-146): spmv, alpha, update (+ forward level
+// 0), forward levels >= 1, backward levels >= 1, stop test + beta, p
+// update.  The mixed form's beta step also decides whether the next
+// iteration starts with a reliable update (the FP32 residual fell by delta
+// since the last one, looks converged, or the cap is reached).  That update
+// runs inside the next SpMV: r = b - A (x64 + x32) in FP64, tested in FP64
+// by the alpha step (stop when |r| / |b| < tol); the r/x update then folds
+// x32 into x64 and the same Krylov sequence goes on from the true residual.
 template <typename T, typename M>
 static void launch_iteration(const PcgArgs<T, M> &a, dfm2h_ilu P, int shape,
-                             const SweepLayout &Lo, hipStream_t s) {
-  const unsigned gs = slice_grid(a.a_nsl);
-  const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(gs), dim3(kPackBlock), 0, s, a);
-  launch_scalar<kScalAlpha>(s, a.S, a.partials, gs, nullptr, 0, a.hist);
-  hipLaunchKernelGGL((k_pcg_update<T, M>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, s, a);
+                             const SweepLayout &Lo, bool fold, hipStream_t s) {
+  hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(slice_grid(a.a_nsl)), dim3(kPackBlock), 0, s, a);
+  launch_alpha_update(a, s);
   launch_sweeps_fused(a, P, shape, Lo, s);
-  launch_scalar<kScalConvBeta>(s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3, Lo.n3,
-                               a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 1);
+  launch_beta_pupdate(a, fold, s);
 }
 
 // Captures `chunk` iterations into G unless its signature matches.
@@ -1143,7 +1370,7 @@ static void run_chunks(PcgWork &W, PcgBufs<T> &B, unsigned max_iters, bool mixed
 
 template <typename T, typename M>
 static std::vector<uintptr_t> graph_sig(const PcgArgs<T, M> &a, dfm2h_bsr A, dfm2h_ilu P,
-                                        unsigned chunk) {
+                                        unsigned chunk, bool fold) {
   return {(uintptr_t)a.a_val,    (uintptr_t)a.a_dia,     (uintptr_t)a.f_val,
           (uintptr_t)a.f_dia,    (uintptr_t)a.b_val,     (uintptr_t)a.dinv0,
           (uintptr_t)a.r,        (uintptr_t)a.partials,  (uintptr_t)a.partials2,
@@ -1152,7 +1379,8 @@ static std::vector<uintptr_t> graph_sig(const PcgArgs<T, M> &a, dfm2h_bsr A, dfm
           (uintptr_t)a.b64,      (uintptr_t)a.hist64,    (uintptr_t)chunk,
           (uintptr_t)A,          (uintptr_t)A->pattern_serial, (uintptr_t)P->pattern_serial,
           (uintptr_t)P->work->fu.serial,
-          (uintptr_t)P->work->shape[solver_kind<T, M>()]};
+          (uintptr_t)P->work->shape[solver_kind<T, M>()], (uintptr_t)fold,
+          (uintptr_t)a.partials5};
 }
 
 // Autotuner of the launch shape: times kRep rounds of one SpMV plus one
@@ -1227,6 +1455,26 @@ static int resolve_shape(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   return v;
 }
 
+// Redundant partial-sum reads, in bytes per iteration, up to which the beta
+// step runs folded into the p update: then every workgroup of the p update
+// reads all the partials the step sums, which grows with the square of the
+// system size, while the one-workgroup kernel it saves costs the same at
+// every size.
+constexpr double kFoldBytesMax = 24.0 * (1 << 20);
+
+// The per-shape fields of the iteration's arguments; returns whether the
+// beta step runs folded into the p update.
+template <typename T, typename M>
+static bool iteration_args(PcgArgs<T, M> &a, const PcgWork &W, int shape, const SweepLayout &Lo) {
+  a.upd_part_off = Lo.upd_off;
+  a.xcd = shape_xcd(shape);
+  a.n_beta2 = update_grid(a.a_nsl);
+  a.n_beta3 = Lo.n3;
+  if (W.fold_mode >= 0) return W.fold_mode != 0;
+  const double bytes = 8.0 * (double)pupdate_grid_folded(a.n) * (a.n_beta2 + a.n_beta3);
+  return bytes <= kFoldBytesMax;
+}
+
 // Runs the fused PCG on B.r (holding b) into B.x; fills status.
 template <typename T, typename M>
 static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
@@ -1243,39 +1491,14 @@ static void run_pcg(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   const int shape = resolve_shape(A, P, W, B, a0);
   const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
   PcgArgs<T, M> a = a0;
-  a.upd_part_off = Lo.upd_off;
-  a.xcd = shape_xcd(shape);
+  const bool fold = iteration_args(a, W, shape, Lo);
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a, 0);
   check_launch("pcg prologue");
-  ensure_graphs(W, B, s, max_nitr, [&](unsigned c) { return graph_sig(a, A, P, c); },
-                [&] { launch_iteration(a, P, shape, Lo, s); });
+  ensure_graphs(W, B, s, max_nitr, [&](unsigned c) { return graph_sig(a, A, P, c, fold); },
+                [&] { launch_iteration(a, P, shape, Lo, fold, s); });
   run_chunks(W, B, max_nitr, false, status);
-}
-
-// One mixed iteration: an FP32 PCG step whose beta step also decides
-// whether the next iteration starts with a reliable update (the FP32
-// residual fell by delta since the last one, looks converged, or the cap is
-// reached).  That update runs inside the next SpMV: r = b - A (x64 + x32) in
-// FP64, tested in FP64 by the alpha step (stop when |r| / |b| < tol); the
-// r/x update then folds x32 into x64 and the same Krylov sequence goes on
-// from the true residual.
-template <typename T>
-static void launch_mixed_iteration(const PcgArgs<T, float> &a, dfm2h_ilu P, int shape,
-                                   const SweepLayout &Lo, hipStream_t s) {
-  const unsigned gs = slice_grid(a.a_nsl);
-  const unsigned gv = vec_grid(a.n);
-  hipLaunchKernelGGL((k_pcg_spmv<T, float>), dim3(gs), dim3(kPackBlock), 0, s, a);
-  launch_scalar<kScalAlphaMixed>(s, a.S, a.partials, gs, a.partials2, gs, a.hist64,
-                                 a.partials4, gs);
-  hipLaunchKernelGGL((k_pcg_update<T, float>), dim3(update_grid(a.a_nsl)), dim3(kPackBlock), 0, s,
-                     a);
-  launch_sweeps_fused(a, P, shape, Lo, s);
-  launch_scalar<kScalBetaMixed>(s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3, Lo.n3,
-                                 a.hist);
-  hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a,
-                     1);
 }
 
 template <typename T>
@@ -1296,15 +1519,14 @@ static void run_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, PcgWork &W, PcgBufs<T> &B,
   const int shape = resolve_shape(A, P, W, B, a0);
   const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
   PcgArgs<T, float> a = a0;
-  a.upd_part_off = Lo.upd_off;
-  a.xcd = shape_xcd(shape);
+  const bool fold = iteration_args(a, W, shape, Lo);
   const unsigned nb0 = launch_sweeps_full(a, P, s);
   launch_scalar<kScalRPr0>(s, a.S, a.partials, nb0, nullptr, 0, a.hist);
   hipLaunchKernelGGL((k_pcg_pupdate<T, float>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, s, a,
                      0);
   check_launch("mixed prologue");
-  ensure_graphs(W, B, s, max_inner, [&](unsigned c) { return graph_sig(a, A, P, c); },
-                [&] { launch_mixed_iteration(a, P, shape, Lo, s); });
+  ensure_graphs(W, B, s, max_inner, [&](unsigned c) { return graph_sig(a, A, P, c, fold); },
+                [&] { launch_iteration(a, P, shape, Lo, fold, s); });
   // one large chunk beyond the cap: the reliable update that follows it
   run_chunks(W, B, max_inner + B.g[0].chunk, true, status);
 }
@@ -1572,6 +1794,14 @@ int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_p
   });
 }
 
+int dfm2h_pcg_set_scalar_fold(dfm2h_ilu P, int mode) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null handle");
+    DFM2H_REQUIRE(mode >= -1 && mode <= 1, DFM2H_ERR_INVALID, "mode must be -1, 0 or 1");
+    work_of(P).fold_mode = mode;
+  });
+}
+
 int dfm2h_pcg_set_graph_chunks(dfm2h_ilu P, unsigned large, unsigned small) {
   return guard(__func__, [&] {
     DFM2H_REQUIRE(P, DFM2H_ERR_INVALID, "null handle");
@@ -1627,21 +1857,22 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       const int shape = std::max(0, W.shape[solver_kind<T, M>()]);
       const SweepLayout Lo = sweep_layout(A, P, kSweepSplit[shape]);
       auto a = a0;
-      a.upd_part_off = Lo.upd_off;
-      a.xcd = shape_xcd(shape);
+      const bool fold = iteration_args(a, W, shape, Lo);
       PcgScal st{};
       DFM2H_HIP(hipMemcpyAsync(&st, B.S.p, sizeof(PcgScal), hipMemcpyDeviceToHost, W.s));
       DFM2H_HIP(hipStreamSynchronize(W.s));
-      PcgScal tst = st;
-      tst.done = 0;
-      tst.upd = 0;
-      tst.tol = 0.0;
-      tst.delta2 = 0.0;
-      tst.max_nitr = 1 << 30;
-      DFM2H_HIP(hipMemcpyAsync(B.S.p, &tst, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
-      const unsigned gs = slice_grid(a.a_nsl), gv = vec_grid(a.n);
-      const unsigned n3 = Lo.n3;
-      // segments: spmv | alpha | update | (two empty) | fwd | bwd | beta+pupdate
+      PcgScal tst[2] = {st, st};
+      for (PcgScal &t : tst) {
+        t.done = 0;
+        t.upd = 0;
+        t.tol = 0.0;
+        t.delta2 = 0.0;
+        t.max_nitr = 1 << 30;
+      }
+      DFM2H_HIP(hipMemcpyAsync(B.S.p, tst, 2 * sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
+      const unsigned gs = slice_grid(a.a_nsl);
+      // segments: spmv | alpha step | update | (two empty) | fwd | bwd |
+      // beta step + p update
       constexpr int kSeg = 8;
       TimingEvents<kSeg + 1> ev;
       double acc[kSeg] = {0};
@@ -1650,7 +1881,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         hipLaunchKernelGGL((k_pcg_spmv<T, M>), dim3(gs), dim3(kPackBlock), 0, W.s, a);
         DFM2H_HIP(hipEventRecord(ev[1], W.s));
         if constexpr (is_mixed<T, M>())
-          launch_scalar<kScalAlphaMixed>(W.s, a.S, a.partials, gs, a.partials2, gs, a.hist64,
+          launch_scalar<kScalAlphaMixed>(W.s, a.S, a.partials, gs, a.partials5, gs, a.hist64,
                                          a.partials4, gs);
         else
           launch_scalar<kScalAlpha>(W.s, a.S, a.partials, gs, nullptr, 0, a.hist);
@@ -1664,14 +1895,7 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
         DFM2H_HIP(hipEventRecord(ev[6], W.s));
         launch_sweeps_fused(a, P, shape, Lo, W.s, 2);
         DFM2H_HIP(hipEventRecord(ev[7], W.s));
-        if constexpr (is_mixed<T, M>())
-          launch_scalar<kScalBetaMixed>(W.s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3,
-                                         n3, a.hist);
-        else
-          launch_scalar<kScalConvBeta>(W.s, a.S, a.partials2, update_grid(a.a_nsl), a.partials3,
-                                        n3, a.hist);
-        hipLaunchKernelGGL((k_pcg_pupdate<T, M>), dim3(pupdate_grid(a.n)), dim3(kVecBlock), 0, W.s,
-                           a, 1);
+        launch_beta_pupdate(a, fold, W.s);
         DFM2H_HIP(hipEventRecord(ev[8], W.s));
         DFM2H_HIP(hipEventSynchronize(ev[8]));
         if (rep == 0) continue;  // warm-up iteration
@@ -1698,9 +1922,10 @@ int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep, do
       ms[4] = acc[7] / nrep;  // (stop test +) beta step + p update
       ms[5] = acc[1] / nrep;  // alpha step
       ms[6] = (double)t_b2b / nrep;  // SpMV, back to back
-      ms[7] = acc[3] / nrep;  // (no separate decision step: 0)
+      ms[7] = fold ? 1.0 : 0.0;  // the beta step ran folded into the p update
       st.done = 1;
-      DFM2H_HIP(hipMemcpyAsync(B.S.p, &st, sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
+      const PcgScal fin[2] = {st, st};
+      DFM2H_HIP(hipMemcpyAsync(B.S.p, fin, 2 * sizeof(PcgScal), hipMemcpyHostToDevice, W.s));
       DFM2H_HIP(hipStreamSynchronize(W.s));
     };
     if (fp32) {

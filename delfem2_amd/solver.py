@@ -339,6 +339,12 @@ class CPreconditionerILU:
         reference counterpart; changes no result)."""
         N.call("dfm2h_pcg_set_graph_chunks", self.h, int(large), int(small))
 
+    def set_scalar_fold(self, mode: int = -1) -> None:
+        """Beta step of the fused solvers as a separate one-workgroup kernel
+        (0), folded into the p update (1), or by system size (-1, the
+        default); no reference counterpart, changes no result."""
+        N.call("dfm2h_pcg_set_scalar_fold", self.h, int(mode))
+
     def set_sweep_shape(self, fp32: bool, shape: int) -> None:
         """Sweep launch shape of the fused solvers (-1 = autotune)."""
         N.call("dfm2h_pcg_set_shape", self.h, int(bool(fp32)), int(shape))
@@ -470,8 +476,10 @@ def pcg_kernel_times(mat: CMatrixSparse, ilu: CPreconditionerILU, fp32: bool,
     on the solver stream), after a solve of that precision."""
     ms = np.zeros(8)
     N.call("dfm2h_pcg_kernel_times", mat.h, ilu.h, int(bool(fp32)), int(nrep), _hptr(ms))
-    return dict(zip(["spmv", "update", "fwd_sweep", "bwd_sweep", "pupdate", "alpha_step",
-                     "spmv_back_to_back", "unused"], ms.tolist()))
+    out = dict(zip(["spmv", "update", "fwd_sweep", "bwd_sweep", "pupdate", "alpha_step",
+                    "spmv_back_to_back"], ms[:7].tolist()))
+    out["folded"] = bool(ms[7] != 0.0)
+    return out
 
 
 def measure_copy_gbs(nbytes: int = 1 << 30, reps: int = 10) -> float:

@@ -337,6 +337,16 @@ int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_p
  * 16 and 2; small == large gives a fixed chunk.  The chunking changes no
  * result.  Requires 1 <= small <= large <= 1024. */
 int dfm2h_pcg_set_graph_chunks(dfm2h_ilu P, unsigned large, unsigned small);
+/* Where the fused solvers' beta step runs (no reference counterpart): the
+ * step (stop test, beta = (r, Pr)_new / (r, Pr)_old, and the mixed solver's
+ * reliable-update decision) as a one-workgroup kernel over the partial sums
+ * of the r/x update and the sweeps (mode 0), or inside the p update, every
+ * workgroup of which then sums all those partials itself (mode 1: one
+ * launch fewer per iteration, partial reads that grow with the square of
+ * the system size).  -1 (the default): folded while those reads stay below
+ * 24 MiB per iteration (823,875 DOF: 13 MiB).  Both forms give the same
+ * bits. */
+int dfm2h_pcg_set_scalar_fold(dfm2h_ilu P, int mode);
 /* Launch shape of the fused solvers (no reference counterpart): shapes
  * 0..5 set the triangular sweeps' {wavefronts per 21-row slice, blocks per
  * load round} from { {1,8}, {1,4}, {2,8}, {2,4}, {1,16}, {2,16} }; with two
@@ -364,10 +374,12 @@ int dfm2h_pcg_get_shape(dfm2h_ilu P, int fp32, int *shape, const char **name,
  * ms[8] = { SpMV, r/x update with forward level 0, forward levels >= 1,
  * backward levels >= 1 (backward level 0 has no work: its (r, Pr) share is
  * summed in the forward sweep), stop test + beta step + p update, alpha
- * step, the SpMV again timed over nrep launches back to back (no events
- * between kernels: its duration plus the dispatch gap), 0 } (the FP32 reliable-update residual runs inside the SpMV when
- * due, its decision inside the beta step).  The kernels run on the solver's
- * scratch vectors; nothing the caller owns is touched. */
+ * step, the SpMV again timed
+ * over nrep launches back to back (no events between kernels: its duration
+ * plus the dispatch gap), 1 if the beta step ran folded else 0 } (the
+ * FP32 reliable-update residual runs inside the SpMV when due, its decision
+ * inside the beta step).  The kernels run on the solver's scratch vectors;
+ * nothing the caller owns is touched. */
 int dfm2h_pcg_kernel_times(dfm2h_bsr A, dfm2h_ilu P, int fp32, unsigned nrep,
                            double *ms);
 /* Measurement (no reference counterpart): read + write bytes per second

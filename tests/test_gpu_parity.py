@@ -417,6 +417,54 @@ def test_graph_chunking_changes_no_result(gpu):
             G.ilu.set_graph_chunks(*bad)
 
 
+def _iso_problem():
+    """The 4^3 cube plus one node no element uses (clamped, right-hand side set
+    after the BCs): a row with neither lower nor upper blocks."""
+    prob = W.cube(4, sigma=0.5, seed=3, ordering="color4")
+    xyz = np.concatenate([prob.xyz, [0.5, 0.5, 2.0]])
+    flag = np.concatenate([prob.flag, [1, 1, 1]]).astype(np.int32)
+    return W.Problem("iso", xyz, prob.tet, prob.myu, prob.lam, flag, g=prob.g)
+
+
+def test_scalar_fold_changes_no_result(gpu):
+    """The beta step as a one-workgroup kernel (mode 0) or folded into the p
+    update (mode 1, every workgroup summing the partials itself) gives the
+    same bits: FP64 and mixed (both vector precisions) solutions and residual
+    histories, iteration counts -- on a colour-4 cube, a natural-order cube
+    (many levels) and a system with an isolated row (the update kernel's
+    (r, Pr) partials).  The default folds at these sizes; bad modes are
+    rejected."""
+    for prob in (W.cube(12, sigma=0.5, seed=11, ordering="color4"),
+                 W.cube(7, sigma=1.0, seed=5), _iso_problem()):
+        G = DeviceSystem(prob)
+        assert G.assemble()
+        if prob.name == "iso":
+            G.b[-3:] = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device="cuda")
+        assert G.factor()
+        out = {}
+        for mode in (0, 1):
+            G.ilu.set_scalar_fold(mode)
+            x, h, _ = G.solve(1e-10, 5000)
+            res = [x.cpu().numpy(), np.array(h)]
+            for vec64 in (False, True):
+                G.ilu.set_mixed_options(vec64)
+                xm, hm, nm = G.solve_mixed(1e-10)
+                res += [xm.cpu().numpy(), np.array(hm), nm]
+            G.ilu.set_mixed_options(False)
+            out[mode] = res
+        for a, b in zip(out[0], out[1]):
+            if isinstance(a, np.ndarray):
+                assert a.shape == b.shape and np.array_equal(bits(a), bits(b)), prob.name
+            else:
+                assert a == b, prob.name
+        G.ilu.set_scalar_fold(-1)
+        G.solve(1e-10, 5000)
+        assert S.pcg_kernel_times(G.A, G.ilu, fp32=False, nrep=2)["folded"]
+    for bad in (-2, 2):
+        with pytest.raises(Exception):
+            G.ilu.set_scalar_fold(bad)
+
+
 def test_sweep_shapes_and_autotuner(gpu):
     """Every launch shape of the fused solvers gives the oracle's solution
     (FP64 sweep shapes 0/1/4 keep the summation order: same iterations as the

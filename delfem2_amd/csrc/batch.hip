@@ -210,8 +210,8 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_spmv(BArgs<T> a, unsigned n
     const SliceLane L = lane_of<V>(s);
     const RowMeta R = row_meta(a.A, L, s);
     const MeshScal *S = a.M + R.mesh;
-    if (S->done) continue;
-    const bool refine = sizeof(T) == 4 && S->upd;  // wavefront-uniform
+    if (ld_state(&S->done)) continue;
+    const bool refine = sizeof(T) == 4 && ld_state(&S->upd);  // wavefront-uniform
     const unsigned i = R.i;
     double part = 0.0, part2 = 0.0, part3 = 0.0;
     if (i != ~0u) {
@@ -253,9 +253,9 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_update(BArgs<T> a, unsigned
     const unsigned mesh = a.A.mesh[s];
     const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
     const MeshScal *S = a.M + mesh;
-    if (S->done) continue;
-    const double alpha = S->alpha;
-    const bool fold = sizeof(T) == 4 && S->upd;
+    if (ld_state(&S->done)) continue;
+    const double alpha = ld_state(&S->alpha);
+    const bool fold = sizeof(T) == 4 && ld_state(&S->upd);
     double part = 0.0, srp = 0.0;
     T rn = 0;
     if (i != ~0u) {
@@ -364,8 +364,8 @@ kb_pupdate(BArgs<T> a, unsigned nsl, int mode) {
     const unsigned s = s0 + k;
     if (s >= nsl) return;
     const MeshScal *S = a.M + a.A.mesh[s];
-    if (S->done) continue;
-    const double beta = S->beta;
+    if (ld_state(&S->done)) continue;
+    const double beta = ld_state(&S->beta);
     const SliceLane L = lane_of<V>(s);
     const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
     if (i == ~0u) continue;
@@ -434,8 +434,8 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
   const unsigned lane = threadIdx.x & (kWave - 1);
   MeshScal *S = a.M + m;
   constexpr bool kInit = MODE == kBInit || MODE == kBInitMixed;
-  if (!kInit && S->done) return;
-  const bool upd = S->upd != 0;
+  if (!kInit && ld_state(&S->done)) return;
+  const bool upd = ld_state(&S->upd) != 0;
   const unsigned a0 = a.a_mptr[m], a1 = a.a_mptr[m + 1];
   double t = 0.0, t2 = 0.0, t3 = 0.0;
   if (MODE == kBRPr0) {
@@ -491,14 +491,14 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
       mark_done(S, G);
     }
   } else if (MODE == kBAlpha) {
-    S->alpha = S->rPr / t;
+    S->alpha = ld_state(&S->rPr) / t;
   } else if (MODE == kBRPr0) {
     S->rPr = t;
   } else if (MODE == kBConvBeta) {
-    const int it = S->it + 1;
+    const int it = ld_state(&S->it) + 1;
     S->it = it;
-    S->res = sqrt(t * S->inv_sq0);
-    if (S->res < G->tol) {
+    S->res = sqrt(t * ld_state(&S->inv_sq0));
+    if (ld_state(&S->res) < G->tol) {
       S->converged = 1;
       mark_done(S, G);
     } else if (bad) {
@@ -507,22 +507,22 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
     } else if (it >= G->max_nitr) {
       mark_done(S, G);
     } else {
-      S->beta = t2 / S->rPr;
+      S->beta = t2 / ld_state(&S->rPr);
       S->rPr = t2;
     }
   } else if (MODE == kBAlphaMixed) {
     if (upd) {
-      const int k = S->nupd + 1;
+      const int k = ld_state(&S->nupd) + 1;
       S->nupd = k;
       S->rmax2 = t2;
-      S->res = sqrt(t2 / S->sq_b);
-      if (S->res < G->tol) {
+      S->res = sqrt(t2 / ld_state(&S->sq_b));
+      if (ld_state(&S->res) < G->tol) {
         S->converged = 1;
         mark_done(S, G);
       } else if (!(t2 == t2) || t2 > 1.0e308) {
         S->nan = 1;
         mark_done(S, G);
-      } else if (S->final_ || k >= G->max_upd) {
+      } else if (ld_state(&S->final_) || k >= G->max_upd) {
         mark_done(S, G);
       }
     }
@@ -530,10 +530,10 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
       S->alpha = 0.0;
       S->restart = 1;
     } else {
-      S->alpha = S->rPr / t;
+      S->alpha = ld_state(&S->rPr) / t;
     }
   } else {  // kBBetaMixed
-    const int it = S->it + 1;
+    const int it = ld_state(&S->it) + 1;
     S->it = it;
     S->upd = 0;
     if (bad) {
@@ -541,13 +541,13 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
       mark_done(S, G);
       return;
     }
-    if (t < G->delta2 * S->rmax2 || sqrt(t / S->sq_b) < G->tol || it >= G->max_nitr ||
-        (G->period > 0 && it - S->it_upd >= G->period)) {
+    if (t < G->delta2 * ld_state(&S->rmax2) || sqrt(t / ld_state(&S->sq_b)) < G->tol || it >= G->max_nitr ||
+        (G->period > 0 && it - ld_state(&S->it_upd) >= G->period)) {
       S->upd = 1;
       S->final_ = it >= G->max_nitr ? 1 : 0;
       S->it_upd = it;
     }
-    S->beta = S->restart ? 0.0 : t2 / S->rPr;
+    S->beta = ld_state(&S->restart) ? 0.0 : t2 / ld_state(&S->rPr);
     S->restart = 0;
     S->rPr = t2;
   }

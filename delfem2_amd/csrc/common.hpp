@@ -259,6 +259,17 @@ __device__ inline void reduce_partial_sets(const double *const (&p)[NS], const u
   partial_sums_block<BS, NS>(s, t);
 }
 
+// Solver state written by earlier kernels (flags, step lengths) is read with
+// vector loads at agent scope, never through the scalar data cache: a
+// uniform load of it can otherwise return a scalar-cache line filled before
+// the write (measured: the mixed PCG's one-workgroup steps, loading the
+// whole state that way, saw stale restart / rmax2 fields and took other
+// iterates).
+template <typename V>
+__device__ __attribute__((always_inline)) inline V ld_state(const V *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace dfm2h

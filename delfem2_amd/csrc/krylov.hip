@@ -84,7 +84,7 @@ template <int STEP>
 __global__ void __launch_bounds__(kScalThreads)
 k_bicg_scalar(BicgScal *S, const double *__restrict__ pa, unsigned na,
               const double *__restrict__ pb, unsigned nb, double *hist) {
-  if (STEP != kBicgStart && S->done) return;
+  if (STEP != kBicgStart && ld_state(&S->done)) return;
   const double *const ps[2] = {pa, pb};
   const unsigned ns[2] = {na, STEP == kBicgOmega || STEP == kBicgNext ? nb : 0u};
   double t[2];
@@ -102,30 +102,30 @@ k_bicg_scalar(BicgScal *S, const double *__restrict__ pa, unsigned na,
     } else if (!(t[0] == t[0]) || t[0] > 1.0e308) {
       S->done = 1;
       S->nan = 1;
-    } else if (S->max_it <= 0) {
+    } else if (ld_state(&S->max_it) <= 0) {
       S->done = 1;
     }
     S->inv_sq0 = 1.0 / t[0];
     S->rho = t[0];  // (r, r^) with r^ = r
   } else if (STEP == kBicgAlpha) {  // t[0] = (q, r^)
-    S->alpha = S->rho / t[0];
+    S->alpha = ld_state(&S->rho) / t[0];
   } else if (STEP == kBicgOmega) {  // t[0] = (t, t), t[1] = (s, t)
     S->omega = t[1] / t[0];
   } else {  // t[0] = (r, r), t[1] = (r, r^)
-    const int it = S->it + 1;
+    const int it = ld_state(&S->it) + 1;
     S->it = it;
-    const double ratio = sqrt(t[0] * S->inv_sq0);
+    const double ratio = sqrt(t[0] * ld_state(&S->inv_sq0));
     hist[it - 1] = ratio;
-    if (ratio < S->tol) {
+    if (ratio < ld_state(&S->tol)) {
       S->done = 1;
       S->converged = 1;
     } else if (!(ratio == ratio) || ratio > 1.0e308) {
       S->done = 1;
       S->nan = 1;
-    } else if (it >= S->max_it) {
+    } else if (it >= ld_state(&S->max_it)) {
       S->done = 1;
     } else {
-      S->beta = t[1] * S->alpha / (S->rho * S->omega);
+      S->beta = t[1] * ld_state(&S->alpha) / (ld_state(&S->rho) * ld_state(&S->omega));
       S->rho = t[1];
     }
   }
@@ -137,7 +137,7 @@ k_bicg_scalar(BicgScal *S, const double *__restrict__ pa, unsigned na,
 template <bool TWO>
 __global__ void __launch_bounds__(kPackBlock)
 k_bicg_matvec(BicgArgs a, const double *z, double *y, const double *u, const double *w) {
-  if (a.S->done) return;
+  if (ld_state(&a.S->done)) return;
   const SliceLane L = slice_lane(0);
   double s0 = 0.0, s1 = 0.0;
   if (L.s < a.nsl && L.c < 3) {
@@ -169,8 +169,8 @@ k_bicg_matvec(BicgArgs a, const double *z, double *y, const double *u, const dou
 
 // s = r; s += -alpha q
 __global__ void __launch_bounds__(kKryBlock) k_bicg_s(BicgArgs a) {
-  if (a.S->done) return;
-  const double nalpha = -a.S->alpha;
+  if (ld_state(&a.S->done)) return;
+  const double nalpha = -ld_state(&a.S->alpha);
   for (size_t i = (size_t)blockIdx.x * kKryBlock + threadIdx.x; i < a.n;
        i += (size_t)gridDim.x * kKryBlock)
     a.s[i] = a.r[i] + nalpha * a.q[i];
@@ -179,8 +179,8 @@ __global__ void __launch_bounds__(kKryBlock) k_bicg_s(BicgArgs a) {
 // x += alpha z1; x += omega z2; r = s; r += -omega t; partials (r, r) and
 // (r, r^)
 __global__ void __launch_bounds__(kKryBlock) k_bicg_xr(BicgArgs a) {
-  if (a.S->done) return;
-  const double alpha = a.S->alpha, omega = a.S->omega;
+  if (ld_state(&a.S->done)) return;
+  const double alpha = ld_state(&a.S->alpha), omega = ld_state(&a.S->omega);
   double srr = 0.0, srs = 0.0;
   for (size_t i = (size_t)blockIdx.x * kKryBlock + threadIdx.x; i < a.n;
        i += (size_t)gridDim.x * kKryBlock) {
@@ -202,8 +202,8 @@ __global__ void __launch_bounds__(kKryBlock) k_bicg_xr(BicgArgs a) {
 
 // p *= beta; p += 1.0 r; p += (-beta omega) q
 __global__ void __launch_bounds__(kKryBlock) k_bicg_p(BicgArgs a) {
-  if (a.S->done) return;
-  const double beta = a.S->beta, bw = -beta * a.S->omega;
+  if (ld_state(&a.S->done)) return;
+  const double beta = ld_state(&a.S->beta), bw = -beta * ld_state(&a.S->omega);
   for (size_t i = (size_t)blockIdx.x * kKryBlock + threadIdx.x; i < a.n;
        i += (size_t)gridDim.x * kKryBlock) {
     double pi = a.p[i] * beta;

@@ -527,7 +527,7 @@ template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
 k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val,
              const T *__restrict__ dia, const T *src, T *vec, const int *done) {
-  if (done && *done) return;
+  if (done && ld_state(done)) return;
   const SliceLane L = slice_lane(slice0);
   if (L.s >= slice0 + nsl) return;  // whole wavefront
   const unsigned i = L.c < 3 ? P.row[L.s * kSliceRows + L.r] : ~0u;
@@ -541,7 +541,7 @@ template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
 k_sliced_bwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val, T *vec,
              const int *done) {
-  if (done && *done) return;
+  if (done && ld_state(done)) return;
   const SliceLane L = slice_lane(slice0);
   if (L.s >= slice0 + nsl || L.c >= 3) return;
   const unsigned i = P.row[L.s * kSliceRows + L.r];

@@ -81,16 +81,6 @@ __device__ __attribute__((always_inline)) inline void copy_state(PcgScal &d, con
   d.pad3 = s.pad3;
 }
 
-// State fields written by earlier kernels are read with vector loads at
-// agent scope, never through the scalar data cache: a uniform load of them
-// can otherwise return a scalar-cache line filled before the write
-// (measured: the mixed solver's one-workgroup steps then saw stale
-// restart / rmax2 fields and took other iterates).
-template <typename V>
-__device__ __attribute__((always_inline)) inline V ld_state(const V *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __attribute__((always_inline)) inline void load_state(PcgScal &d, const PcgScal *s) {
   d.rPr = ld_state(&s->rPr);
   d.alpha = ld_state(&s->alpha);

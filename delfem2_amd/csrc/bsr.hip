@@ -700,6 +700,102 @@ k_assemble_diag_newmark(AsmArgs A, const unsigned *__restrict__ ptr,
   if (live && l >= 9 && l < 12) A.vec_b[(size_t)i * 3 + (l - 9)] = sum;
 }
 
+// The stiffness-warped diagonal blocks and right-hand side with four lanes
+// per element, one per element node jn: lane (element, jn) forms E = R_i
+// K_e(li, jn) R_i^T, the node's u1 = P_j + u_j + dt v_j - R_i P_j and the
+// nine products E[a][k] u1[k] the right-hand side subtracts, and the lane
+// with jn = li also the element's diagonal block (that E plus the lumped
+// mass).  Three lanes per element then subtract the products in the order
+// of element_rhs_term (node by node, component by component) and divide by
+// dt, and the diagonal block and vec_b receive the elements in ascending
+// order.  Same values as k_assemble_diag_lists<kAsmStiffwarp>, with a
+// quarter of its per-lane work.
+__global__ void __launch_bounds__(64, 4)
+k_assemble_diag_stiffwarp(AsmArgs A, const unsigned *__restrict__ ptr,
+                          const unsigned *__restrict__ ent, const double *__restrict__ rec) {
+  constexpr unsigned EPR = kDiagLanes / 4;  // elements per round
+  __shared__ double prod[64][10];           // 9 products per lane, padded
+  __shared__ double dblk[2 * EPR][10];      // per element of the round: its block, its volume
+  __shared__ double eres[2 * EPR][3];       // per element of the round: its vec_b addend
+  const unsigned lane = threadIdx.x, grp = lane / kDiagLanes, g0 = grp * kDiagLanes;
+  const unsigned l = lane & (kDiagLanes - 1u), u = l / 4, jn = l % 4;
+  const unsigned i = wg_id(true) * (64 / kDiagLanes) + grp;
+  const bool live = i < A.nblk;
+  const size_t t = A.ncrs + (live ? i : 0u);
+  const unsigned q0 = live ? ptr[t] : 0u, q1 = live ? ptr[t + 1] : 0u;
+  double sum = 0.0;  // lane l < 9: diagonal entry l; 9..11: vec_b component l - 9
+  if (live && l < 9) sum = A.vdia[(size_t)i * 9 + l];
+  if (live && l >= 9 && l < 12) sum = A.vec_b[(size_t)i * 3 + (l - 9)];
+  double Ri[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Ri[k] = live ? A.rot[(size_t)i * 9 + k] : 0.0;
+  unsigned rounds = q1 - q0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    rounds = max(rounds, (unsigned)__shfl_xor((int)rounds, off, 64));
+  rounds = (rounds + EPR - 1) / EPR;
+  const unsigned *const tet_nodes = reinterpret_cast<const unsigned *>(A.tet);
+  for (unsigned c = 0; c < rounds; ++c) {
+    const unsigned qb = q0 + c * EPR, q = qb + u;
+    if (q < q1) {
+      const unsigned en = ent[q];
+      const unsigned e = en & 0x0FFFFFFFu;
+      const unsigned li = (en >> 28) & 3u;
+      const unsigned node = tet_nodes[(size_t)e * 4 + jn];
+      const double *r = rec + (size_t)e * kTetRec;
+      const double vol = r[0], myu = r[13], lam = r[14];
+      const double di[3] = {r[1 + 3 * li], r[2 + 3 * li], r[3 + 3 * li]};
+      const double dj[3] = {r[1 + 3 * jn], r[2 + 3 * jn], r[3 + 3 * jn]};
+      double kb[9], E[9], xu[3], xv[3], Pj[3];
+      emat_block<true>(kb, di, dj, vol, myu, lam);
+      rotate_block(E, kb, Ri);
+      load_node3(xu, A.disp, node);
+      load_node3(xv, A.velo, node);
+      load_node3(Pj, A.xyz, node);
+      const double Pj1[3] = {Ri[0] * Pj[0] + Ri[1] * Pj[1] + Ri[2] * Pj[2],
+                             Ri[3] * Pj[0] + Ri[4] * Pj[1] + Ri[5] * Pj[2],
+                             Ri[6] * Pj[0] + Ri[7] * Pj[1] + Ri[8] * Pj[2]};
+      const double uj1[3] = {Pj[0] + xu[0] + A.dt * xv[0] - Pj1[0],
+                             Pj[1] + xu[1] + A.dt * xv[1] - Pj1[1],
+                             Pj[2] + xu[2] + A.dt * xv[2] - Pj1[2]};
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) prod[lane][3 * a + k] = E[3 * a + k] * uj1[k];
+      if (jn == li) {  // element_block<kAsmStiffwarp> of the block (li, li)
+        const double m = A.rho * vol * 0.25 / (A.dt * A.dt);
+        E[0] += m;
+        E[4] += m;
+        E[8] += m;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dblk[grp * EPR + u][k] = E[k];
+        dblk[grp * EPR + u][9] = vol;
+      }
+    }
+    __syncthreads();
+    if (l < 3 * EPR && qb + l / 3 < q1) {  // element k = l / 3, row a = l % 3: its eres_a
+      const unsigned k = l / 3, a = l % 3, b = g0 + 4 * k;
+      const double g = a == 0 ? A.g0 : a == 1 ? A.g1 : A.g2;
+      double er = dblk[grp * EPR + k][9] * A.rho * g * 0.25;
+      for (int j = 0; j < 4; ++j) {
+        er -= prod[b + j][3 * a + 0];
+        er -= prod[b + j][3 * a + 1];
+        er -= prod[b + j][3 * a + 2];
+      }
+      eres[grp * EPR + k][a] = er / A.dt;
+    }
+    __syncthreads();
+    if (l < 12 && qb < q1) {  // the elements in ascending order
+      const unsigned m = min(q1 - qb, EPR);
+      for (unsigned k = 0; k < m; ++k)
+        sum += l < 9 ? dblk[grp * EPR + k][l] : eres[grp * EPR + k][l - 9];
+    }
+    __syncthreads();
+  }
+  if (live && l < 9) A.vdia[(size_t)i * 9 + l] = sum;
+  if (live && l >= 9 && l < 12) A.vec_b[(size_t)i * 3 + (l - 9)] = sum;
+}
+
 // SetFixedBC (ls_block_sparse.cpp:397-472): entries in a fixed row or a
 // fixed column become 0; the diagonal entry of a fixed DOF becomes 1.
 __global__ void k_set_fixed_bc(unsigned nblk, size_t ncrs, const unsigned *__restrict__ rowof,
@@ -934,8 +1030,10 @@ static void run_assembly(int kind, dfm2h_bsr A, double *vec_b, const double *myu
       DFM2H_ASM_OFFDIAG(kAsmNewmark);
       hipLaunchKernelGGL(k_assemble_diag_newmark, gd, bd, 0, stream(), a, pp, pe, pr);
     }
-    else if (kind == kAsmStiffwarp)
-      DFM2H_ASM_LISTS(kAsmStiffwarp);
+    else if (kind == kAsmStiffwarp) {
+      DFM2H_ASM_OFFDIAG(kAsmStiffwarp);
+      hipLaunchKernelGGL(k_assemble_diag_stiffwarp, gd, bd, 0, stream(), a, pp, pe, pr);
+    }
     else if (kind == kAsmBEuler)
       DFM2H_ASM_LISTS(kAsmBEuler);
     else

@@ -328,3 +328,39 @@ def test_values_changed_reaches_the_solver_copies(gpu):
     x2, h2, _ = G.solve(1e-10, 5000)
     assert np.array_equal(bits(np.array(h1)), bits(np.array(h2)))
     assert np.array_equal(bits((x1 * 0.5).cpu().numpy()), bits(x2.cpu().numpy()))
+
+
+def test_copy_value_after_matrix_reinitialized(gpu):
+    """A second SetPattern on a matrix is refused (the reference asserts an
+    empty pattern, ls_block_sparse.h:77); Initialize and SetPattern again
+    (a new handle, possibly at the old address) between two CopyValue calls:
+    the second CopyValue maps A's new slots onto P's unchanged ILU(0)
+    pattern, copying the blocks both patterns share and zeroing the rest
+    (ls_ilu_block_sparse.cpp:1009-1047 rebuilds its map from the matrix's
+    current pattern on every call)."""
+    p1 = W.cube(5, sigma=0.5, seed=71)
+    p2 = W.cube(5, sigma=0.5, seed=71, ordering="color4")
+    ind1, ps1 = host_pattern(p1)
+    ind2, ps2 = host_pattern(p2)
+    A = matrix(ind1, ps1, p1.np_)
+    A.upload(W.uniform(72, ps1.size * 9), W.uniform(73, p1.np_ * 9))
+    P = S.CPreconditionerILU()
+    P.SetPattern0(A)
+    P.CopyValue(A)
+    with pytest.raises(N.DFM2HError):
+        A.SetPattern(ind2, ps2)
+    A.Initialize(p2.np_, 3, True)
+    A.SetPattern(ind2, ps2)
+    vc2 = W.uniform(74, ps2.size * 9)
+    vd2 = W.uniform(75, p2.np_ * 9)
+    A.upload(vc2, vd2)
+    P.CopyValue(A)
+    lrow, _, lc, ld = P.download()
+    rows2 = np.repeat(np.arange(p2.np_), np.diff(ind2.astype(np.int64)))
+    dstP = slot_lookup(ind1, lrow, p1.np_)(rows2, ps2.astype(np.int64))
+    exp = np.zeros(ps1.size * 9).reshape(-1, 9)
+    hit = dstP >= 0
+    exp[dstP[hit]] = vc2.reshape(-1, 9)[hit]
+    assert np.any(~hit) and np.any(hit)
+    assert np.array_equal(bits(lc), bits(exp.ravel()))
+    assert np.array_equal(bits(ld), bits(vd2))

@@ -33,7 +33,11 @@ namespace dfm2h {
 
 constexpr int kKryBlock = 256;
 constexpr unsigned kKryGridMax = 2048;
-constexpr unsigned kChunk = 8;  // iterations per graph launch
+// iterations per graph launch: the host polls one graph behind, so up to
+// two chunks run past convergence as immediate returns (24 launches each):
+// 2 keeps that below 0.1 ms, and a chunk of 2 iterations (~0.4 ms at
+// 823,875 DOF) still hides the host's launch and poll
+constexpr unsigned kChunk = 2;
 constexpr int kScalThreads = 1024;
 
 static unsigned kry_grid(size_t n) {

@@ -354,8 +354,8 @@ void ilu_forward_levels_double(dfm2h_ilu P, const double *src, double *vec, hipS
   for (size_t l = 0; l + 1 < P->fplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->fplan.group_slices(l);
     hipLaunchKernelGGL((k_sliced_fwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
-                       P->fplan.group_ptr[l], nsl, view(P->fplan), P->f64.val.p, P->f64.dia.p,
-                       src, vec, done);
+                       P->fplan.group_ptr[l], nsl, view(P->fplan), P->fplan.gview[l],
+                       P->f64.val.p, P->f64.dia.p, src, vec, done);
   }
   check_launch("k_sliced_fwd");
 }
@@ -365,7 +365,8 @@ void ilu_backward_levels_double(dfm2h_ilu P, double *vec, hipStream_t s, const i
   for (size_t l = 0; l + 1 < P->bplan.group_ptr.size(); ++l) {
     const unsigned nsl = P->bplan.group_slices(l);
     hipLaunchKernelGGL((k_sliced_bwd<double, 3>), dim3(slice_grid(nsl)), dim3(kPackBlock), 0, s,
-                       P->bplan.group_ptr[l], nsl, view(P->bplan), P->b64.val.p, vec, done);
+                       P->bplan.group_ptr[l], nsl, view(P->bplan), P->bplan.gview[l],
+                       P->b64.val.p, vec, done);
   }
   check_launch("k_sliced_bwd");
 }

@@ -472,15 +472,11 @@ __device__ inline RowSlots row_slots(const PlanView &P, const GroupView &g, cons
   return R;
 }
 
-// blocks per load round (measured on MI355X, 64^3 cube, 4-colour order:
-// SpMV 4 for FP64 / 8 for FP32; sweeps 8 -- 16 was 12-20% slower per level)
+// blocks per load round of the plain MatVec (measured on MI355X, 64^3
+// cube, 4-colour order: 4 for FP64 / 8 for FP32)
 template <typename T>
 constexpr int unroll_spmv() {
   return sizeof(T) == 8 ? 4 : 8;
-}
-template <typename T>
-constexpr int unroll_sweep() {
-  return 8;
 }
 
 // y = alpha*A*x + beta*y over the slices [slice0, slice0+nsl) (all rows)
@@ -501,54 +497,62 @@ k_sliced_matvec(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__
   y[VS * (size_t)i + L.c] = acc;
 }
 
-// forward sweep of one level: vec_i = D_i^-1 (src_i - sum L_ij vec_j)
+// forward sweep of one level: vec_i = D_i^-1 (src_i - sum L_ij vec_j), over
+// the level's slices [slice0, slice0 + nsl) with its group shape gv (the
+// row and slot offsets computed instead of loaded when the group is
+// uniform); the row's own entry, its D^-1 row and the first round of
+// blocks are requested together.  done (nullable): the kernel returns at
+// once when *done is set (a fused solver's convergence flag; read first, so
+// that launches after convergence cost only their launch).  Products and
+// order of sliced_row_sum_m.  4 blocks per load round (measured at 64^3:
+// SolvePrecond 95.4 us with the previous form, 93.1 with 8, 91.4 with 4).
 template <typename T, int VS>
-__device__ inline void sliced_fwd_row(const SliceLane &L, bool valid, unsigned i, const PlanView &P,
-                                      const T *__restrict__ val, const T *__restrict__ dia,
-                                      const T *src, T *vec, T &out) {
+__global__ void __launch_bounds__(kPackBlock)
+k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, GroupView gv, const T *__restrict__ val,
+             const T *__restrict__ dia, const T *src, T *vec, const int *done) {
+  constexpr int U = 4;
+  if (done && ld_state(done)) return;
+  const SliceLane L = slice_lane(slice0);
+  const bool sl = L.s < slice0 + nsl;
+  if (!sl) return;  // whole wavefront
+  const RowSlots R = row_slots(P, gv, L, slice0, sl);
+  const unsigned i = R.i;
   T t = 0;
-  if (valid) {
+  Trip<T> d{};
+  SlotRound<T, U> R0;
+  if (i != ~0u) {
     t = src[VS * (size_t)i + L.c];
-    t = sliced_row_sum<T, true, unroll_sweep<T>(), VS>(
-        t, (T)1, L, P.len[L.s * kSliceRows + L.r], P.kmax[L.s], P.off[L.s], P.col, val, vec);
+    d = load_trip<T>(dia, L.s, L.lane);
+    load_slot_round<T, U, 1>(R0, L, 0, R.kl, R.off, P.col, val);
   }
+  if (i != ~0u)
+    t = sliced_row_sum_pre<T, T, true, U, 1, VS>(t, (T)1, R0, L, 0, R.len, R.kl, R.kmax, R.off,
+                                                 P.col, val, vec);
   const T t0 = __shfl(t, (int)L.r, kWave);
   const T t1 = __shfl(t, (int)L.r + kSliceRows, kWave);
   const T t2 = __shfl(t, (int)L.r + 2 * kSliceRows, kWave);
-  if (valid) {
-    out = dia_row_dot(dia, L, t0, t1, t2);
-    vec[VS * (size_t)i + L.c] = out;
-  }
+  if (i != ~0u) vec[VS * (size_t)i + L.c] = d.a * t0 + d.b * t1 + d.c * t2;
 }
 
-// done (nullable): the kernel returns at once when *done is set (a fused
-// solver's convergence flag)
+// backward sweep of one level: vec_i -= sum U_ij vec_j (U scaled by D^-1);
+// same structure as k_sliced_fwd
 template <typename T, int VS>
 __global__ void __launch_bounds__(kPackBlock)
-k_sliced_fwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val,
-             const T *__restrict__ dia, const T *src, T *vec, const int *done) {
+k_sliced_bwd(unsigned slice0, unsigned nsl, PlanView P, GroupView gv, const T *__restrict__ val,
+             T *vec, const int *done) {
+  constexpr int U = 4;
   if (done && ld_state(done)) return;
   const SliceLane L = slice_lane(slice0);
-  if (L.s >= slice0 + nsl) return;  // whole wavefront
-  const unsigned i = L.c < 3 ? P.row[L.s * kSliceRows + L.r] : ~0u;
-  const bool valid = i != ~0u;
-  T out;
-  sliced_fwd_row<T, VS>(L, valid, i, P, val, dia, src, vec, out);
-}
-
-// backward sweep of one level: vec_i -= sum U_ij vec_j (U scaled by D^-1)
-template <typename T, int VS>
-__global__ void __launch_bounds__(kPackBlock)
-k_sliced_bwd(unsigned slice0, unsigned nsl, PlanView P, const T *__restrict__ val, T *vec,
-             const int *done) {
-  if (done && ld_state(done)) return;
-  const SliceLane L = slice_lane(slice0);
-  if (L.s >= slice0 + nsl || L.c >= 3) return;
-  const unsigned i = P.row[L.s * kSliceRows + L.r];
+  const bool sl = L.s < slice0 + nsl;
+  if (!sl) return;  // whole wavefront
+  const RowSlots R = row_slots(P, gv, L, slice0, sl);
+  const unsigned i = R.i;
   if (i == ~0u) return;
   T t = vec[VS * (size_t)i + L.c];
-  t = sliced_row_sum<T, true, unroll_sweep<T>(), VS>(
-      t, (T)1, L, P.len[L.s * kSliceRows + L.r], P.kmax[L.s], P.off[L.s], P.col, val, vec);
+  SlotRound<T, U> R0;
+  load_slot_round<T, U, 1>(R0, L, 0, R.kl, R.off, P.col, val);
+  t = sliced_row_sum_pre<T, T, true, U, 1, VS>(t, (T)1, R0, L, 0, R.len, R.kl, R.kmax, R.off,
+                                               P.col, val, vec);
   vec[VS * (size_t)i + L.c] = t;
 }
 

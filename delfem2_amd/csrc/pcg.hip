@@ -593,10 +593,16 @@ inline unsigned update_grid(unsigned nsl) {
 
 template <typename T, typename M>
 __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
-  if (ld_state(&a.S->done)) return;
-  const double alpha = ld_state(&a.S->alpha);
-  // an update just ran
-  const bool fold = sizeof(T) == 4 && is_mixed<T, M>() && ld_state(&a.S->upd);
+  // the state: before the rows' loads in FP64, after them in the mixed
+  // solver (measured at 64^3: mixed solve 59.0 -> 58.7 ms with it after,
+  // FP64 103.2 -> 103.4 ms)
+  constexpr bool kLate = is_mixed<T, M>();
+  double alpha = 0.0;
+  bool fold = false;
+  if constexpr (!kLate) {
+    if (ld_state(&a.S->done)) return;
+    alpha = ld_state(&a.S->alpha);
+  }
   // no gathers here: the dispatch order serves (an XCD-ordered numbering
   // measured slower on MI355X: FP64 64^3 14.2 -> 16.9 us)
   const unsigned wv = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
@@ -620,6 +626,11 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
       x0[h] = a.x[e];
       p0[h] = a.p[e];
     }
+  }
+  if constexpr (kLate) {
+    if (ld_state(&a.S->done)) return;
+    alpha = ld_state(&a.S->alpha);
+    fold = sizeof(T) == 4 && ld_state(&a.S->upd);  // an update just ran: fold x32 into x64
   }
   double s2 = 0.0, srp = 0.0;
 #pragma unroll

@@ -218,8 +218,13 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_spmv(BArgs<T> a, unsigned n
       // the row's own node and diagonal block first: off the gathers' chain
       const Trip<T> pi = load_node<T, kVSB>(a.p, i);
       const Trip<T> d = load_trip<T>(a.a_dia, s, L.lane);
-      T acc = sliced_row_sum<T, false, V::template US<T>(), kVSB>(
-          (T)0, (T)1, L, R.len, R.kmax, R.off, a.A.col, a.a_val, a.p);
+      T acc;
+      if constexpr (sizeof(T) == 4)  // FP32 copy: difference form, d = row-sum block
+        acc = sliced_row_diff_sum<T, T, V::template US<T>(), kVSB>(
+            (T)0, pi, L, R.len, R.len, R.kmax, R.off, a.A.col, a.a_val, a.p);
+      else
+        acc = sliced_row_sum<T, false, V::template US<T>(), kVSB>(
+            (T)0, (T)1, L, R.len, R.kmax, R.off, a.A.col, a.a_val, a.p);
       acc += (T)d.a * pi.a + (T)d.b * pi.b + (T)d.c * pi.c;  // dia_row_dot's products
       const size_t e = kVSB * (size_t)i + L.c;
       a.Pr[e] = acc;
@@ -818,7 +823,11 @@ void ensure_batch_vals(dfm2h_batch B, PackedVals<T> &av, PackedVals<T> &fv, Pack
                        PackedVals<T> &dv) {
   dfm2h_bsr A = B->A;
   dfm2h_ilu P = B->P;
-  ensure_packed(B->aplan, av, A->vcrs.p, A->vdia.p, A->version, B->s);
+  if (sizeof(T) == 4)  // the FP32 copy in the difference form (kb_spmv)
+    ensure_packed_rowsum(B->aplan, av, A->vcrs.p, A->vdia.p, A->pat.colind_d.p, A->version,
+                         B->s);
+  else
+    ensure_packed(B->aplan, av, A->vcrs.p, A->vdia.p, A->version, B->s);
   ensure_packed(B->fplan, fv, P->vcrs.p, P->vdia.p, P->version, B->s);
   ensure_packed(B->bplan, bv, P->vcrs.p, nullptr, P->version, B->s);
   // D^-1 of the forward-level-0 rows in the matrix-slice layout

@@ -61,6 +61,28 @@ void ensure_packed(const PackedPlan &P, PackedVals<T> &V, const double *crs, con
   V.version = version;
 }
 
+// The same copy in the difference form of the row sums: off-diagonal blocks
+// as ensure_packed, the row-sum blocks D_i + sum_j A_ij (rowptr: the CRS row
+// starts, on the device) in the diagonal slot.
+template <typename T>
+void ensure_packed_rowsum(const PackedPlan &P, PackedVals<T> &V, const double *crs,
+                          const double *dia, const unsigned *rowptr, uint64_t version,
+                          hipStream_t s) {
+  if (V.version == version && (V.val.p || P.nq == 0)) return;
+  V.val.alloc(P.nq * kSliceVals);
+  if (P.nq) {
+    hipLaunchKernelGGL(k_pack_blocks<T>, dim3(pack_grid(P.nq)), dim3(256), 0, s, P.nq, P.src.p,
+                       crs, V.val.p);
+  }
+  V.dia.alloc((size_t)P.nslices * kSliceVals);
+  if (P.nslices) {
+    hipLaunchKernelGGL(k_pack_rowsum<T>, dim3(pack_grid(P.nslices)), dim3(256), 0, s, P.nslices,
+                       P.row.p, rowptr, crs, dia, V.dia.p);
+  }
+  check_launch("k_pack_rowsum");
+  V.version = version;
+}
+
 // Groups rows by dependency level.  Forward: lev(i) = 1 + max lev(j) over
 // the lower part of row i (0 when empty); backward: the same over the upper
 // part, rows visited from the last one.  Rows inside a level stay ascending.

@@ -194,6 +194,35 @@ __global__ void k_pack_dia(unsigned nslices, const unsigned *__restrict__ row,
   put_lane(dia, s, lane, i != ~0u ? dia_crs + (size_t)i * 9 + c * 3 : nullptr);
 }
 
+// Row-sum blocks of the difference form (sliced_row_diff_sum): in the
+// diagonal slot of row i, S_i = D_i + sum_j A_ij over the row's CRS blocks
+// [rowptr[i], rowptr[i+1]), summed in FP64 and rounded once to T.
+template <typename T>
+__global__ void k_pack_rowsum(unsigned nslices, const unsigned *__restrict__ row,
+                              const unsigned *__restrict__ rowptr,
+                              const double *__restrict__ crs, const double *__restrict__ dia_crs,
+                              T *__restrict__ dia) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // over nslices*63
+  if (t >= (size_t)nslices * 63) return;
+  const size_t s = t / 63;
+  const unsigned lane = (unsigned)(t - s * 63);
+  const unsigned c = lane / kSliceRows, r = lane % kSliceRows;
+  const unsigned i = row[s * kSliceRows + r];
+  if (i == ~0u) {
+    put_lane(dia, s, lane, (const double *)nullptr);
+    return;
+  }
+  const double *d = dia_crs + (size_t)i * 9 + c * 3;
+  double v[3] = {d[0], d[1], d[2]};
+  for (unsigned q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+    const double *b = crs + (size_t)q * 9 + c * 3;
+    v[0] += b[0];
+    v[1] += b[1];
+    v[2] += b[2];
+  }
+  put_lane(dia, s, lane, v);
+}
+
 // ---------------------------------------------------------------------------
 // row kernels
 
@@ -334,6 +363,55 @@ __device__ inline T sliced_row_sum_m(T acc, T alpha, const SliceLane &L, unsigne
           acc += alpha * d;
       }
     }
+  }
+  return acc;
+}
+
+// Difference form of a row sum (the mixed solver's FP32 copy of A):
+//   acc + sum_k ((v0*(x0-o0) + v1*(x1-o1)) + v2*(x2-o2))
+// over the row's off-diagonal blocks, o = the row's own node values.  The
+// diagonal slot then holds the row-sum block S_i = D_i + sum_j A_ij
+// (k_pack_rowsum), and acc + S_i o is (A x)_i.  A stiffness matrix maps
+// rigid translations to zero, so on smooth vectors (A x)_i is a small
+// difference of large products: in this form the rounding of the stored
+// blocks to FP32 multiplies x_j - x_i instead of x_j, and S_i (near zero
+// away from fixed nodes) is rounded once from its FP64 sum.  Same loads and
+// load rounds as sliced_row_sum_m.
+template <typename T, typename M, int U, int VS>
+__device__ inline T sliced_row_diff_sum(T acc, const Trip<T> &o, const SliceLane &L, unsigned K,
+                                        unsigned KL, unsigned kmax, unsigned q0,
+                                        const unsigned *__restrict__ col,
+                                        const M *__restrict__ val, const T *x) {
+  for (unsigned k0 = 0; k0 < kmax; k0 += U) {
+    unsigned j[U];
+    Trip<M> v[U];
+    Trip<T> xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u < KL) {
+        const size_t q = q0 + k0 + u;
+        j[u] = col[q * kSliceRows + L.r];
+        v[u] = load_trip<M>(val, q, L.lane);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + u < KL) xv[u] = load_node<T, VS>(x, j[u]);
+    // the differences in a pass of their own, after every gather of the
+    // round is requested (formed inside each product, the compiler issued
+    // the gathers one at a time)
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u < K) {
+        xv[u].a -= o.a;
+        xv[u].b -= o.b;
+        xv[u].c -= o.c;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + u < K)
+        acc += (T)v[u].a * xv[u].a + (T)v[u].b * xv[u].b + (T)v[u].c * xv[u].c;
   }
   return acc;
 }

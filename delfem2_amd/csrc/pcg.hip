@@ -539,8 +539,12 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
     const Trip<T> pi = load_node<T, kVS>(a.p, i);
     const Trip<M> d = load_trip<M>(a.a_dia, L.s, L.lane);
     T acc = 0;
-    acc = sliced_row_sum_m<T, M, false, spmv_unroll<T, M>(), kVS>(
-        acc, (T)1, L, R.len, R.kl, R.kmax, R.off, a.a_plan.col, a.a_val, a.p);
+    if constexpr (is_mixed<T, M>())  // FP32 copy: difference form, d = row-sum block
+      acc = sliced_row_diff_sum<T, M, spmv_unroll<T, M>(), kVS>(
+          acc, pi, L, R.len, R.kl, R.kmax, R.off, a.a_plan.col, a.a_val, a.p);
+    else
+      acc = sliced_row_sum_m<T, M, false, spmv_unroll<T, M>(), kVS>(
+          acc, (T)1, L, R.len, R.kl, R.kmax, R.off, a.a_plan.col, a.a_val, a.p);
     acc += (T)d.a * pi.a + (T)d.b * pi.b + (T)d.c * pi.c;  // dia_row_dot_m's products
     a.Pr[kVS * (size_t)i + L.c] = acc;  // Ap lives in Pr (lsitrsol.h:111)
     part = (double)(L.c == 0 ? pi.a : L.c == 1 ? pi.b : pi.c) * (double)acc;
@@ -1537,9 +1541,12 @@ static void fence_in(PcgWork &W) {
   DFM2H_HIP(hipStreamWaitEvent(W.s, W.ev_in, 0));
 }
 
-// Packed FP32 copies of A's values and P's factors (rounded from FP64)
+// Packed FP32 copies of A's values (in the difference form of
+// sliced_row_diff_sum, which k_pcg_spmv uses on them) and of P's factors,
+// rounded from FP64
 static void ensure_f32(dfm2h_bsr A, dfm2h_ilu P, hipStream_t s) {
-  ensure_packed(A->plan, A->pk32, A->vcrs.p, A->vdia.p, A->version, s);
+  ensure_packed_rowsum(A->plan, A->pk32, A->vcrs.p, A->vdia.p, A->pat.colind_d.p, A->version,
+                       s);
   ensure_packed(P->fplan, P->f32, P->vcrs.p, P->vdia.p, P->version, s);
   ensure_packed(P->bplan, P->b32, P->vcrs.p, nullptr, P->version, s);
   ensure_dinv(A, P, P->work->fu.d32, s);

@@ -269,6 +269,29 @@ def test_pcg_mixed_vs_oracle(gpu, case, vec64):
     assert np.linalg.norm(xg - xc) / np.linalg.norm(xc) <= MIXED_X_TOL
 
 
+@pytest.mark.parametrize("n", [24, 40])
+def test_mixed_iterations_match_fp64(gpu, n):
+    """The FP32 matrix copy is streamed in the difference form (row-sum
+    blocks in the diagonal slot, off-diagonal blocks times x_j - x_i), so its
+    rounding does not slow the inner iteration down: the mixed solver needs
+    the FP64 solver's iteration count, +-3.  With the plain FP32 copy the
+    same solves took 219 (24^3) and 405 (40^3) inner iterations against FP64's
+    205 and 346 (measured on MI355X), more with other update settings."""
+    prob = W.cube(n, sigma=0.5, ordering="color4")
+    G = DeviceSystem(prob)
+    assert G.assemble() and G.factor()
+    x64, h64, _ = G.solve(1e-8, 20000)
+    it64 = len(h64) - 1
+    for vec64 in (False, True):
+        G.ilu.set_mixed_options(vec64)
+        x, hist, n_inner = G.solve_mixed(1e-8, delta=1e-1)
+        print(f"[mixed {n}^3 vec{64 if vec64 else 32}] inner {n_inner}, FP64 {it64}", flush=True)
+        assert hist[-1] / hist[0] < 1e-8
+        assert abs(n_inner - it64) <= 3
+        assert ((x - x64).norm() / x64.norm()).item() <= MIXED_X_TOL
+    G.ilu.set_mixed_options(False)
+
+
 def test_vector_ops(gpu):
     n = 100003
     x = W.uniform(1, n) - 0.5

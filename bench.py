@@ -10,7 +10,11 @@ and right-hand side resident in HBM.  The JSON line reports the
 FP32+refinement time (value) beside the FP64 time, DOF*iterations/s, the
 HBM roofline of the dominant kernel and the CPU reference (the reference's
 own code, compiled) solving the same cube in full, in its natural node order
-and in the GPU's.
+and in the GPU's.  The same line then carries, under other_workloads, one
+run of each other workload (--workload cube128, batch512, newmark64,
+stiffwarp64, adjoint96: the 128^3 mixed solve to 1e-10, the 512-mesh batch,
+the two time steps and the compliance gradient), each measured by a child
+process after the headline's timed steps (--others none skips them).
 """
 from __future__ import annotations
 
@@ -18,6 +22,7 @@ import argparse
 import json
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -163,7 +168,6 @@ def cpu_baseline(n: int, sigma: float, ordering: str) -> dict:
     reference's own (natural) node order -- how its callers run it -- and in
     the GPU workload's order, as two child processes side by side (one core
     each).  value = the natural-order time-to-1e-8 (Decompose + PCG)."""
-    import subprocess
 
     orders = ["natural"] + ([ordering] if ordering != "natural" else [])
     procs = {o: subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
@@ -758,6 +762,10 @@ def main():
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--spmv-reps", type=int, default=50,
                     help="launches per kernel class when timing the fused kernels")
+    ap.add_argument("--others", default=",".join(OTHER_WORKLOADS),
+                    help="cube only: after the headline, also run these workloads (comma "
+                         "list, or 'none'), each as a child process with --steps 1 --warmup 1, "
+                         "and fold their results into the same JSON line")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MB (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -970,7 +978,42 @@ def main():
         "assembly_ms": round(t_asm * 1e3, 3),  # warm median; the first call also builds lists
         "assembly_first_ms": round(t_asm_first * 1e3, 3),
     }
+    if args.others != "none":
+        out["other_workloads"] = other_workloads(args)
     print(json.dumps(out), flush=True)
+
+
+# the BASELINE-style workloads besides the headline, run after it by default
+OTHER_WORKLOADS = ("cube128", "batch512", "newmark64", "stiffwarp64", "adjoint96")
+# keys of a child's JSON line left out of the folded copy (per-kernel tables
+# and autotuner timings; the child prints them when run on its own)
+OTHER_DROP = ("kernels", "autotune", "sweep_shape", "n_gpus", "scaling", "vs_baseline",
+              "higher_is_better", "steps", "warmup")
+
+
+def other_workloads(args) -> dict:
+    """Runs each workload of --others as `bench.py --workload W --steps 1
+    --warmup 1` in a child process (one at a time, after the headline's
+    timed steps) and returns its JSON line without the bulky tables; a child
+    that fails or times out is recorded with its error."""
+    res = {}
+    for w in [x for x in args.others.split(",") if x]:
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", w, "--steps", "1",
+               "--warmup", "1", "--cpu-iters", str(args.cpu_iters), "--others", "none"]
+        print(f"[bench] workload {w}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=600)
+            lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+            if r.returncode != 0 or not lines:
+                res[w] = {"error": f"exit status {r.returncode}"}
+                continue
+            d = json.loads(lines[-1])
+            res[w] = {k: v for k, v in d.items() if k not in OTHER_DROP}
+            res[w]["wall_s"] = round(time.perf_counter() - t0, 1)
+        except subprocess.TimeoutExpired:
+            res[w] = {"error": "timed out after 600 s"}
+    return res
 
 
 if __name__ == "__main__":

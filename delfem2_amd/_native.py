@@ -106,6 +106,7 @@ SIGNATURES = {
     "dfm2h_measure_copy": [_sz, _u, _vp],
     "dfm2h_pcg_set_shape": [_vp, _i, _i],
     "dfm2h_pcg_set_mixed_options": [_vp, _i, _u],
+    "dfm2h_pcg_mixed_fell_back": [_vp, _pi],
     "dfm2h_pcg_set_graph_chunks": [_vp, _u, _u],
     "dfm2h_pcg_set_scalar_fold": [_vp, _i],
     "dfm2h_pcg_get_shape": [_vp, _i, _pi, C.POINTER(C.c_char_p), _vp, _sz, _psz],

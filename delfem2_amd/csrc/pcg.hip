@@ -51,6 +51,9 @@ struct PcgScal {
   int period;     // force an update after this many inner iterations (0: off)
   int restart;    // the search direction restarts from the true residual
   int pad3;
+  double best2;   // smallest FP64 |r|^2 of a reliable update that at least halved |r|
+  int nstall;     // reliable updates since then
+  int stalled;    // kStallUpdates of them in a row: the FP32 iteration stopped converging
 };
 
 // Field by field copy of the state (a whole-struct copy keeps a per-thread
@@ -79,6 +82,9 @@ __device__ __attribute__((always_inline)) inline void copy_state(PcgScal &d, con
   d.period = s.period;
   d.restart = s.restart;
   d.pad3 = s.pad3;
+  d.best2 = s.best2;
+  d.nstall = s.nstall;
+  d.stalled = s.stalled;
 }
 
 __device__ __attribute__((always_inline)) inline void load_state(PcgScal &d, const PcgScal *s) {
@@ -105,6 +111,9 @@ __device__ __attribute__((always_inline)) inline void load_state(PcgScal &d, con
   d.period = ld_state(&s->period);
   d.restart = ld_state(&s->restart);
   d.pad3 = ld_state(&s->pad3);
+  d.best2 = ld_state(&s->best2);
+  d.nstall = ld_state(&s->nstall);
+  d.stalled = ld_state(&s->stalled);
 }
 
 constexpr int kVecBlock = 256;
@@ -268,6 +277,11 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T, M> a) {
 
 // Scalar steps, one workgroup each, over the partials the previous kernels
 // wrote (lsitrsol.h:95-100, 116-117, 122-125, 132-135).
+// Reliable updates in a row without halving the smallest FP64 residual
+// after which the mixed solver counts as stalled (with the default forced
+// update every 50 inner iterations: 500 iterations).
+constexpr int kStallUpdates = 10;
+
 enum ScalMode {
   kScalInit = 0,
   kScalAlpha = 1,
@@ -329,6 +343,9 @@ __device__ __attribute__((always_inline)) inline void scal_step(PcgScal &S, doub
     if (writer) hist[0] = sqrt(t);
     S.sq_b = t;
     S.rmax2 = t;
+    S.best2 = t;
+    S.nstall = 0;
+    S.stalled = 0;
     S.sq = t;
     S.it = 0;
     S.nupd = 0;
@@ -361,6 +378,15 @@ __device__ __attribute__((always_inline)) inline void scal_step(PcgScal &S, doub
         S.nan = 1;
       } else if (S.final_ || k >= S.max_upd) {
         S.done = 1;
+      } else if (t2 < 0.25 * S.best2) {
+        S.best2 = t2;
+        S.nstall = 0;
+      } else if (++S.nstall >= kStallUpdates) {
+        // the FP32 iteration no longer reduces the FP64 residual (the FP32
+        // matrix is too coarse for this system): stop, the host finishes
+        // with the FP64 solver
+        S.done = 1;
+        S.stalled = 1;
       }
     }
     if (upd && (!(t > 0.0) || t3 > 1.0e-2 * t2)) {
@@ -404,6 +430,9 @@ __device__ __attribute__((always_inline)) inline void store_step_fields(PcgScal 
   } else if constexpr (MODE == kScalAlphaMixed) {
     d.nupd = s.nupd;
     d.rmax2 = s.rmax2;
+    d.best2 = s.best2;
+    d.nstall = s.nstall;
+    d.stalled = s.stalled;
     d.done = s.done;
     d.converged = s.converged;
     d.nan = s.nan;
@@ -991,6 +1020,10 @@ struct PcgWork {
   unsigned chunk_large = kChunkLarge, chunk_small = kChunkSmall;
   // beta step folded into the p update: -1 by size, 0 never, 1 always
   int fold_mode = -1;
+  // the last mixed solve stalled and was finished by the FP64 solver, on
+  // these (allocated on the first such solve)
+  bool last_mixed_fallback = false;
+  DBuf<double> fb_r, fb_e;
   PcgWork() {
     DFM2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     DFM2H_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
@@ -1727,6 +1760,40 @@ int dfm2h_vec_set_zero(double *x, size_t n) {
   });
 }
 
+namespace dfm2h {
+
+// The fused FP64 solve behind dfm2h_solve_pcg: r (device) holds b on entry
+// and the final residual on return, x receives the solution; h the it + 1
+// residual norms.  Synchronises the solver stream.
+static PcgScal fp64_solve(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
+                          unsigned max_nitr, std::vector<double> &h) {
+  PcgWork &W = work_of(P);
+  PcgBufs<double> &B = W.f64;
+  ensure_fusion(A, P, W);
+  ensure_bufs(B, A, P, max_nitr);
+  const size_t nn = A->nblk;
+  fence_in(W);
+  hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s, B.r.p,
+                     r, nn);
+  ensure_f64(A, P, W.s);
+  const PcgArgs<double, double> a =
+      make_args<double, double>(A, P, B, A->pk64, P->f64, P->b64, W.fu.d64);
+  PcgScal st{};
+  run_pcg(A, P, W, B, a, tol, max_nitr, st);
+  hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, r,
+                     B.r.p, nn);
+  hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x,
+                     B.x.p, nn);
+  check_launch("k_unpad4");
+  h.assign((size_t)st.it + 1, 0.0);
+  DFM2H_HIP(hipMemcpyAsync(h.data(), B.hist.p, h.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, W.s));
+  DFM2H_HIP(hipStreamSynchronize(W.s));
+  return st;
+}
+
+}  // namespace dfm2h
+
 int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
                     unsigned max_nitr, double *hist, size_t hist_cap, size_t *n_hist) {
   return guard(__func__, [&] {
@@ -1734,29 +1801,9 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r, double *x, double tol,
     DFM2H_REQUIRE(r && x, DFM2H_ERR_INVALID, "null vector");
     DFM2H_DEVICE_ARG(r);
     DFM2H_DEVICE_ARG(x);
-    PcgWork &W = work_of(P);
-    PcgBufs<double> &B = W.f64;
-    ensure_fusion(A, P, W);
-    ensure_bufs(B, A, P, max_nitr);
-    const size_t nn = A->nblk;
-    fence_in(W);
-    hipLaunchKernelGGL((k_pad4<double, double>), dim3(vec_grid(kVS * nn)), dim3(256), 0, W.s, B.r.p,
-                       r, nn);
-    ensure_f64(A, P, W.s);
-    const PcgArgs<double, double> a =
-        make_args<double, double>(A, P, B, A->pk64, P->f64, P->b64, W.fu.d64);
-    PcgScal st{};
-    run_pcg(A, P, W, B, a, tol, max_nitr, st);
-    hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, r,
-                       B.r.p, nn);
-    hipLaunchKernelGGL((k_unpad4<double, double>), dim3(vec_grid(3 * nn)), dim3(256), 0, W.s, x,
-                       B.x.p, nn);
-    check_launch("k_unpad4");
-    // history: it+1 norms, plus the reference's final push when not converged
-    std::vector<double> h((size_t)st.it + 1);
-    DFM2H_HIP(hipMemcpyAsync(h.data(), B.hist.p, h.size() * sizeof(double),
-                             hipMemcpyDeviceToHost, W.s));
-    DFM2H_HIP(hipStreamSynchronize(W.s));
+    std::vector<double> h;
+    const PcgScal st = fp64_solve(A, P, r, x, tol, max_nitr, h);
+    // the reference's final push of the history when not converged
     if (!st.converged && !st.nan) h.push_back(h.back());
     if (hist)
       for (size_t k = 0; k < h.size() && k < hist_cap; ++k) hist[k] = h[k];
@@ -1785,11 +1832,49 @@ int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b, double *x, 
     DFM2H_HIP(hipMemcpyAsync(h.data(), W.hist64.p, h.size() * sizeof(double),
                              hipMemcpyDeviceToHost, W.s));
     DFM2H_HIP(hipStreamSynchronize(W.s));
+    DFM2H_REQUIRE(!st.nan, DFM2H_ERR_NAN, "residual became NaN or Inf");
+    size_t inner = (size_t)st.it;
+    W.last_mixed_fallback = st.stalled != 0;
+    if (st.stalled) {
+      // The FP32 iteration stopped reducing the FP64 residual: finish with
+      // the FP64 solver on the correction, e = A^-1 (b - A x), x += e.  Its
+      // residual norms are those of x + e_k, so they continue the history.
+      const size_t n = 3 * (size_t)A->nblk;
+      if (W.fb_r.n != n) {
+        W.fb_r.alloc(n);
+        W.fb_e.alloc(n);
+      }
+      DFM2H_HIP(hipMemcpyAsync(W.fb_r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, W.s));
+      DFM2H_HIP(hipStreamSynchronize(W.s));
+      const auto ok = [](int rc) {
+        DFM2H_REQUIRE(rc == DFM2H_OK, rc, dfm2h_last_error());
+      };
+      ok(dfm2h_bsr_matvec(A, W.fb_r.p, -1.0, x, 1.0));
+      double rr = 0.0;
+      ok(dfm2h_vec_dot(W.fb_r.p, W.fb_r.p, n, &rr));
+      // stop when |b - A (x + e)| / |b| < tol
+      const double tol_e = rr > 0.0 ? tol * h[0] / std::sqrt(rr) : 1.0;
+      std::vector<double> he;
+      const PcgScal se = fp64_solve(A, P, W.fb_r.p, W.fb_e.p, tol_e, max_inner, he);
+      DFM2H_REQUIRE(!se.nan, DFM2H_ERR_NAN, "residual became NaN or Inf");
+      hipLaunchKernelGGL(k_add_scaled, dim3(vec_grid(n)), dim3(kVecBlock), 0, W.s, x, 1.0,
+                         (const double *)W.fb_e.p, n);
+      check_launch("k_add_scaled");
+      DFM2H_HIP(hipStreamSynchronize(W.s));
+      h.insert(h.end(), he.begin(), he.end());
+      inner += (size_t)se.it;
+    }
     if (hist)
       for (size_t k = 0; k < h.size() && k < hist_cap; ++k) hist[k] = h[k];
     if (n_hist) *n_hist = h.size();
-    if (n_inner) *n_inner = (size_t)st.it;
-    DFM2H_REQUIRE(!st.nan, DFM2H_ERR_NAN, "residual became NaN or Inf");
+    if (n_inner) *n_inner = inner;
+  });
+}
+
+int dfm2h_pcg_mixed_fell_back(dfm2h_ilu P, int *fell_back) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(P && fell_back, DFM2H_ERR_INVALID, "null argument");
+    *fell_back = P->work && P->work->last_mixed_fallback ? 1 : 0;
   });
 }
 

@@ -333,6 +333,14 @@ class CPreconditionerILU:
         N.call("dfm2h_pcg_set_mixed_options", self.h, int(bool(fp64_vectors)),
                int(update_period))
 
+    def mixed_fell_back(self) -> bool:
+        """Whether the last Solve_PCG_Mixed stalled (10 reliable updates in a
+        row without halving the best FP64 residual) and was finished by the
+        FP64 solver on the correction (no reference counterpart)."""
+        f = C.c_int(0)
+        N.call("dfm2h_pcg_mixed_fell_back", self.h, C.byref(f))
+        return bool(f.value)
+
     def set_graph_chunks(self, large: int = 16, small: int = 2) -> None:
         """Iterations per HIP-graph launch of the fused solvers: small chunks
         at the start and near the predicted end, large ones in between (no
@@ -495,9 +503,11 @@ def Solve_PCG_Mixed(b_vec: torch.Tensor, x_vec: torch.Tensor, conv_ratio_tol: fl
                     mat: CMatrixSparse, ilu: CPreconditionerILU, delta: float = 1e-1,
                     max_outer: int = 200, max_inner: int = 20000):
     """FP32 ILU(0)-PCG inside FP64 refinement by reliable updates (no
-    reference counterpart): refine whenever the FP32 residual fell by `delta`.
-    Returns (FP64 residual-norm history, FP32 iterations)."""
-    cap = int(max_outer) + 2
+    reference counterpart): refine whenever the FP32 residual fell by `delta`;
+    when the refinement stalls, the FP64 solver finishes (ilu.mixed_fell_back()).
+    Returns (FP64 residual-norm history, FP32 iterations plus, after a
+    fallback, the FP64 solver's)."""
+    cap = int(max_outer) + int(max_inner) + 4  # room for a fallback's FP64 history
     hist = np.zeros(cap)
     nh = C.c_size_t(0)
     ni = C.c_size_t(0)

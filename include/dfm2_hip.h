@@ -310,15 +310,29 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r_dev, double *x_dev,
  * r = b - A x is recomputed in FP64 (FP64 values of A), the stop test
  * |r|/|b| < tol is made in FP64, and the FP32 iteration continues from the
  * true residual with its search direction kept (no Krylov restart).
+ * The FP32 copy of A is held in difference form: the off-diagonal blocks
+ * multiply x_j - x_i, and the diagonal slot holds the row's block sum
+ * D_i + sum_j A_ij (summed in FP64, rounded once), so that the rounding of
+ * the stored blocks costs no extra iterations on smooth vectors.
+ * When 10 reliable updates in a row have not halved the smallest FP64
+ * residual so far (the FP32 matrix too coarse for the system, e.g. element
+ * stiffnesses spread over ~7 decades), the FP32 iteration stops and the
+ * FP64 solver (dfm2h_solve_pcg, up to max_inner iterations) finishes on
+ * the correction: x += A^-1 (b - A x); dfm2h_pcg_mixed_fell_back tells.
  * b_dev is not modified; x_dev receives the solution.  hist receives the
- * FP64 residual norms (entry 0 = |b|, then one per refinement step); n_inner
- * the number of FP32 iterations.  max_outer caps the refinement steps,
- * max_inner the FP32 iterations. */
+ * FP64 residual norms (entry 0 = |b|, then one per refinement step, then,
+ * after a fallback, the FP64 solver's history, whose norms are those of
+ * b - A x); n_inner the number of FP32 iterations plus the FP64 solver's.
+ * max_outer caps the refinement steps, max_inner the FP32 iterations. */
 int dfm2h_solve_pcg_mixed(dfm2h_bsr A, dfm2h_ilu P, const double *b_dev,
                           double *x_dev, double tol, double delta,
                           unsigned max_outer, unsigned max_inner,
                           double *hist, size_t hist_cap, size_t *n_hist,
                           size_t *n_inner);
+
+/* Whether P's last dfm2h_solve_pcg_mixed stalled and was finished by the
+ * FP64 solver (no reference counterpart): *fell_back = 1 or 0. */
+int dfm2h_pcg_mixed_fell_back(dfm2h_ilu P, int *fell_back);
 
 /* Options of the mixed solver (no reference counterpart).
  * fp64_vectors = 0 (default): r, x, p, Pr in FP32 (the solution kept as

@@ -750,7 +750,7 @@ def main():
     ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
     ap.add_argument("--delta", type=float, default=1e-1,
                     help="mixed: refine in FP64 whenever the FP32 residual fell by this factor")
-    ap.add_argument("--update-period", type=int, default=50,
+    ap.add_argument("--update-period", type=int, default=100,
                     help="mixed: force a reliable update after this many inner iterations "
                          "(0: only the delta rule)")
     ap.add_argument("--mixed-vectors", type=int, default=32, choices=[32, 64],

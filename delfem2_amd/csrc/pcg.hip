@@ -52,8 +52,8 @@ struct PcgScal {
   int restart;    // the search direction restarts from the true residual
   int pad3;
   double best2;   // smallest FP64 |r|^2 of a reliable update that at least halved |r|
-  int nstall;     // reliable updates since then
-  int stalled;    // kStallUpdates of them in a row: the FP32 iteration stopped converging
+  int it_best;    // inner iteration of that update
+  int stalled;    // kStallIters inner iterations since it: the FP32 iteration stopped converging
 };
 
 // Field by field copy of the state (a whole-struct copy keeps a per-thread
@@ -83,7 +83,7 @@ __device__ __attribute__((always_inline)) inline void copy_state(PcgScal &d, con
   d.restart = s.restart;
   d.pad3 = s.pad3;
   d.best2 = s.best2;
-  d.nstall = s.nstall;
+  d.it_best = s.it_best;
   d.stalled = s.stalled;
 }
 
@@ -112,7 +112,7 @@ __device__ __attribute__((always_inline)) inline void load_state(PcgScal &d, con
   d.restart = ld_state(&s->restart);
   d.pad3 = ld_state(&s->pad3);
   d.best2 = ld_state(&s->best2);
-  d.nstall = ld_state(&s->nstall);
+  d.it_best = ld_state(&s->it_best);
   d.stalled = ld_state(&s->stalled);
 }
 
@@ -277,10 +277,11 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T, M> a) {
 
 // Scalar steps, one workgroup each, over the partials the previous kernels
 // wrote (lsitrsol.h:95-100, 116-117, 122-125, 132-135).
-// Reliable updates in a row without halving the smallest FP64 residual
-// after which the mixed solver counts as stalled (with the default forced
-// update every 50 inner iterations: 500 iterations).
-constexpr int kStallUpdates = 10;
+// Inner iterations after the last reliable update that halved the smallest
+// FP64 residual after which the mixed solver counts as stalled (checked at
+// every reliable update; the default forced update every 100 inner
+// iterations checks it at least that often).
+constexpr int kStallIters = 500;
 
 enum ScalMode {
   kScalInit = 0,
@@ -344,7 +345,7 @@ __device__ __attribute__((always_inline)) inline void scal_step(PcgScal &S, doub
     S.sq_b = t;
     S.rmax2 = t;
     S.best2 = t;
-    S.nstall = 0;
+    S.it_best = 0;
     S.stalled = 0;
     S.sq = t;
     S.it = 0;
@@ -380,8 +381,8 @@ __device__ __attribute__((always_inline)) inline void scal_step(PcgScal &S, doub
         S.done = 1;
       } else if (t2 < 0.25 * S.best2) {
         S.best2 = t2;
-        S.nstall = 0;
-      } else if (++S.nstall >= kStallUpdates) {
+        S.it_best = S.it;
+      } else if (S.it - S.it_best >= kStallIters) {
         // the FP32 iteration no longer reduces the FP64 residual (the FP32
         // matrix is too coarse for this system): stop, the host finishes
         // with the FP64 solver
@@ -431,7 +432,7 @@ __device__ __attribute__((always_inline)) inline void store_step_fields(PcgScal 
     d.nupd = s.nupd;
     d.rmax2 = s.rmax2;
     d.best2 = s.best2;
-    d.nstall = s.nstall;
+    d.it_best = s.it_best;
     d.stalled = s.stalled;
     d.done = s.done;
     d.converged = s.converged;
@@ -1014,7 +1015,7 @@ struct PcgWork {
   std::vector<double> tune_ms[3];
   // mixed solver options: Krylov vector precision, forced-update period
   int mixed_vectors64 = 0;
-  unsigned mixed_period = 50;
+  unsigned mixed_period = 100;
   // iterations per graph launch (the small chunk at the start and near the
   // predicted end, the large one in between)
   unsigned chunk_large = kChunkLarge, chunk_small = kChunkSmall;

@@ -325,18 +325,19 @@ class CPreconditionerILU:
         N.call("dfm2h_ilu_levels", self.h, C.byref(f), C.byref(b))
         return f.value, b.value
 
-    def set_mixed_options(self, fp64_vectors: bool = False, update_period: int = 50) -> None:
+    def set_mixed_options(self, fp64_vectors: bool = False, update_period: int = 100) -> None:
         """Solve_PCG_Mixed options: Krylov vector precision (default FP32;
         the matrix and factors are streamed in FP32 either way) and a forced
         reliable update after `update_period` inner iterations without one
-        (0: off; default 50)."""
+        (0: off; default 100)."""
         N.call("dfm2h_pcg_set_mixed_options", self.h, int(bool(fp64_vectors)),
                int(update_period))
 
     def mixed_fell_back(self) -> bool:
-        """Whether the last Solve_PCG_Mixed stalled (10 reliable updates in a
-        row without halving the best FP64 residual) and was finished by the
-        FP64 solver on the correction (no reference counterpart)."""
+        """Whether the last Solve_PCG_Mixed stalled (500 inner iterations
+        without a reliable update that halved the best FP64 residual) and was
+        finished by the FP64 solver on the correction (no reference
+        counterpart)."""
         f = C.c_int(0)
         N.call("dfm2h_pcg_mixed_fell_back", self.h, C.byref(f))
         return bool(f.value)

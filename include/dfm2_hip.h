@@ -314,9 +314,10 @@ int dfm2h_solve_pcg(dfm2h_bsr A, dfm2h_ilu P, double *r_dev, double *x_dev,
  * multiply x_j - x_i, and the diagonal slot holds the row's block sum
  * D_i + sum_j A_ij (summed in FP64, rounded once), so that the rounding of
  * the stored blocks costs no extra iterations on smooth vectors.
- * When 10 reliable updates in a row have not halved the smallest FP64
- * residual so far (the FP32 matrix too coarse for the system, e.g. element
- * stiffnesses spread over ~7 decades), the FP32 iteration stops and the
+ * When a reliable update finds that 500 inner iterations have passed since
+ * the last one that halved the smallest FP64 residual so far (the FP32
+ * matrix too coarse for the system, e.g. element stiffnesses spread over ~7
+ * decades), the FP32 iteration stops and the
  * FP64 solver (dfm2h_solve_pcg, up to max_inner iterations) finishes on
  * the correction: x += A^-1 (b - A x); dfm2h_pcg_mixed_fell_back tells.
  * b_dev is not modified; x_dev receives the solution.  hist receives the
@@ -340,8 +341,11 @@ int dfm2h_pcg_mixed_fell_back(dfm2h_ilu P, int *fell_back);
  * the matrix and factor values still streamed in FP32 (widened before the
  * products).  update_period > 0 forces a reliable update after that many
  * inner iterations without one (0: only the delta rule).  Defaults: FP32
- * vectors, period 50 (measured on MI355X: with delta = 0.1 it keeps the
- * inner iteration count of the 64^3 cube at the FP64 solver's). */
+ * vectors, period 100 (measured on MI355X with delta = 0.1: the inner
+ * iteration count of the 64^3 cube stays at the FP64 solver's with 9
+ * reliable updates instead of 13 at period 50 -- 58.0 against 58.5 ms --,
+ * and the 128^3 cube to 1e-10 takes 980 against 987 ms; period 200 and no
+ * forced updates need more iterations there). */
 int dfm2h_pcg_set_mixed_options(dfm2h_ilu P, int fp64_vectors, unsigned update_period);
 /* HIP-graph chunking of the fused solvers (no reference counterpart): the
  * iterations are captured in graphs of `large` and of `small` iterations;

@@ -540,12 +540,13 @@ __device__ inline double residual_row(const PcgArgs<T, M> &a, const SliceLane &L
   return r * r;
 }
 
-// blocks per load round of the SpMV
-// (FP64: 8 or 16 per round measured no faster at 64^3: 71.1 / 78.7 us
-// against 71.6 us with 4)
+// blocks per load round of the SpMV: 8 with FP32 values and vectors, 4
+// when either is FP64 (FP64: 8 or 16 per round measured no faster at 64^3:
+// 71.1 / 78.7 us against 71.6 us with 4; FP32 values with FP64 vectors at 8
+// took 162 VGPRs, three wavefronts per SIMD)
 template <typename T, typename M>
 constexpr int spmv_unroll() {
-  return sizeof(M) == 4 ? 8 : 4;
+  return sizeof(M) == 4 && sizeof(T) == 4 ? 8 : 4;
 }
 
 // Ap -> Pr, partial (p, Ap).  Mixed form with a reliable update due: also

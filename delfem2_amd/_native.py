@@ -118,6 +118,7 @@ SIGNATURES = {
     "dfm2h_batch_set_variant": [_vp, _i, _i],
     "dfm2h_batch_set_update_period": [_vp, _u],
     "dfm2h_batch_get_variant": [_vp, _i, _pi, C.POINTER(C.c_char_p)],
+    "dfm2h_batch_fell_back": [_vp, _vp],
 }
 _RESTYPE = {"dfm2h_last_error": C.c_char_p, "dfm2h_get_stream": _vp}
 

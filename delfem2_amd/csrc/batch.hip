@@ -36,14 +36,20 @@ constexpr unsigned kChunk = 16;    // iterations per captured graph
 
 struct MeshScal {
   double rPr, alpha, beta, inv_sq0, sq_b, rmax2, res;
-  int it, done, converged, nan, upd, nupd, final_, it_upd, restart, pad;
+  double best2;  // FP32 form: smallest FP64 |r|^2 of an update that at least halved |r|
+  int it, done, converged, nan, upd, nupd, final_, it_upd, restart;
+  int it_best;   // inner iteration of that update
+  int stalled;   // kStallIters inner iterations since it: stopped, for the FP64 finish
+  int pad;
 };
 
 struct BatchGlobal {
   double tol, delta2;
   int max_nitr, max_upd;
   unsigned ndone;
-  int period;  // FP32 form: force a reliable update after this many iterations
+  int period;      // FP32 form: force a reliable update after this many iterations
+  int keep_scale;  // FP64 form on a correction: the stop test stays relative to the
+                   // |b|^2 a preceding mixed solve left in sq_b (set for no mesh: 0)
 };
 
 struct BPlan {
@@ -394,6 +400,13 @@ __global__ void kb_fill(double *y, double v, size_t n) {
     y[i] = v;
 }
 
+// y += x
+__global__ void kb_add(double *__restrict__ y, const double *__restrict__ x, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] += x[i];
+}
+
 // ---------------------------------------------------------------------------
 // per-mesh scalar steps: one wavefront per mesh sums that mesh's slice
 // partials (lane-strided, then a fixed butterfly) and updates its state
@@ -483,10 +496,19 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
     S->converged = 0;
     S->nan = 0;
     S->done = 0;
-    S->sq_b = t;
+    S->stalled = 0;
+    S->it_best = 0;
+    S->best2 = t;
     S->rmax2 = t;
-    S->inv_sq0 = 1.0 / t;
-    S->res = 1.0;
+    const double sb = MODE == kBInit && ld_state(&G->keep_scale) ? ld_state(&S->sq_b) : 0.0;
+    if (sb > 0.0) {
+      S->inv_sq0 = 1.0 / sb;
+      S->res = sqrt(t / sb);
+    } else {
+      S->sq_b = t;
+      S->inv_sq0 = 1.0 / t;
+      S->res = 1.0;
+    }
     if (t < 1.0e-30) {
       S->converged = 1;
       S->res = 0.0;
@@ -528,6 +550,14 @@ __global__ void __launch_bounds__(kMeshPerBlock * kWave) kb_scalar(BArgs<float> 
         S->nan = 1;
         mark_done(S, G);
       } else if (ld_state(&S->final_) || k >= G->max_upd) {
+        mark_done(S, G);
+      } else if (t2 < 0.25 * ld_state(&S->best2)) {
+        S->best2 = t2;
+        S->it_best = ld_state(&S->it);
+      } else if (ld_state(&S->it) - ld_state(&S->it_best) >= kStallIters) {
+        // the FP32 iteration no longer reduces this mesh's FP64 residual:
+        // stop it; the host finishes it with the FP64 form
+        S->stalled = 1;
         mark_done(S, G);
       }
     }
@@ -608,6 +638,10 @@ struct dfm2h_batch_s {
   DBuf<BatchGlobal> G;
   int variant[2] = {-1, -1};  // [fp32] chosen launch shape, -1 = autotune
   unsigned period = 25;       // FP32 form: forced reliable-update period
+  // meshes the last FP32-form solve finished with the FP64 form, and the
+  // full residual b - A x it was formed from (allocated on the first one)
+  std::vector<int> fell_back;
+  DBuf<double> fb_r;
   hipStream_t s = nullptr;
   hipEvent_t ev_in = nullptr, ev_poll[2] = {nullptr, nullptr}, ev_t0 = nullptr, ev_t1 = nullptr;
   BatchGlobal *host_g = nullptr;  // pinned, 2 entries
@@ -858,9 +892,10 @@ void ensure_batch_bufs(dfm2h_batch B, BatchBufs<T> &S) {
 // b is already in B->b64.  Returns when every mesh is done.
 template <typename T>
 void run_batch(dfm2h_batch B, BatchBufs<T> &S, const BArgs<T> &a, int v, double tol,
-               double delta, unsigned max_nitr) {
+               double delta, unsigned max_nitr, int keep_scale = 0) {
   hipStream_t s = B->s;
   BatchGlobal g{};
+  g.keep_scale = keep_scale;
   g.tol = tol;
   g.delta2 = delta * delta;
   g.max_nitr = (int)max_nitr;
@@ -1149,6 +1184,14 @@ int dfm2h_batch_tune(dfm2h_batch B, int fp32, int *best, double *ms, size_t ms_c
   });
 }
 
+int dfm2h_batch_fell_back(dfm2h_batch B, int *flags) {
+  return guard(__func__, [&] {
+    DFM2H_REQUIRE(B && flags, DFM2H_ERR_INVALID, "null argument");
+    for (unsigned m = 0; m < B->nmesh; ++m)
+      flags[m] = m < B->fell_back.size() ? B->fell_back[m] : 0;
+  });
+}
+
 int dfm2h_batch_solve(dfm2h_batch B, const double *b, double *x, double tol, int fp32,
                       double delta, unsigned max_nitr, unsigned *iters, double *relres,
                       int *state) {
@@ -1191,6 +1234,51 @@ int dfm2h_batch_solve(dfm2h_batch B, const double *b, double *x, double tol, int
     DFM2H_HIP(hipMemcpyAsync(h.data(), B->M.p, h.size() * sizeof(MeshScal),
                              hipMemcpyDeviceToHost, B->s));
     DFM2H_HIP(hipStreamSynchronize(B->s));
+    B->fell_back.assign(B->nmesh, 0);
+    std::vector<unsigned> stalled;
+    for (unsigned m = 0; m < B->nmesh; ++m)
+      if (fp32 && h[m].stalled && !h[m].nan) stalled.push_back(m);
+    if (!stalled.empty()) {
+      // The FP32 iteration stopped reducing these meshes' FP64 residuals:
+      // finish them with the FP64 form on the correction, e = A^-1 (b - A x)
+      // with the right-hand side zero on every other mesh (they stop at
+      // once) and each stop test relative to the mesh's own |b| (kept in
+      // sq_b); then x += e.
+      int v64 = B->variant[0];
+      if (v64 < 0) v64 = tune_lookup("batch", batch_bucket(B), 0, kNumVariants);
+      if (v64 < 0) v64 = autotune<double>(B, B->s64, nullptr, 0);  // before b64 is set
+      B->variant[0] = v64;
+      if (B->fb_r.n != n) B->fb_r.alloc(n);
+      DFM2H_HIP(hipMemcpyAsync(B->fb_r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice,
+                               stream()));
+      const int rc = dfm2h_bsr_matvec(B->A, B->fb_r.p, -1.0, x, 1.0);
+      DFM2H_REQUIRE(rc == DFM2H_OK, rc, dfm2h_last_error());
+      DFM2H_HIP(hipEventRecord(B->ev_in, stream()));
+      DFM2H_HIP(hipStreamWaitEvent(B->s, B->ev_in, 0));
+      hipLaunchKernelGGL(kb_fill, dim3(std::min<size_t>(2048, ceil_div(n, 256))), dim3(256), 0,
+                         B->s, B->b64.p, 0.0, n);
+      check_launch("kb_fill");
+      for (unsigned m : stalled) {
+        const size_t r0 = (size_t)kVSB * B->node_offset[m];
+        const size_t r1 = (size_t)kVSB * B->node_offset[m + 1];
+        DFM2H_HIP(hipMemcpyAsync(B->b64.p + r0, B->fb_r.p + r0, (r1 - r0) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, B->s));
+        B->fell_back[m] = 1;
+      }
+      const BArgs<double> a = prepare<double>(B, B->s64);
+      run_batch<double>(B, B->s64, a, v64, tol, 0.5, max_nitr, 1);
+      hipLaunchKernelGGL(kb_add, dim3(std::min<size_t>(2048, ceil_div(n, 256))), dim3(256), 0,
+                         B->s, x, (const double *)B->s64.x.p, n);
+      check_launch("kb_add");
+      std::vector<MeshScal> hf(B->nmesh);
+      DFM2H_HIP(hipMemcpyAsync(hf.data(), B->M.p, hf.size() * sizeof(MeshScal),
+                               hipMemcpyDeviceToHost, B->s));
+      DFM2H_HIP(hipStreamSynchronize(B->s));
+      for (unsigned m : stalled) {
+        hf[m].it += h[m].it;  // FP32 iterations, then the FP64 form's
+        h[m] = hf[m];
+      }
+    }
     bool any_nan = false;
     for (unsigned m = 0; m < B->nmesh; ++m) {
       if (iters) iters[m] = (unsigned)h[m].it;

@@ -157,6 +157,12 @@ struct DBuf {
 
 constexpr int kWave = 64;
 
+// Inner iterations after the last reliable update that halved the smallest
+// FP64 residual so far after which a mixed solve counts as stalled and is
+// finished by the FP64 solver (checked at every reliable update; the forced
+// updates check it at least every period).
+constexpr int kStallIters = 500;
+
 __device__ inline double wave_reduce_sum(double v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);

@@ -277,12 +277,6 @@ __global__ void __launch_bounds__(kVecBlock) k_pcg_init(PcgArgs<T, M> a) {
 
 // Scalar steps, one workgroup each, over the partials the previous kernels
 // wrote (lsitrsol.h:95-100, 116-117, 122-125, 132-135).
-// Inner iterations after the last reliable update that halved the smallest
-// FP64 residual after which the mixed solver counts as stalled (checked at
-// every reliable update; the default forced update every 100 inner
-// iterations checks it at least that often).
-constexpr int kStallIters = 500;
-
 enum ScalMode {
   kScalInit = 0,
   kScalAlpha = 1,

@@ -553,6 +553,13 @@ class BatchSolver:
                float(delta), int(max_nitr), _hptr(it), _hptr(rr), _hptr(st))
         return it, rr, st
 
+    def fell_back(self) -> np.ndarray:
+        """Per mesh: whether the last FP32-form solve stalled on it and
+        finished it with the FP64 form (no reference counterpart)."""
+        f = np.zeros(self.nmesh, np.int32)
+        N.call("dfm2h_batch_fell_back", self.h, _hptr(f))
+        return f.astype(bool)
+
     def tune(self, fp32: bool):
         """Runs the launch-shape autotuner; returns (best variant, ms per
         iteration of every variant)."""

@@ -441,12 +441,22 @@ int dfm2h_batch_info(dfm2h_batch B, size_t *nmesh, size_t *nslices,
  * reference's test); fp32 = 1: FP32 SpMV / sweeps / vectors inside FP64
  * reliable updates (x64 += x32, r = b - A x64 in FP64 whenever the FP32
  * residual fell by delta), stop when the FP64 |b - A x| / |b| < tol.
- * max_nitr caps the (inner) iterations of each mesh.  Optional host outputs
- * per mesh: iters (inner iterations), relres (last relative residual tested),
+ * max_nitr caps the (inner) iterations of each mesh.  The FP32 matrix copy
+ * is held in the difference form of dfm2h_solve_pcg_mixed.  A mesh whose
+ * FP32 iteration stalls (500 inner iterations without a reliable update that
+ * halved its smallest FP64 residual) stops, and after the batch the FP64 form
+ * finishes every such mesh on its correction, x += A^-1 (b - A x), stopping
+ * when |b - A x| / |b| < tol (dfm2h_batch_fell_back tells which).  Optional
+ * host outputs per mesh: iters (inner iterations; after an FP64 finish, the
+ * FP32 ones plus the FP64 ones), relres (last relative residual tested),
  * state (1 converged, 0 iteration cap, -1 NaN).  Returns DFM2H_ERR_NAN if any
  * mesh produced NaN / Inf; meshes that hit the cap are reported in state.
  * The first solve of a (DOF bucket, precision) with no variant set runs the
  * autotuner (dfm2h_batch_tune). */
+/* Per mesh (host, nmesh entries, no reference counterpart): 1 when the last
+ * dfm2h_batch_solve in the FP32 form finished that mesh with the FP64 form
+ * after its FP32 iteration stalled, else 0. */
+int dfm2h_batch_fell_back(dfm2h_batch B, int *flags);
 int dfm2h_batch_solve(dfm2h_batch B, const double *b_dev, double *x_dev,
                       double tol, int fp32, double delta, unsigned max_nitr,
                       unsigned *iters, double *relres, int *state);

@@ -216,8 +216,9 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_spmv(BArgs<T> a, unsigned n
     const SliceLane L = lane_of<V>(s);
     const RowMeta R = row_meta(a.A, L, s);
     const MeshScal *S = a.M + R.mesh;
-    if (ld_state(&S->done)) continue;
+    const int dn = ld_state(&S->done);  // requested together with upd
     const bool refine = sizeof(T) == 4 && ld_state(&S->upd);  // wavefront-uniform
+    if (dn) continue;
     const unsigned i = R.i;
     double part = 0.0, part2 = 0.0, part3 = 0.0;
     if (i != ~0u) {
@@ -264,9 +265,11 @@ __global__ void __launch_bounds__(V::WPB * kWave) kb_update(BArgs<T> a, unsigned
     const unsigned mesh = a.A.mesh[s];
     const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
     const MeshScal *S = a.M + mesh;
-    if (ld_state(&S->done)) continue;
+    // the state fields requested together (one memory round trip)
+    const int dn = ld_state(&S->done);
     const double alpha = ld_state(&S->alpha);
     const bool fold = sizeof(T) == 4 && ld_state(&S->upd);
+    if (dn) continue;
     double part = 0.0, srp = 0.0;
     T rn = 0;
     if (i != ~0u) {
@@ -312,7 +315,7 @@ kb_fwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
     if (s >= slice0 + nsl) return;
     const SliceLane L = lane_of<V>(s);
     const RowMeta R = row_meta(a.F, L, s);
-    if (a.M[R.mesh].done) continue;
+    if (ld_state(&a.M[R.mesh].done)) continue;
     const unsigned i = R.i;
     const bool valid = i != ~0u;
     T t = 0, rv = 0;
@@ -349,7 +352,7 @@ kb_bwd(BArgs<T> a, unsigned slice0, unsigned nsl) {
     if (s >= slice0 + nsl) return;
     const SliceLane L = lane_of<V>(s);
     const RowMeta R = row_meta(a.B, L, s);
-    if (a.M[R.mesh].done) continue;
+    if (ld_state(&a.M[R.mesh].done)) continue;
     const unsigned i = R.i;
     double part = 0.0;
     if (i != ~0u) {
@@ -375,8 +378,9 @@ kb_pupdate(BArgs<T> a, unsigned nsl, int mode) {
     const unsigned s = s0 + k;
     if (s >= nsl) return;
     const MeshScal *S = a.M + a.A.mesh[s];
-    if (ld_state(&S->done)) continue;
+    const int dn = ld_state(&S->done);  // requested together with beta
     const double beta = ld_state(&S->beta);
+    if (dn) continue;
     const SliceLane L = lane_of<V>(s);
     const unsigned i = L.c < 3 ? a.A.row[s * kSliceRows + L.r] : ~0u;
     if (i == ~0u) continue;

@@ -192,7 +192,11 @@ __device__ inline double block_reduce_sum(double v) {
 // per-thread sums, then a fixed tree).  Grid reductions write one partial per
 // workgroup and a separate one-workgroup kernel sums them: the kernel
 // boundary makes the partials visible without a device-scope fence in every
-// workgroup (which on a multi-XCD part writes back L2 each time).
+// workgroup (which on a multi-XCD part writes back L2 each time: measured at
+// 823,875 DOF, the PCG's alpha step run by the SpMV's last workgroup, with a
+// release fence and a counter increment in each of its 3,270 workgroups,
+// took the mixed solve from 58.9 to 104.8 ms and the FP64 solve from 104.0
+// to 180.6 ms, same bits).
 template <int BS>
 __device__ inline double reduce_partials(const double *partials, unsigned n) {
   double s = 0.0;

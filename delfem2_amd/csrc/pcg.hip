@@ -591,21 +591,21 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_spmv(PcgArgs<T, M> a) {
 }
 
 // Forward level 0 for the level-0 rows of matrix slice L.s, given every
-// lane's new r component rv (wavefront-uniform call): Pr_i = D_i^-1 r_i, the
-// same products in the same order as the sweep.  Rows that are also
-// backward level 0 add r_i . Pr_i to srp.
+// lane's new r component rv (wavefront-uniform call), the slice's level-0
+// and isolated-row masks m / iso and the lane's D^-1 row d (loaded only when
+// m is non-zero): Pr_i = D_i^-1 r_i, the same products in the same order as
+// the sweep.  Rows that are also backward level 0 add r_i . Pr_i to srp.
 template <typename T, typename M>
 __device__ inline void fwd_level0(const PcgArgs<T, M> &a, const SliceLane &L, unsigned i, T rv,
-                                  double &srp) {
-  const unsigned m = a.l0mask[L.s];
+                                  unsigned m, unsigned iso, const Trip<M> &d, double &srp) {
   if (!m) return;
   const T t0 = __shfl(rv, (int)L.r, kWave);
   const T t1 = __shfl(rv, (int)L.r + kSliceRows, kWave);
   const T t2 = __shfl(rv, (int)L.r + 2 * kSliceRows, kWave);
   if (i != ~0u && ((m >> L.r) & 1u)) {
-    const T z = dia_row_dot_m<T, M>(a.dinv0, L, t0, t1, t2);
+    const T z = (T)d.a * t0 + (T)d.b * t1 + (T)d.c * t2;  // dia_row_dot_m's products
     a.Pr[kVS * (size_t)i + L.c] = z;
-    if ((a.isomask[L.s] >> L.r) & 1u) srp += (double)rv * (double)z;
+    if ((iso >> L.r) & 1u) srp += (double)rv * (double)z;
   }
 }
 
@@ -635,18 +635,29 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
   // no gathers here: the dispatch order serves (an XCD-ordered numbering
   // measured slower on MI355X: FP64 64^3 14.2 -> 16.9 us)
   const unsigned wv = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
+  const bool iso_rows = a.upd_part_off != ~0u;  // rows with neither lower nor upper blocks exist
   SliceLane L[kUpdSlices];
   bool sl[kUpdSlices];
-  unsigned i[kUpdSlices];
+  unsigned i[kUpdSlices], m[kUpdSlices], iso[kUpdSlices];
   T r0[kUpdSlices], q0[kUpdSlices], x0[kUpdSlices], p0[kUpdSlices];
+  Trip<M> dv[kUpdSlices];
+  // Both slices' row indices first (a group without the uniform shape loads
+  // them: one wait then covers both), then the vector entries of both
+  // slices, the slices' level-0 masks as scalar loads (the slice index is
+  // wavefront-uniform) and -- once the masks are in, the entries still in
+  // flight -- the D^-1 rows of forward level 0: requested with the entries,
+  // not after the stores.
 #pragma unroll
   for (unsigned h = 0; h < kUpdSlices; ++h) {
     L[h].lane = threadIdx.x & (kWave - 1);
-    L[h].s = wv * kUpdSlices + h;
+    L[h].s = __builtin_amdgcn_readfirstlane(wv * kUpdSlices + h);
     L[h].c = L[h].lane / kSliceRows;
     L[h].r = L[h].lane - L[h].c * kSliceRows;
     sl[h] = L[h].s < a.a_nsl;
     i[h] = row_of(a.a_plan, a.a_gv, L[h], 0, sl[h]);
+  }
+#pragma unroll
+  for (unsigned h = 0; h < kUpdSlices; ++h) {
     r0[h] = q0[h] = x0[h] = p0[h] = 0;
     if (i[h] != ~0u) {
       const size_t e = kVS * (size_t)i[h] + L[h].c;
@@ -655,6 +666,17 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
       x0[h] = a.x[e];
       p0[h] = a.p[e];
     }
+  }
+#pragma unroll
+  for (unsigned h = 0; h < kUpdSlices; ++h) {
+    m[h] = sl[h] ? __ldg(a.l0mask + L[h].s) : 0u;
+    iso[h] = sl[h] && iso_rows ? __ldg(a.isomask + L[h].s) : 0u;
+  }
+#pragma unroll
+  for (unsigned h = 0; h < kUpdSlices; ++h) {
+    dv[h] = Trip<M>{};
+    if (m[h] && L[h].c < 3 && ((m[h] >> L[h].r) & 1u))  // the lanes of level-0 rows
+      dv[h] = load_trip<M>(a.dinv0, L[h].s, L[h].lane);
   }
   if constexpr (kLate) {
     if (ld_state(&a.S->done)) return;
@@ -677,15 +699,14 @@ __global__ void __launch_bounds__(kPackBlock) k_pcg_update(PcgArgs<T, M> a) {
       }
       s2 += (double)rn * (double)rn;
     }
-    if (sl[h]) fwd_level0(a, L[h], i[h], rn, srp);
+    if (sl[h]) fwd_level0(a, L[h], i[h], rn, m[h], iso[h], dv[h], srp);
   }
   s2 = block_reduce_sum<kPackBlock>(s2);
-  const bool iso = a.upd_part_off != ~0u;  // rows with neither lower nor upper blocks exist
-  if (iso) srp = block_reduce_sum<kPackBlock>(srp);
+  if (iso_rows) srp = block_reduce_sum<kPackBlock>(srp);
   if (threadIdx.x == 0) {
     const unsigned w = blockIdx.x;
     a.partials2[w] = s2;
-    if (iso) a.partials3[a.upd_part_off + w] = srp;
+    if (iso_rows) a.partials3[a.upd_part_off + w] = srp;
   }
 }
 

@@ -750,7 +750,8 @@ def main():
                          "stiffness-warped time steps (BiCGStab) of the 823,875-DOF cube")
     ap.add_argument("--n", type=int, default=64, help="hexes per axis (64 = 823,875 DOF)")
     ap.add_argument("--sigma", type=float, default=0.5)
-    ap.add_argument("--ordering", default="color4", choices=["color8", "color4", "natural"])
+    ap.add_argument("--ordering", default="color4",
+                    choices=["color8", "color4", "multicolor", "natural"])
     ap.add_argument("--delta", type=float, default=1e-1,
                     help="mixed: refine in FP64 whenever the FP32 residual fell by this factor")
     ap.add_argument("--update-period", type=int, default=100,

@@ -53,6 +53,7 @@ SIGNATURES = {
     "dfm2h_memcpy_dtoh": [_vp, _vp, _sz],
     "dfm2h_memcpy_dtod": [_vp, _vp, _sz],
     "dfm2h_psup_tet": [_vp, _sz, _sz, _vp, _vp, _sz, _psz],
+    "dfm2h_multicolor_order": [_sz, _vp, _vp, _vp, _vp],
     "dfm2h_bsr_create": [_pvp, _sz, _u, _i],
     "dfm2h_bsr_destroy": [_vp],
     "dfm2h_bsr_set_pattern": [_vp, _vp, _sz, _vp, _sz],

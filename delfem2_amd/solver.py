@@ -89,6 +89,22 @@ def JArray_PSuP_MeshElem_Device(mesh: "TetMesh"):
     return ind, arr[: n.value]
 
 
+def MultiColorOrder(psup_ind: np.ndarray, psup: np.ndarray):
+    """Node relabelling for the level schedule (no reference counterpart):
+    new2old with the nodes colour by colour, no two neighbours sharing a
+    colour, and the colour count -- the ILU(0) sweeps' level bound in the
+    relabelled mesh (`workload.permute_mesh` applies it).  The pattern must
+    be symmetric without self entries, as `JArray_PSuP_MeshElem` builds it."""
+    ind = np.ascontiguousarray(psup_ind, np.uint32)
+    ps = np.ascontiguousarray(psup, np.uint32)
+    n = ind.size - 1
+    new2old = np.zeros(max(n, 1), np.uint32)
+    nc = C.c_uint(0)
+    N.call("dfm2h_multicolor_order", n, _hptr(ind), _hptr(ps) if ps.size else None,
+           _hptr(new2old), C.byref(nc))
+    return new2old[:n].astype(np.int64), int(nc.value)
+
+
 class CMatrixSparse:
     """3x3 block-CSR matrix with separate diagonal blocks, values in HBM."""
 

@@ -136,12 +136,24 @@ class Problem:
         return self.xyz.size
 
 
+def multicolor_permutation(tet: np.ndarray, np_: int):
+    """new->old node map of the library's multicolour ordering for any tet
+    mesh (dfm2h_multicolor_order on the mesh's one-ring pattern): nodes
+    colour by colour, no two neighbours sharing a colour -- the ILU(0) sweeps'
+    level count is the colour count.  Returns (new2old, ncolor)."""
+    from . import solver
+    ind, ps = solver.JArray_PSuP_MeshElem(tet, np_)
+    return solver.MultiColorOrder(ind, ps)
+
+
 def _order(xyz, tet, nx, ny, nz, ordering):
     if ordering == "natural":
         return xyz, tet, None
-    if ordering in ("color8", "color4"):
-        perm = color8_permutation if ordering == "color8" else color4_permutation
-        n2o = perm(nx, ny, nz)
+    if ordering in ("color8", "color4", "multicolor"):
+        if ordering == "multicolor":
+            n2o = multicolor_permutation(tet, xyz.size // 3)[0]
+        else:
+            n2o = (color8_permutation if ordering == "color8" else color4_permutation)(nx, ny, nz)
         xyz2, tet2, _ = permute_mesh(xyz, tet, n2o)
         return xyz2, tet2, n2o
     raise ValueError(ordering)

@@ -21,6 +21,8 @@
  *   Solve_PCG            lsitrsol.h:80-148, fused on the device
  *   Solve_CG             lsitrsol.h:36-74 (device views)
  *   LinearSystemSolver_BlockSparseILU  ls_solver_block_sparse_ilu.h:18-103
+ *   MultiColorOrder      (no reference counterpart) node relabelling that
+ *                        bounds the sweeps' levels by a colour count
  *
  * With these types the reference's own template
  *   delfem2::Solve_PCG(VEC&&, VEC&&, VEC&&, VEC&&, double, unsigned, const MAT&, const PREC&)
@@ -194,6 +196,23 @@ inline std::vector<double> Solve_PCG(ViewAsVectorXd &&r_vec, ViewAsVectorXd &&x_
                         hist.size(), &nh));
   hist.resize(nh);
   return hist;
+}
+
+// Node relabelling for the level schedule (no reference counterpart; the C
+// ABI's dfm2h_multicolor_order): new2old lists the nodes colour by colour,
+// no two neighbours sharing a colour, from the symmetric one-ring pattern
+// (psup_ind, psup) that JArray_PSuP_MeshElem builds.  Returns the colour
+// count: the ILU(0) sweeps' levels in the relabelled mesh.
+inline unsigned int MultiColorOrder(std::vector<unsigned int> &new2old,
+                                    const std::vector<unsigned int> &psup_ind,
+                                    const std::vector<unsigned int> &psup) {
+  static const unsigned int kZero = 0;
+  const size_t np = psup_ind.empty() ? 0 : psup_ind.size() - 1;
+  new2old.resize(np);
+  unsigned int nc = 0;
+  check(dfm2h_multicolor_order(np, psup_ind.empty() ? &kZero : psup_ind.data(), psup.data(),
+                               new2old.data(), &nc));
+  return nc;
 }
 
 // fem_solidlinear.h:192-237 on device arrays; per-element Lame arrays may be

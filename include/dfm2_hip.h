@@ -76,6 +76,20 @@ int dfm2h_memcpy_dtod(void *dst_dev, const void *src_dev, size_t bytes);
 int dfm2h_psup_tet(const unsigned *tet, size_t ntet, size_t np,
                    unsigned *psup_ind, unsigned *psup, size_t psup_cap,
                    size_t *npsup);
+/* Node relabelling for the level schedule (no reference counterpart: the
+ * reference factorises in the caller's node order, where a voxel cube of n^3
+ * hexes has 3n+1 dependency levels per triangular sweep).  From a symmetric
+ * one-ring pattern without self entries (as dfm2h_psup_tet builds it),
+ * colours the nodes so that no two neighbours share a colour (first fit in
+ * DSATUR order) and writes new2old[np]: the nodes colour by colour,
+ * ascending old index inside a colour.  A mesh relabelled by it (node k of
+ * the new mesh = node new2old[k] of the old) gives ILU(0) at most *ncolor
+ * levels per sweep -- 4 on a Kuhn-split voxel grid.  The relabelled system is
+ * the same equations; ILU(0)'s fill pattern, and so the iteration count,
+ * depend on the order.  Host arrays; DFM2H_ERR_INVALID for a pattern that is
+ * not symmetric, has self entries or entries out of range. */
+int dfm2h_multicolor_order(size_t np, const unsigned *psup_ind, const unsigned *psup,
+                           unsigned *new2old, unsigned *ncolor);
 
 /* ---- CMatrixSparse<double> (ls_block_sparse.h) ---------------------------- */
 /* ls_block_sparse.h:45-55 Initialize(nblk, len, is_dia); len must be 3 and
